@@ -1,0 +1,207 @@
+/*
+ * xpcs_mt.h -- C ABI of the MI355X multi-tau correlator.
+ *
+ * Plain pointers and sizes only; no C++ types cross this boundary and no
+ * exception escapes it.  Every entry point returns an xpcs_status (0 = OK);
+ * xpcs_last_error() gives a thread-local message for the last failure.
+ *
+ * The reference (AdvancedPhotonSource/xpcs-eigen) has no FFI for this path; its
+ * in-process interface is the static class xpcs::Corr (src/xpcs/corr.h:66-126)
+ * fed by a Reader -> Filter -> SparseData chain and parameterised by the
+ * Configuration singleton.  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Units.  Per-pixel sums are exact integers in "summed-bin" units: at level L
+ * the binned value of frame k is x_L[k] = sum of the 2^L level-0 counts it
+ * covers (the reference stores x_L / 2^L), with N = lastframe_L - tau' and
+ * tau' = tau / 2^L:
+ *     G2 = sum_{t<N} x_L[t] x_L[t+tau'],  IP = sum_{t<N} x_L[t],
+ *     IF = sum_{tau'<=t<lastframe_L} x_L[t].
+ * The reference's float outputs are G2/4^L/N, IP/2^L/N, IF/2^L/N (times the
+ * flat-field and 1/avg factors of each pixel).
+ *
+ * Ownership.  A plan owns all device memory it allocates.  Host buffers passed
+ * in are read during the call and never retained.  Inputs are never mutated.
+ *
+ * Threading.  One plan per host thread; plans on different streams may run
+ * concurrently.  The library keeps no mutable global state besides the
+ * thread-local error message.
+ */
+#ifndef XPCS_MT_H
+#define XPCS_MT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t xpcs_status;
+#define XPCS_OK 0
+#define XPCS_ERR_INVALID_ARG 1   /* bad sizes, NULL pointers, inconsistent desc */
+#define XPCS_ERR_HIP 2           /* a HIP runtime call failed */
+#define XPCS_ERR_FORMAT 3        /* malformed / truncated IMM data */
+#define XPCS_ERR_RANGE 4         /* pixel index outside the detector */
+#define XPCS_ERR_UNSUPPORTED 5   /* option combination outside the exact path */
+#define XPCS_ERR_STATE 6         /* call out of order (e.g. read before execute) */
+#define XPCS_ERR_NOMEM 7         /* device allocation failed */
+#define XPCS_ERR_NO_DEVICE 8     /* no HIP device visible */
+
+/* desc.flags */
+#define XPCS_MT_TIMING 0x1u      /* record HIP events around every kernel */
+
+typedef struct xpcs_mt_plan xpcs_mt_plan;
+
+/*
+ * Analysis description.  Replaces the Configuration singleton fields the
+ * multi-tau path reads (src/xpcs/configuration.cpp:80-242, :594-606).
+ */
+typedef struct xpcs_mt_desc {
+    int32_t width;              /* x_dimension (columns)                      */
+    int32_t height;             /* y_dimension (rows)                         */
+    int32_t frames;             /* output frames = getFrameTodoCount()        */
+    int32_t delays_per_level;   /* delays_per_level (>= 1)                    */
+    int32_t stride;             /* stride_frames (>= 1)                       */
+    int32_t average;            /* avg_frames (>= 1)                          */
+    int32_t frame_from;         /* getFrameStartTodo(): first data frame, 0-based */
+    int32_t dark_frames;        /* getDarkFrames(): leading dark frames (dense files) */
+    float lld;                  /* getDarkThreshold()                         */
+    float sigma;                /* getDarkSigma()                             */
+    const int32_t* dqmap;       /* [height*width] dynamic q-map, row-major    */
+    const int32_t* sqmap;       /* [height*width] static q-map, row-major     */
+    const double* flatfield;    /* [height*width] or NULL (flatfield_enabled) */
+    uint32_t flags;             /* XPCS_MT_* */
+} xpcs_mt_desc;
+
+typedef struct xpcs_mt_info {
+    int32_t taus;               /* T = Corr::delaysPerLevel(...).size()       */
+    int32_t max_level;          /* Corr::calculateLevelMax(frames, dpl)        */
+    int32_t dynamic_bins;       /* Q (rows of norm-0-g2 per tau)               */
+    int32_t static_bins;        /* S (largest static bin)                      */
+    int32_t pixels;             /* P masked pixels (dqmap>=1 && sqmap>=1)      */
+    int32_t mapped_pixels;      /* pixels that reach normalisation (m_mapping) */
+    int32_t groups;             /* (q, s) groups of m_mapping                  */
+    int32_t pad;
+    int64_t events;             /* level-0 events after the last ingest        */
+    int64_t long_pixels;        /* pixels handled by the dense-row kernel      */
+} xpcs_mt_info;
+
+/* Library version string. */
+const char* xpcs_version(void);
+
+/* Thread-local message for the last non-OK status. */
+const char* xpcs_last_error(void);
+
+/*
+ * Tau schedule.  Replace Corr::calculateLevelMax (src/xpcs/corr.cpp:1156-1160),
+ * Corr::delaysPerLevel (:1133-1154) and Corr::calculateDelayCount (:1162-1164).
+ * xpcs_delays_per_level writes up to cap entries and stores the full length in
+ * *count.
+ */
+int32_t xpcs_level_max(int32_t frames, int32_t dpl);
+int32_t xpcs_delay_count(int32_t dpl, int32_t level);
+xpcs_status xpcs_delays_per_level(int32_t frames, int32_t dpl, int32_t max_level,
+                                  int32_t* level, int32_t* tau, int32_t cap, int32_t* count);
+
+/*
+ * Plan: validates the description, builds the q-partition tables
+ * (Configuration::BuildQMap, src/xpcs/configuration.cpp:244-381) and the
+ * schedule, and allocates the device tables on the current HIP device.
+ */
+xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_plan** plan);
+xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* plan);
+xpcs_status xpcs_mt_plan_info(const xpcs_mt_plan* plan, xpcs_mt_info* info);
+/* Schedule (level, tau) of the plan, T entries each. */
+xpcs_status xpcs_mt_schedule(const xpcs_mt_plan* plan, int32_t* level, int32_t* tau);
+/* Flat detector index of each compact pixel, P entries (m_mapping order). */
+xpcs_status xpcs_mt_pixel_order(const xpcs_mt_plan* plan, int32_t* flat_index);
+
+/*
+ * Ingest a whole IMM file held in host memory.  Replaces the load loop of
+ * src/xpcs/main.cpp:222-268 with io::Imm (src/xpcs/io/imm.cpp:59-129): dark
+ * frames first (uncompressed files), skip to frame_from, then
+ * frames*stride*average raw frames.  Compressed files take the sparse path
+ * (SparseFilter, src/xpcs/filter/sparse_filter.cpp:115-193), uncompressed files
+ * the dense path (DenseFilter, src/xpcs/filter/dense_filter.cpp:121-210).
+ * The needed bytes are copied to the device on the plan's stream.
+ */
+xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* plan, const void* imm, size_t nbytes);
+
+/*
+ * Ingest an IMM image already resident in device memory.  payload_off[i] and
+ * dlen[i] describe raw frame i (byte offset of its payload in dev_imm and its
+ * entry count, as Imm::NextFrames reads them); compressed != 0 selects the
+ * sparse layout.  The device buffer must stay valid until execute returns.
+ */
+xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* plan, const void* dev_imm, size_t nbytes,
+                                      const int64_t* payload_off, const uint32_t* dlen,
+                                      int64_t raw_frames, int32_t compressed);
+
+/*
+ * Index the headers of an IMM file in host memory (io::Header fields of
+ * src/xpcs/io/imm.h:63-144).  Writes up to cap entries, total in *frames.
+ */
+xpcs_status xpcs_imm_index(const void* imm, size_t nbytes, int64_t* payload_off, uint32_t* dlen,
+                           double* elapsed, int32_t* corecotick, int64_t cap, int64_t* frames,
+                           int32_t* compressed);
+
+/*
+ * Ingest pixel-major rows (the SparseData layout of
+ * src/xpcs/data_structure/sparse_data.h:57-79 as filled by a Filter):
+ * row_off[W*H+1] into frame[] (ascending per row) and count[] (integer
+ * values).  Rows of masked-out pixels are ignored.
+ */
+xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* plan, const int64_t* row_off, const int32_t* frame,
+                                const uint32_t* count);
+
+/*
+ * Run the path on `stream` (a hipStream_t; NULL = the plan's own stream):
+ * (a) frame->pixel transpose, (b)+(c) multi-tau binning and lag sums
+ * (Corr::multiTau2, src/xpcs/corr.cpp:315-431), (d) partition reduction and
+ * normalisation (Corr::normalizeG2s, src/xpcs/corr.cpp:927-1091).
+ * Asynchronous; the read functions synchronise.
+ */
+xpcs_status xpcs_mt_execute(xpcs_mt_plan* plan, void* stream);
+xpcs_status xpcs_mt_synchronize(xpcs_mt_plan* plan);
+
+/* Exact per-pixel sums, [P][T] int64 in compact pixel order (summed-bin units). */
+xpcs_status xpcs_mt_read_sums_i64(xpcs_mt_plan* plan, int64_t* G2, int64_t* IP, int64_t* IF);
+
+/*
+ * Per-pixel sums in the reference's float units and layout [T][W*H]
+ * (the G2s/IPs/IFs arrays of src/xpcs/main.cpp:193-201 after multiTau2; the
+ * --g2out datasets).  Pixels without events are 0.
+ */
+xpcs_status xpcs_mt_read_ref_f32(xpcs_mt_plan* plan, float* G2, float* IP, float* IF);
+
+/* g2 and stderr, [T][Q] float64 (norm-0-g2 / norm-0-stderr before the f32 cast). */
+xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* plan, double* g2, double* stderr_);
+
+/* Device pointers of the [T][Q] results (valid until the next execute/destroy). */
+xpcs_status xpcs_mt_device_results(xpcs_mt_plan* plan, const double** g2, const double** stderr_);
+
+/*
+ * Side statistics of the load loop (SparseFilter/DenseFilter accessors,
+ * src/xpcs/filter/filter.h:62-96, scaled as src/xpcs/main.cpp:339-411 does):
+ * pixel_sum [W*H] (pixelSum), frame_sum [2][frames] (frameSum),
+ * partition_mean [S] (partition-mean-total),
+ * partition_partial [floor(frames/window)][S] (partition-mean-partial).
+ * window = static_mean_window_size.  Any pointer may be NULL.
+ */
+xpcs_status xpcs_mt_read_side(xpcs_mt_plan* plan, int32_t window, float* pixel_sum,
+                              float* frame_sum, float* partition_mean, float* partition_partial);
+
+/*
+ * Kernel timing (desc.flags & XPCS_MT_TIMING): per kernel name, the summed
+ * event-measured milliseconds and launch count since the last reset.
+ * Names are written as a '\n'-separated list into names (cap bytes).
+ */
+xpcs_status xpcs_mt_kernel_times(xpcs_mt_plan* plan, char* names, size_t cap, double* ms,
+                                 int64_t* launches, int32_t max_kernels, int32_t* n_kernels);
+xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,295 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class ImmFrame(C.Structure):
+    _fields_ = [("payload_off", C.c_int64), ("dlen", C.c_uint32), ("compressed", C.c_int32),
+                ("elapsed", C.c_double), ("corecotick", C.c_int32), ("pad", C.c_int32)]
+
+
+FRAME_DTYPE = np.dtype([("payload_off", np.int64), ("dlen", np.uint32), ("compressed", np.int32),
+                        ("elapsed", np.float64), ("corecotick", np.int32), ("pad", np.int32)])
+framep = np.ctypeslib.ndpointer(FRAME_DTYPE, flags="C_CONTIGUOUS")
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = HERE / "liboracle.so"
+        if not so.exists():
+            build()
+        L = C.CDLL(str(so))
+        L.xo_level_max.argtypes = [C.c_int, C.c_int]
+        L.xo_level_max.restype = C.c_int
+        L.xo_delays_per_level.argtypes = [C.c_int, C.c_int, C.c_int, i32p, i32p, C.c_int]
+        L.xo_delays_per_level.restype = C.c_int
+        L.xo_build_qmap.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, i32p, i32p,
+                                    i32p, i32p, C.c_int, C.c_int, C.c_int, i32p]
+        L.xo_build_qmap.restype = C.c_int
+        L.xo_imm_index.argtypes = [u8p, C.c_int64, framep, C.c_int64]
+        L.xo_imm_index.restype = C.c_int64
+        L.xo_select_frames.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       i64p, i32p, C.c_int64]
+        L.xo_select_frames.restype = C.c_int64
+        L.xo_sparse_transpose.argtypes = [u8p, framep, i64p, i32p, C.c_int64, C.c_int, i32p,
+                                          C.c_int32, C.c_void_p, C.c_int, i64p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_int64]
+        L.xo_sparse_transpose.restype = C.c_int64
+        L.xo_dark_image.argtypes = [u8p, framep, C.c_int, C.c_int, C.c_void_p, f64p, f64p, i64p]
+        L.xo_dark_image.restype = None
+        L.xo_dense_transpose.argtypes = [u8p, framep, i64p, i32p, C.c_int64, C.c_int, i32p,
+                                         C.c_int32, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_float, C.c_float, i64p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.xo_dense_transpose.restype = C.c_int64
+        L.xo_multitau_exact.argtypes = [C.c_int64, i64p, i32p, i64p, C.c_int, C.c_int, i32p, i32p,
+                                        i64p, i64p, i64p, C.c_int]
+        L.xo_multitau_exact.restype = None
+        L.xo_multitau_compat.argtypes = [C.c_int64, i64p, i32p, f32p, i32p, C.c_int64, C.c_int,
+                                         C.c_int, i32p, i32p, C.c_int, f32p, f32p, f32p, C.c_int]
+        L.xo_multitau_compat.restype = None
+        L.xo_normalize_f64.argtypes = [C.c_int, i32p, C.c_int64, i64p, i64p, i64p, C.c_int, i32p,
+                                       i32p, i32p, C.c_int, C.c_void_p, f64p, f64p]
+        L.xo_normalize_f64.restype = None
+        L.xo_normalize_compat.argtypes = [C.c_int, C.c_int64, f32p, f32p, f32p, i32p, C.c_int,
+                                          i32p, i32p, i32p, i32p, C.c_int, C.c_int, f32p, f32p]
+        L.xo_normalize_compat.restype = None
+        L.xo_cpu_pipeline.argtypes = [u8p, C.c_int64, C.c_int, C.c_int, i32p, i32p, C.c_int,
+                                      C.c_int, C.c_int, f32p, f32p, f64p]
+        L.xo_cpu_pipeline.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- schedule ---------------------------------------------------------------
+def level_max(frames, dpl):
+    return lib().xo_level_max(frames, dpl)
+
+
+def schedule(frames, dpl):
+    """(level[T], tau[T]) of Corr::delaysPerLevel for frames, dpl."""
+    ml = level_max(frames, dpl)
+    lev = np.zeros(4096, np.int32)
+    tau = np.zeros(4096, np.int32)
+    n = lib().xo_delays_per_level(frames, dpl, ml, lev, tau, 4096)
+    return lev[:n].copy(), tau[:n].copy()
+
+
+def schedule_N(frames, level, tau):
+    """N = lastframe_L - tau' per schedule entry."""
+    lf = np.array([frames >> int(L) for L in level], dtype=np.int64)
+    return (lf - (tau >> level)).astype(np.int32)
+
+
+# ---- q map --------------------------------------------------------------------
+def build_qmap(dqmap, sqmap):
+    dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+    sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+    h, w = np.asarray(dqmap).shape
+    gcap = 1 << 16
+    order = np.zeros(w * h, np.int32)
+    gq, gs, gb, ge = (np.zeros(gcap, np.int32) for _ in range(4))
+    qcap = int(max(dq.max(), 0)) + 2
+    scap = int(max(sq.max(), 0)) + 2
+    qfirst = np.zeros(qcap, np.int32)
+    ppb = np.zeros(scap, np.int32)
+    counts = np.zeros(6, np.int32)
+    rc = lib().xo_build_qmap(w, h, dq, sq, order, gq, gs, gb, ge, qfirst, ppb, gcap, qcap, scap, counts)
+    assert rc == 0
+    npix, n_mapped, ng, nq, ns, nkeys = (int(x) for x in counts)
+    pix2c = np.full(w * h, -1, np.int32)
+    pix2c[order[:npix]] = np.arange(npix, dtype=np.int32)
+    return dict(order=order[:npix].copy(), pix2c=pix2c, n_mapped=n_mapped, ngroups=ng,
+                group_q=gq[:ng].copy(), group_s=gs[:ng].copy(), group_begin=gb[:ng].copy(),
+                group_end=ge[:ng].copy(), q_first_group=qfirst[:nq + 1].copy(),
+                pixels_per_bin=ppb[:ns].copy(), nq=nq, ns=ns, nkeys=nkeys, npix=npix,
+                width=w, height=h)
+
+
+# ---- IMM / filters --------------------------------------------------------------
+def imm_index(buf):
+    b = np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf
+    b = np.ascontiguousarray(b)
+    cap = max(16, b.size // 1024 + 1)
+    fr = np.zeros(cap, FRAME_DTYPE)
+    n = lib().xo_imm_index(b, b.size, fr, cap)
+    if n < 0:
+        raise ValueError("truncated IMM payload")
+    return fr[:n].copy()
+
+
+def select_frames(n_raw, frames, darks=0, frame_from=0, stride=1, avg=1):
+    cap = frames * avg + 1
+    sel = np.zeros(cap, np.int64)
+    of = np.zeros(cap, np.int32)
+    n = lib().xo_select_frames(n_raw, darks, frame_from, frames, stride, avg, sel, of, cap)
+    if n < 0:
+        raise ValueError("IMM has fewer frames than the selection needs")
+    return sel[:n].copy(), of[:n].copy()
+
+
+def sparse_transpose(buf, frames_idx, sel, out_frame, qm, flatfield=None, avg=1):
+    b = np.ascontiguousarray(np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf)
+    P = qm["npix"]
+    off = np.zeros(P + 1, np.int64)
+    ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+    npd = qm["width"] * qm["height"]
+    n = lib().xo_sparse_transpose(b, frames_idx, sel, out_frame, sel.size, npd, qm["pix2c"], P,
+                                  _ptr(ff), avg, off, None, None, None, 0)
+    if n < 0:
+        raise ValueError("pixel index out of range")
+    fr = np.zeros(max(n, 1), np.int32)
+    cnt = np.zeros(max(n, 1), np.int64)
+    val = np.zeros(max(n, 1), np.float32)
+    n2 = lib().xo_sparse_transpose(b, frames_idx, sel, out_frame, sel.size, npd, qm["pix2c"], P,
+                                   _ptr(ff), avg, off, _ptr(fr), _ptr(cnt), _ptr(val), max(n, 1))
+    assert n2 == n
+    return dict(off=off, frame=fr[:n], cnt=cnt[:n], val=val[:n])
+
+
+def dark_image(buf, frames_idx, darks, npix_det, flatfield=None):
+    b = np.ascontiguousarray(np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf)
+    avgd = np.zeros(npix_det, np.float64)
+    std = np.zeros(npix_det, np.float64)
+    s = np.zeros(npix_det, np.int64)
+    ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+    lib().xo_dark_image(b, frames_idx, darks, npix_det, _ptr(ff), avgd, std, s)
+    return avgd, std, s
+
+
+def dense_transpose(buf, frames_idx, sel, out_frame, qm, darks=0, lld=0.0, sigma=0.0,
+                    flatfield=None, avg=1):
+    b = np.ascontiguousarray(np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf)
+    P = qm["npix"]
+    npd = qm["width"] * qm["height"]
+    off = np.zeros(P + 1, np.int64)
+    ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+    davg = dstd = dsum = None
+    if darks > 0:
+        davg, dstd, dsum = dark_image(b, frames_idx, darks, npd, ff)
+    args = (b, frames_idx, sel, out_frame, sel.size, npd, qm["pix2c"], P, _ptr(ff), avg, darks,
+            _ptr(davg), _ptr(dstd), _ptr(dsum), C.c_float(lld), C.c_float(sigma), off)
+    n = lib().xo_dense_transpose(*args, None, None, None, 0)
+    if n < 0:
+        raise ValueError("dense frame size mismatch")
+    fr = np.zeros(max(n, 1), np.int32)
+    cnt = np.zeros(max(n, 1), np.int64)
+    val = np.zeros(max(n, 1), np.float32)
+    n2 = lib().xo_dense_transpose(*args, _ptr(fr), _ptr(cnt), _ptr(val), max(n, 1))
+    assert n2 == n
+    return dict(off=off, frame=fr[:n], cnt=cnt[:n], val=val[:n], dark_avg=davg, dark_std=dstd,
+                dark_sum=dsum)
+
+
+# ---- correlator ------------------------------------------------------------------
+def multitau_exact(csr, frames, level, tau, nthreads=0):
+    P = csr["off"].size - 1
+    T = level.size
+    G2 = np.zeros((P, T), np.int64)
+    IP = np.zeros((P, T), np.int64)
+    IF = np.zeros((P, T), np.int64)
+    lib().xo_multitau_exact(P, csr["off"], np.ascontiguousarray(csr["frame"], np.int32),
+                            np.ascontiguousarray(csr["cnt"], np.int64), frames, T,
+                            level, tau, G2, IP, IF, nthreads)
+    return G2, IP, IF
+
+
+def multitau_compat(csr, c2pix, npix_det, frames, level, tau, stale_tail=True, nthreads=0):
+    P = csr["off"].size - 1
+    T = level.size
+    G2 = np.zeros((T, npix_det), np.float32)
+    IP = np.zeros((T, npix_det), np.float32)
+    IF = np.zeros((T, npix_det), np.float32)
+    lib().xo_multitau_compat(P, csr["off"], np.ascontiguousarray(csr["frame"], np.int32),
+                             np.ascontiguousarray(csr["val"], np.float32),
+                             np.ascontiguousarray(c2pix, np.int32), npix_det, frames, T, level,
+                             tau, int(bool(stale_tail)), G2, IP, IF, nthreads)
+    return G2, IP, IF
+
+
+def normalize_f64(G2, IP, IF, Nvec, qm, weight=None):
+    T = G2.shape[1]
+    nq = qm["nq"]
+    g2 = np.zeros((T, nq), np.float64)
+    se = np.zeros((T, nq), np.float64)
+    w = None if weight is None else np.ascontiguousarray(weight, np.float64)
+    lib().xo_normalize_f64(T, np.ascontiguousarray(Nvec, np.int32), G2.shape[0],
+                           np.ascontiguousarray(G2), np.ascontiguousarray(IP),
+                           np.ascontiguousarray(IF), qm["ngroups"], qm["group_q"],
+                           qm["group_begin"], qm["group_end"], nq, _ptr(w), g2, se)
+    return g2, se
+
+
+def normalize_compat(G2r, IPr, IFr, qm):
+    T, npd = G2r.shape
+    nq = qm["nq"]
+    g2 = np.zeros((T, nq), np.float32)
+    se = np.zeros((T, nq), np.float32)
+    lib().xo_normalize_compat(T, npd, G2r, IPr, IFr, qm["order"], qm["ngroups"], qm["group_q"],
+                              qm["group_s"], qm["group_begin"], qm["group_end"], nq, qm["ns"],
+                              g2, se)
+    return g2, se
+
+
+def cpu_pipeline(buf, width, height, dqmap, sqmap, frames, dpl, nthreads=0):
+    b = np.ascontiguousarray(np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else buf)
+    dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+    sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+    lev, tau = schedule(frames, dpl)
+    nq = int(dq.max())
+    g2 = np.zeros((lev.size, nq), np.float32)
+    se = np.zeros((lev.size, nq), np.float32)
+    st = np.zeros(4, np.float64)
+    rc = lib().xo_cpu_pipeline(b, b.size, width, height, dq, sq, frames, dpl, nthreads, g2, se, st)
+    if rc < 0:
+        raise RuntimeError(f"cpu pipeline failed rc={rc}")
+    return g2, se, st
+
+
+# ---- convenience: whole exact path from IMM bytes ------------------------------------
+def run_exact(buf, dqmap, sqmap, frames, dpl, darks=0, frame_from=0, stride=1, avg=1,
+              flatfield=None, lld=0.0, sigma=0.0, nthreads=0):
+    qm = build_qmap(dqmap, sqmap)
+    fr = imm_index(buf)
+    sel, of = select_frames(fr.size, frames, darks if (fr.size and not fr[0]["compressed"]) else 0,
+                            frame_from, stride, avg)
+    if fr.size and fr[0]["compressed"]:
+        csr = sparse_transpose(buf, fr, sel, of, qm, flatfield, avg)
+    else:
+        csr = dense_transpose(buf, fr, sel, of, qm, darks, lld, sigma, flatfield, avg)
+    lev, tau = schedule(frames, dpl)
+    G2, IP, IF = multitau_exact(csr, frames, lev, tau, nthreads)
+    Nv = schedule_N(frames, lev, tau)
+    weight = None
+    if flatfield is not None:
+        weight = np.asarray(flatfield, np.float64).reshape(-1)[qm["order"]]
+    g2, se = normalize_f64(G2, IP, IF, Nv, qm, weight)
+    return dict(qm=qm, csr=csr, level=lev, tau=tau, N=Nv, G2=G2, IP=IP, IF=IF, g2=g2, stderr=se)

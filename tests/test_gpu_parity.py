@@ -1,0 +1,144 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (bit-exact integer sums,
+g2/stderr within 1e-6 relative of the FP64 oracle).  Calls go through the C ABI."""
+import numpy as np
+import pytest
+
+import cases
+import synth
+
+pytestmark = pytest.mark.gpu
+
+G2_RTOL = 1e-6      # g2(q, tau) vs the FP64 oracle
+REF_RTOL = 1e-4     # reference float units vs the float replica of the reference
+
+
+def run_gpu(xmt, c, **kw):
+    plan = xmt.Plan(c["W"], c["H"], c["F"], c["dpl"], c["dq"], c["sq"], **kw)
+    plan.ingest_imm(c["imm"])
+    plan.execute()
+    return plan
+
+
+def assert_g2_close(got, want, rtol=G2_RTOL):
+    assert got.shape == want.shape
+    nan_w = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan_w), "NaN pattern differs"
+    g, w = got[~nan_w], want[~nan_w]
+    rel = np.abs(g - w) / np.maximum(np.abs(w), 1e-300)
+    assert rel.max(initial=0.0) <= rtol, f"max rel err {rel.max()}"
+
+
+def check_against_oracle(xmt, orc, c, **kw):
+    print(f"case W={c['W']} H={c['H']} F={c['F']} dpl={c['dpl']} kw={kw}", flush=True)
+    plan = run_gpu(xmt, c, **kw)
+    okw = {k: v for k, v in kw.items() if k in ("stride", "avg", "frame_from", "darks", "flatfield",
+                                               "lld", "sigma")}
+    if "average" in kw:
+        okw["avg"] = kw["average"]
+    if "dark_frames" in kw:
+        okw["darks"] = kw["dark_frames"]
+    r = orc.run_exact(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"], **okw)
+    lev, tau = plan.schedule()
+    assert np.array_equal(lev, r["level"]) and np.array_equal(tau, r["tau"])
+    assert np.array_equal(plan.pixel_order(), r["qm"]["order"])
+    G2, IP, IF = plan.sums()
+    assert np.array_equal(IP, r["IP"]), "IP differs"
+    assert np.array_equal(IF, r["IF"]), "IF differs"
+    assert np.array_equal(G2, r["G2"]), "G2 differs"
+    g2, se = plan.g2()
+    assert_g2_close(g2, r["g2"])
+    assert_g2_close(se, r["stderr"])
+    return plan, r
+
+
+def test_small_sparse(xmt, orc):
+    check_against_oracle(xmt, orc, cases.sparse_case())
+
+
+def test_counterexample_exact(xmt, orc):
+    c = cases.counterexample_case()
+    plan, r = check_against_oracle(xmt, orc, c)
+    lev, tau = plan.schedule()
+    i = int(np.flatnonzero((lev == 1) & (tau == 10))[0])
+    G2, _, _ = plan.sums()
+    assert G2[0, i] == 16          # 4 pairs of binned value 2*2 (summed-bin units)
+
+
+def test_config1_plain(xmt, orc):
+    """256x256, 1000 frames, mu=0.012, tau_c=20, dpl=4, 18 static / 6 dynamic rings."""
+    c = cases.sparse_case(W=256, H=256, F=1000, dpl=4, mu=0.012, tau_c=20.0, seed=1001, S=18, Q=6)
+    plan, r = check_against_oracle(xmt, orc, c)
+    assert plan.T == 34
+
+
+def test_long_rows(xmt, orc):
+    """High occupancy: rows longer than the on-chip capacity take the dense-row kernel."""
+    c = cases.sparse_case(W=16, H=16, F=3000, dpl=4, mu=2.0, tau_c=30.0, seed=3)
+    plan, _ = check_against_oracle(xmt, orc, c)
+    assert plan.info().long_pixels > 0
+
+
+def test_mixed_rows(xmt, orc):
+    c = cases.sparse_case(W=24, H=24, F=2000, dpl=8, mu=0.3, tau_c=5.0, seed=11)
+    check_against_oracle(xmt, orc, c)
+
+
+def test_stride_average_skip(xmt, orc):
+    c = cases.sparse_case(W=20, H=20, F=1300, dpl=4, mu=0.2, tau_c=8.0, seed=5)
+    c["F"] = 200
+    check_against_oracle(xmt, orc, c, stride=2, average=3, frame_from=4)
+
+
+def test_dense_with_darks(xmt, orc):
+    W = H = 24
+    F = 400
+    dq, sq = synth.annular_qmaps(W, H, 9, 3)
+    sig = synth.ou_counts(W, H, F, 5.0, 20.0, seed=1003)
+    dark = synth.dark_frames(W, H, 2, seed=2003)
+    frames = np.concatenate([dark, sig + dark[0]], axis=0)
+    imm = synth.imm_bytes_dense(frames, W, H)
+    c = dict(W=W, H=H, F=F, dpl=4, dq=dq, sq=sq, imm=imm)
+    plan, r = check_against_oracle(xmt, orc, c, dark_frames=2)
+    assert plan.info().long_pixels == plan.P
+
+
+def test_flatfield(xmt, orc):
+    c = cases.sparse_case(W=32, H=32, F=600, dpl=4, mu=0.1, tau_c=10.0, seed=9)
+    ff = np.random.default_rng(0).uniform(0.5, 1.5, (32, 32))
+    check_against_oracle(xmt, orc, c, flatfield=ff)
+
+
+def test_ref_units_vs_float_replica(xmt, orc):
+    """--g2out arrays vs the float replica of multiTau2 without the stale-tail search."""
+    c = cases.sparse_case(W=32, H=32, F=500, dpl=4, mu=0.05, tau_c=10.0, seed=7)
+    plan = run_gpu(xmt, c)
+    r = orc.run_exact(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"])
+    G2r, IPr, IFr = plan.ref_f32()
+    G2c, IPc, IFc = orc.multitau_compat(r["csr"], r["qm"]["order"], c["W"] * c["H"], c["F"],
+                                        r["level"], r["tau"], stale_tail=False)
+    for got, want in ((G2r, G2c), (IPr, IPc), (IFr, IFc)):
+        rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+        assert rel.max() <= REF_RTOL
+
+
+def test_pixel_out_of_range_is_an_error(xmt):
+    W = H = 8
+    ev = [(np.array([3, 70], np.int32), np.array([1, 1], np.uint16))] * 20
+    imm = synth.imm_bytes_events(ev, W, H)
+    dq, sq = synth.annular_qmaps(W, H, 2, 1)
+    plan = xmt.Plan(W, H, 20, 2, dq, sq)
+    plan.ingest_imm(imm)
+    plan.execute()
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.g2()
+    assert e.value.status == 4
+
+
+def test_repeated_execute_is_stable(xmt, orc):
+    c = cases.sparse_case(W=32, H=32, F=400, dpl=4, mu=0.08, tau_c=6.0, seed=21)
+    plan = run_gpu(xmt, c)
+    a = plan.sums()
+    plan.execute()
+    b = plan.sums()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -1,0 +1,698 @@
+// C ABI of the multi-tau library: plans, ingest, execute, results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+using xpcs::fail;
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(e_ == hipErrorOutOfMemory ? XPCS_ERR_NOMEM : XPCS_ERR_HIP,         \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));                    \
+    } while (0)
+
+namespace xpcs {
+int sparse_cap();
+
+template <typename T>
+static xpcs_status dalloc(T** p, size_t n)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(XPCS_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    return XPCS_OK;
+}
+
+template <typename T>
+static xpcs_status upload(T** p, const std::vector<T>& v, hipStream_t s)
+{
+    xpcs_status st = dalloc(p, v.size());
+    if (st) return st;
+    if (!v.empty()) {
+        hipError_t e = hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail(XPCS_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    return XPCS_OK;
+}
+
+template <typename T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name)
+{
+    if (!(p->h.flags & XPCS_MT_TIMING)) return;
+    KernelTimer t;
+    t.name = name;
+    if (p->event_pool.size() >= 2) {
+        t.a = p->event_pool.back(); p->event_pool.pop_back();
+        t.b = p->event_pool.back(); p->event_pool.pop_back();
+    } else {
+        (void)hipEventCreate(&t.a);
+        (void)hipEventCreate(&t.b);
+    }
+    (void)hipEventRecord(t.a, s);
+    p->pending.push_back(t);
+}
+
+void timer_end(xpcs_mt_plan* p, hipStream_t s)
+{
+    if (!(p->h.flags & XPCS_MT_TIMING)) return;
+    (void)hipEventRecord(p->pending.back().b, s);
+}
+
+static void collect_timers(xpcs_mt_plan* p)
+{
+    for (auto& t : p->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            auto& e = p->times[t.name];
+            e.first += ms;
+            e.second += 1;
+        }
+        p->event_pool.push_back(t.a);
+        p->event_pool.push_back(t.b);
+    }
+    p->pending.clear();
+}
+
+}  // namespace xpcs
+
+using namespace xpcs;
+
+extern "C" const char* xpcs_version(void) { return "xpcs-mt 0.1 (gfx950)"; }
+
+extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_plan** out)
+{
+    if (!desc || !out) return fail(XPCS_ERR_INVALID_ARG, "plan_create: NULL argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(XPCS_ERR_NO_DEVICE, "plan_create: no HIP device visible");
+    xpcs_mt_plan* p = new (std::nothrow) xpcs_mt_plan();
+    if (!p) return fail(XPCS_ERR_NOMEM, "plan_create: host allocation failed");
+    xpcs_status st = build_tables(*desc, p->h);
+    if (st) {
+        delete p;
+        return st;
+    }
+    HostTables& h = p->h;
+    (void)hipGetDevice(&p->device);
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete p;
+        return fail(XPCS_ERR_HIP, "plan_create: stream creation failed");
+    }
+    p->own_stream = true;
+    hipStream_t s = p->stream;
+    std::vector<int32_t> gsize((size_t)h.ngroups);
+    for (int g = 0; g < h.ngroups; g++) gsize[(size_t)g] = h.group_end[(size_t)g] - h.group_begin[(size_t)g];
+    std::vector<int32_t> qchunk(h.q_chunk_first.begin(), h.q_chunk_first.end());
+#define TRY(x)                        \
+    do {                              \
+        st = (x);                     \
+        if (st) {                     \
+            xpcs_mt_plan_destroy(p);  \
+            return st;                \
+        }                             \
+    } while (0)
+    TRY(upload(&p->d_pix2c, h.pix2c, s));
+    TRY(upload(&p->d_taus, h.taus, s));
+    TRY(upload(&p->d_levels, h.levels, s));
+    TRY(upload(&p->d_chunks, h.chunks, s));
+    TRY(upload(&p->d_group_chunk_first, h.group_chunk_first, s));
+    TRY(upload(&p->d_group_q, h.group_q, s));
+    TRY(upload(&p->d_group_size, gsize, s));
+    TRY(upload(&p->d_q_chunk_first, qchunk, s));
+    TRY(upload(&p->d_q_first_group, h.q_first_group, s));
+    TRY(upload(&p->d_c2pix, h.order, s));
+    TRY(upload(&p->d_sbin_of_c, h.sbin_of_c, s));
+    if (!h.weight.empty()) TRY(upload(&p->d_weight, h.weight, s));
+    const size_t PT = (size_t)h.P * (size_t)h.T;
+    TRY(dalloc(&p->d_G2, PT));
+    TRY(dalloc(&p->d_IP, PT));
+    TRY(dalloc(&p->d_IF, PT));
+    TRY(dalloc(&p->d_pix_total, (size_t)h.P));
+    TRY(dalloc(&p->d_part, std::max<size_t>(1, h.chunks.size() * (size_t)h.T)));
+    TRY(dalloc(&p->d_g2s, std::max<size_t>(1, (size_t)h.ngroups * h.T)));
+    TRY(dalloc(&p->d_g2, std::max<size_t>(1, (size_t)h.nq * h.T)));
+    TRY(dalloc(&p->d_se, std::max<size_t>(1, (size_t)h.nq * h.T)));
+    TRY(dalloc(&p->d_off, (size_t)h.P + 1));
+    TRY(dalloc(&p->d_cnt, (size_t)h.P));
+    TRY(dalloc(&p->d_bsum, (size_t)h.P / 4096 + 2));
+    TRY(dalloc(&p->d_long_list, (size_t)h.P));
+    TRY(dalloc(&p->d_counters, 4));
+    // dense-row scratch: one row of F 64-bit values per resident workgroup
+    {
+        size_t per = (size_t)h.F * sizeof(uint64_t);
+        size_t budget = (size_t)1 << 30;
+        int blocks = (int)std::min<size_t>(512, std::max<size_t>(1, budget / per));
+        p->long_blocks = blocks;
+        TRY(dalloc(&p->d_scratch, (size_t)blocks * h.F));
+    }
+    if (hipMemsetAsync(p->d_cnt, 0, sizeof(uint32_t) * (size_t)h.P, s) != hipSuccess ||
+        hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        xpcs_mt_plan_destroy(p);
+        return fail(XPCS_ERR_HIP, "plan_create: initialisation failed");
+    }
+#undef TRY
+    *out = p;
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
+{
+    if (!p) return XPCS_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    dfree(p->d_pix2c); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_chunks);
+    dfree(p->d_group_chunk_first); dfree(p->d_group_q); dfree(p->d_group_size);
+    dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
+    dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
+    dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
+    dfree(p->d_cnt); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_counters);
+    dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total);
+    dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_frame_sum);
+    for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : p->event_pool) (void)hipEventDestroy(e);
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_plan_info(const xpcs_mt_plan* p, xpcs_mt_info* info)
+{
+    if (!p || !info) return fail(XPCS_ERR_INVALID_ARG, "plan_info: NULL argument");
+    const HostTables& h = p->h;
+    info->taus = h.T;
+    info->max_level = h.max_level;
+    info->dynamic_bins = h.nq;
+    info->static_bins = h.ns;
+    info->pixels = h.P;
+    info->mapped_pixels = h.n_mapped;
+    info->groups = h.ngroups;
+    info->pad = 0;
+    info->events = p->events;
+    uint32_t cnt[2] = {0, 0};
+    if (p->executed && p->d_counters)
+        (void)hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost);
+    info->long_pixels = p->mode == 2 ? h.P : cnt[0];
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_schedule(const xpcs_mt_plan* p, int32_t* level, int32_t* tau)
+{
+    if (!p || !level || !tau) return fail(XPCS_ERR_INVALID_ARG, "schedule: NULL argument");
+    std::memcpy(level, p->h.level.data(), sizeof(int32_t) * (size_t)p->h.T);
+    std::memcpy(tau, p->h.tau.data(), sizeof(int32_t) * (size_t)p->h.T);
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_pixel_order(const xpcs_mt_plan* p, int32_t* flat)
+{
+    if (!p || !flat) return fail(XPCS_ERR_INVALID_ARG, "pixel_order: NULL argument");
+    std::memcpy(flat, p->h.order.data(), sizeof(int32_t) * (size_t)p->h.P);
+    return XPCS_OK;
+}
+
+// ---- IMM -------------------------------------------------------------------------
+// io::Header offsets (src/xpcs/io/imm.h:63-144): compression @4, elapsed @128,
+// dlen @152, corecotick @620; the first header decides compression for the
+// whole file (Imm::Imm, src/xpcs/io/imm.cpp:59-68).
+extern "C" xpcs_status xpcs_imm_index(const void* imm, size_t nbytes, int64_t* payload_off,
+                                      uint32_t* dlen, double* elapsed, int32_t* corecotick,
+                                      int64_t cap, int64_t* frames, int32_t* compressed)
+{
+    if (!imm || !frames) return fail(XPCS_ERR_INVALID_ARG, "imm_index: NULL argument");
+    const uint8_t* b = static_cast<const uint8_t*>(imm);
+    *frames = 0;
+    if (nbytes < 1024) return fail(XPCS_ERR_FORMAT, "imm_index: file shorter than one header");
+    int32_t comp;
+    std::memcpy(&comp, b + 4, 4);
+    if (compressed) *compressed = comp != 0;
+    const size_t per = comp != 0 ? 6 : 2;
+    size_t off = 0;
+    int64_t n = 0;
+    while (off + 1024 <= nbytes) {
+        uint32_t dl;
+        std::memcpy(&dl, b + off + 152, 4);
+        size_t pay = off + 1024;
+        if (pay + (size_t)dl * per > nbytes)
+            return fail(XPCS_ERR_FORMAT, "imm_index: frame %lld payload (%u entries) overruns the file",
+                        (long long)n, dl);
+        if (n < cap) {
+            if (payload_off) payload_off[n] = (int64_t)pay;
+            if (dlen) dlen[n] = dl;
+            if (elapsed) std::memcpy(&elapsed[n], b + off + 128, 8);
+            if (corecotick) std::memcpy(&corecotick[n], b + off + 620, 4);
+        }
+        n++;
+        off = pay + (size_t)dl * per;
+    }
+    if (off != nbytes && nbytes - off < 1024 && nbytes - off > 0)
+        return fail(XPCS_ERR_FORMAT, "imm_index: %zu trailing bytes after frame %lld",
+                    nbytes - off, (long long)n);
+    *frames = n;
+    return XPCS_OK;
+}
+
+namespace {
+
+// Raw frames used by the load loop of src/xpcs/main.cpp:222-268: darks are the
+// first frames of the file (uncompressed files only), then skip to frame_from,
+// then for output frame f the raw frames r0 + f*R + j*stride, j < avg,
+// R = stride*avg (SparseFilter/DenseFilter sum them and divide by avg).
+xpcs_status select_frames(const HostTables& h, int64_t n_raw, bool compressed, int64_t* first_data,
+                          std::vector<int64_t>& sel, std::vector<int32_t>& out_frame)
+{
+    int64_t r = 0;
+    if (!compressed && h.darks > 0) r += h.darks;
+    if (h.frame_from > 0 && r < h.frame_from) r = h.frame_from;
+    *first_data = r;
+    const int64_t R = (int64_t)h.stride * h.avg;
+    sel.clear();
+    out_frame.clear();
+    sel.reserve((size_t)h.F * h.avg);
+    for (int64_t f = 0; f < h.F; f++)
+        for (int64_t j = 0; j < h.avg; j++) {
+            const int64_t raw = r + f * R + j * h.stride;
+            if (raw >= n_raw)
+                return fail(XPCS_ERR_FORMAT, "ingest: IMM holds %lld frames, output frame %lld needs raw frame %lld",
+                            (long long)n_raw, (long long)f, (long long)raw);
+            sel.push_back(raw);
+            out_frame.push_back((int32_t)f);
+        }
+    return XPCS_OK;
+}
+
+// Dark image of the first `darks` frames (DarkImage::Compute,
+// src/xpcs/data_structure/dark_image.cpp:81-106, Welford in double) plus the
+// exact integer sums used for the n-scaled threshold of the dense path.
+xpcs_status dense_prepare(xpcs_mt_plan* p, const uint8_t* const* dark_frames)
+{
+    HostTables& h = p->h;
+    const int64_t npd = (int64_t)h.W * h.H;
+    p->nscale = h.darks > 0 ? h.darks : 1;
+    std::vector<int64_t> dsum((size_t)h.P, 0), thr((size_t)h.P, 0);
+    p->dark_avg.assign((size_t)npd, 0.0);
+    p->dark_std.assign((size_t)npd, 0.0);
+    if (h.darks > 0) {
+        if (!h.ff_det.empty())
+            return fail(XPCS_ERR_UNSUPPORTED,
+                        "dense path: a flat field together with dark frames has no exact integer form");
+        std::vector<int64_t> isum((size_t)npd, 0);
+        for (int i = 0; i < h.darks; i++) {
+            const uint16_t* v = reinterpret_cast<const uint16_t*>(dark_frames[i]);
+            for (int64_t j = 0; j < npd; j++) {
+                const double x = (double)(float)(int16_t)v[j];
+                const double prev = p->dark_avg[(size_t)j];
+                p->dark_avg[(size_t)j] += (x - p->dark_avg[(size_t)j]) / (double)(i + 1);
+                p->dark_std[(size_t)j] += (x - prev) * (x - p->dark_avg[(size_t)j]);
+                isum[(size_t)j] += v[j];
+            }
+        }
+        for (int64_t j = 0; j < npd; j++) p->dark_std[(size_t)j] = std::sqrt(p->dark_std[(size_t)j] / h.darks);
+        for (int c = 0; c < h.P; c++) {
+            const int64_t j = h.order[(size_t)c];
+            const float t = (float)((double)h.lld + (double)h.sigma * p->dark_std[(size_t)j]);
+            dsum[(size_t)c] = isum[(size_t)j];
+            thr[(size_t)c] = (int64_t)std::floor((double)p->nscale * (double)t);
+        }
+    }
+    xpcs_status st = upload(&p->d_dark_sum, dsum, p->stream);
+    if (st) return st;
+    return upload(&p->d_thr, thr, p->stream);
+}
+
+xpcs_status finish_selection(xpcs_mt_plan* p, const std::vector<int64_t>& sel_off,
+                             const std::vector<uint32_t>& sel_dlen, const std::vector<int32_t>& sel_frame)
+{
+    p->nsel = (int64_t)sel_off.size();
+    xpcs_status st = upload(&p->d_sel_off, sel_off, p->stream);
+    if (st) return st;
+    st = upload(&p->d_sel_dlen, sel_dlen, p->stream);
+    if (st) return st;
+    st = upload(&p->d_sel_frame, sel_frame, p->stream);
+    if (st) return st;
+    int64_t raw = 0;
+    for (uint32_t d : sel_dlen) raw += d;
+    p->raw_events = raw;
+    return XPCS_OK;
+}
+
+}  // namespace
+
+extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size_t nbytes)
+{
+    if (!p || !imm) return fail(XPCS_ERR_INVALID_ARG, "ingest_imm: NULL argument");
+    HostTables& h = p->h;
+    int64_t n = 0;
+    int32_t comp = 0;
+    xpcs_status st = xpcs_imm_index(imm, nbytes, nullptr, nullptr, nullptr, nullptr, 0, &n, &comp);
+    if (st) return st;
+    std::vector<int64_t> off((size_t)n);
+    std::vector<uint32_t> dl((size_t)n);
+    std::vector<double> el((size_t)n);
+    std::vector<int32_t> ct((size_t)n);
+    st = xpcs_imm_index(imm, nbytes, off.data(), dl.data(), el.data(), ct.data(), n, &n, &comp);
+    if (st) return st;
+    std::vector<int64_t> sel;
+    std::vector<int32_t> ofr;
+    int64_t first = 0;
+    st = select_frames(h, n, comp != 0, &first, sel, ofr);
+    if (st) return st;
+    const uint8_t* b = static_cast<const uint8_t*>(imm);
+    if (!comp) {
+        for (int64_t r : sel)
+            if ((int64_t)dl[(size_t)r] != (int64_t)h.W * h.H)
+                return fail(XPCS_ERR_FORMAT, "ingest_imm: uncompressed frame %lld has %u values, detector has %lld",
+                            (long long)r, dl[(size_t)r], (long long)h.W * h.H);
+        std::vector<const uint8_t*> darks;
+        for (int i = 0; i < h.darks; i++) {
+            if (i >= n) return fail(XPCS_ERR_FORMAT, "ingest_imm: fewer frames than dark frames");
+            if ((int64_t)dl[(size_t)i] != (int64_t)h.W * h.H)
+                return fail(XPCS_ERR_FORMAT, "ingest_imm: dark frame %d has the wrong size", i);
+            darks.push_back(b + off[(size_t)i]);
+        }
+        st = dense_prepare(p, darks.data());
+        if (st) return st;
+    }
+    // timestamps of every raw frame the load loop reads (SparseFilter::Apply
+    // records clock/ticks for each frame of each block, :134-140)
+    p->clock.clear();
+    p->ticks.clear();
+    const int64_t R = (int64_t)h.stride * h.avg;
+    for (int64_t k = 0; k < (int64_t)h.F * R && first + k < n; k++) {
+        p->clock.push_back(el[(size_t)(first + k)]);
+        p->ticks.push_back((double)ct[(size_t)(first + k)]);
+    }
+    // copy the byte range covering the selected frames
+    const int64_t lo_frame = sel.front(), hi_frame = sel.back();
+    const size_t lo = (size_t)off[(size_t)lo_frame] - 1024;
+    const size_t hi = (size_t)off[(size_t)hi_frame] + (size_t)dl[(size_t)hi_frame] * (comp ? 6 : 2);
+    const size_t len = hi - lo;
+    if (len > p->imm_cap) {
+        st = dalloc(&p->d_imm_owned, len);
+        if (st) return st;
+        p->imm_cap = len;
+    }
+    HIP_TRY(hipMemcpyAsync(p->d_imm_owned, b + lo, len, hipMemcpyHostToDevice, p->stream));
+    p->d_imm = p->d_imm_owned;
+    std::vector<int64_t> so(sel.size());
+    std::vector<uint32_t> sd(sel.size());
+    for (size_t k = 0; k < sel.size(); k++) {
+        so[k] = off[(size_t)sel[k]] - (int64_t)lo;
+        sd[k] = dl[(size_t)sel[k]];
+    }
+    st = finish_selection(p, so, sd, ofr);
+    if (st) return st;
+    p->mode = comp ? 1 : 2;
+    p->executed = false;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* dev_imm, size_t nbytes,
+                                                 const int64_t* payload_off, const uint32_t* dlen,
+                                                 int64_t raw_frames, int32_t compressed)
+{
+    if (!p || !dev_imm || !payload_off || !dlen || raw_frames <= 0)
+        return fail(XPCS_ERR_INVALID_ARG, "ingest_imm_device: bad arguments");
+    HostTables& h = p->h;
+    std::vector<int64_t> sel;
+    std::vector<int32_t> ofr;
+    int64_t first = 0;
+    xpcs_status st = select_frames(h, raw_frames, compressed != 0, &first, sel, ofr);
+    if (st) return st;
+    const size_t per = compressed ? 6 : 2;
+    for (int64_t r : sel) {
+        if (payload_off[r] < 0 || (size_t)payload_off[r] + (size_t)dlen[r] * per > nbytes)
+            return fail(XPCS_ERR_FORMAT, "ingest_imm_device: frame %lld lies outside the buffer", (long long)r);
+        if (!compressed && (int64_t)dlen[r] != (int64_t)h.W * h.H)
+            return fail(XPCS_ERR_FORMAT, "ingest_imm_device: uncompressed frame %lld has the wrong size", (long long)r);
+    }
+    if (!compressed) {
+        std::vector<std::vector<uint8_t>> dark_host((size_t)h.darks);
+        std::vector<const uint8_t*> darks;
+        for (int i = 0; i < h.darks; i++) {
+            dark_host[(size_t)i].resize((size_t)h.W * h.H * 2);
+            HIP_TRY(hipMemcpy(dark_host[(size_t)i].data(), static_cast<const uint8_t*>(dev_imm) + payload_off[i],
+                              dark_host[(size_t)i].size(), hipMemcpyDeviceToHost));
+            darks.push_back(dark_host[(size_t)i].data());
+        }
+        st = dense_prepare(p, darks.data());
+        if (st) return st;
+    }
+    p->clock.clear();
+    p->ticks.clear();
+    p->d_imm = static_cast<const uint8_t*>(dev_imm);
+    std::vector<int64_t> so(sel.size());
+    std::vector<uint32_t> sd(sel.size());
+    for (size_t k = 0; k < sel.size(); k++) {
+        so[k] = payload_off[sel[k]];
+        sd[k] = dlen[sel[k]];
+    }
+    st = finish_selection(p, so, sd, ofr);
+    if (st) return st;
+    p->mode = compressed ? 1 : 2;
+    p->executed = false;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_off, const int32_t* frame,
+                                           const uint32_t* count)
+{
+    if (!p || !row_off) return fail(XPCS_ERR_INVALID_ARG, "ingest_rows: NULL argument");
+    if ((!frame || !count) && row_off[(int64_t)p->h.W * p->h.H] > 0)
+        return fail(XPCS_ERR_INVALID_ARG, "ingest_rows: NULL frame/count arrays");
+    HostTables& h = p->h;
+    std::vector<int64_t> off((size_t)h.P + 1, 0);
+    for (int c = 0; c < h.P; c++) {
+        const int64_t j = h.order[(size_t)c];
+        off[(size_t)c + 1] = off[(size_t)c] + (row_off[j + 1] - row_off[j]);
+    }
+    std::vector<uint64_t> ev((size_t)off[(size_t)h.P]);
+    for (int c = 0; c < h.P; c++) {
+        const int64_t j = h.order[(size_t)c];
+        int64_t o = off[(size_t)c];
+        for (int64_t e = row_off[j]; e < row_off[j + 1]; e++) {
+            if (frame[e] < 0 || frame[e] >= h.F)
+                return fail(XPCS_ERR_RANGE, "ingest_rows: frame %d of pixel %lld outside [0, %d)", frame[e],
+                            (long long)j, h.F);
+            ev[(size_t)o++] = ((uint64_t)(uint32_t)frame[e] << 32) | count[e];
+        }
+    }
+    xpcs_status st = upload(&p->d_ev, ev, p->stream);
+    if (st) return st;
+    p->ev_cap = (int64_t)ev.size();
+    HIP_TRY(hipMemcpyAsync(p->d_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, p->stream));
+    p->events = off[(size_t)h.P];
+    p->nscale = 1;
+    p->mode = 3;
+    p->executed = false;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "execute: NULL plan");
+    if (p->mode == 0) return fail(XPCS_ERR_STATE, "execute: nothing ingested");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    HostTables& h = p->h;
+    HIP_TRY(hipMemsetAsync(p->d_counters, 0, 16, s));
+    if (p->mode == 1) {
+        if (p->raw_events > p->ev_cap) {
+            xpcs_status st = dalloc(&p->d_ev, (size_t)p->raw_events);
+            if (st) return st;
+            p->ev_cap = p->raw_events;
+        }
+        launch_hist(p, s);
+        launch_scan(p, s);
+        launch_scatter(p, s);
+    }
+    if (p->mode == 2) {
+        const size_t need = (size_t)h.P * (size_t)h.F;
+        if (need > p->rows_cap) {
+            xpcs_status st = dalloc(&p->d_rows, need);
+            if (st) return st;
+            p->rows_cap = need;
+        }
+        launch_dense_transpose(p, s);
+        launch_mt_long(p, s, true);
+    } else {
+        launch_mt_sparse(p, s);
+        launch_mt_long(p, s, false);
+    }
+    launch_normalize(p, s);
+    HIP_TRY(hipGetLastError());
+    p->stream_last = s;
+    p->executed = true;
+    return XPCS_OK;
+}
+
+static xpcs_status sync_and_check(xpcs_mt_plan* p)
+{
+    hipStream_t s = p->stream_last ? p->stream_last : p->stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    collect_timers(p);
+    uint32_t cnt[2] = {0, 0};
+    HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[1] & 1u) return fail(XPCS_ERR_RANGE, "execute: an IMM pixel index lies outside the detector");
+    if (p->mode == 1) {
+        int64_t e = 0;
+        HIP_TRY(hipMemcpy(&e, p->d_off + p->h.P, sizeof(e), hipMemcpyDeviceToHost));
+        p->events = e;
+    }
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_synchronize(xpcs_mt_plan* p)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "synchronize: NULL plan");
+    if (!p->executed) return XPCS_OK;
+    return sync_and_check(p);
+}
+
+extern "C" xpcs_status xpcs_mt_read_sums_i64(xpcs_mt_plan* p, int64_t* G2, int64_t* IP, int64_t* IF)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_sums: NULL plan");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "read_sums: execute first");
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    const size_t n = (size_t)p->h.P * p->h.T * sizeof(int64_t);
+    if (G2) HIP_TRY(hipMemcpy(G2, p->d_G2, n, hipMemcpyDeviceToHost));
+    if (IP) HIP_TRY(hipMemcpy(IP, p->d_IP, n, hipMemcpyDeviceToHost));
+    if (IF) HIP_TRY(hipMemcpy(IF, p->d_IF, n, hipMemcpyDeviceToHost));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_read_ref_f32(xpcs_mt_plan* p, float* G2, float* IP, float* IF)
+{
+    if (!p || !G2 || !IP || !IF) return fail(XPCS_ERR_INVALID_ARG, "read_ref_f32: NULL argument");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "read_ref_f32: execute first");
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    const size_t n = (size_t)p->h.W * p->h.H * p->h.T;
+    float* d = nullptr;
+    HIP_TRY(hipMalloc(&d, 3 * n * sizeof(float)));
+    hipError_t e = hipMemsetAsync(d, 0, 3 * n * sizeof(float), p->stream);
+    if (e == hipSuccess) {
+        launch_ref_f32(p, p->stream, d, d + n, d + 2 * n);
+        e = hipStreamSynchronize(p->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpy(G2, d, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(IP, d + n, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(IF, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(XPCS_ERR_HIP, "read_ref_f32: %s", hipGetErrorString(e));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* p, double* g2, double* se)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_g2: NULL plan");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "read_g2: execute first");
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    const size_t n = (size_t)p->h.nq * p->h.T * sizeof(double);
+    if (g2) HIP_TRY(hipMemcpy(g2, p->d_g2, n, hipMemcpyDeviceToHost));
+    if (se) HIP_TRY(hipMemcpy(se, p->d_se, n, hipMemcpyDeviceToHost));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_device_results(xpcs_mt_plan* p, const double** g2, const double** se)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "device_results: NULL plan");
+    if (g2) *g2 = p->d_g2;
+    if (se) *se = p->d_se;
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_read_side(xpcs_mt_plan* p, int32_t window, float* pixel_sum,
+                                         float* frame_sum, float* partition_mean, float* partition_partial)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_side: NULL plan");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "read_side: execute first");
+    (void)window;
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    if (frame_sum || partition_partial)
+        return fail(XPCS_ERR_UNSUPPORTED, "read_side: frame sums and partial partition means are not built yet");
+    const HostTables& h = p->h;
+    std::vector<uint64_t> tot((size_t)h.P);
+    HIP_TRY(hipMemcpy(tot.data(), p->d_pix_total, tot.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const double inv = 1.0 / ((double)h.avg * (double)p->nscale);
+    if (pixel_sum) {
+        std::fill(pixel_sum, pixel_sum + (size_t)h.W * h.H, 0.0f);
+        for (int c = 0; c < h.P; c++) {
+            const double w = h.weight.empty() ? 1.0 : h.weight[(size_t)c];
+            pixel_sum[h.order[(size_t)c]] = (float)((double)tot[(size_t)c] * w * inv / (double)h.F);
+        }
+    }
+    if (partition_mean) {
+        std::vector<double> acc((size_t)h.ns, 0.0);
+        for (int c = 0; c < h.P; c++) {
+            const double w = h.weight.empty() ? 1.0 : h.weight[(size_t)c];
+            acc[(size_t)h.sbin_of_c[(size_t)c] - 1] += (double)tot[(size_t)c] * w * inv;
+        }
+        for (int s = 0; s < h.ns; s++)
+            partition_mean[s] = (float)(acc[(size_t)s] / ((double)h.pixels_per_bin[(size_t)s] * (double)h.F));
+    }
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_kernel_times(xpcs_mt_plan* p, char* names, size_t cap, double* ms,
+                                            int64_t* launches, int32_t max_kernels, int32_t* n_kernels)
+{
+    if (!p || !n_kernels) return fail(XPCS_ERR_INVALID_ARG, "kernel_times: NULL argument");
+    if (p->executed) {
+        xpcs_status st = sync_and_check(p);
+        if (st) return st;
+    }
+    std::string all;
+    int k = 0;
+    for (auto& kv : p->times) {
+        if (k < max_kernels) {
+            if (ms) ms[k] = kv.second.first;
+            if (launches) launches[k] = kv.second.second;
+        }
+        all += kv.first;
+        all += '\n';
+        k++;
+    }
+    *n_kernels = k;
+    if (names && cap > 0) {
+        size_t n = std::min(cap - 1, all.size());
+        std::memcpy(names, all.data(), n);
+        names[n] = 0;
+    }
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* p)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "kernel_times_reset: NULL plan");
+    if (p->executed) (void)hipStreamSynchronize(p->stream_last ? p->stream_last : p->stream);
+    collect_timers(p);
+    p->times.clear();
+    return XPCS_OK;
+}

@@ -1,0 +1,179 @@
+// Internal declarations of the MI355X multi-tau library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "xpcs_mt.h"
+
+namespace xpcs {
+
+constexpr int kWave = 64;
+// Longest per-pixel event list the LDS kernel keeps on chip; longer rows go to
+// the dense-row kernel.
+constexpr int kSparseCap = 1024;
+// Largest schedule supported by the per-wave LDS accumulators.
+constexpr int kMaxTaus = 256;
+
+// One schedule entry as the kernels need it.
+struct TauParam {
+    int32_t level;   // L
+    int32_t tp;      // tau' = tau >> L
+    int32_t lf;      // lastframe_L = floor(F / 2^L)
+    int32_t N;       // lastframe_L - tau'
+};
+
+// One level of the schedule: its taus are [first, first+count), with
+// contiguous tau' values lo..hi.
+struct LevelParam {
+    int32_t first;
+    int32_t count;
+    int32_t lo;
+    int32_t hi;
+    int32_t lf;
+    int32_t shift;   // 2^shift = level-0 frames per level frame (== level)
+};
+
+// A normalisation chunk: pixels [begin, end) of one (q, s) group.
+struct NormChunk {
+    int32_t group;
+    int32_t begin;
+    int32_t end;
+    int32_t q;
+};
+
+struct NormPartial {   // per (chunk, tau)
+    int64_t g2, ip, ifs;   // integer sums (bit patterns of doubles when weighted)
+    double n, mean, m2;    // Welford of x_p = N G2/(IP IF)
+};
+
+// Host-side tables derived from the description.
+struct HostTables {
+    int W = 0, H = 0, F = 0, dpl = 0, stride = 1, avg = 1, frame_from = 0, darks = 0;
+    float lld = 0.f, sigma = 0.f;
+    uint32_t flags = 0;
+    // schedule
+    int T = 0, max_level = 0;
+    std::vector<int32_t> level, tau;
+    std::vector<TauParam> taus;
+    std::vector<LevelParam> levels;
+    // q map
+    int P = 0, n_mapped = 0, ngroups = 0, nq = 0, ns = 0;
+    std::vector<int32_t> order, pix2c, group_q, group_s, group_begin, group_end, q_first_group;
+    std::vector<int32_t> pixels_per_bin, sbin_of_c;
+    std::vector<double> ff_det;      // [W*H] or empty
+    std::vector<double> weight;      // [P] flat-field weight per compact pixel or empty
+    std::vector<NormChunk> chunks;
+    std::vector<int32_t> group_chunk_first;   // [ngroups+1]
+    std::vector<int32_t> q_chunk_first;       // [nq+1]
+};
+
+struct KernelTimer {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace xpcs
+
+struct xpcs_mt_plan {
+    xpcs::HostTables h;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t stream_last = nullptr;   // stream of the last execute
+    bool own_stream = false;
+
+    // device tables
+    int32_t* d_pix2c = nullptr;
+    xpcs::TauParam* d_taus = nullptr;
+    xpcs::LevelParam* d_levels = nullptr;
+    xpcs::NormChunk* d_chunks = nullptr;
+    int32_t* d_group_chunk_first = nullptr;
+    int32_t* d_group_q = nullptr;
+    int32_t* d_group_size = nullptr;
+    int32_t* d_q_chunk_first = nullptr;
+    int32_t* d_q_first_group = nullptr;
+    double* d_weight = nullptr;
+    int32_t* d_c2pix = nullptr;
+    int32_t* d_sbin_of_c = nullptr;
+
+    // ingest state: 0 none, 1 sparse IMM, 2 dense IMM, 3 rows
+    int mode = 0;
+    const uint8_t* d_imm = nullptr;      // raw IMM bytes (owned when own_imm)
+    uint8_t* d_imm_owned = nullptr;
+    size_t imm_cap = 0;
+    int64_t nsel = 0;                    // selected raw frames
+    int64_t* d_sel_off = nullptr;        // payload byte offset per selected raw frame
+    uint32_t* d_sel_dlen = nullptr;
+    int32_t* d_sel_frame = nullptr;      // output frame per selected raw frame
+    int64_t sel_cap = 0;
+    int64_t raw_events = 0;              // sum of dlen over selected frames (sparse)
+    std::vector<double> clock, ticks;    // timestamps of the data frames (host)
+    // dense ingest
+    int64_t nscale = 1;                  // value scale (dark frames)
+    int64_t* d_dark_sum = nullptr;       // [P] per compact pixel
+    int64_t* d_thr = nullptr;            // [P]
+    std::vector<double> dark_avg, dark_std;   // [W*H] (host, --darkout)
+
+    // events (pixel-major CSR over compact pixels)
+    int64_t ev_cap = 0;
+    uint64_t* d_ev = nullptr;            // (frame << 32) | count
+    int64_t* d_off = nullptr;            // [P+1]
+    uint32_t* d_cnt = nullptr;           // [P] zero between executes
+    int64_t* d_bsum = nullptr;           // scan block sums
+    int64_t events = 0;                  // known after execute (host copy)
+    // dense rows
+    uint32_t* d_rows = nullptr;          // [P][F]
+    size_t rows_cap = 0;
+
+    // long pixels
+    int32_t* d_long_list = nullptr;      // [P]
+    uint32_t* d_counters = nullptr;      // [0] long count, [1] error flags
+    uint64_t* d_scratch = nullptr;
+    size_t scratch_cap = 0;
+    int long_blocks = 0;
+
+    // outputs
+    int64_t* d_G2 = nullptr;             // [P][T]
+    int64_t* d_IP = nullptr;
+    int64_t* d_IF = nullptr;
+    uint64_t* d_pix_total = nullptr;     // [P] sum of level-0 values
+    xpcs::NormPartial* d_part = nullptr; // [nchunks][T]
+    double* d_g2s = nullptr;             // [ngroups][T]
+    double* d_g2 = nullptr;              // [T][Q]
+    double* d_se = nullptr;              // [T][Q]
+    // side statistics
+    uint64_t* d_frame_sum = nullptr;     // [F]
+    bool executed = false;
+
+    // timing
+    std::vector<xpcs::KernelTimer> pending;
+    std::map<std::string, std::pair<double, int64_t>> times;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace xpcs {
+
+// host_tables.cpp
+xpcs_status build_tables(const xpcs_mt_desc& d, HostTables& h);
+void set_error(const char* fmt, ...);
+xpcs_status fail(xpcs_status st, const char* fmt, ...);
+
+// kernel launchers (kernels_*.hip); all asynchronous on `s`
+void launch_hist(xpcs_mt_plan* p, hipStream_t s);
+void launch_scan(xpcs_mt_plan* p, hipStream_t s);
+void launch_scatter(xpcs_mt_plan* p, hipStream_t s);
+void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
+void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
+void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
+void launch_normalize(xpcs_mt_plan* p, hipStream_t s);
+void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF);
+
+// timing helpers (api.cpp)
+void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name);
+void timer_end(xpcs_mt_plan* p, hipStream_t s);
+
+}  // namespace xpcs

@@ -1,0 +1,375 @@
+// (b)+(c) Multi-tau binning and lag sums (Corr::multiTau2,
+// src/xpcs/corr.cpp:315-431), exact in integers.
+//
+// Level L frame k holds x_L[k] = x_{L-1}[2k] + x_{L-1}[2k+1], k < lf_L =
+// floor(F/2^L).  For a schedule entry with lag tau' = tau >> L and
+// N = lf_L - tau':
+//   G2 = sum_{t<N} x_L[t] x_L[t+tau'],
+//   IP = sum_{t<N} x_L[t]       = S0(2^L (lf_L - tau')),
+//   IF = sum_{tau'<=t<lf_L} x_L[t] = S0(2^L lf_L) - S0(2^L tau'),
+// where S0(b) is the sum of the level-0 counts of frames < b.
+//
+// k_mt_sparse: one wavefront per pixel row of at most CAP events.  The row is
+// sorted on chip (register bitonic network up to 64 events, LDS network
+// beyond), duplicates merged, IP/IF read off the level-0 prefix sums by binary
+// search, and every level is binned in place in LDS (ballot/mbcnt
+// compaction).  Pairs at lag tau' are found by looking ahead at most
+// hi = max tau' entries; their products go to per-wave LDS accumulators.
+//
+// k_mt_long: one workgroup per row that is too long for LDS (and every row of
+// the dense path).  The row becomes a dense series x_0[0..F) in HBM scratch;
+// each level is a coalesced sweep with per-thread lag accumulators, then an
+// in-place pairwise binning pass.
+#include "device_util.h"
+#include "internal.h"
+
+namespace xpcs {
+
+// In-place compaction of one level: keys are shifted by `shift`, entries with
+// key >= lf are dropped, runs of equal keys are summed.  Returns the new count.
+// Reads of a chunk complete before its writes, and writes only go to slots at
+// or below the slot being read, so the pass is safe in place.
+__device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uint32_t lf)
+{
+    const int lane = lane_id();
+    int carry = 0;
+    uint32_t prevk = 0xFFFFFFFFu;
+    for (int b0 = 0; b0 < n; b0 += kWave) {
+        const int i = b0 + lane;
+        const bool valid = i < n;
+        const uint32_t k = valid ? (key[i] >> shift) : 0xFFFFFFFFu;
+        uint32_t kp = __shfl_up(k, 1, 64);
+        if (lane == 0) kp = prevk;
+        const bool head = valid && k < lf && k != kp;
+        uint32_t sum = 0;
+        if (head) {
+            sum = val[i];
+            for (int j = i + 1; j < n && (key[j] >> shift) == k; j++) sum += val[j];
+        }
+        prevk = __shfl(k, kWave - 1, 64);
+        const uint64_t mask = __ballot(head);
+        const int pos = carry + mask_rank(mask);
+        const bool stop = __ballot(valid && k >= lf) != 0;
+        wave_sync();
+        if (head) {
+            key[pos] = k;
+            val[pos] = sum;
+        }
+        carry += __popcll(mask);
+        wave_sync();
+        if (stop) break;
+    }
+    return carry;
+}
+
+template <int CAP, int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES)
+k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P,
+            const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
+            int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
+            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
+            int32_t* __restrict__ long_list, uint32_t* __restrict__ counters)
+{
+    __shared__ uint32_t s_key[WAVES][CAP];
+    __shared__ uint32_t s_val[WAVES][CAP];
+    __shared__ unsigned long long s_aux[WAVES][CAP];   // prefix sums, then lag accumulators
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    uint32_t* key = s_key[w];
+    uint32_t* val = s_val[w];
+    unsigned long long* aux = s_aux[w];
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+
+    for (int64_t p = (int64_t)blockIdx.x * WAVES + w; p < P; p += nw) {
+        const int64_t base = off[p];
+        const int64_t n64 = off[p + 1] - base;
+        int64_t* g2row = G2 + p * T;
+        int64_t* iprow = IP + p * T;
+        int64_t* ifrow = IF + p * T;
+        if (n64 > CAP) {
+            if (lane == 0) {
+                uint32_t k = atomicAdd(&counters[0], 1u);
+                long_list[k] = (int32_t)p;
+            }
+            continue;
+        }
+        if (n64 == 0) {
+            for (int i = lane; i < T; i += kWave) {
+                g2row[i] = 0;
+                iprow[i] = 0;
+                ifrow[i] = 0;
+            }
+            if (lane == 0) pix_total[p] = 0;
+            continue;
+        }
+        const int n = (int)n64;
+        const int m = next_pow2(n);
+        // ---- load and sort by frame ----
+        if (m <= kWave) {
+            uint32_t k = 0xFFFFFFFFu, v = 0;
+            if (lane < n) {
+                uint64_t e = ev[base + lane];
+                k = (uint32_t)(e >> 32);
+                v = (uint32_t)e;
+            }
+            for (int kk = 2; kk <= m; kk <<= 1) {
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    uint32_t pk = __shfl_xor(k, j, 64);
+                    uint32_t pv = __shfl_xor(v, j, 64);
+                    const bool up = (lane & kk) == 0;
+                    const bool lower = (lane & j) == 0;
+                    const bool take = (lower == up) ? (pk < k) : (pk > k);
+                    if (take) {
+                        k = pk;
+                        v = pv;
+                    }
+                }
+            }
+            key[lane] = k;
+            val[lane] = v;
+        } else {
+            for (int i = lane; i < m; i += kWave) {
+                uint32_t k = 0xFFFFFFFFu, v = 0;
+                if (i < n) {
+                    uint64_t e = ev[base + i];
+                    k = (uint32_t)(e >> 32);
+                    v = (uint32_t)e;
+                }
+                key[i] = k;
+                val[i] = v;
+            }
+            wave_sync();
+            for (int kk = 2; kk <= m; kk <<= 1) {
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    for (int i = lane; i < m; i += kWave) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const uint32_t ki = key[i], kl = key[l];
+                            const bool up = (i & kk) == 0;
+                            if (ki != kl && ((ki > kl) == up)) {
+                                key[i] = kl;
+                                key[l] = ki;
+                                const uint32_t vi = val[i];
+                                val[i] = val[l];
+                                val[l] = vi;
+                            }
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+        }
+        wave_sync();
+        // ---- level 0: merge repeated frames, prefix sums, IP/IF ----
+        const int n0 = level_compact(key, val, n, 0, (uint32_t)F);
+        uint64_t carry = 0;
+        for (int b0 = 0; b0 < n0; b0 += kWave) {
+            const int i = b0 + lane;
+            const uint64_t v = i < n0 ? (uint64_t)val[i] : 0ull;
+            const uint64_t incl = wave_incl_scan_u64(v) + carry;
+            if (i < n0) aux[i] = incl;
+            carry = __shfl(incl, kWave - 1, 64);
+        }
+        wave_sync();
+        for (int i = lane; i < T; i += kWave) {
+            const TauParam tp = taus[i];
+            const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
+            const uint32_t b_if = (uint32_t)tp.tp << tp.level;
+            const uint32_t b_end = (uint32_t)tp.lf << tp.level;
+            const int j_ip = lds_lower_bound(key, n0, b_ip);
+            const int j_if = lds_lower_bound(key, n0, b_if);
+            const int j_end = lds_lower_bound(key, n0, b_end);
+            const uint64_t s_ip = j_ip > 0 ? aux[j_ip - 1] : 0ull;
+            const uint64_t s_if = j_if > 0 ? aux[j_if - 1] : 0ull;
+            const uint64_t s_end = j_end > 0 ? aux[j_end - 1] : 0ull;
+            iprow[i] = (int64_t)s_ip;
+            ifrow[i] = (int64_t)(s_end - s_if);
+        }
+        if (lane == 0) pix_total[p] = carry;
+        wave_sync();
+        for (int i = lane; i < T; i += kWave) aux[i] = 0ull;
+        wave_sync();
+        // ---- levels: bin in place, then lag products ----
+        int nl = n0;
+        for (int L = 0; L < nlev; L++) {
+            const LevelParam lp = levels[L];
+            if (L > 0) nl = level_compact(key, val, nl, 1, (uint32_t)lp.lf);
+            if (nl < 2) break;
+            const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
+            for (int i = lane; i < nl; i += kWave) {
+                const uint32_t ki = key[i];
+                const uint64_t vi = val[i];
+                for (int j = i + 1; j < nl; j++) {
+                    const uint32_t d = key[j] - ki;
+                    if (d > hi) break;
+                    if (d >= lo) atomicAdd(&aux[lp.first + (int)(d - lo)], (unsigned long long)(vi * val[j]));
+                }
+            }
+            wave_sync();
+        }
+        for (int i = lane; i < T; i += kWave) g2row[i] = (int64_t)aux[i];
+        wave_sync();
+    }
+}
+
+// ---- rows too long for LDS, and the dense path ------------------------------------
+constexpr int kLongBlock = 256;
+constexpr int kMaxLagBlock = 16;
+
+__device__ uint64_t block_sum_u64(uint64_t x, unsigned long long* s_red)
+{
+    x = wave_sum_u64(x);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane_id() == 0) s_red[w] = x;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += s_red[i];
+    __syncthreads();
+    return t;
+}
+
+// Lag sums of one level over a dense series src[0, lf) for lags lo..hi; the
+// series is read from `src` (level 0 of the dense path) or scratch.
+template <typename SrcT>
+__device__ void long_level(const SrcT* __restrict__ src, int lf, const LevelParam& lp,
+                           int64_t* __restrict__ g2row, int64_t* __restrict__ iprow,
+                           int64_t* __restrict__ ifrow, unsigned long long* s_red,
+                           unsigned long long* s_lag, uint64_t* total0)
+{
+    // sum over the level
+    uint64_t s = 0;
+    for (int t = threadIdx.x; t < lf; t += blockDim.x) s += (uint64_t)src[t];
+    const uint64_t S = block_sum_u64(s, s_red);
+    if (total0 && threadIdx.x == 0) *total0 = S;
+    // products, kMaxLagBlock lags at a time
+    for (int l0 = lp.lo; l0 <= lp.hi; l0 += kMaxLagBlock) {
+        uint64_t acc[kMaxLagBlock];
+#pragma unroll
+        for (int d = 0; d < kMaxLagBlock; d++) acc[d] = 0;
+        const int nlag = min(kMaxLagBlock, lp.hi - l0 + 1);
+        for (int t = threadIdx.x; t < lf; t += blockDim.x) {
+            const uint64_t xt = (uint64_t)src[t];
+            if (xt == 0) continue;
+#pragma unroll
+            for (int d = 0; d < kMaxLagBlock; d++) {
+                const int u = t + l0 + d;
+                if (d < nlag && u < lf) acc[d] += xt * (uint64_t)src[u];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < kMaxLagBlock; d++) {
+            const uint64_t v = block_sum_u64(acc[d], s_red);
+            if (threadIdx.x == 0 && d < nlag) s_lag[d] = v;
+        }
+        __syncthreads();
+        for (int d = threadIdx.x; d < nlag; d += blockDim.x) g2row[lp.first + (l0 - lp.lo) + d] = (int64_t)s_lag[d];
+        __syncthreads();
+    }
+    // IP = S - sum of the last tau' values, IF = S - sum of the first tau' values
+    for (int d = threadIdx.x; d < lp.count; d += blockDim.x) {
+        const int tp = lp.lo + d;
+        uint64_t head = 0, tail = 0;
+        for (int t = 0; t < tp && t < lf; t++) head += (uint64_t)src[t];
+        for (int t = max(lf - tp, 0); t < lf; t++) tail += (uint64_t)src[t];
+        iprow[lp.first + d] = (int64_t)(S - tail);
+        ifrow[lp.first + d] = (int64_t)(S - head);
+    }
+    __syncthreads();
+}
+
+// x[k] = src[2k] + src[2k+1] for k < lf_out, chunked so it may run in place.
+template <typename SrcT>
+__device__ void long_bin(const SrcT* src, uint64_t* x, int lf_out)
+{
+    for (int k0 = 0; k0 < lf_out; k0 += blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        uint64_t v = 0;
+        if (k < lf_out) v = (uint64_t)src[2 * k] + (uint64_t)src[2 * k + 1];
+        __syncthreads();
+        if (k < lf_out) x[k] = v;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kLongBlock)
+k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
+          const uint32_t* __restrict__ rows, int32_t P, int32_t T,
+          const LevelParam* __restrict__ levels, int32_t nlev, int32_t F,
+          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense,
+          uint64_t* __restrict__ scratch, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
+          int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total)
+{
+    __shared__ unsigned long long s_red[kLongBlock / 64];
+    __shared__ unsigned long long s_lag[kMaxLagBlock];
+    uint64_t* x = scratch + (size_t)blockIdx.x * (size_t)F;
+    const int64_t nrows = dense ? (int64_t)P : (int64_t)counters[0];
+    for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int64_t p = dense ? r : (int64_t)long_list[r];
+        int64_t* g2row = G2 + p * T;
+        int64_t* iprow = IP + p * T;
+        int64_t* ifrow = IF + p * T;
+        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+            g2row[i] = 0;
+            iprow[i] = 0;
+            ifrow[i] = 0;
+        }
+        __syncthreads();
+        if (dense) {
+            const uint32_t* row = rows + p * (int64_t)F;
+            long_level<uint32_t>(row, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
+            if (nlev > 1) long_bin<uint32_t>(row, x, levels[1].lf);
+        } else {
+            for (int t = threadIdx.x; t < F; t += blockDim.x) x[t] = 0;
+            __syncthreads();
+            const int64_t b = off[p], e = off[p + 1];
+            for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+                const uint64_t v = ev[i];
+                atomicAdd(reinterpret_cast<unsigned long long*>(&x[v >> 32]),
+                          (unsigned long long)(uint32_t)v);
+            }
+            __threadfence_block();
+            __syncthreads();
+            long_level<uint64_t>(x, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
+            if (nlev > 1) long_bin<uint64_t>(x, x, levels[1].lf);
+        }
+        for (int L = 1; L < nlev; L++) {
+            const LevelParam lp = levels[L];
+            long_level<uint64_t>(x, lp.lf, lp, g2row, iprow, ifrow, s_red, s_lag, nullptr);
+            if (L + 1 < nlev) long_bin<uint64_t>(x, x, levels[L + 1].lf);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------
+constexpr int kSparseWaves = 8;
+constexpr int kSparseCapUsed = 512;
+
+void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
+{
+    const HostTables& h = p->h;
+    int64_t blocks = ((int64_t)h.P + kSparseWaves - 1) / kSparseWaves;
+    if (blocks > 8192) blocks = 8192;
+    timer_begin(p, s, "k_mt_sparse");
+    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<(int)blocks, kWave * kSparseWaves, 0, s>>>(
+        p->d_ev, p->d_off, h.P, p->d_taus, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_long_list, p->d_counters);
+    timer_end(p, s);
+}
+
+void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
+{
+    const HostTables& h = p->h;
+    timer_begin(p, s, dense_rows ? "k_mt_long_dense" : "k_mt_long");
+    k_mt_long<<<p->long_blocks, kLongBlock, 0, s>>>(p->d_ev, p->d_off, p->d_rows, h.P, h.T,
+                                                    p->d_levels, (int32_t)h.levels.size(), h.F,
+                                                    p->d_long_list, p->d_counters, dense_rows ? 1 : 0,
+                                                    p->d_scratch, p->d_G2, p->d_IP, p->d_IF,
+                                                    p->d_pix_total);
+    timer_end(p, s);
+}
+
+int sparse_cap() { return kSparseCapUsed; }
+
+}  // namespace xpcs

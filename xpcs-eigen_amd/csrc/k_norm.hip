@@ -1,0 +1,195 @@
+// (d) Partition reduction and g2 normalisation (Corr::normalizeG2s,
+// src/xpcs/corr.cpp:927-1091) on the exact per-pixel sums.
+//
+//   k_norm_partial  one workgroup per chunk (<= 256 pixels of one (q, s) group):
+//                   integer sums of G2/IP/IF per tau (flat-field weighted sums in
+//                   FP64 when a flat field is set) and a sequential Welford of
+//                   x_p = N G2_p / (IP_p IF_p) with 0/0 -> 0 over the chunk.
+//   k_norm_group    per (group, tau): sums of the group's chunks in order;
+//                   g2s = c_s N sum G2 / (sum IP sum IF)            (FP64)
+//   k_norm_q        per (q, tau): mean of the non-NaN g2s of q; the chunk
+//                   Welford states merged in order (Chan et al.), giving
+//                   stderr = sqrt(1/n) sqrt(M2/n) with every pixel of q counted.
+// The reduction order is fixed, so results are deterministic.
+#include "device_util.h"
+#include "internal.h"
+
+namespace xpcs {
+
+__global__ void __launch_bounds__(256)
+k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
+               int32_t T, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
+               const int64_t* __restrict__ IF, const double* __restrict__ weight,
+               NormPartial* __restrict__ part)
+{
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const NormChunk ch = chunks[c];
+        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+            const double N = (double)taus[i].N;
+            int64_t sg = 0, sp = 0, sf = 0;
+            double wg = 0.0, wp = 0.0, wf = 0.0;
+            double n = 0.0, mean = 0.0, m2 = 0.0;
+            for (int p = ch.begin; p < ch.end; p++) {
+                const int64_t g = G2[(int64_t)p * T + i];
+                const int64_t a = IP[(int64_t)p * T + i];
+                const int64_t b = IF[(int64_t)p * T + i];
+                double x;
+                if (weight) {
+                    const double w = weight[p];
+                    wg += w * w * (double)g;
+                    wp += w * (double)a;
+                    wf += w * (double)b;
+                    x = (w == 0.0) ? 0.0 : N * (double)g / ((double)a * (double)b);
+                } else {
+                    sg += g;
+                    sp += a;
+                    sf += b;
+                    x = N * (double)g / ((double)a * (double)b);
+                }
+                if (x != x) x = 0.0;
+                n += 1.0;
+                const double d = x - mean;
+                mean += d / n;
+                m2 += d * (x - mean);
+            }
+            NormPartial r;
+            if (weight) {
+                r.g2 = __double_as_longlong(wg);
+                r.ip = __double_as_longlong(wp);
+                r.ifs = __double_as_longlong(wf);
+            } else {
+                r.g2 = sg;
+                r.ip = sp;
+                r.ifs = sf;
+            }
+            r.n = n;
+            r.mean = mean;
+            r.m2 = m2;
+            part[c * T + i] = r;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ group_chunk_first,
+             const int32_t* __restrict__ group_size, int32_t ngroups, const TauParam* __restrict__ taus,
+             int32_t T, int weighted, double* __restrict__ g2s)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)ngroups * T) return;
+    const int g = (int)(k / T);
+    const int i = (int)(k % T);
+    double sg, sp, sf;
+    if (weighted) {
+        sg = sp = sf = 0.0;
+        for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
+            const NormPartial& r = part[(int64_t)c * T + i];
+            sg += __longlong_as_double(r.g2);
+            sp += __longlong_as_double(r.ip);
+            sf += __longlong_as_double(r.ifs);
+        }
+    } else {
+        int64_t ig = 0, ip = 0, iff = 0;
+        for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
+            const NormPartial& r = part[(int64_t)c * T + i];
+            ig += r.g2;
+            ip += r.ip;
+            iff += r.ifs;
+        }
+        sg = (double)ig;
+        sp = (double)ip;
+        sf = (double)iff;
+    }
+    g2s[(int64_t)g * T + i] = (double)group_size[g] * (double)taus[i].N * sg / (sp * sf);
+}
+
+__global__ void __launch_bounds__(256)
+k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
+         const int32_t* __restrict__ q_first_group, const int32_t* __restrict__ q_chunk_first,
+         int32_t nq, int32_t T, double* __restrict__ g2, double* __restrict__ se)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nq * T) return;
+    const int q = (int)(k / T);    // 0-based
+    const int i = (int)(k % T);
+    double acc = 0.0, cnt = 0.0;
+    for (int g = q_first_group[q]; g < q_first_group[q + 1]; g++) {
+        const double v = g2s[(int64_t)g * T + i];
+        if (v == v) {
+            acc += v;
+            cnt += 1.0;
+        }
+    }
+    g2[(int64_t)i * nq + q] = acc / cnt;
+    double n = 0.0, mean = 0.0, m2 = 0.0;
+    for (int c = q_chunk_first[q]; c < q_chunk_first[q + 1]; c++) {
+        const NormPartial& r = part[(int64_t)c * T + i];
+        if (r.n == 0.0) continue;
+        if (n == 0.0) {
+            n = r.n;
+            mean = r.mean;
+            m2 = r.m2;
+            continue;
+        }
+        const double nt = n + r.n;
+        const double d = r.mean - mean;
+        mean += d * (r.n / nt);
+        m2 += r.m2 + d * d * (n * r.n / nt);
+        n = nt;
+    }
+    se[(int64_t)i * nq + q] = sqrt(1.0 / n) * sqrt(m2 / n);
+}
+
+// Per-pixel sums in the reference's float units, layout [T][W*H].
+__global__ void __launch_bounds__(256)
+k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const int64_t* __restrict__ IF,
+          const int32_t* __restrict__ c2pix, int32_t P, const TauParam* __restrict__ taus, int32_t T,
+          const double* __restrict__ weight, double inv_scale, int64_t npix_det,
+          float* __restrict__ oG2, float* __restrict__ oIP, float* __restrict__ oIF)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)P * T) return;
+    const int i = (int)(k / P);
+    const int64_t c = k % P;
+    const TauParam tp = taus[i];
+    const double s = (weight ? weight[c] : 1.0) * inv_scale;
+    const double lv = (double)(1ll << tp.level);
+    const double N = (double)tp.N;
+    const int64_t o = (int64_t)i * npix_det + c2pix[c];
+    oG2[o] = (float)((double)G2[c * T + i] * s * s / (lv * lv * N));
+    oIP[o] = (float)((double)IP[c * T + i] * s / (lv * N));
+    oIF[o] = (float)((double)IF[c * T + i] * s / (lv * N));
+}
+
+void launch_normalize(xpcs_mt_plan* p, hipStream_t s)
+{
+    const HostTables& h = p->h;
+    const int32_t nchunks = (int32_t)h.chunks.size();
+    timer_begin(p, s, "k_norm");
+    if (nchunks > 0)
+        k_norm_partial<<<nchunks < 65536 ? nchunks : 65536, 128, 0, s>>>(
+            p->d_chunks, nchunks, p->d_taus, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_weight, p->d_part);
+    const int64_t ng = (int64_t)h.ngroups * h.T;
+    if (ng > 0)
+        k_norm_group<<<(int)((ng + 255) / 256), 256, 0, s>>>(p->d_part, p->d_group_chunk_first,
+                                                             p->d_group_size, h.ngroups, p->d_taus, h.T,
+                                                             p->d_weight ? 1 : 0, p->d_g2s);
+    const int64_t nqt = (int64_t)h.nq * h.T;
+    if (nqt > 0)
+        k_norm_q<<<(int)((nqt + 255) / 256), 256, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group,
+                                                          p->d_q_chunk_first, h.nq, h.T, p->d_g2,
+                                                          p->d_se);
+    timer_end(p, s);
+}
+
+void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF)
+{
+    const HostTables& h = p->h;
+    const int64_t n = (int64_t)h.P * h.T;
+    const double inv_scale = 1.0 / ((double)h.avg * (double)p->nscale);
+    k_ref_f32<<<(int)((n + 255) / 256), 256, 0, s>>>(p->d_G2, p->d_IP, p->d_IF, p->d_c2pix, h.P,
+                                                     p->d_taus, h.T, p->d_weight, inv_scale,
+                                                     (int64_t)h.W * h.H, G2, IP, IF);
+}
+
+}  // namespace xpcs

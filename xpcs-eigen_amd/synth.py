@@ -1,0 +1,191 @@
+"""Synthetic XPCS inputs: OU speckle photon counts, annular q-maps, IMM files.
+
+Speckle model (per pixel): a complex Ornstein-Uhlenbeck field
+``E[t+1] = a*E[t] + sqrt(1-a^2)*xi[t]`` with ``xi ~ CN(0,1)``, ``a = exp(-1/tau_c)``
+and ``E[0] ~ CN(0,1)``; intensity ``I = mu*|E|^2``; counts ``Poisson(I)`` as uint16.
+For such a field the level-0 intensity autocorrelation of a single pixel tends
+to ``1 + exp(-2*tau/tau_c)``.
+
+Q-maps are concentric annuli about the detector centre with ``R_max = W/2``:
+``sbin = 1 + floor(S*r/R_max)`` and ``qbin = 1 + floor((sbin-1)/(S/Q))``, so
+static rings nest inside dynamic rings; pixels outside ``R_max`` are 0 (masked).
+
+IMM layout (1024-byte header per frame, then the payload) follows the header
+of the reference reader (src/xpcs/io/imm.h:63-144): compression @4, rows @108,
+cols @112, bytes @116, elapsed @128 (f64), dlen @152, buffer_number @160,
+immversion @616, corecotick @620.  Compressed payload: int32 index[dlen] then
+uint16 value[dlen]; uncompressed payload: uint16 value[W*H].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HEADER_BYTES = 1024
+
+
+def annular_qmaps(width: int, height: int, n_static: int, n_dynamic: int):
+    """Return (dqmap, sqmap) int32 [height, width] nested annuli (R_max = W/2)."""
+    if n_static % n_dynamic:
+        raise ValueError("n_static must be a multiple of n_dynamic")
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    cx, cy = width / 2.0 - 0.5, height / 2.0 - 0.5
+    r = np.sqrt((xx - cx) ** 2 + (yy - cy) ** 2)
+    rmax = width / 2.0
+    sbin = 1 + np.floor(n_static * r / rmax).astype(np.int64)
+    sbin[r >= rmax] = 0
+    per = n_static // n_dynamic
+    qbin = np.where(sbin > 0, 1 + (sbin - 1) // per, 0)
+    return qbin.astype(np.int32), sbin.astype(np.int32)
+
+
+def ou_counts(width, height, frames, mu, tau_c, seed, max_count=65535):
+    """Dense counts [frames, height*width] uint16 from the OU speckle model.
+
+    ``tau_c`` is a scalar or a per-pixel array (length height*width).
+    Draw order per frame: real normals, imaginary normals, Poisson.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    npix = width * height
+    tau = np.broadcast_to(np.asarray(tau_c, dtype=np.float64), (npix,))
+    a = np.exp(-1.0 / tau)
+    b = np.sqrt(1.0 - a * a)
+    s = np.sqrt(0.5)
+    e = (rng.standard_normal(npix) + 1j * rng.standard_normal(npix)) * s
+    out = np.empty((frames, npix), dtype=np.uint16)
+    for t in range(frames):
+        if t > 0:
+            xi = (rng.standard_normal(npix) + 1j * rng.standard_normal(npix)) * s
+            e = a * e + b * xi
+        lam = mu * (e.real * e.real + e.imag * e.imag)
+        c = rng.poisson(lam)
+        np.minimum(c, max_count, out=c)
+        out[t] = c
+    return out
+
+
+def _header(compression, rows, cols, dlen, t, exposure=1.0e-3):
+    h = np.zeros(HEADER_BYTES, dtype=np.uint8)
+    def put(off, val, dt):
+        h[off:off + np.dtype(dt).itemsize] = np.frombuffer(np.array(val, dtype=dt).tobytes(), np.uint8)
+    put(0, 2 if compression else 0, np.int32)        # mode
+    put(4, compression, np.int32)
+    put(108, rows, np.int32)
+    put(112, cols, np.int32)
+    put(116, 2, np.int32)                             # bytes per value
+    put(128, t * exposure, np.float64)                # elapsed
+    put(136, exposure, np.float64)                    # preset
+    put(152, dlen, np.uint32)
+    put(156, 1, np.int32)                             # roi_number
+    put(160, t, np.int32)                             # buffer_number
+    put(616, 12, np.int32)                            # immversion
+    put(620, t, np.int32)                             # corecotick
+    return h
+
+
+def imm_bytes_sparse(counts: np.ndarray, width: int, height: int) -> bytes:
+    """Compressed IMM (compression=6): per frame the non-zero pixels."""
+    parts = []
+    for t in range(counts.shape[0]):
+        idx = np.flatnonzero(counts[t]).astype(np.int32)
+        val = counts[t][idx].astype(np.uint16)
+        parts.append(_header(6, height, width, idx.size, t).tobytes())
+        parts.append(idx.tobytes())
+        parts.append(val.tobytes())
+    return b"".join(parts)
+
+
+def imm_bytes_events(frames_events, width: int, height: int) -> bytes:
+    """Compressed IMM from a list of (index int32[], value uint16[]) per frame."""
+    parts = []
+    for t, (idx, val) in enumerate(frames_events):
+        idx = np.asarray(idx, dtype=np.int32)
+        val = np.asarray(val, dtype=np.uint16)
+        parts.append(_header(6, height, width, idx.size, t).tobytes())
+        parts.append(idx.tobytes())
+        parts.append(val.tobytes())
+    return b"".join(parts)
+
+
+def imm_bytes_dense(counts: np.ndarray, width: int, height: int) -> bytes:
+    """Uncompressed IMM (compression=0): every pixel of every frame."""
+    parts = []
+    for t in range(counts.shape[0]):
+        parts.append(_header(0, height, width, width * height, t).tobytes())
+        parts.append(np.ascontiguousarray(counts[t], dtype=np.uint16).tobytes())
+    return b"".join(parts)
+
+
+def dark_frames(width, height, n, mean=2.0, sd=0.5, seed=0):
+    """n identical integer dark frames D = clip(round(N(mean, sd)), 0)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = np.clip(np.rint(rng.normal(mean, sd, width * height)), 0, None).astype(np.uint16)
+    return np.repeat(d[None, :], n, axis=0)
+
+
+def ring_tau(dqmap: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    """Per-pixel correlation time, log-spaced over the dynamic rings in [lo, hi]."""
+    q = dqmap.reshape(-1)
+    nq = max(int(q.max()), 1)
+    taus = np.geomspace(lo, hi, nq)
+    out = np.full(q.shape, lo, dtype=np.float64)
+    m = q > 0
+    out[m] = taus[q[m] - 1]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Device-side generator for the large workloads (torch, any device).  Same
+# model, different random stream from the numpy generator.
+# ---------------------------------------------------------------------------
+def ou_events_torch(width, height, frames, mu, tau_c, seed, device="cuda", chunk=64):
+    """Yield (t0, counts[chunk, npix] uint8/int16 tensor) blocks on `device`."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    npix = width * height
+    tau = torch.as_tensor(np.broadcast_to(np.asarray(tau_c, np.float64), (npix,)).copy(),
+                          dtype=torch.float32, device=device)
+    a = torch.exp(-1.0 / tau)
+    b = torch.sqrt(1.0 - a * a)
+    s = float(np.sqrt(0.5))
+    er = torch.randn(npix, generator=g, device=device) * s
+    ei = torch.randn(npix, generator=g, device=device) * s
+    t = 0
+    while t < frames:
+        n = min(chunk, frames - t)
+        out = torch.empty((n, npix), dtype=torch.int32, device=device)
+        for k in range(n):
+            if t + k > 0:
+                er = a * er + b * (torch.randn(npix, generator=g, device=device) * s)
+                ei = a * ei + b * (torch.randn(npix, generator=g, device=device) * s)
+            lam = mu * (er * er + ei * ei)
+            out[k] = torch.poisson(lam, generator=g).clamp_(max=65535).to(torch.int32)
+        yield t, out
+        t += n
+
+
+def imm_device_sparse(width, height, frames, mu, tau_c, seed, device="cuda"):
+    """Build a compressed IMM image directly in device memory.
+
+    Returns (imm uint8 tensor on device, payload_off int64[F], dlen uint32[F]).
+    Headers carry compression=6, dlen, buffer_number/corecotick = t and
+    elapsed = t*1e-3.
+    """
+    import torch
+    chunks = []
+    offs, dls = [], []
+    pos = 0
+    for t0, c in ou_events_torch(width, height, frames, mu, tau_c, seed, device):
+        for k in range(c.shape[0]):
+            t = t0 + k
+            idx = torch.nonzero(c[k], as_tuple=False).reshape(-1).to(torch.int32)
+            val = c[k][idx].to(torch.int16)
+            hdr = torch.from_numpy(_header(6, height, width, int(idx.numel()), t)).to(device)
+            chunks.append(hdr)
+            chunks.append(idx.view(torch.uint8))
+            chunks.append(val.view(torch.uint8))
+            n = int(idx.numel())
+            offs.append(pos + HEADER_BYTES)
+            dls.append(n)
+            pos += HEADER_BYTES + 6 * n
+    return torch.cat(chunks), np.array(offs, np.int64), np.array(dls, np.uint32)

@@ -1,0 +1,259 @@
+"""Python front end of libxpcs_mt.so (the C ABI declared in include/xpcs_mt.h).
+
+Mirrors the reference's multi-tau interface (xpcs::Corr, src/xpcs/corr.h:66-126)
+for tests, benchmarks and scripting:
+
+    plan = Plan(width, height, frames, dpl, dqmap, sqmap)
+    plan.ingest_imm(imm_bytes)            # Reader + Filter (main.cpp:222-268)
+    plan.execute()                        # multiTau2 + normalizeG2s
+    g2, stderr = plan.g2()                # norm-0-g2 / norm-0-stderr, [T, Q]
+
+There is no CPU fallback: loading fails loudly when the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libxpcs_mt.so"
+
+XPCS_MT_TIMING = 0x1
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "FORMAT", 4: "RANGE", 5: "UNSUPPORTED",
+          6: "STATE", 7: "NOMEM", 8: "NO_DEVICE"}
+
+EXPORTS = [
+    "xpcs_version", "xpcs_last_error", "xpcs_level_max", "xpcs_delay_count",
+    "xpcs_delays_per_level", "xpcs_mt_plan_create", "xpcs_mt_plan_destroy", "xpcs_mt_plan_info",
+    "xpcs_mt_schedule", "xpcs_mt_pixel_order", "xpcs_mt_ingest_imm", "xpcs_mt_ingest_imm_device",
+    "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_execute", "xpcs_mt_synchronize",
+    "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
+    "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset",
+]
+
+
+class XpcsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Desc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("frames", C.c_int32),
+                ("delays_per_level", C.c_int32), ("stride", C.c_int32), ("average", C.c_int32),
+                ("frame_from", C.c_int32), ("dark_frames", C.c_int32), ("lld", C.c_float),
+                ("sigma", C.c_float), ("dqmap", C.c_void_p), ("sqmap", C.c_void_p),
+                ("flatfield", C.c_void_p), ("flags", C.c_uint32)]
+
+
+class Info(C.Structure):
+    _fields_ = [("taus", C.c_int32), ("max_level", C.c_int32), ("dynamic_bins", C.c_int32),
+                ("static_bins", C.c_int32), ("pixels", C.c_int32), ("mapped_pixels", C.c_int32),
+                ("groups", C.c_int32), ("pad", C.c_int32), ("events", C.c_int64),
+                ("long_pixels", C.c_int64)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libxpcs_mt.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` "
+                                    "(or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        vp, i32, i64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
+        sig = {
+            "xpcs_version": ([], C.c_char_p),
+            "xpcs_last_error": ([], C.c_char_p),
+            "xpcs_level_max": ([i32, i32], i32),
+            "xpcs_delay_count": ([i32, i32], i32),
+            "xpcs_delays_per_level": ([i32, i32, i32, vp, vp, i32, vp], i32),
+            "xpcs_mt_plan_create": ([vp, vp], i32),
+            "xpcs_mt_plan_destroy": ([vp], i32),
+            "xpcs_mt_plan_info": ([vp, vp], i32),
+            "xpcs_mt_schedule": ([vp, vp, vp], i32),
+            "xpcs_mt_pixel_order": ([vp, vp], i32),
+            "xpcs_mt_ingest_imm": ([vp, vp, C.c_size_t], i32),
+            "xpcs_mt_ingest_imm_device": ([vp, vp, C.c_size_t, vp, vp, i64, i32], i32),
+            "xpcs_imm_index": ([vp, C.c_size_t, vp, vp, vp, vp, i64, vp, vp], i32),
+            "xpcs_mt_ingest_rows": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_execute": ([vp, vp], i32),
+            "xpcs_mt_synchronize": ([vp], i32),
+            "xpcs_mt_read_sums_i64": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_read_ref_f32": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_read_g2": ([vp, vp, vp], i32),
+            "xpcs_mt_device_results": ([vp, vp, vp], i32),
+            "xpcs_mt_read_side": ([vp, i32, vp, vp, vp, vp], i32),
+            "xpcs_mt_kernel_times": ([vp, vp, C.c_size_t, vp, vp, i32, vp], i32),
+            "xpcs_mt_kernel_times_reset": ([vp], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def check(st):
+    if st != 0:
+        raise XpcsError(st, lib().xpcs_last_error().decode())
+
+
+def level_max(frames, dpl):
+    return lib().xpcs_level_max(frames, dpl)
+
+
+def delays_per_level(frames, dpl, max_level=None):
+    if max_level is None:
+        max_level = level_max(frames, dpl)
+    n = C.c_int32(0)
+    check(lib().xpcs_delays_per_level(frames, dpl, max_level, None, None, 0, C.byref(n)))
+    lev = np.zeros(n.value, np.int32)
+    tau = np.zeros(n.value, np.int32)
+    check(lib().xpcs_delays_per_level(frames, dpl, max_level, _p(lev), _p(tau), n.value, C.byref(n)))
+    return lev, tau
+
+
+def imm_index(buf):
+    b = np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else np.ascontiguousarray(buf)
+    n = C.c_int64(0)
+    comp = C.c_int32(0)
+    check(lib().xpcs_imm_index(_p(b), b.size, None, None, None, None, 0, C.byref(n), C.byref(comp)))
+    off = np.zeros(n.value, np.int64)
+    dl = np.zeros(n.value, np.uint32)
+    el = np.zeros(n.value, np.float64)
+    ct = np.zeros(n.value, np.int32)
+    check(lib().xpcs_imm_index(_p(b), b.size, _p(off), _p(dl), _p(el), _p(ct), n.value, C.byref(n),
+                               C.byref(comp)))
+    return dict(payload_off=off, dlen=dl, elapsed=el, corecotick=ct, compressed=bool(comp.value))
+
+
+class Plan:
+    """One multi-tau job on the current HIP device."""
+
+    def __init__(self, width, height, frames, dpl, dqmap, sqmap, stride=1, average=1,
+                 frame_from=0, dark_frames=0, lld=0.0, sigma=0.0, flatfield=None, timing=False):
+        self._dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+        self._sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+        self._ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+        if self._dq.size != width * height or self._sq.size != width * height:
+            raise ValueError("q-maps must have height*width entries")
+        d = Desc(width, height, frames, dpl, stride, average, frame_from, dark_frames, lld, sigma,
+                 _p(self._dq), _p(self._sq), _p(self._ff), XPCS_MT_TIMING if timing else 0)
+        self._h = C.c_void_p()
+        check(lib().xpcs_mt_plan_create(C.byref(d), C.byref(self._h)))
+        info = self.info()
+        self.T, self.Q, self.S, self.P = info.taus, info.dynamic_bins, info.static_bins, info.pixels
+        self.width, self.height, self.frames = width, height, frames
+
+    def close(self):
+        if self._h:
+            lib().xpcs_mt_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        i = Info()
+        check(lib().xpcs_mt_plan_info(self._h, C.byref(i)))
+        return i
+
+    def schedule(self):
+        lev = np.zeros(self.T, np.int32)
+        tau = np.zeros(self.T, np.int32)
+        check(lib().xpcs_mt_schedule(self._h, _p(lev), _p(tau)))
+        return lev, tau
+
+    def pixel_order(self):
+        o = np.zeros(self.P, np.int32)
+        check(lib().xpcs_mt_pixel_order(self._h, _p(o)))
+        return o
+
+    def ingest_imm(self, buf):
+        b = np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else np.ascontiguousarray(buf)
+        check(lib().xpcs_mt_ingest_imm(self._h, _p(b), b.size))
+
+    def ingest_imm_device(self, dev_ptr, nbytes, payload_off, dlen, compressed=True):
+        off = np.ascontiguousarray(payload_off, np.int64)
+        dl = np.ascontiguousarray(dlen, np.uint32)
+        check(lib().xpcs_mt_ingest_imm_device(self._h, C.c_void_p(dev_ptr), nbytes, _p(off), _p(dl),
+                                              off.size, 1 if compressed else 0))
+
+    def ingest_rows(self, row_off, frame, count):
+        ro = np.ascontiguousarray(row_off, np.int64)
+        fr = np.ascontiguousarray(frame, np.int32)
+        ct = np.ascontiguousarray(count, np.uint32)
+        check(lib().xpcs_mt_ingest_rows(self._h, _p(ro), _p(fr), _p(ct)))
+
+    def execute(self, stream=None):
+        check(lib().xpcs_mt_execute(self._h, None if stream is None else C.c_void_p(stream)))
+
+    def synchronize(self):
+        check(lib().xpcs_mt_synchronize(self._h))
+
+    def sums(self):
+        G2 = np.zeros((self.P, self.T), np.int64)
+        IP = np.zeros((self.P, self.T), np.int64)
+        IF = np.zeros((self.P, self.T), np.int64)
+        check(lib().xpcs_mt_read_sums_i64(self._h, _p(G2), _p(IP), _p(IF)))
+        return G2, IP, IF
+
+    def ref_f32(self):
+        n = self.width * self.height
+        G2 = np.zeros((self.T, n), np.float32)
+        IP = np.zeros((self.T, n), np.float32)
+        IF = np.zeros((self.T, n), np.float32)
+        check(lib().xpcs_mt_read_ref_f32(self._h, _p(G2), _p(IP), _p(IF)))
+        return G2, IP, IF
+
+    def g2(self):
+        g2 = np.zeros((self.T, self.Q), np.float64)
+        se = np.zeros((self.T, self.Q), np.float64)
+        check(lib().xpcs_mt_read_g2(self._h, _p(g2), _p(se)))
+        return g2, se
+
+    def device_results(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        check(lib().xpcs_mt_device_results(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def side(self, window=0, frame_sum=False, partial=False):
+        ps = np.zeros(self.width * self.height, np.float32)
+        pm = np.zeros(self.S, np.float32)
+        fs = np.zeros((2, self.frames), np.float32) if frame_sum else None
+        pp = np.zeros((max(self.frames // max(window, 1), 0), self.S), np.float32) if partial else None
+        check(lib().xpcs_mt_read_side(self._h, window, _p(ps), _p(fs), _p(pm), _p(pp)))
+        return dict(pixel_sum=ps, partition_mean=pm, frame_sum=fs, partition_partial=pp)
+
+    def kernel_times(self):
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(64, np.float64)
+        ln = np.zeros(64, np.int64)
+        n = C.c_int32(0)
+        check(lib().xpcs_mt_kernel_times(self._h, names, 4096, _p(ms), _p(ln), 64, C.byref(n)))
+        keys = names.value.decode().split("\n")[: n.value]
+        return {k: (float(ms[i]), int(ln[i])) for i, k in enumerate(keys)}
+
+    def kernel_times_reset(self):
+        check(lib().xpcs_mt_kernel_times_reset(self._h))

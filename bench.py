@@ -34,14 +34,26 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped):
+def transpose_hist_entries(P, nsel):
+    """Entries of the (bucket, chunk) histogram the sparse transpose uses."""
+    bshift = 8
+    while ((P + (1 << bshift) - 1) >> bshift) > 8192:
+        bshift += 1
+    nb = (P + (1 << bshift) - 1) >> bshift
+    cf = (nsel + 511) // 512 if nsel > 512 else 1
+    nblk = max(1, (nsel + cf - 1) // cf)
+    return nb * nblk
+
+
+def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel):
     """Bytes each kernel must move at minimum (inputs read once, outputs written once)."""
+    nh = transpose_hist_entries(P, nsel)
     return {
-        "k_hist": 6 * E_raw,                              # int32 index + u16 count per raw event
-        "k_scan": 4 * P + 8 * (P + 1),                    # counters in, offsets out
-        "k_scatter": 6 * E_raw + 8 * E0,                  # raw events in, 8-B pixel-major events out
-        "k_mt_sparse": 8 * E0 + 8 * (P + 1) + 3 * 8 * P * T,   # events + offsets in, int64 sums out
-        "k_mt_long": 0,
+        "k_tr_count": 6 * E_raw + 4 * nh,                 # raw events in, histogram out
+        "k_tr_scan": 4 * nh + 8 * (nh + 1),               # histogram in, segment offsets out
+        "k_tr_bucket": 6 * E_raw + 8 * E0,                # raw events in, bucketed events out
+        "k_tr_rows": 8 * E0 + 8 * E0 + 8 * (P + 1),       # bucketed events in, rows + offsets out
+        "k_mt_reg": 8 * E0 + 8 * (P + 1) + 3 * 8 * P * T,  # rows + offsets in, int64 sums out
         "k_norm": 3 * 8 * n_mapped * T,                   # int64 sums in
     }.get(name, 0)
 
@@ -101,7 +113,7 @@ def main():
     kern = {}
     for k, (ms, n) in ktimes.items():
         avg_ms = ms / max(n, 1)
-        b = algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped)
+        b = algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen))
         kern[k] = dict(avg_ms=avg_ms, launches=n, alg_bytes=b,
                        gbs=(b / (avg_ms * 1e-3) / 1e9) if b and avg_ms > 0 else None)
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])

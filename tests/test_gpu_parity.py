@@ -78,6 +78,19 @@ def test_long_rows(xmt, orc):
     assert plan.info().long_pixels > 0
 
 
+def test_mid_rows(xmt, orc):
+    """Rows of 65..512 events take the LDS kernel."""
+    c = cases.sparse_case(W=16, H=16, F=2000, dpl=4, mu=0.08, tau_c=12.0, seed=13)
+    check_against_oracle(xmt, orc, c)
+
+
+def test_many_taus(xmt, orc):
+    """More than 128 taus: every non-empty row takes the LDS kernel (dpl=16)."""
+    c = cases.sparse_case(W=12, H=12, F=5000, dpl=16, mu=0.01, tau_c=30.0, seed=17)
+    plan, _ = check_against_oracle(xmt, orc, c)
+    assert plan.T > 128
+
+
 def test_mixed_rows(xmt, orc):
     c = cases.sparse_case(W=24, H=24, F=2000, dpl=8, mu=0.3, tau_c=5.0, seed=11)
     check_against_oracle(xmt, orc, c)

@@ -155,9 +155,8 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     TRY(dalloc(&p->d_g2, std::max<size_t>(1, (size_t)h.nq * h.T)));
     TRY(dalloc(&p->d_se, std::max<size_t>(1, (size_t)h.nq * h.T)));
     TRY(dalloc(&p->d_off, (size_t)h.P + 1));
-    TRY(dalloc(&p->d_cnt, (size_t)h.P));
-    TRY(dalloc(&p->d_bsum, (size_t)h.P / 4096 + 2));
     TRY(dalloc(&p->d_long_list, (size_t)h.P));
+    TRY(dalloc(&p->d_mid_list, (size_t)h.P));
     TRY(dalloc(&p->d_counters, 4));
     // dense-row scratch: one row of F 64-bit values per resident workgroup
     {
@@ -167,8 +166,7 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
         p->long_blocks = blocks;
         TRY(dalloc(&p->d_scratch, (size_t)blocks * h.F));
     }
-    if (hipMemsetAsync(p->d_cnt, 0, sizeof(uint32_t) * (size_t)h.P, s) != hipSuccess ||
-        hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
+    if (hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         xpcs_mt_plan_destroy(p);
         return fail(XPCS_ERR_HIP, "plan_create: initialisation failed");
@@ -187,7 +185,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
     dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
-    dfree(p->d_cnt); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_counters);
+    dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
     dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total);
     dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_frame_sum);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -522,12 +520,25 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
     if (p->mode == 1) {
         if (p->raw_events > p->ev_cap) {
             xpcs_status st = dalloc(&p->d_ev, (size_t)p->raw_events);
+            if (!st) st = dalloc(&p->d_b1, (size_t)p->raw_events);
             if (st) return st;
             p->ev_cap = p->raw_events;
         }
-        launch_hist(p, s);
-        launch_scan(p, s);
-        launch_scatter(p, s);
+        p->tshape = transpose_shape(h.P, p->nsel);
+        const int64_t nh = (int64_t)p->tshape.nb * p->tshape.nblk;
+        if (nh + 1 > p->hist_cap) {
+            xpcs_status st = dalloc(&p->d_hist, (size_t)nh);
+            if (!st) st = dalloc(&p->d_hoff, (size_t)nh + 1);
+            if (st) return st;
+            p->hist_cap = nh + 1;
+        }
+        const int64_t nbs = nh / 4096 + 2;
+        if (nbs > p->bsum_cap) {
+            xpcs_status st = dalloc(&p->d_bsum, (size_t)nbs);
+            if (st) return st;
+            p->bsum_cap = nbs;
+        }
+        launch_transpose(p, s);
     }
     if (p->mode == 2) {
         const size_t need = (size_t)h.P * (size_t)h.F;

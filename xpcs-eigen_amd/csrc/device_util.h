@@ -18,6 +18,9 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// Wavefront index within the workgroup, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int mask_rank(uint64_t mask)
 {
@@ -83,6 +86,81 @@ __device__ __forceinline__ uint32_t load_u32_a2(const uint8_t* p)
 {
     const uint16_t* q = reinterpret_cast<const uint16_t*>(p);
     return (uint32_t)q[0] | ((uint32_t)q[1] << 16);
+}
+
+}  // namespace xpcs
+
+namespace xpcs {
+
+// ---- DPP (gfx9 data-parallel primitives; no LDS traffic) --------------------------
+enum : int {
+    kDppRowShl = 0x100,    // + n: lane i reads lane i+n of its row of 16
+    kDppRowShr = 0x110,    // + n: lane i reads lane i-n of its row of 16
+    kDppWaveShl1 = 0x130,  // lane i reads lane i+1 (lane 63 keeps `old`)
+    kDppWaveShr1 = 0x138,  // lane i reads lane i-1 (lane 0 keeps `old`)
+    kDppBcast15 = 0x142,   // rows 1..3 read lane 15 of the previous row
+    kDppBcast31 = 0x143,   // rows 2..3 read lane 31
+    kDppQuadXor1 = 0xB1,   // quad_perm [1,0,3,2]
+    kDppQuadXor2 = 0x4E,   // quad_perm [2,3,0,1]
+};
+
+template <int CTRL, int ROW_MASK = 0xF, bool BOUND_ZERO = false>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+}
+
+// Inclusive prefix sum over the 64 lanes (u32).
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x)
+{
+    x += dpp_u32<kDppRowShr + 1, 0xF, true>(0u, x);
+    x += dpp_u32<kDppRowShr + 2, 0xF, true>(0u, x);
+    x += dpp_u32<kDppRowShr + 4, 0xF, true>(0u, x);
+    x += dpp_u32<kDppRowShr + 8, 0xF, true>(0u, x);
+    x += dpp_u32<kDppBcast15, 0xA>(0u, x);
+    x += dpp_u32<kDppBcast31, 0xC>(0u, x);
+    return x;
+}
+
+// Value of lane i+1 (lane 63 gets `old`).
+__device__ __forceinline__ uint32_t next_lane_u32(uint32_t x, uint32_t old)
+{
+    return dpp_u32<kDppWaveShl1>(old, x);
+}
+
+// Value of lane i-1 (lane 0 gets `old`).
+__device__ __forceinline__ uint32_t prev_lane_u32(uint32_t x, uint32_t old)
+{
+    return dpp_u32<kDppWaveShr1>(old, x);
+}
+
+// Value of lane (i ^ J) for J in {1, 2, 4, 8} without LDS, else ds_bpermute.
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t x)
+{
+    if constexpr (J == 1) {
+        return dpp_u32<kDppQuadXor1>(0u, x);
+    } else if constexpr (J == 2) {
+        return dpp_u32<kDppQuadXor2>(0u, x);
+    } else if constexpr (J == 4 || J == 8) {
+        const uint32_t up = dpp_u32<kDppRowShl + J>(0u, x);    // from lane i+J
+        const uint32_t dn = dpp_u32<kDppRowShr + J>(0u, x);    // from lane i-J
+        return (lane_id() & J) ? dn : up;
+    } else {
+        return (uint32_t)__shfl_xor((int)x, J, 64);
+    }
+}
+
+// Lane l receives `x` from the lane whose dst == l (lanes that nobody targets
+// keep garbage).
+__device__ __forceinline__ uint32_t push_to_lane(int dst, uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)x);
+}
+
+__device__ __forceinline__ uint32_t read_lane_u32(uint32_t x, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
 }
 
 }  // namespace xpcs

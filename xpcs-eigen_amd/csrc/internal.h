@@ -72,6 +72,14 @@ struct HostTables {
     std::vector<int32_t> q_chunk_first;       // [nq+1]
 };
 
+// Launch shape of the bucketed sparse transpose.
+struct TransposeShape {
+    int bshift = 8;       // 2^bshift pixels per bucket
+    int32_t nb = 0;       // buckets
+    int64_t cf = 1;       // raw frames per chunk
+    int32_t nblk = 1;     // chunks
+};
+
 struct KernelTimer {
     std::string name;
     hipEvent_t a, b;
@@ -122,8 +130,13 @@ struct xpcs_mt_plan {
     int64_t ev_cap = 0;
     uint64_t* d_ev = nullptr;            // (frame << 32) | count
     int64_t* d_off = nullptr;            // [P+1]
-    uint32_t* d_cnt = nullptr;           // [P] zero between executes
+    uint64_t* d_b1 = nullptr;            // bucketed events (transpose stage)
+    uint32_t* d_hist = nullptr;          // [nb][nblk] events per (bucket, chunk)
+    int64_t* d_hoff = nullptr;           // [nb*nblk+1] segment starts
+    int64_t hist_cap = 0;
     int64_t* d_bsum = nullptr;           // scan block sums
+    int64_t bsum_cap = 0;
+    xpcs::TransposeShape tshape;
     int64_t events = 0;                  // known after execute (host copy)
     // dense rows
     uint32_t* d_rows = nullptr;          // [P][F]
@@ -131,7 +144,8 @@ struct xpcs_mt_plan {
 
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]
-    uint32_t* d_counters = nullptr;      // [0] long count, [1] error flags
+    int32_t* d_mid_list = nullptr;       // [P] rows of 65..CAP events
+    uint32_t* d_counters = nullptr;      // [0] long rows, [1] error flags, [2] mid rows
     uint64_t* d_scratch = nullptr;
     size_t scratch_cap = 0;
     int long_blocks = 0;
@@ -163,9 +177,8 @@ void set_error(const char* fmt, ...);
 xpcs_status fail(xpcs_status st, const char* fmt, ...);
 
 // kernel launchers (kernels_*.hip); all asynchronous on `s`
-void launch_hist(xpcs_mt_plan* p, hipStream_t s);
-void launch_scan(xpcs_mt_plan* p, hipStream_t s);
-void launch_scatter(xpcs_mt_plan* p, hipStream_t s);
+TransposeShape transpose_shape(int64_t P, int64_t nsel);
+void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);

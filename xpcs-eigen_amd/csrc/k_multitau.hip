@@ -62,38 +62,149 @@ __device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uin
     return carry;
 }
 
-template <int CAP, int WAVES>
-__global__ void __launch_bounds__(kWave * WAVES)
-k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P,
-            const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
-            int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
-            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
-            int32_t* __restrict__ long_list, uint32_t* __restrict__ counters)
+// ---- register path: rows of at most 64 events, one event per lane ------------------
+constexpr int kRegMaxTaus = 128;
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+template <int J>
+__device__ __forceinline__ void bitonic_step(uint32_t& k, uint32_t& v, int kk)
 {
-    __shared__ uint32_t s_key[WAVES][CAP];
-    __shared__ uint32_t s_val[WAVES][CAP];
-    __shared__ unsigned long long s_aux[WAVES][CAP];   // prefix sums, then lag accumulators
+    const uint32_t pk = xor_lane_u32<J>(k);
+    const uint32_t pv = xor_lane_u32<J>(v);
     const int lane = lane_id();
-    const int w = threadIdx.x / kWave;
-    uint32_t* key = s_key[w];
-    uint32_t* val = s_val[w];
-    unsigned long long* aux = s_aux[w];
+    const bool up = (lane & kk) == 0;
+    const bool lower = (lane & J) == 0;
+    const bool take = (lower == up) ? (pk < k) : (pk > k);
+    if (take) {
+        k = pk;
+        v = pv;
+    }
+}
+
+// Ascending sort of (k, v) over the first m lanes (m a power of two <= 64;
+// lanes >= m hold kNoKey).
+__device__ __forceinline__ void bitonic_sort64(uint32_t& k, uint32_t& v, int m)
+{
+    if (m >= 2) { bitonic_step<1>(k, v, 2); }
+    if (m >= 4) { bitonic_step<2>(k, v, 4); bitonic_step<1>(k, v, 4); }
+    if (m >= 8) { bitonic_step<4>(k, v, 8); bitonic_step<2>(k, v, 8); bitonic_step<1>(k, v, 8); }
+    if (m >= 16) {
+        bitonic_step<8>(k, v, 16); bitonic_step<4>(k, v, 16);
+        bitonic_step<2>(k, v, 16); bitonic_step<1>(k, v, 16);
+    }
+    if (m >= 32) {
+        bitonic_step<16>(k, v, 32); bitonic_step<8>(k, v, 32); bitonic_step<4>(k, v, 32);
+        bitonic_step<2>(k, v, 32); bitonic_step<1>(k, v, 32);
+    }
+    if (m >= 64) {
+        bitonic_step<32>(k, v, 64); bitonic_step<16>(k, v, 64); bitonic_step<8>(k, v, 64);
+        bitonic_step<4>(k, v, 64); bitonic_step<2>(k, v, 64); bitonic_step<1>(k, v, 64);
+    }
+}
+
+// One binning step in registers: keys >> shift, keys >= lf dropped, equal keys
+// summed.  `pairs_only` when runs are at most two long (every level after the
+// first, whose keys are distinct).  Returns the new count; lanes past it hold
+// (kNoKey, 0).  Without repeated or dropped keys no data moves.
+__device__ __forceinline__ int reg_level(uint32_t& k, uint32_t& v, int nl, int shift, uint32_t lf,
+                                         bool pairs_only)
+{
+    const int lane = lane_id();
+    const bool valid = lane < nl;
+    const uint32_t kk = valid ? (k >> shift) : kNoKey;
+    const uint32_t kp = prev_lane_u32(kk, kNoKey);
+    const bool dup = valid && kk == kp;
+    const bool drop = valid && kk >= lf;
+    const uint64_t dupmask = __ballot(dup);
+    const uint64_t dropmask = __ballot(drop);
+    if (dupmask == 0 && dropmask == 0) {
+        k = kk;
+        return nl;
+    }
+    const bool head = valid && !dup && !drop;
+    uint32_t sum;
+    if (pairs_only) {
+        const uint32_t kn = next_lane_u32(kk, kNoKey);
+        const uint32_t vn = next_lane_u32(v, 0u);
+        sum = v + ((lane + 1 < nl && kn == kk) ? vn : 0u);
+    } else {
+        const uint32_t pre = wave_incl_scan_u32(valid ? v : 0u);
+        const uint64_t starts = __ballot(valid && !dup);
+        const uint64_t above = lane == 63 ? 0ull : (starts & ~((2ull << lane) - 1ull));
+        const int j = above ? (int)__builtin_ctzll(above) : nl;      // end of this lane's run
+        const uint32_t pj = (uint32_t)__shfl((int)pre, j - 1 < 0 ? 0 : j - 1, 64);
+        const uint32_t pi = prev_lane_u32(pre, 0u);
+        sum = pj - pi;
+    }
+    const uint64_t hm = __ballot(head);
+    const int newn = __popcll(hm);
+    const int dst = head ? mask_rank(hm) : 63;
+    uint32_t k2 = push_to_lane(dst, kk);
+    uint32_t v2 = push_to_lane(dst, sum);
+    if (lane >= newn) {
+        k2 = kNoKey;
+        v2 = 0u;
+    }
+    k = k2;
+    v = v2;
+    return newn;
+}
+
+// Sum of the level-0 values of keys < b: keys/prefix are sorted across the
+// first n lanes (kNoKey beyond).  Lane-parallel binary search; every lane takes
+// part in every permute.
+__device__ __forceinline__ uint32_t reg_prefix_below(uint32_t k, uint32_t pre, int n, uint32_t b)
+{
+    int lo = 0, len = n;
+#pragma unroll
+    for (int step = 0; step < 7; step++) {
+        const int half = len >> 1;
+        const int idx = min(lo + half, kWave - 1);
+        const uint32_t kmid = (uint32_t)__shfl((int)k, idx, 64);
+        const bool live = len > 0;
+        const bool right = live && kmid < b;
+        lo = right ? lo + half + 1 : lo;
+        len = live ? (right ? len - half - 1 : half) : 0;
+    }
+    const uint32_t s = (uint32_t)__shfl((int)pre, lo > 0 ? lo - 1 : 0, 64);
+    return lo > 0 ? s : 0u;
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES)
+k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P, int32_t T,
+         const TauParam* __restrict__ taus, const LevelParam* __restrict__ levels, int32_t nlev,
+         int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
+         uint64_t* __restrict__ pix_total, int32_t* __restrict__ mid_list,
+         int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap)
+{
+    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus];
+    const int lane = lane_id();
+    const int w = wave_id();
+    unsigned long long* acc = s_acc[w];
     const int64_t nw = (int64_t)gridDim.x * WAVES;
+    const bool reg_ok = T <= kRegMaxTaus;
 
     for (int64_t p = (int64_t)blockIdx.x * WAVES + w; p < P; p += nw) {
         const int64_t base = off[p];
         const int64_t n64 = off[p + 1] - base;
-        int64_t* g2row = G2 + p * T;
-        int64_t* iprow = IP + p * T;
-        int64_t* ifrow = IF + p * T;
-        if (n64 > CAP) {
+        if (n64 > kWave || (!reg_ok && n64 > 0)) {
             if (lane == 0) {
-                uint32_t k = atomicAdd(&counters[0], 1u);
-                long_list[k] = (int32_t)p;
+                if (n64 <= mid_cap) {
+                    const uint32_t q = atomicAdd(&counters[2], 1u);
+                    mid_list[q] = (int32_t)p;
+                } else {
+                    const uint32_t q = atomicAdd(&counters[0], 1u);
+                    long_list[q] = (int32_t)p;
+                }
             }
             continue;
         }
-        if (n64 == 0) {
+        int64_t* g2row = G2 + p * T;
+        int64_t* iprow = IP + p * T;
+        int64_t* ifrow = IF + p * T;
+        const int n = (int)n64;
+        if (n == 0) {
             for (int i = lane; i < T; i += kWave) {
                 g2row[i] = 0;
                 iprow[i] = 0;
@@ -102,7 +213,85 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, in
             if (lane == 0) pix_total[p] = 0;
             continue;
         }
-        const int n = (int)n64;
+        for (int i = lane; i < T; i += kWave) acc[i] = 0ull;
+        uint32_t k = kNoKey, v = 0u;
+        if (lane < n) {
+            const uint64_t e = ev[base + lane];
+            k = (uint32_t)(e >> 32);
+            v = (uint32_t)e;
+        }
+        bitonic_sort64(k, v, next_pow2(n));
+        int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
+        // IP and IF of every tau from the level-0 prefix sums:
+        // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau')
+        {
+            const uint32_t pre = wave_incl_scan_u32(v);
+            if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
+            for (int i0 = 0; i0 < T; i0 += kWave) {
+                const int i = i0 + lane;
+                const TauParam tp = taus[i < T ? i : T - 1];
+                const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
+                const uint32_t b_if = (uint32_t)tp.tp << tp.level;
+                const uint32_t b_end = (uint32_t)tp.lf << tp.level;
+                const uint32_t s_ip = reg_prefix_below(k, pre, nl, b_ip);
+                const uint32_t s_if = reg_prefix_below(k, pre, nl, b_if);
+                const uint32_t s_end = reg_prefix_below(k, pre, nl, b_end);
+                if (i < T) {
+                    iprow[i] = (int64_t)s_ip;
+                    ifrow[i] = (int64_t)(s_end - s_if);
+                }
+            }
+        }
+        for (int L = 0; L < nlev; L++) {
+            const LevelParam lp = levels[L];
+            if (L > 0) nl = reg_level(k, v, nl, 1, (uint32_t)lp.lf, true);
+            if (nl < 2) break;
+            // lag products: partner of lane i is lane i+m, m = 1..hi
+            const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
+            uint32_t kj = k, vj = v;
+            for (uint32_t m = 1; m <= hi; m++) {
+                kj = next_lane_u32(kj, kNoKey);
+                vj = next_lane_u32(vj, 0u);
+                const uint32_t d = kj - k;
+                const bool active = lane < nl && kj != kNoKey && d <= hi;
+                if (__ballot(active) == 0ull) break;
+                if (active && d >= lo)
+                    atomicAdd(&acc[lp.first + (int)(d - lo)], (unsigned long long)v * (unsigned long long)vj);
+            }
+        }
+        wave_sync();
+        for (int i = lane; i < T; i += kWave) g2row[i] = (int64_t)acc[i];
+        wave_sync();
+    }
+}
+
+template <int CAP, int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES)
+k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
+            const int32_t* __restrict__ rows, const uint32_t* __restrict__ counters,
+            const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
+            int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
+            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total)
+{
+    __shared__ uint32_t s_key[WAVES][CAP];
+    __shared__ uint32_t s_val[WAVES][CAP];
+    __shared__ unsigned long long s_aux[WAVES][CAP];   // prefix sums, then lag accumulators
+    const int lane = lane_id();
+    const int w = wave_id();
+    uint32_t* key = s_key[w];
+    uint32_t* val = s_val[w];
+    unsigned long long* aux = s_aux[w];
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+
+    const int64_t nrows = (int64_t)counters[2];
+    for (int64_t r = (int64_t)blockIdx.x * WAVES + w; r < nrows; r += nw) {
+        const int64_t p = rows[r];
+        const int64_t base = off[p];
+        const int64_t n64 = off[p + 1] - base;
+        int64_t* g2row = G2 + p * T;
+        int64_t* iprow = IP + p * T;
+        int64_t* ifrow = IF + p * T;
+        const int n = (int)n64;         // 0 < n <= CAP by construction of the list
         const int m = next_pow2(n);
         // ---- load and sort by frame ----
         if (m <= kWave) {
@@ -345,16 +534,22 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
 // ---- launchers ------------------------------------------------------------------
 constexpr int kSparseWaves = 8;
 constexpr int kSparseCapUsed = 512;
+constexpr int kRegWaves = 8;
 
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
 {
     const HostTables& h = p->h;
-    int64_t blocks = ((int64_t)h.P + kSparseWaves - 1) / kSparseWaves;
-    if (blocks > 8192) blocks = 8192;
-    timer_begin(p, s, "k_mt_sparse");
-    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<(int)blocks, kWave * kSparseWaves, 0, s>>>(
-        p->d_ev, p->d_off, h.P, p->d_taus, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
-        p->d_IP, p->d_IF, p->d_pix_total, p->d_long_list, p->d_counters);
+    int64_t blocks = ((int64_t)h.P + kRegWaves - 1) / kRegWaves;
+    if (blocks > 16384) blocks = 16384;
+    timer_begin(p, s, "k_mt_reg");
+    k_mt_reg<kRegWaves><<<(int)blocks, kWave * kRegWaves, 0, s>>>(
+        p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed);
+    timer_end(p, s);
+    timer_begin(p, s, "k_mt_lds");
+    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<1024, kWave * kSparseWaves, 0, s>>>(
+        p->d_ev, p->d_off, p->d_mid_list, p->d_counters, p->d_taus, h.T, p->d_levels,
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total);
     timer_end(p, s);
 }
 

@@ -1,14 +1,10 @@
 // (a) Frame -> pixel transpose kernels.
 //
 // Sparse IMM (SparseFilter::Apply, src/xpcs/filter/sparse_filter.cpp:115-193):
-// a counting sort of the frame-major events by compact pixel id.
-//   k_hist     one workgroup per selected raw frame; masked, non-zero events
-//              bump their pixel's counter (the mask lookup drops the rest).
-//   k_scan_*   exclusive scan of the counters -> 64-bit row offsets.
-//   k_scatter  same traversal; each event claims a slot of its row with an
-//              atomic decrement of the counter (which leaves the counters at 0
-//              for the next job) and stores (frame << 32 | count).
-// Rows come out in arbitrary slot order; the correlator sorts each row on chip.
+// a two-level counting sort of the frame-major events by compact pixel id
+// (buckets of pixels first, then rows within a bucket), with masked pixels and
+// zero counts dropped on the way.  Rows come out in arbitrary order within a
+// row; the correlator sorts each row on chip.
 //
 // Dense IMM (DenseFilter::Apply, src/xpcs/filter/dense_filter.cpp:121-210):
 // an LDS-tiled transpose of whole frames into pixel-major rows [P][F] of
@@ -18,51 +14,129 @@
 
 namespace xpcs {
 
-__global__ void __launch_bounds__(256)
-k_hist(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
-       const uint32_t* __restrict__ sel_dlen, int64_t nsel, const int32_t* __restrict__ pix2c,
-       uint32_t npix_det, uint32_t* __restrict__ cnt, uint32_t* __restrict__ counters)
+// ---- bucketed transpose (no per-event global atomics) -------------------------------
+// Pixels are grouped into buckets of 2^bshift consecutive compact ids.
+//   k_tr_count   per chunk of CF raw frames: LDS histogram of events per bucket,
+//                written to hist[bucket][chunk].
+//   (scan)       exclusive scan of hist in bucket-major order: the start of
+//                every (bucket, chunk) segment of the bucketed array B1.
+//   k_tr_bucket  per chunk: each event takes a slot of its (bucket, chunk)
+//                segment through an LDS cursor and is written to B1 as
+//                (frame << 32 | local pixel << 16 | count).
+//   k_tr_rows    per bucket: LDS counting sort of its events by local pixel;
+//                the bucket's rows are contiguous, so the global row offsets
+//                follow from the bucket start, and events land in their rows
+//                through LDS cursors.
+// Only LDS atomics are used; every global write goes to a small, hot region.
+constexpr int kTrBlock = 256;
+__device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total);
+
+__device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j,
+                                         const int32_t* __restrict__ pix2c, uint32_t npix_det,
+                                         uint32_t* __restrict__ counters, int32_t& c, uint32_t& v)
 {
-    for (int64_t k = blockIdx.x; k < nsel; k += gridDim.x) {
+    const uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
+    v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen)[j];
+    if (idx >= npix_det) {
+        if (counters) atomicOr(&counters[1], 1u);
+        return false;
+    }
+    c = pix2c[idx];
+    return c >= 0 && v != 0;
+}
+
+__global__ void __launch_bounds__(kTrBlock)
+k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
+           const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf,
+           const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
+           int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters)
+{
+    extern __shared__ uint32_t s_h[];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0u;
+    __syncthreads();
+    const int64_t k0 = (int64_t)blockIdx.x * cf;
+    const int64_t k1 = min(nsel, k0 + cf);
+    for (int64_t k = k0; k < k1; k++) {
         const uint8_t* base = imm + sel_off[k];
         const uint32_t dlen = sel_dlen[k];
-        const uint16_t* vals = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen);
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
         for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
-            uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
-            uint16_t v = vals[j];
-            if (idx >= npix_det) {
-                atomicOr(&counters[1], 1u);
-                continue;
-            }
-            int32_t c = pix2c[idx];
-            if (c < 0 || v == 0) continue;
-            atomicAdd(&cnt[c], 1u);
+            int32_t c;
+            uint32_t v;
+            if (tr_event(base, dlen, al4, j, pix2c, npix_det, counters, c, v)) atomicAdd(&s_h[c >> bshift], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)b * nblk + blockIdx.x] = s_h[b];
+}
+
+__global__ void __launch_bounds__(kTrBlock)
+k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
+            const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
+            int64_t cf, const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
+            int32_t nblk, const int64_t* __restrict__ hoff, uint64_t* __restrict__ b1)
+{
+    extern __shared__ uint32_t s_cur[];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s_cur[b] = 0u;
+    __syncthreads();
+    const uint32_t lmask = (1u << bshift) - 1u;
+    const int64_t k0 = (int64_t)blockIdx.x * cf;
+    const int64_t k1 = min(nsel, k0 + cf);
+    for (int64_t k = k0; k < k1; k++) {
+        const uint8_t* base = imm + sel_off[k];
+        const uint32_t dlen = sel_dlen[k];
+        const uint64_t f = (uint64_t)(uint32_t)sel_frame[k] << 32;
+        const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
+        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
+            int32_t c;
+            uint32_t v;
+            if (!tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c, v)) continue;
+            const int b = c >> bshift;
+            const uint32_t slot = atomicAdd(&s_cur[b], 1u);
+            b1[hoff[(int64_t)b * nblk + blockIdx.x] + slot] =
+                f | ((uint64_t)((uint32_t)c & lmask) << 16) | (uint64_t)v;
         }
     }
 }
 
-__global__ void __launch_bounds__(256)
-k_scatter(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
-          const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
-          const int32_t* __restrict__ pix2c, uint32_t npix_det, uint32_t* __restrict__ cnt,
-          const int64_t* __restrict__ off, uint64_t* __restrict__ ev)
+__global__ void __launch_bounds__(kTrBlock)
+k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int32_t nblk, int bshift,
+          int32_t nb, int32_t P, int64_t* __restrict__ off, uint64_t* __restrict__ ev)
 {
-    for (int64_t k = blockIdx.x; k < nsel; k += gridDim.x) {
-        const uint8_t* base = imm + sel_off[k];
-        const uint32_t dlen = sel_dlen[k];
-        const uint64_t f = (uint64_t)(uint32_t)sel_frame[k] << 32;
-        const uint16_t* vals = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen);
-        const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
-            uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
-            uint16_t v = vals[j];
-            if (idx >= npix_det) continue;
-            int32_t c = pix2c[idx];
-            if (c < 0 || v == 0) continue;
-            uint32_t r = atomicSub(&cnt[c], 1u);
-            ev[off[c] + (int64_t)r - 1] = f | (uint64_t)v;
+    extern __shared__ unsigned long long s_row[];   // [2^bshift] counts, then cursors
+    __shared__ int64_t s_warp[33];
+    const int nloc = 1 << bshift;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int64_t s0 = hoff[(int64_t)b * nblk];
+        const int64_t s1 = hoff[(int64_t)(b + 1) * nblk];   // hoff[nb*nblk] is the total
+        for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_row[i] = 0ull;
+        __syncthreads();
+        for (int64_t e = s0 + threadIdx.x; e < s1; e += blockDim.x)
+            atomicAdd(&s_row[(uint32_t)(b1[e] >> 16) & (uint32_t)(nloc - 1)], 1ull);
+        __syncthreads();
+        // exclusive scan of the nloc counts (nloc is a multiple of blockDim.x)
+        const int per = nloc / (int)blockDim.x;
+        int64_t acc = 0;
+        for (int u = 0; u < per; u++) acc += (int64_t)s_row[threadIdx.x * per + u];
+        int64_t tot;
+        int64_t ex = block_excl_scan(acc, s_warp, tot) + s0;
+        for (int u = 0; u < per; u++) {
+            const int i = threadIdx.x * per + u;
+            const int64_t cnt = (int64_t)s_row[i];
+            const int64_t c = ((int64_t)b << bshift) + i;
+            if (c < P) off[c] = ex;
+            s_row[i] = (unsigned long long)ex;
+            ex += cnt;
         }
+        if (b == nb - 1 && threadIdx.x == 0) off[P] = s1;
+        __syncthreads();
+        for (int64_t e = s0 + threadIdx.x; e < s1; e += blockDim.x) {
+            const uint64_t x = b1[e];
+            const uint32_t l = (uint32_t)(x >> 16) & (uint32_t)(nloc - 1);
+            const uint64_t pos = atomicAdd(&s_row[l], 1ull);
+            ev[pos] = (x & 0xFFFFFFFF00000000ull) | (x & 0xFFFFull);
+        }
+        __syncthreads();
     }
 }
 
@@ -215,34 +289,49 @@ static int grid_for(int64_t n, int64_t cap)
     return (int)(n < cap ? n : cap);
 }
 
-void launch_hist(xpcs_mt_plan* p, hipStream_t s)
+// Exclusive scan of n u32 values into out[0..n] (out[n] = total).
+static void scan_u32(const uint32_t* in, int64_t n, int64_t* out, int64_t* bsum, hipStream_t s)
 {
-    timer_begin(p, s, "k_hist");
-    k_hist<<<grid_for(p->nsel, 1 << 20), 256, 0, s>>>(p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel,
-                                                    p->d_pix2c, (uint32_t)((int64_t)p->h.W * p->h.H),
-                                                    p->d_cnt, p->d_counters);
-    timer_end(p, s);
-}
-
-void launch_scan(xpcs_mt_plan* p, hipStream_t s)
-{
-    const int64_t n = p->h.P;
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
-    timer_begin(p, s, "k_scan");
-    k_scan_reduce<<<(int)nb, kScanBlock, 0, s>>>(p->d_cnt, n, p->d_bsum);
-    k_scan_bsum<<<1, 1024, 0, s>>>(p->d_bsum, nb);
-    k_scan_tile<<<(int)nb, kScanBlock, 0, s>>>(p->d_cnt, n, p->d_bsum, p->d_off);
-    k_scan_total<<<1, 64, 0, s>>>(p->d_cnt, n, p->d_off);
-    timer_end(p, s);
+    k_scan_reduce<<<(int)nb, kScanBlock, 0, s>>>(in, n, bsum);
+    k_scan_bsum<<<1, 1024, 0, s>>>(bsum, nb);
+    k_scan_tile<<<(int)nb, kScanBlock, 0, s>>>(in, n, bsum, out);
+    k_scan_total<<<1, 64, 0, s>>>(in, n, out);
 }
 
-void launch_scatter(xpcs_mt_plan* p, hipStream_t s)
+TransposeShape transpose_shape(int64_t P, int64_t nsel)
 {
-    timer_begin(p, s, "k_scatter");
-    k_scatter<<<grid_for(p->nsel, 1 << 20), 256, 0, s>>>(p->d_imm, p->d_sel_off, p->d_sel_dlen,
-                                                       p->d_sel_frame, p->nsel, p->d_pix2c,
-                                                       (uint32_t)((int64_t)p->h.W * p->h.H), p->d_cnt,
-                                                       p->d_off, p->d_ev);
+    TransposeShape t;
+    t.bshift = 8;
+    while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 8192) t.bshift++;
+    t.nb = (int32_t)((P + (int64_t(1) << t.bshift) - 1) >> t.bshift);
+    t.cf = nsel > 512 ? (nsel + 511) / 512 : 1;
+    t.nblk = (int32_t)((nsel + t.cf - 1) / t.cf);
+    if (t.nblk < 1) t.nblk = 1;
+    return t;
+}
+
+void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
+{
+    const TransposeShape& t = p->tshape;
+    const uint32_t npd = (uint32_t)((int64_t)p->h.W * p->h.H);
+    const int64_t nh = (int64_t)t.nb * t.nblk;
+    timer_begin(p, s, "k_tr_count");
+    k_tr_count<<<t.nblk, kTrBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, p->d_pix2c, npd, t.bshift, t.nb, t.nblk,
+        p->d_hist, p->d_counters);
+    timer_end(p, s);
+    timer_begin(p, s, "k_tr_scan");
+    scan_u32(p->d_hist, nh, p->d_hoff, p->d_bsum, s);
+    timer_end(p, s);
+    timer_begin(p, s, "k_tr_bucket");
+    k_tr_bucket<<<t.nblk, kTrBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, p->d_pix2c, npd, t.bshift,
+        t.nb, t.nblk, p->d_hoff, p->d_b1);
+    timer_end(p, s);
+    timer_begin(p, s, "k_tr_rows");
+    k_tr_rows<<<t.nb < 4096 ? t.nb : 4096, kTrBlock, (size_t)sizeof(unsigned long long) << t.bshift, s>>>(
+        p->d_b1, p->d_hoff, t.nblk, t.bshift, t.nb, p->h.P, p->d_off, p->d_ev);
     timer_end(p, s);
 }
 

@@ -37,10 +37,10 @@ WORKLOADS = {
 def transpose_hist_entries(P, nsel):
     """Entries of the (bucket, chunk) histogram the sparse transpose uses."""
     bshift = 8
-    while ((P + (1 << bshift) - 1) >> bshift) > 8192:
+    while ((P + (1 << bshift) - 1) >> bshift) > 4096:
         bshift += 1
     nb = (P + (1 << bshift) - 1) >> bshift
-    cf = (nsel + 511) // 512 if nsel > 512 else 1
+    cf = (nsel + 1023) // 1024 if nsel > 1024 else 1
     nblk = max(1, (nsel + cf - 1) // cf)
     return nb * nblk
 

@@ -29,10 +29,20 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
             int64_t sg = 0, sp = 0, sf = 0;
             double wg = 0.0, wp = 0.0, wf = 0.0;
             double n = 0.0, mean = 0.0, m2 = 0.0;
-            for (int p = ch.begin; p < ch.end; p++) {
-                const int64_t g = G2[(int64_t)p * T + i];
-                const int64_t a = IP[(int64_t)p * T + i];
-                const int64_t b = IF[(int64_t)p * T + i];
+            for (int p0 = ch.begin; p0 < ch.end; p0 += 4) {
+              int64_t gg[4], aa[4], bb[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                  const int pu = min(p0 + u, ch.end - 1);
+                  gg[u] = G2[(int64_t)pu * T + i];
+                  aa[u] = IP[(int64_t)pu * T + i];
+                  bb[u] = IF[(int64_t)pu * T + i];
+              }
+              for (int u = 0; u < 4 && p0 + u < ch.end; u++) {
+                const int p = p0 + u;
+                const int64_t g = gg[u];
+                const int64_t a = aa[u];
+                const int64_t b = bb[u];
                 double x;
                 if (weight) {
                     const double w = weight[p];
@@ -51,6 +61,7 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
                 const double d = x - mean;
                 mean += d / n;
                 m2 += d * (x - mean);
+              }
             }
             NormPartial r;
             if (weight) {

@@ -29,6 +29,7 @@ namespace xpcs {
 //                through LDS cursors.
 // Only LDS atomics are used; every global write goes to a small, hot region.
 constexpr int kTrBlock = 256;
+constexpr int kTrChunkBlock = 1024;   // workgroup of the per-chunk kernels
 __device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total);
 
 __device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j,
@@ -45,7 +46,7 @@ __device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, boo
     return c >= 0 && v != 0;
 }
 
-__global__ void __launch_bounds__(kTrBlock)
+__global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf,
            const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
@@ -60,24 +61,37 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         const uint8_t* base = imm + sel_off[k];
         const uint32_t dlen = sel_dlen[k];
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
-            int32_t c;
-            uint32_t v;
-            if (tr_event(base, dlen, al4, j, pix2c, npix_det, counters, c, v)) atomicAdd(&s_h[c >> bshift], 1u);
+        const uint32_t bd = blockDim.x;
+        for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
+            int32_t c[4];
+            uint32_t v[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + u * bd;
+                ok[u] = j < dlen && tr_event(base, dlen, al4, j, pix2c, npix_det, counters, c[u], v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
         }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)b * nblk + blockIdx.x] = s_h[b];
 }
 
-__global__ void __launch_bounds__(kTrBlock)
+__global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
             const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
             int64_t cf, const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
             int32_t nblk, const int64_t* __restrict__ hoff, uint64_t* __restrict__ b1)
 {
-    extern __shared__ uint32_t s_cur[];
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s_cur[b] = 0u;
+    extern __shared__ unsigned long long s_base[];   // [nb] segment starts of this chunk
+    uint32_t* s_cur = reinterpret_cast<uint32_t*>(s_base + nb);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        s_cur[b] = 0u;
+        s_base[b] = (unsigned long long)hoff[(int64_t)b * nblk + blockIdx.x];
+    }
     __syncthreads();
     const uint32_t lmask = (1u << bshift) - 1u;
     const int64_t k0 = (int64_t)blockIdx.x * cf;
@@ -87,14 +101,25 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
         const uint32_t dlen = sel_dlen[k];
         const uint64_t f = (uint64_t)(uint32_t)sel_frame[k] << 32;
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
-            int32_t c;
-            uint32_t v;
-            if (!tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c, v)) continue;
-            const int b = c >> bshift;
-            const uint32_t slot = atomicAdd(&s_cur[b], 1u);
-            b1[hoff[(int64_t)b * nblk + blockIdx.x] + slot] =
-                f | ((uint64_t)((uint32_t)c & lmask) << 16) | (uint64_t)v;
+        const uint32_t bd = blockDim.x;
+        for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
+            int32_t c[4];
+            uint32_t v[4];
+            bool ok[4];
+            uint32_t slot[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + u * bd;
+                ok[u] = j < dlen && tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c[u], v[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (ok[u]) slot[u] = atomicAdd(&s_cur[c[u] >> bshift], 1u);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (ok[u])
+                    b1[s_base[c[u] >> bshift] + slot[u]] =
+                        f | ((uint64_t)((uint32_t)c[u] & lmask) << 16) | (uint64_t)v[u];
         }
     }
 }
@@ -111,8 +136,15 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
         const int64_t s1 = hoff[(int64_t)(b + 1) * nblk];   // hoff[nb*nblk] is the total
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_row[i] = 0ull;
         __syncthreads();
-        for (int64_t e = s0 + threadIdx.x; e < s1; e += blockDim.x)
-            atomicAdd(&s_row[(uint32_t)(b1[e] >> 16) & (uint32_t)(nloc - 1)], 1ull);
+        const int64_t bd = blockDim.x;
+        for (int64_t e0 = s0 + threadIdx.x; e0 < s1; e0 += 4 * bd) {
+            uint64_t x[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) x[u] = (e0 + u * bd < s1) ? b1[e0 + u * bd] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (x[u] != ~0ull) atomicAdd(&s_row[(uint32_t)(x[u] >> 16) & (uint32_t)(nloc - 1)], 1ull);
+        }
         __syncthreads();
         // exclusive scan of the nloc counts (nloc is a multiple of blockDim.x)
         const int per = nloc / (int)blockDim.x;
@@ -130,11 +162,16 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
         }
         if (b == nb - 1 && threadIdx.x == 0) off[P] = s1;
         __syncthreads();
-        for (int64_t e = s0 + threadIdx.x; e < s1; e += blockDim.x) {
-            const uint64_t x = b1[e];
-            const uint32_t l = (uint32_t)(x >> 16) & (uint32_t)(nloc - 1);
-            const uint64_t pos = atomicAdd(&s_row[l], 1ull);
-            ev[pos] = (x & 0xFFFFFFFF00000000ull) | (x & 0xFFFFull);
+        for (int64_t e0 = s0 + threadIdx.x; e0 < s1; e0 += 4 * bd) {
+            uint64_t x[4], pos[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) x[u] = (e0 + u * bd < s1) ? b1[e0 + u * bd] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (x[u] != ~0ull) pos[u] = atomicAdd(&s_row[(uint32_t)(x[u] >> 16) & (uint32_t)(nloc - 1)], 1ull);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (x[u] != ~0ull) ev[pos[u]] = (x[u] & 0xFFFFFFFF00000000ull) | (x[u] & 0xFFFFull);
         }
         __syncthreads();
     }
@@ -303,9 +340,9 @@ TransposeShape transpose_shape(int64_t P, int64_t nsel)
 {
     TransposeShape t;
     t.bshift = 8;
-    while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 8192) t.bshift++;
+    while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 4096) t.bshift++;
     t.nb = (int32_t)((P + (int64_t(1) << t.bshift) - 1) >> t.bshift);
-    t.cf = nsel > 512 ? (nsel + 511) / 512 : 1;
+    t.cf = nsel > 1024 ? (nsel + 1023) / 1024 : 1;
     t.nblk = (int32_t)((nsel + t.cf - 1) / t.cf);
     if (t.nblk < 1) t.nblk = 1;
     return t;
@@ -317,7 +354,7 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
     const uint32_t npd = (uint32_t)((int64_t)p->h.W * p->h.H);
     const int64_t nh = (int64_t)t.nb * t.nblk;
     timer_begin(p, s, "k_tr_count");
-    k_tr_count<<<t.nblk, kTrBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+    k_tr_count<<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
         p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, p->d_pix2c, npd, t.bshift, t.nb, t.nblk,
         p->d_hist, p->d_counters);
     timer_end(p, s);
@@ -325,7 +362,7 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
     scan_u32(p->d_hist, nh, p->d_hoff, p->d_bsum, s);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_bucket");
-    k_tr_bucket<<<t.nblk, kTrBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+    k_tr_bucket<<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(
         p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, p->d_pix2c, npd, t.bshift,
         t.nb, t.nblk, p->d_hoff, p->d_b1);
     timer_end(p, s);

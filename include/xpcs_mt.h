@@ -105,6 +105,21 @@ xpcs_status xpcs_delays_per_level(int32_t frames, int32_t dpl, int32_t max_level
                                   int32_t* level, int32_t* tau, int32_t cap, int32_t* count);
 
 /*
+ * q-partition tables, no device needed (Configuration::BuildQMap,
+ * src/xpcs/configuration.cpp:244-381).  order[] lists the flat index of every
+ * masked pixel (dqmap >= 1 && sqmap >= 1): first the pixels of m_mapping in its
+ * iteration order (q ascending, s ascending, raster order), then the pixels the
+ * sbin->qbin dedupe drops (raster order).  Groups (q, s, [begin, end)) index
+ * order[].  counts[0..4] = pixels, mapped pixels, groups, max q, max s.  Pass
+ * NULL arrays to query counts first; order must hold `pixels` entries and the
+ * group arrays group_cap entries.
+ */
+xpcs_status xpcs_build_qmap(int32_t width, int32_t height, const int32_t* dqmap,
+                            const int32_t* sqmap, int32_t* order, int32_t* group_q,
+                            int32_t* group_s, int32_t* group_begin, int32_t* group_end,
+                            int32_t group_cap, int32_t* counts);
+
+/*
  * Plan: validates the description, builds the q-partition tables
  * (Configuration::BuildQMap, src/xpcs/configuration.cpp:244-381) and the
  * schedule, and allocates the device tables on the current HIP device.

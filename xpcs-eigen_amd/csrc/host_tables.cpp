@@ -92,7 +92,7 @@ namespace xpcs {
 // groups are erased; their pixels leave the mapping (the reference pushes them
 // into a discarded copy, :327-328).  Compact pixel ids follow the mapping
 // order; dropped pixels follow in raster order.
-static xpcs_status build_qmap(const int32_t* dq, const int32_t* sq, HostTables& h)
+xpcs_status build_qmap(const int32_t* dq, const int32_t* sq, HostTables& h)
 {
     const int64_t npd = (int64_t)h.W * h.H;
     std::map<int, std::map<int, std::vector<int32_t>>> mapping;
@@ -239,3 +239,31 @@ xpcs_status build_tables(const xpcs_mt_desc& d, HostTables& h)
 }
 
 }  // namespace xpcs
+
+extern "C" xpcs_status xpcs_build_qmap(int32_t width, int32_t height, const int32_t* dqmap,
+                                       const int32_t* sqmap, int32_t* order, int32_t* group_q,
+                                       int32_t* group_s, int32_t* group_begin, int32_t* group_end,
+                                       int32_t group_cap, int32_t* counts)
+{
+    if (width <= 0 || height <= 0 || !dqmap || !sqmap || !counts)
+        return xpcs::fail(XPCS_ERR_INVALID_ARG, "build_qmap: bad arguments");
+    xpcs::HostTables h;
+    h.W = width;
+    h.H = height;
+    xpcs_status st = xpcs::build_qmap(dqmap, sqmap, h);
+    if (st) return st;
+    counts[0] = h.P;
+    counts[1] = h.n_mapped;
+    counts[2] = h.ngroups;
+    counts[3] = h.nq;
+    counts[4] = h.ns;
+    if (order) std::copy(h.order.begin(), h.order.end(), order);
+    const int ng = std::min(h.ngroups, group_cap);
+    for (int g = 0; g < ng; g++) {
+        if (group_q) group_q[g] = h.group_q[(size_t)g];
+        if (group_s) group_s[g] = h.group_s[(size_t)g];
+        if (group_begin) group_begin[g] = h.group_begin[(size_t)g];
+        if (group_end) group_end[g] = h.group_end[(size_t)g];
+    }
+    return XPCS_OK;
+}

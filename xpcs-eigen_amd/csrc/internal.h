@@ -173,6 +173,7 @@ namespace xpcs {
 
 // host_tables.cpp
 xpcs_status build_tables(const xpcs_mt_desc& d, HostTables& h);
+xpcs_status build_qmap(const int32_t* dq, const int32_t* sq, HostTables& h);
 void set_error(const char* fmt, ...);
 xpcs_status fail(xpcs_status st, const char* fmt, ...);
 

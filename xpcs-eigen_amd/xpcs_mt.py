@@ -31,7 +31,7 @@ EXPORTS = [
     "xpcs_mt_schedule", "xpcs_mt_pixel_order", "xpcs_mt_ingest_imm", "xpcs_mt_ingest_imm_device",
     "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_execute", "xpcs_mt_synchronize",
     "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
-    "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset",
+    "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
             "xpcs_mt_read_side": ([vp, i32, vp, vp, vp, vp], i32),
             "xpcs_mt_kernel_times": ([vp, vp, C.c_size_t, vp, vp, i32, vp], i32),
             "xpcs_mt_kernel_times_reset": ([vp], i32),
+            "xpcs_build_qmap": ([i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -137,6 +138,22 @@ def imm_index(buf):
     check(lib().xpcs_imm_index(_p(b), b.size, _p(off), _p(dl), _p(el), _p(ct), n.value, C.byref(n),
                                C.byref(comp)))
     return dict(payload_off=off, dlen=dl, elapsed=el, corecotick=ct, compressed=bool(comp.value))
+
+
+def build_qmap(dqmap, sqmap):
+    """Configuration::BuildQMap tables (host only)."""
+    dq = np.ascontiguousarray(dqmap, np.int32)
+    sq = np.ascontiguousarray(sqmap, np.int32)
+    h, w = dq.shape
+    counts = np.zeros(5, np.int32)
+    check(lib().xpcs_build_qmap(w, h, _p(dq), _p(sq), None, None, None, None, None, 0, _p(counts)))
+    P, nm, ng = int(counts[0]), int(counts[1]), int(counts[2])
+    order = np.zeros(max(P, 1), np.int32)
+    g = [np.zeros(max(ng, 1), np.int32) for _ in range(4)]
+    check(lib().xpcs_build_qmap(w, h, _p(dq), _p(sq), _p(order), _p(g[0]), _p(g[1]), _p(g[2]),
+                                _p(g[3]), ng, _p(counts)))
+    return dict(order=order[:P], n_mapped=nm, ngroups=ng, group_q=g[0][:ng], group_s=g[1][:ng],
+                group_begin=g[2][:ng], group_end=g[3][:ng], nq=int(counts[3]), ns=int(counts[4]))
 
 
 class Plan:

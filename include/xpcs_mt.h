@@ -50,6 +50,8 @@ typedef int32_t xpcs_status;
 
 /* desc.flags */
 #define XPCS_MT_TIMING 0x1u      /* record HIP events around every kernel */
+#define XPCS_MT_SIDE_STATS 0x2u  /* also sum counts per (frame, static bin) for frameSum /
+                                    partition-mean-partial (xpcs_mt_read_side) */
 
 typedef struct xpcs_mt_plan xpcs_mt_plan;
 
@@ -192,6 +194,13 @@ xpcs_status xpcs_mt_read_ref_f32(xpcs_mt_plan* plan, float* G2, float* IP, float
 
 /* g2 and stderr, [T][Q] float64 (norm-0-g2 / norm-0-stderr before the f32 cast). */
 xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* plan, double* g2, double* stderr_);
+
+/*
+ * Dark image of a dense ingest (DarkImage, src/xpcs/data_structure/dark_image.cpp:81-106):
+ * per-pixel Welford mean and standard deviation, [W*H] float64 each (zeros
+ * without dark frames); the --darkout DarkAvg/DarkStd datasets.
+ */
+xpcs_status xpcs_mt_read_dark(xpcs_mt_plan* plan, double* dark_avg, double* dark_std);
 
 /* Device pointers of the [T][Q] results (valid until the next execute/destroy). */
 xpcs_status xpcs_mt_device_results(xpcs_mt_plan* plan, const double** g2, const double** stderr_);

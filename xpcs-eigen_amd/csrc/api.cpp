@@ -158,6 +158,13 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     TRY(dalloc(&p->d_long_list, (size_t)h.P));
     TRY(dalloc(&p->d_mid_list, (size_t)h.P));
     TRY(dalloc(&p->d_counters, 4));
+    if (h.flags & XPCS_MT_SIDE_STATS) {
+        if (h.ns > 4096 || (int64_t)h.F * h.ns > (int64_t(1) << 26)) {
+            xpcs_mt_plan_destroy(p);
+            return fail(XPCS_ERR_UNSUPPORTED, "plan: side statistics need S <= 4096 and F*S <= 2^26");
+        }
+        TRY(dalloc(&p->d_fs, (size_t)h.F * h.ns));
+    }
     // dense-row scratch: one row of F 64-bit values per resident workgroup
     {
         size_t per = (size_t)h.F * sizeof(uint64_t);
@@ -187,7 +194,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
     dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
     dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total);
-    dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_frame_sum);
+    dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : p->event_pool) (void)hipEventDestroy(e);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
@@ -539,6 +546,10 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             p->bsum_cap = nbs;
         }
         launch_transpose(p, s);
+        if (p->d_fs) {
+            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
+            launch_side(p, s);
+        }
     }
     if (p->mode == 2) {
         const size_t need = (size_t)h.P * (size_t)h.F;
@@ -548,6 +559,10 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             p->rows_cap = need;
         }
         launch_dense_transpose(p, s);
+        if (p->d_fs) {
+            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
+            launch_side(p, s);
+        }
         launch_mt_long(p, s, true);
     } else {
         launch_mt_sparse(p, s);
@@ -646,9 +661,36 @@ extern "C" xpcs_status xpcs_mt_read_side(xpcs_mt_plan* p, int32_t window, float*
     (void)window;
     xpcs_status st = sync_and_check(p);
     if (st) return st;
-    if (frame_sum || partition_partial)
-        return fail(XPCS_ERR_UNSUPPORTED, "read_side: frame sums and partial partition means are not built yet");
     const HostTables& h = p->h;
+    if ((frame_sum || partition_partial) && !p->d_fs)
+        return fail(XPCS_ERR_STATE, "read_side: frame sums need a plan created with XPCS_MT_SIDE_STATS");
+    if (partition_partial && window <= 0)
+        return fail(XPCS_ERR_INVALID_ARG, "read_side: static window must be >= 1");
+    const double vscale = 1.0 / ((double)h.avg * (double)p->nscale);
+    if (frame_sum || partition_partial) {
+        std::vector<double> fs((size_t)h.F * h.ns);
+        HIP_TRY(hipMemcpy(fs.data(), p->d_fs, fs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (frame_sum) {
+            // [2][F]: frame number (1-based), sum over masked pixels / (W*H)
+            for (int f = 0; f < h.F; f++) {
+                double t = 0.0;
+                for (int b = 0; b < h.ns; b++) t += fs[(size_t)f * h.ns + b];
+                frame_sum[f] = (float)(f + 1);
+                frame_sum[h.F + f] = (float)(t * vscale / ((double)h.W * h.H));
+            }
+        }
+        if (partition_partial) {
+            // [floor(F/window)][S] means over each window of `window` frames
+            const int nw = h.F / window;
+            for (int w = 0; w < nw; w++)
+                for (int b = 0; b < h.ns; b++) {
+                    double t = 0.0;
+                    for (int f = w * window; f < (w + 1) * window; f++) t += fs[(size_t)f * h.ns + b];
+                    partition_partial[(size_t)w * h.ns + b] =
+                        (float)(t * vscale / ((double)h.pixels_per_bin[(size_t)b] * window));
+                }
+        }
+    }
     std::vector<uint64_t> tot((size_t)h.P);
     HIP_TRY(hipMemcpy(tot.data(), p->d_pix_total, tot.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
     const double inv = 1.0 / ((double)h.avg * (double)p->nscale);
@@ -705,5 +747,16 @@ extern "C" xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* p)
     if (p->executed) (void)hipStreamSynchronize(p->stream_last ? p->stream_last : p->stream);
     collect_timers(p);
     p->times.clear();
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_read_dark(xpcs_mt_plan* p, double* dark_avg, double* dark_std)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_dark: NULL plan");
+    const size_t n = (size_t)p->h.W * p->h.H;
+    for (size_t i = 0; i < n; i++) {
+        if (dark_avg) dark_avg[i] = p->dark_avg.size() == n ? p->dark_avg[i] : 0.0;
+        if (dark_std) dark_std[i] = p->dark_std.size() == n ? p->dark_std[i] : 0.0;
+    }
     return XPCS_OK;
 }

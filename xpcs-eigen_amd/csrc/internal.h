@@ -160,7 +160,7 @@ struct xpcs_mt_plan {
     double* d_g2 = nullptr;              // [T][Q]
     double* d_se = nullptr;              // [T][Q]
     // side statistics
-    uint64_t* d_frame_sum = nullptr;     // [F]
+    double* d_fs = nullptr;              // [F][S] side statistics (XPCS_MT_SIDE_STATS)
     bool executed = false;
 
     // timing
@@ -183,6 +183,7 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
+void launch_side(xpcs_mt_plan* p, hipStream_t s);
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s);
 void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF);
 

@@ -319,6 +319,80 @@ k_dense_transpose(const uint8_t* __restrict__ imm, const int64_t* __restrict__ s
     }
 }
 
+// ---- side statistics: per (output frame, static bin) sums ---------------------------
+// fs[f][s-1] = sum over masked pixels with sqmap == s of count * flatfield
+// (SparseFilter/DenseFilter accumulate partitions_mean_ / partial means and the
+// frame sums from the same values, src/xpcs/filter/sparse_filter.cpp:168-191).
+constexpr int kSideMaxBins = 4096;
+
+__global__ void __launch_bounds__(256)
+k_side_sparse(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
+              const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
+              const int32_t* __restrict__ pix2c, uint32_t npix_det, const int32_t* __restrict__ sbin_of_c,
+              const double* __restrict__ weight, int32_t S, double* __restrict__ fs)
+{
+    __shared__ double s_acc[kSideMaxBins];
+    for (int64_t k = blockIdx.x; k < nsel; k += gridDim.x) {
+        for (int b = threadIdx.x; b < S; b += blockDim.x) s_acc[b] = 0.0;
+        __syncthreads();
+        const uint8_t* base = imm + sel_off[k];
+        const uint32_t dlen = sel_dlen[k];
+        const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
+        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
+            int32_t c;
+            uint32_t v;
+            if (!tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c, v)) continue;
+            atomicAdd(&s_acc[sbin_of_c[c] - 1], (double)v * (weight ? weight[c] : 1.0));
+        }
+        __syncthreads();
+        double* row = fs + (int64_t)sel_frame[k] * S;
+        for (int b = threadIdx.x; b < S; b += blockDim.x)
+            if (s_acc[b] != 0.0) atomicAdd(&row[b], s_acc[b]);
+        __syncthreads();
+    }
+}
+
+// Dense rows [P][F]: thread = frame, loop over a chunk of compact pixels (which
+// are grouped by static bin), flushing a running sum whenever the bin changes.
+__global__ void __launch_bounds__(256)
+k_side_dense(const uint32_t* __restrict__ rows, int32_t P, int32_t F, const int32_t* __restrict__ sbin_of_c,
+             const double* __restrict__ weight, int32_t S, int32_t chunk, double* __restrict__ fs)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(P, c0 + chunk);
+    if (f >= F) return;
+    double acc = 0.0;
+    int cur = c0 < c1 ? sbin_of_c[c0] : 0;
+    for (int c = c0; c < c1; c++) {
+        const int sb = sbin_of_c[c];
+        if (sb != cur) {
+            if (acc != 0.0) atomicAdd(&fs[(int64_t)f * S + cur - 1], acc);
+            acc = 0.0;
+            cur = sb;
+        }
+        acc += (double)rows[(int64_t)c * F + f] * (weight ? weight[c] : 1.0);
+    }
+    if (acc != 0.0) atomicAdd(&fs[(int64_t)f * S + cur - 1], acc);
+}
+
+void launch_side(xpcs_mt_plan* p, hipStream_t s)
+{
+    const HostTables& h = p->h;
+    timer_begin(p, s, "k_side");
+    if (p->mode == 1) {
+        k_side_sparse<<<p->nsel < 65536 ? (int)p->nsel : 65536, 256, 0, s>>>(
+            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, p->d_pix2c,
+            (uint32_t)((int64_t)h.W * h.H), p->d_sbin_of_c, p->d_weight, h.ns, p->d_fs);
+    } else if (p->mode == 2) {
+        const int chunk = 256;
+        dim3 grid((unsigned)((h.F + 255) / 256), (unsigned)((h.P + chunk - 1) / chunk));
+        k_side_dense<<<grid, 256, 0, s>>>(p->d_rows, h.P, h.F, p->d_sbin_of_c, p->d_weight, h.ns, chunk,
+                                          p->d_fs);
+    }
+    timer_end(p, s);
+}
+
 // ---- launchers ------------------------------------------------------------------
 static int grid_for(int64_t n, int64_t cap)
 {

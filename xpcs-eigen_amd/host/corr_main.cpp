@@ -1,0 +1,318 @@
+// `corr` -- drop-in replacement of the reference's multi-tau command line
+// (src/xpcs/main.cpp:86-479) running the correlator on the GPU through the C ABI.
+//
+//   corr <config.hdf5> [data.imm] [--imm=PATH] [--entry=/xpcs] [--exchange=GROUP]
+//        [--inpath=P --outpath=Q] [--g2out] [--darkout] [--quiet]
+//
+// Flags use the gflags syntax of the reference (--name=value, --name value,
+// --bool / --nobool).  The IMM path comes from --imm, else the optional second
+// positional argument (as README.md documents), else <entry>/input_file_local,
+// with the --inpath prefix replaced by --outpath.  Results are written into the
+// configuration file under <entry>/output_data (or --exchange) with the
+// reference's dataset names, shapes and dtypes.
+//
+// Exit status: 0 ok, 1 usage or configuration error, 2 option outside this
+// build (two-time analysis, UFXC/Rigaku/HDF5 readers, --frameout,
+// normalize_by_framesum), 3 unreadable or malformed IMM, 4 GPU error,
+// 5 cannot write results.  (The reference exits 0 on most failures.)
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdlib>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "config.h"
+#include "h5lite.h"
+#include "xpcs_mt.h"
+
+namespace {
+
+bool g_quiet = false;
+
+void info(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void info(const char* fmt, ...)
+{
+    if (g_quiet) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::printf("[info] ");
+    std::vprintf(fmt, ap);
+    std::printf("\n");
+    std::fflush(stdout);
+    va_end(ap);
+}
+
+// RAII scope timer printing "<name> took <t> ms" (the reference's Benchmark,
+// src/xpcs/benchmark.h:56-86).
+struct Scope {
+    std::string name;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit Scope(std::string n) : name(std::move(n)) {}
+    ~Scope()
+    {
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        info("%s took %.3f ms", name.c_str(), ms);
+    }
+};
+
+int die(int code, const std::string& msg)
+{
+    std::fprintf(stderr, "corr: %s\n", msg.c_str());
+    return code;
+}
+
+struct Flags {
+    std::map<std::string, std::string> kv;
+    std::vector<std::string> pos;
+    std::string bad;
+};
+
+const char* kBoolFlags[] = {"g2out", "darkout", "ufxc", "rigaku", "hdf5", "frame_threading", "transposed", "quiet"};
+const char* kValueFlags[] = {"imm", "inpath", "outpath", "exchange", "entry", "frameout"};
+
+bool is_in(const char* const* set, size_t n, const std::string& s)
+{
+    for (size_t i = 0; i < n; i++)
+        if (s == set[i]) return true;
+    return false;
+}
+
+Flags parse(int argc, char** argv)
+{
+    Flags f;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a.size() < 2 || a[0] != '-') {
+            f.pos.push_back(a);
+            continue;
+        }
+        std::string body = a.substr(a[1] == '-' ? 2 : 1);
+        std::string name = body, val;
+        const size_t eq = body.find('=');
+        bool has_val = eq != std::string::npos;
+        if (has_val) {
+            name = body.substr(0, eq);
+            val = body.substr(eq + 1);
+        }
+        if (is_in(kBoolFlags, sizeof(kBoolFlags) / sizeof(*kBoolFlags), name)) {
+            f.kv[name] = has_val ? val : "true";
+        } else if (name.rfind("no", 0) == 0 && is_in(kBoolFlags, sizeof(kBoolFlags) / sizeof(*kBoolFlags), name.substr(2))) {
+            f.kv[name.substr(2)] = "false";
+        } else if (is_in(kValueFlags, sizeof(kValueFlags) / sizeof(*kValueFlags), name)) {
+            if (!has_val) {
+                if (i + 1 >= argc) {
+                    f.bad = "flag --" + name + " needs a value";
+                    return f;
+                }
+                val = argv[++i];
+            }
+            f.kv[name] = val;
+        } else {
+            f.bad = "unknown flag " + a;
+            return f;
+        }
+    }
+    return f;
+}
+
+bool flag_true(const Flags& f, const char* name)
+{
+    auto it = f.kv.find(name);
+    return it != f.kv.end() && (it->second == "true" || it->second == "1" || it->second == "yes");
+}
+
+std::string flag_str(const Flags& f, const char* name)
+{
+    auto it = f.kv.find(name);
+    return it == f.kv.end() ? std::string() : it->second;
+}
+
+struct Mapped {
+    void* p = nullptr;
+    size_t n = 0;
+    ~Mapped()
+    {
+        if (p && p != MAP_FAILED) munmap(p, n);
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) return die(1, "usage: corr <config.hdf5> [data.imm] [--imm=PATH] [--entry=/xpcs] [--exchange=GROUP] [--g2out] [--darkout]");
+    Flags fl = parse(argc, argv);
+    if (!fl.bad.empty()) return die(1, fl.bad);
+    if (fl.pos.empty()) return die(1, "missing configuration file argument");
+    g_quiet = flag_true(fl, "quiet");
+    Scope total("Total");
+    if (flag_true(fl, "ufxc") || flag_true(fl, "rigaku") || flag_true(fl, "hdf5"))
+        return die(2, "only IMM input is built in this tool (--ufxc/--rigaku/--hdf5 readers are not)");
+    if (!flag_str(fl, "frameout").empty() && std::atoi(flag_str(fl, "frameout").c_str()) > 0)
+        return die(2, "--frameout (debug frame dump) is not built in this tool");
+
+    std::string entry = flag_str(fl, "entry").empty() ? "/xpcs" : flag_str(fl, "entry");
+    info("H5 metadata path %s", entry.c_str());
+    std::string why;
+    if (!xpcs::h5::available(&why)) return die(1, "HDF5 unavailable: " + why);
+    xpcs::JobConfig cfg;
+    {
+        Scope s("Configuration Total");
+        if (!xpcs::load_config(fl.pos[0], entry, &cfg, &why)) return die(1, why);
+    }
+    if (cfg.analysis_type == "Twotime") return die(2, "analysis_type Twotime (two-time correlation) is not built in this tool");
+    if (cfg.normalize_by_framesum)
+        return die(2, "normalize_by_framesum rescales counts by frame sums; the exact integer path does not build it");
+    if (!flag_str(fl, "imm").empty()) cfg.imm_path = flag_str(fl, "imm");
+    else if (fl.pos.size() > 1) cfg.imm_path = fl.pos[1];
+    const std::string inp = flag_str(fl, "inpath"), outp = flag_str(fl, "outpath");
+    if (!inp.empty() && !outp.empty()) {
+        const size_t pos = cfg.imm_path.find(inp);
+        if (pos != std::string::npos) cfg.imm_path.replace(pos, inp.size(), outp);
+    }
+    if (!flag_str(fl, "exchange").empty()) cfg.output_group = flag_str(fl, "exchange");
+    info("Processing IMM file at path %s..", cfg.imm_path.c_str());
+
+    // map the IMM file
+    Mapped imm;
+    {
+        int fd = open(cfg.imm_path.c_str(), O_RDONLY);
+        if (fd < 0) return die(3, "cannot open IMM file " + cfg.imm_path);
+        struct stat st;
+        fstat(fd, &st);
+        imm.n = (size_t)st.st_size;
+        imm.p = imm.n ? mmap(nullptr, imm.n, PROT_READ, MAP_PRIVATE, fd, 0) : nullptr;
+        close(fd);
+        if (!imm.p || imm.p == MAP_FAILED) return die(3, "cannot map IMM file " + cfg.imm_path);
+        info("File size %.3f MB", imm.n / 1048576.0);
+    }
+
+    const int32_t F = cfg.frames();
+    info("Data frames=%d stride=%lld average=%lld", F, (long long)cfg.stride, (long long)cfg.average);
+    xpcs_mt_desc d{};
+    d.width = cfg.width;
+    d.height = cfg.height;
+    d.frames = F;
+    d.delays_per_level = cfg.delays_per_level;
+    d.stride = (int32_t)cfg.stride;
+    d.average = (int32_t)cfg.average;
+    d.frame_from = cfg.frame_from();
+    d.dark_frames = cfg.dark_frames;
+    d.lld = cfg.lld;
+    d.sigma = cfg.sigma;
+    d.dqmap = cfg.dqmap.data();
+    d.sqmap = cfg.sqmap.data();
+    d.flatfield = cfg.flatfield_enabled ? cfg.flatfield.data() : nullptr;
+    d.flags = XPCS_MT_SIDE_STATS;
+    xpcs_mt_plan* plan = nullptr;
+    int st = xpcs_mt_plan_create(&d, &plan);
+    if (st == XPCS_ERR_UNSUPPORTED) return die(2, xpcs_last_error());
+    if (st) return die(st == XPCS_ERR_NO_DEVICE || st == XPCS_ERR_HIP || st == XPCS_ERR_NOMEM ? 4 : 1, xpcs_last_error());
+    struct PlanGuard {
+        xpcs_mt_plan* p;
+        ~PlanGuard() { xpcs_mt_plan_destroy(p); }
+    } guard{plan};
+    xpcs_mt_info pi{};
+    xpcs_mt_plan_info(plan, &pi);
+    const int T = pi.taus, Q = pi.dynamic_bins, S = pi.static_bins;
+
+    {
+        Scope s("Loading data");
+        st = xpcs_mt_ingest_imm(plan, imm.p, imm.n);
+        if (st) return die(st == XPCS_ERR_UNSUPPORTED ? 2 : 3, xpcs_last_error());
+    }
+    {
+        Scope s("Computing G2 MultiTau + Normalizing Data");
+        st = xpcs_mt_execute(plan, nullptr);
+        if (!st) st = xpcs_mt_synchronize(plan);
+        if (st) return die(st == XPCS_ERR_RANGE ? 3 : 4, xpcs_last_error());
+    }
+    std::vector<double> g2((size_t)T * Q), se((size_t)T * Q);
+    std::vector<float> pix((size_t)cfg.width * cfg.height), fsum((size_t)2 * F), pmean((size_t)S);
+    const int nwin = cfg.static_window > 0 ? F / cfg.static_window : 0;
+    std::vector<float> ppart((size_t)std::max(nwin, 0) * S);
+    st = xpcs_mt_read_g2(plan, g2.data(), se.data());
+    if (!st)
+        st = xpcs_mt_read_side(plan, cfg.static_window, pix.data(), fsum.data(), pmean.data(),
+                               nwin > 0 ? ppart.data() : nullptr);
+    if (st) return die(4, xpcs_last_error());
+    std::vector<int32_t> lev((size_t)T), tau((size_t)T);
+    xpcs_mt_schedule(plan, lev.data(), tau.data());
+
+    // timestamps of the data frames (row 0: 1-based index, row 1: header value)
+    const int32_t FR = cfg.real_frames();
+    std::vector<double> clk((size_t)2 * FR, 0.0), tck((size_t)2 * FR, 0.0);
+    {
+        int64_t n = 0;
+        int32_t comp = 0;
+        xpcs_imm_index(imm.p, imm.n, nullptr, nullptr, nullptr, nullptr, 0, &n, &comp);
+        std::vector<double> el((size_t)n);
+        std::vector<int32_t> ct((size_t)n);
+        xpcs_imm_index(imm.p, imm.n, nullptr, nullptr, el.data(), ct.data(), n, &n, &comp);
+        int64_t first = (!comp && cfg.dark_frames > 0) ? cfg.dark_frames : 0;
+        if (cfg.frame_from() > first) first = cfg.frame_from();
+        for (int32_t i = 0; i < FR; i++) {
+            clk[(size_t)i] = tck[(size_t)i] = (double)(i + 1);
+            if (first + i < n) {
+                clk[(size_t)FR + i] = el[(size_t)(first + i)];
+                tck[(size_t)FR + i] = (double)ct[(size_t)(first + i)];
+            }
+        }
+    }
+
+    {
+        Scope s("Writing results");
+        xpcs::h5::File out;
+        if (!out.open(cfg.path, false, &why)) return die(5, why);
+        std::string G = cfg.output_group;
+        if (G.empty() || G[0] != '/') G = "/" + G;
+        auto P = [&](const char* n) { return G + "/" + n; };
+        using V = std::vector<xpcs::h5::hsize_t>;
+        std::vector<float> g2f(g2.size()), sef(se.size()), tauf((size_t)T);
+        for (size_t i = 0; i < g2.size(); i++) {
+            g2f[i] = (float)g2[i];
+            sef[i] = (float)se[i];
+        }
+        for (int i = 0; i < T; i++) tauf[(size_t)i] = (float)tau[(size_t)i];
+        bool ok = out.write_f32(P("pixelSum"), pix.data(), V{(unsigned long long)cfg.height, (unsigned long long)cfg.width}) &&
+                  out.write_f32(P("frameSum"), fsum.data(), V{2, (unsigned long long)F}) &&
+                  out.write_f32(P("partition-mean-total"), pmean.data(), V{1, (unsigned long long)S}) &&
+                  out.write_f32(P("partition-mean-partial"), ppart.data(), V{(unsigned long long)std::max(nwin, 0), (unsigned long long)S}) &&
+                  out.write_f32(P("partition_norm_factor"), &cfg.norm_factor, V{1, 1}) &&
+                  out.write_f64(P("timestamp_clock"), clk.data(), V{2, (unsigned long long)FR}) &&
+                  out.write_f64(P("timestamp_tick"), tck.data(), V{2, (unsigned long long)FR}) &&
+                  out.write_f32(P("tau"), tauf.data(), V{1, (unsigned long long)T}) &&
+                  out.write_f32(P("norm-0-g2"), g2f.data(), V{(unsigned long long)T, (unsigned long long)Q}) &&
+                  out.write_f32(P("norm-0-stderr"), sef.data(), V{(unsigned long long)T, (unsigned long long)Q});
+        if (ok && flag_true(fl, "g2out")) {
+            const size_t n = (size_t)cfg.width * cfg.height * T;
+            std::vector<float> a(n), b(n), c(n);
+            st = xpcs_mt_read_ref_f32(plan, a.data(), b.data(), c.data());
+            if (st) return die(4, xpcs_last_error());
+            V dd{(unsigned long long)T, (unsigned long long)cfg.width * cfg.height};
+            ok = out.write_f32(P("G2"), a.data(), dd) && out.write_f32(P("IP"), b.data(), dd) &&
+                 out.write_f32(P("IF"), c.data(), dd);
+        }
+        if (ok && flag_true(fl, "darkout") && cfg.dark_frames > 0) {
+            std::vector<double> da((size_t)cfg.width * cfg.height), ds((size_t)cfg.width * cfg.height);
+            st = xpcs_mt_read_dark(plan, da.data(), ds.data());
+            if (!st) {
+                V dd{(unsigned long long)cfg.height, (unsigned long long)cfg.width};
+                ok = out.write_f64(P("DarkAvg"), da.data(), dd) && out.write_f64(P("DarkStd"), ds.data(), dd);
+            }
+        }
+        if (!ok) return die(5, "cannot write results into " + cfg.path + " group " + G);
+    }
+    info("pixels=%d taus=%d q=%d events=%lld", pi.pixels, T, Q, (long long)pi.events);
+    return 0;
+}

@@ -32,6 +32,7 @@ EXPORTS = [
     "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_execute", "xpcs_mt_synchronize",
     "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
     "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
+    "xpcs_mt_read_dark",
 ]
 
 
@@ -93,6 +94,7 @@ def lib():
             "xpcs_mt_kernel_times": ([vp, vp, C.c_size_t, vp, vp, i32, vp], i32),
             "xpcs_mt_kernel_times_reset": ([vp], i32),
             "xpcs_build_qmap": ([i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp], i32),
+            "xpcs_mt_read_dark": ([vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -31,6 +31,8 @@ WORKLOADS = {
     "sparse-1mpx": dict(W=1024, H=1024, F=10000, dpl=8, mu=0.005, tau=(2.0, 2000.0), S=36, Q=12,
                         seed=1002),
     "plain-256": dict(W=256, H=256, F=1000, dpl=4, mu=0.012, tau=(20.0, 20.0), S=18, Q=6, seed=1001),
+    "dense-512": dict(W=512, H=512, F=20000, dpl=4, mu=5.0, tau=(50.0, 50.0), S=27, Q=9, seed=1003,
+                      dense=True, darks=2),
 }
 
 
@@ -45,7 +47,7 @@ def transpose_hist_entries(P, nsel):
     return nb * nblk
 
 
-def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel):
+def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel, F=0, dense_raw=0):
     """Bytes each kernel must move at minimum (inputs read once, outputs written once)."""
     nh = transpose_hist_entries(P, nsel)
     return {
@@ -55,6 +57,8 @@ def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel):
         "k_tr_rows": 8 * E0 + 8 * E0 + 8 * (P + 1),       # bucketed events in, rows + offsets out
         "k_mt_reg": 8 * E0 + 8 * (P + 1) + 3 * 8 * P * T,  # rows + offsets in, int64 sums out
         "k_norm": 3 * 8 * n_mapped * T,                   # int64 sums in
+        "k_dense_transpose": 2 * dense_raw + 4 * P * F,   # u16 frames in, u32 pixel rows out
+        "k_mt_long_dense": 4 * P * F + 3 * 8 * P * T,     # u32 rows in, int64 sums out
     }.get(name, 0)
 
 
@@ -64,7 +68,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sparse-1mpx", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-frames", type=int, default=1000,
+    ap.add_argument("--cpu-frames", type=int, default=2000,
                     help="frames of the CPU baseline sample (full detector)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -84,13 +88,19 @@ def main():
     tau_c = synth.ring_tau(dq, lo, hi) if hi != lo else lo
     dev = torch.device("cuda:0")
     t0 = time.time()
-    imm, payload_off, dlen = synth.imm_device_sparse(W, H, F, wl["mu"], tau_c, wl["seed"], device=dev)
+    dense = wl.get("dense", False)
+    if dense:
+        imm, payload_off, dlen = synth.imm_device_dense(W, H, F, wl["mu"], tau_c, wl["seed"],
+                                                        darks=wl["darks"], device=dev)
+    else:
+        imm, payload_off, dlen = synth.imm_device_sparse(W, H, F, wl["mu"], tau_c, wl["seed"], device=dev)
     torch.cuda.synchronize()
     nbytes = imm.numel()
     gen_s = time.time() - t0
 
-    plan = xpcs_mt.Plan(W, H, F, dpl, dq, sq, timing=True)
-    plan.ingest_imm_device(imm.data_ptr(), nbytes, payload_off, dlen, compressed=True)
+    darks = wl.get("darks", 0)
+    plan = xpcs_mt.Plan(W, H, F, dpl, dq, sq, dark_frames=darks, timing=True)
+    plan.ingest_imm_device(imm.data_ptr(), nbytes, payload_off, dlen, compressed=not dense)
     for _ in range(args.warmup):
         plan.execute()
         plan.synchronize()
@@ -113,7 +123,7 @@ def main():
     kern = {}
     for k, (ms, n) in ktimes.items():
         avg_ms = ms / max(n, 1)
-        b = algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen))
+        b = algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen), F, int(dlen[darks:].sum()) * wl.get('avg', 1) if dense else 0)
         kern[k] = dict(avg_ms=avg_ms, launches=n, alg_bytes=b,
                        gbs=(b / (avg_ms * 1e-3) / 1e9) if b and avg_ms > 0 else None)
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
@@ -131,7 +141,7 @@ def main():
     # PCIe-inclusive end-to-end (host IMM bytes -> g2 on host), one pass
     host_imm = imm.cpu().numpy()
     t2 = time.perf_counter()
-    plan2 = xpcs_mt.Plan(W, H, F, dpl, dq, sq)
+    plan2 = xpcs_mt.Plan(W, H, F, dpl, dq, sq, dark_frames=darks)
     t3 = time.perf_counter()
     plan2.ingest_imm(host_imm)
     plan2.execute()
@@ -141,7 +151,7 @@ def main():
     e2e = dict(frames_per_s=F / (t4 - t3), seconds=t4 - t3, plan_seconds=t3 - t2)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not dense:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle
         fs = min(args.cpu_frames, F)

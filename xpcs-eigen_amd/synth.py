@@ -189,3 +189,32 @@ def imm_device_sparse(width, height, frames, mu, tau_c, seed, device="cuda"):
             dls.append(n)
             pos += HEADER_BYTES + 6 * n
     return torch.cat(chunks), np.array(offs, np.int64), np.array(dls, np.uint32)
+
+
+def imm_device_dense(width, height, frames, mu, tau_c, seed, darks=2, dark_mean=2.0, dark_sd=0.5,
+                     device="cuda"):
+    """Uncompressed IMM image in device memory: `darks` identical integer dark
+    frames D = clip(round(N(dark_mean, dark_sd)), 0), then `frames` frames of
+    OU-speckle counts + D.  Returns (imm uint8 tensor, payload_off, dlen)."""
+    import torch
+    npix = width * height
+    rec = HEADER_BYTES + 2 * npix
+    total = darks + frames
+    imm = torch.empty(total * rec, dtype=torch.uint8, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 7)
+    d = torch.clamp(torch.round(torch.randn(npix, generator=g, device=device) * dark_sd + dark_mean), min=0)
+    d16 = d.to(torch.int32)
+    view = imm.view(total, rec)
+    for t in range(darks):
+        view[t, :HEADER_BYTES] = torch.from_numpy(_header(0, height, width, npix, t)).to(device)
+        view[t, HEADER_BYTES:] = d16.to(torch.int16).view(torch.uint8)
+    t = darks
+    for t0, c in ou_events_torch(width, height, frames, mu, tau_c, seed, device):
+        for k in range(c.shape[0]):
+            row = view[t]
+            row[:HEADER_BYTES] = torch.from_numpy(_header(0, height, width, npix, t)).to(device)
+            row[HEADER_BYTES:] = (c[k] + d16).clamp_(max=65535).to(torch.int32).to(torch.int16).view(torch.uint8)
+            t += 1
+    offs = np.arange(total, dtype=np.int64) * rec + HEADER_BYTES
+    return imm, offs, np.full(total, npix, np.uint32)

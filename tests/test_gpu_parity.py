@@ -115,6 +115,33 @@ def test_dense_with_darks(xmt, orc):
     assert plan.info().long_pixels == plan.P
 
 
+def dense_case(W, H, F_raw, dpl, mu, tau_c, seed, darks=2, S=9, Q=3):
+    dq, sq = synth.annular_qmaps(W, H, S, Q)
+    sig = synth.ou_counts(W, H, F_raw, mu, tau_c, seed=seed)
+    dark = synth.dark_frames(W, H, darks, seed=seed + 1000) if darks else np.zeros((0, W * H), np.uint16)
+    base = dark[0] if darks else 0
+    frames = np.concatenate([dark, sig + base], axis=0)
+    return dict(W=W, H=H, F=F_raw, dpl=dpl, dq=dq, sq=sq, imm=synth.imm_bytes_dense(frames, W, H))
+
+
+def test_dense_dpl8_two_passes(xmt, orc):
+    """Streaming dense kernel, dpl=8, 9 levels (two passes of the level cascade)."""
+    c = dense_case(16, 16, 3000, 8, 3.0, 40.0, seed=41)
+    check_against_oracle(xmt, orc, c, dark_frames=2)
+
+
+def test_dense_fallback_dpl2(xmt, orc):
+    """dpl outside the streaming kernel's instantiations takes the dense-row kernel."""
+    c = dense_case(16, 16, 500, 2, 4.0, 10.0, seed=43)
+    check_against_oracle(xmt, orc, c, dark_frames=2)
+
+
+def test_dense_no_darks_average(xmt, orc):
+    c = dense_case(12, 20, 900, 4, 1.0, 15.0, seed=47, darks=0)
+    c["F"] = 300
+    check_against_oracle(xmt, orc, c, stride=1, average=3)
+
+
 def test_flatfield(xmt, orc):
     c = cases.sparse_case(W=32, H=32, F=600, dpl=4, mu=0.1, tau_c=10.0, seed=9)
     ff = np.random.default_rng(0).uniform(0.5, 1.5, (32, 32))

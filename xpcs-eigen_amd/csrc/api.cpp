@@ -158,6 +158,21 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     TRY(dalloc(&p->d_long_list, (size_t)h.P));
     TRY(dalloc(&p->d_mid_list, (size_t)h.P));
     TRY(dalloc(&p->d_counters, 4));
+    {
+        // level-0 prefix boundaries of the dense streaming path
+        std::vector<int4> bnd;
+        for (int i = 0; i < h.T; i++) {
+            const TauParam& tp = h.taus[(size_t)i];
+            bnd.push_back(make_int4((tp.lf - tp.tp) << tp.level, 0, i, 0));
+            bnd.push_back(make_int4(tp.tp << tp.level, 1, i, 0));
+            bnd.push_back(make_int4(tp.lf << tp.level, 2, i, 0));
+        }
+        std::stable_sort(bnd.begin(), bnd.end(), [](const int4& a, const int4& b) {
+            return a.x != b.x ? a.x < b.x : a.y < b.y;
+        });
+        p->nbnd = (int32_t)bnd.size();
+        TRY(upload(&p->d_bnd, bnd, s));
+    }
     if (h.flags & XPCS_MT_SIDE_STATS) {
         if (h.ns > 4096 || (int64_t)h.F * h.ns > (int64_t(1) << 26)) {
             xpcs_mt_plan_destroy(p);
@@ -194,7 +209,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
     dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
     dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total);
-    dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs);
+    dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs); dfree(p->d_series_a); dfree(p->d_series_b); dfree(p->d_bnd);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : p->event_pool) (void)hipEventDestroy(e);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
@@ -551,7 +566,23 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             launch_side(p, s);
         }
     }
-    if (p->mode == 2) {
+    const bool stream_ok = dense_stream_supported(h) &&
+                           (double)h.F * 65535.0 * (double)p->nscale * (double)h.avg < 4294967295.0;
+    if (p->mode == 2 && stream_ok) {
+        // thread-per-pixel streaming multi-tau over the raw frames
+        const size_t need = h.levels.size() > 6 ? (size_t)h.P * (size_t)h.levels[6].lf : 1;
+        if (need > p->series_cap) {
+            xpcs_status st = dalloc(&p->d_series_a, need);
+            if (!st) st = dalloc(&p->d_series_b, need);
+            if (st) return st;
+            p->series_cap = need;
+        }
+        if (p->d_fs) {
+            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
+            launch_side_dense_raw(p, s);
+        }
+        launch_mt_dense(p, s);
+    } else if (p->mode == 2) {
         const size_t need = (size_t)h.P * (size_t)h.F;
         if (need > p->rows_cap) {
             xpcs_status st = dalloc(&p->d_rows, need);

@@ -138,9 +138,15 @@ struct xpcs_mt_plan {
     int64_t bsum_cap = 0;
     xpcs::TransposeShape tshape;
     int64_t events = 0;                  // known after execute (host copy)
-    // dense rows
+    // dense rows (fallback path)
     uint32_t* d_rows = nullptr;          // [P][F]
     size_t rows_cap = 0;
+    // dense streaming path: coarsened series between passes, boundary events
+    uint32_t* d_series_a = nullptr;
+    uint32_t* d_series_b = nullptr;
+    size_t series_cap = 0;
+    int4* d_bnd = nullptr;               // (frame boundary, op, tau slot, 0), sorted
+    int32_t nbnd = 0;
 
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]
@@ -184,6 +190,9 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_side(xpcs_mt_plan* p, hipStream_t s);
+bool dense_stream_supported(const HostTables& h);
+void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
+void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s);
 void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF);
 

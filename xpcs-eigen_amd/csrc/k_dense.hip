@@ -124,29 +124,58 @@ k_mt_dense_pass(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel
         th = thr ? thr[c] : 0;
     }
     const uint32_t* in_row = RAW ? nullptr : series_in + c * (int64_t)n_in;
-    for (int t = 0; t < n_in; t++) {
-        uint32_t x;
+    // Frames are fetched kPrefetch at a time so that many independent loads
+    // are in flight per thread; processing order stays frame order.
+    constexpr int kPrefetch = 16;
+    for (int t0 = 0; t0 < n_in; t0 += kPrefetch) {
+        uint32_t xs[kPrefetch];
         if constexpr (RAW) {
-            uint32_t acc = 0;
-            for (int a = 0; a < avg; a++) {
-                const uint16_t* fr = reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)t * avg + a]);
-                const int64_t y = nscale * (int64_t)fr[gid] - ds;
-                if (y > th && y > 0) acc += (uint32_t)y;
-            }
-            x = acc;
-            S += x;
-            // boundary captures after frame t (running sum covers frames <= t)
-            while (bi < nbnd && bnd[bi].x == t + 1) {
-                const int4 b = bnd[bi];
-                if (b.y == 0) IP[c * T + b.z] = (int64_t)S;
-                else if (b.y == 1) IF[c * T + b.z] = (int64_t)S;
-                else IF[c * T + b.z] = (int64_t)S - IF[c * T + b.z];
-                bi++;
+            if (avg == 1) {
+                uint16_t raw[kPrefetch];
+#pragma unroll
+                for (int u = 0; u < kPrefetch; u++) {
+                    const int t = min(t0 + u, n_in - 1);
+                    raw[u] = reinterpret_cast<const uint16_t*>(imm + sel_off[t])[gid];
+                }
+#pragma unroll
+                for (int u = 0; u < kPrefetch; u++) {
+                    const int64_t y = nscale * (int64_t)raw[u] - ds;
+                    xs[u] = (y > th && y > 0) ? (uint32_t)y : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kPrefetch; u++) {
+                    const int t = min(t0 + u, n_in - 1);
+                    uint32_t acc = 0;
+                    for (int a = 0; a < avg; a++) {
+                        const uint16_t* fr = reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)t * avg + a]);
+                        const int64_t y = nscale * (int64_t)fr[gid] - ds;
+                        if (y > th && y > 0) acc += (uint32_t)y;
+                    }
+                    xs[u] = acc;
+                }
             }
         } else {
-            x = in_row[t];
+#pragma unroll
+            for (int u = 0; u < kPrefetch; u++) xs[u] = in_row[min(t0 + u, n_in - 1)];
         }
-        stream_push<DPL, K, RAW, 0>(s, x, cnt, nlag, lo, out_next, lf_out);
+        const int nu = min(kPrefetch, n_in - t0);
+        for (int u = 0; u < nu; u++) {
+            const uint32_t x = xs[u];
+            if constexpr (RAW) {
+                const int t = t0 + u;
+                S += x;
+                // boundary captures after frame t (running sum covers frames <= t)
+                while (bi < nbnd && bnd[bi].x == t + 1) {
+                    const int4 b = bnd[bi];
+                    if (b.y == 0) IP[c * T + b.z] = (int64_t)S;
+                    else if (b.y == 1) IF[c * T + b.z] = (int64_t)S;
+                    else IF[c * T + b.z] = (int64_t)S - IF[c * T + b.z];
+                    bi++;
+                }
+            }
+            stream_push<DPL, K, RAW, 0>(s, x, cnt, nlag, lo, out_next, lf_out);
+        }
     }
     if constexpr (RAW) {
         pix_total[c] = S;

@@ -84,7 +84,9 @@ typedef struct xpcs_mt_info {
     int32_t pixels;             /* P masked pixels (dqmap>=1 && sqmap>=1)      */
     int32_t mapped_pixels;      /* pixels that reach normalisation (m_mapping) */
     int32_t groups;             /* (q, s) groups of m_mapping                  */
-    int32_t pad;
+    int32_t exact_pixels;       /* dense streaming path: pixels whose lag sums
+                                   were recomputed in 64-bit integers (total
+                                   count above 94906265, beyond exact FP64) */
     int64_t events;             /* level-0 events after the last ingest        */
     int64_t long_pixels;        /* pixels handled by the dense-row kernel      */
 } xpcs_mt_info;

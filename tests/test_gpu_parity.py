@@ -130,6 +130,29 @@ def test_dense_dpl8_two_passes(xmt, orc):
     check_against_oracle(xmt, orc, c, dark_frames=2)
 
 
+@pytest.mark.parametrize("F,dpl", [(13, 4), (37, 4), (1007, 4), (4099, 4), (29, 8), (2061, 8)])
+def test_dense_partial_groups(xmt, orc, F, dpl):
+    """Frame counts that are not multiples of the kernel's 16-frame group, on
+    the raw pass and on the series passes (including series shorter than one
+    group)."""
+    c = dense_case(12, 12, F, dpl, 2.0, 12.0, seed=53 + F)
+    check_against_oracle(xmt, orc, c, dark_frames=2)
+
+
+def test_dense_hot_pixels_exact_redo(xmt, orc):
+    """Pixels whose total count exceeds the exact-FP64 bound (94906265) are
+    recomputed with 64-bit integer lag sums; results stay bit-exact."""
+    W, H, F = 16, 16, 2500
+    dq, sq = synth.annular_qmaps(W, H, 9, 3)
+    sig = synth.ou_counts(W, H, F, 3.0, 20.0, seed=61).astype(np.int64)
+    rng = np.random.default_rng(62)
+    hot = rng.choice(np.flatnonzero((dq.reshape(-1) > 0) & (sq.reshape(-1) > 0)), 7, replace=False)
+    sig[:, hot] = rng.integers(45000, 65000, size=(F, hot.size))
+    c = dict(W=W, H=H, F=F, dpl=4, dq=dq, sq=sq, imm=synth.imm_bytes_dense(sig.astype(np.uint16), W, H))
+    plan, r = check_against_oracle(xmt, orc, c)
+    assert plan.info().exact_pixels == hot.size
+
+
 def test_dense_fallback_dpl2(xmt, orc):
     """dpl outside the streaming kernel's instantiations takes the dense-row kernel."""
     c = dense_case(16, 16, 500, 2, 4.0, 10.0, seed=43)

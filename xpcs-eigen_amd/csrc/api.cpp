@@ -228,11 +228,11 @@ extern "C" xpcs_status xpcs_mt_plan_info(const xpcs_mt_plan* p, xpcs_mt_info* in
     info->pixels = h.P;
     info->mapped_pixels = h.n_mapped;
     info->groups = h.ngroups;
-    info->pad = 0;
     info->events = p->events;
-    uint32_t cnt[2] = {0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
     if (p->executed && p->d_counters)
         (void)hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost);
+    info->exact_pixels = p->mode == 2 ? (int32_t)cnt[3] : 0;
     info->long_pixels = p->mode == 2 ? h.P : cnt[0];
     return XPCS_OK;
 }
@@ -566,11 +566,14 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             launch_side(p, s);
         }
     }
+    // streaming path: level values and pixel totals in 32 bits, n-scaled raw
+    // values in signed 32 bits
     const bool stream_ok = dense_stream_supported(h) &&
-                           (double)h.F * 65535.0 * (double)p->nscale * (double)h.avg < 4294967295.0;
+                           (double)h.F * 65535.0 * (double)p->nscale * (double)h.avg < 4294967295.0 &&
+                           65535.0 * (double)p->nscale < 2147483647.0;
     if (p->mode == 2 && stream_ok) {
         // thread-per-pixel streaming multi-tau over the raw frames
-        const size_t need = h.levels.size() > 6 ? (size_t)h.P * (size_t)h.levels[6].lf : 1;
+        const size_t need = dense_series_elems(h);
         if (need > p->series_cap) {
             xpcs_status st = dalloc(&p->d_series_a, need);
             if (!st) st = dalloc(&p->d_series_b, need);

@@ -151,7 +151,7 @@ struct xpcs_mt_plan {
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]
     int32_t* d_mid_list = nullptr;       // [P] rows of 65..CAP events
-    uint32_t* d_counters = nullptr;      // [0] long rows, [1] error flags, [2] mid rows
+    uint32_t* d_counters = nullptr;      // [0] long rows, [1] error flags, [2] mid rows, [3] dense exact-redo pixels
     uint64_t* d_scratch = nullptr;
     size_t scratch_cap = 0;
     int long_blocks = 0;
@@ -191,6 +191,7 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_side(xpcs_mt_plan* p, hipStream_t s);
 bool dense_stream_supported(const HostTables& h);
+size_t dense_series_elems(const HostTables& h);
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s);

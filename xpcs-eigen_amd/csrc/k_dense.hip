@@ -4,199 +4,381 @@
 //
 // One thread owns one detector pixel and walks the frames in order.  A wave's
 // loads of a frame are 64 consecutive pixels, so the raw IMM frames are read
-// once, fully coalesced, with no transposed copy.  Each level keeps the last
-// 2*dpl values in a register shift register; a new level-L value x at index k
-// adds x * x_L[k-d] to G2 for every lag d of the level, then enters the
-// register.  Level L+1 values are formed from pairs of level-L values; the
-// pairing parity is the same for every lane (it only depends on the frame
-// index), so the cascade never diverges.
+// once, fully coalesced, with no transposed copy.  Frames are taken 16 at a
+// time; within such a group the pairing of every level up to 2^4 is fixed at
+// compile time, so levels 0..4 are fully unrolled (no shift-register moves,
+// no branches) and only the levels above follow the group index (a
+// wave-uniform branch).  Each level keeps its last 2*dpl values; a new level-L
+// value x at index k adds x * x_L[k-d] to the lag-d sum for every lag d of the
+// level.
+//
+// Lag sums are accumulated with FP64 fused multiply-adds, which is exact here:
+// every value is a non-negative integer, and every product and partial sum of
+// a pixel is at most S^2, S = the pixel's total count, so while S^2 < 2^53 no
+// result is ever rounded.  A pixel with a larger total is listed by the first
+// pass and recomputed with 64-bit integer sums by k_mt_dense_exact.
 //
 // IP and IF come from level-0 prefix sums captured at the schedule's frame
 // boundaries (IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau')); the
-// boundary list is sorted on the host and walked with a scalar pointer.
+// boundary list is sorted on the host and walked with a wave-uniform index.
 //
 // Levels are processed K at a time (register budget).  The first pass reads the
 // raw frames (dark subtraction, threshold and mask fused) and, when more
-// levels remain, writes the level-K series [P][lf_K]; each further pass reads
-// the previous series.
+// levels remain, writes the level-K series frame-major ([lf_K][W*H], the
+// column of a pixel is its raster index); each further pass reads the
+// previous series with the same coalesced pattern.
+#include <climits>
+
 #include "device_util.h"
 #include "internal.h"
 
 namespace xpcs {
 
-// Boundary events of the level-0 prefix: after frame `frame` (i.e. when the
-// running sum covers frames < frame+1 ... see host), apply op to slot.
-// op 0: IP[slot] = S;  op 1: IF[slot] = S;  op 2: IF[slot] = S - IF[slot].
+// Largest pixel total S with S^2 < 2^53 (every lag sum exact in FP64).
+constexpr uint32_t kExactF64Total = 94906265u;
+constexpr int kGroup = 16;
+
+template <int DPL>
+constexpr int dense_levels_per_pass()
+{
+    return DPL == 4 ? 6 : 4;
+}
+
+template <int DPL, bool FIRST, int l>
+constexpr int lag_lo()
+{
+    // level 0 of the schedule has lags 1..2*dpl, every other level dpl+1..2*dpl
+    return (FIRST && l == 0) ? 1 : DPL + 1;
+}
 
 template <int DPL, int K>
-struct StreamState {
-    uint32_t win[K][2 * DPL];    // win[l][j] = value j+1 steps back at level L0+l
-    uint32_t pend[K];            // first value of an incomplete pair at level L0+l
-    uint64_t acc[K][2 * DPL];    // G2 accumulators per (level, lag-1)
+struct FState {
+    double win[K][2 * DPL];   // win[l][j] = value j+1 steps back at level L0+l
+    double acc[K][2 * DPL];   // lag sums, acc[l][d-1] for lag d
+    double pend[K];           // first value of an incomplete pair (levels above the group)
 };
 
-// Push one value into level l of the pass (static l), accumulate its lag
-// products, and cascade the completed pair into level l+1.
+struct SeriesOut {
+    uint32_t* col;            // element k of the output series at col[k * stride]
+    int64_t stride;
+    uint32_t lf;              // values of the output level
+};
+
+// One value of level l (index k, wave-uniform): lag products, window shift,
+// and pairing into level l+1 by the parity of k.
 template <int DPL, int K, bool FIRST, int l>
-__device__ __forceinline__ void stream_push(StreamState<DPL, K>& s, uint32_t x, uint32_t (&cnt)[K + 1],
-                                            const int (&nlag)[K], const int (&lo)[K], uint32_t* out_next,
-                                            int lf_next)
+__device__ __forceinline__ void push_one(FState<DPL, K>& s, double v, uint32_t k, const SeriesOut& o)
 {
     if constexpr (l < K) {
-        // lag products with the last values of this level (level 0 of the
-        // schedule has lags 1..2*dpl, every other level dpl+1..2*dpl)
-        constexpr int dlo = (FIRST && l == 0) ? 1 : DPL + 1;
+        constexpr int dlo = lag_lo<DPL, FIRST, l>();
 #pragma unroll
-        for (int d = dlo; d <= 2 * DPL; d++) {
-            const int rel = d - lo[l];
-            if (rel >= 0 && rel < nlag[l]) s.acc[l][d - 1] += (uint64_t)x * (uint64_t)s.win[l][d - 1];
-        }
+        for (int d = dlo; d <= 2 * DPL; d++) s.acc[l][d - 1] = __builtin_fma(v, s.win[l][d - 1], s.acc[l][d - 1]);
 #pragma unroll
         for (int j = 2 * DPL - 1; j > 0; j--) s.win[l][j] = s.win[l][j - 1];
-        s.win[l][0] = x;
-        const uint32_t k = cnt[l]++;
-        if ((k & 1u) == 0u) {
-            s.pend[l] = x;
-        } else {
-            const uint32_t y = s.pend[l] + x;
-            if constexpr (l + 1 < K) {
-                stream_push<DPL, K, FIRST, l + 1>(s, y, cnt, nlag, lo, out_next, lf_next);
-            } else {
-                const uint32_t kn = cnt[K]++;
-                if (out_next && (int)kn < lf_next) out_next[kn] = y;
+        s.win[l][0] = v;
+        if (k & 1u) push_one<DPL, K, FIRST, l + 1>(s, s.pend[l] + v, k >> 1, o);
+        else s.pend[l] = v;
+    } else {
+        if (o.col && k < o.lf) o.col[(int64_t)k * o.stride] = (uint32_t)v;
+    }
+}
+
+// N consecutive values of level l, the first with index kb (N = 16 >> l).
+// TAIL: the last, partial group; values with index >= lf of their level are
+// zeroed (they do not exist in the series) and nothing reaches the levels
+// above the group.
+template <int DPL, int K, bool FIRST, bool TAIL, int l, int N>
+__device__ __forceinline__ void push_block(FState<DPL, K>& s, double (&v)[N], uint32_t kb, uint32_t n_in,
+                                           const SeriesOut& o)
+{
+    if constexpr (TAIL) {
+        const uint32_t lim = n_in >> l;
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (kb + j >= lim) v[j] = 0.0;
+    }
+    if constexpr (l < K) {
+        constexpr int dlo = lag_lo<DPL, FIRST, l>();
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+#pragma unroll
+            for (int d = dlo; d <= 2 * DPL; d++) {
+                const double w = (j >= d) ? v[j >= d ? j - d : 0] : s.win[l][j >= d ? 0 : d - j - 1];
+                s.acc[l][d - 1] = __builtin_fma(v[j], w, s.acc[l][d - 1]);
             }
         }
+        double nw[2 * DPL];
+#pragma unroll
+        for (int i = 0; i < 2 * DPL; i++) nw[i] = (i < N) ? v[i < N ? N - 1 - i : 0] : s.win[l][i < N ? 0 : i - N];
+#pragma unroll
+        for (int i = 0; i < 2 * DPL; i++) s.win[l][i] = nw[i];
+        if constexpr (N >= 2) {
+            double u[N / 2];
+#pragma unroll
+            for (int j = 0; j < N / 2; j++) u[j] = v[2 * j] + v[2 * j + 1];
+            push_block<DPL, K, FIRST, TAIL, l + 1, N / 2>(s, u, kb >> 1, n_in, o);
+        } else if constexpr (!TAIL) {
+            if (kb & 1u) push_one<DPL, K, FIRST, l + 1>(s, s.pend[l] + v[0], kb >> 1, o);
+            else s.pend[l] = v[0];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (o.col && kb + j < o.lf) o.col[(int64_t)(kb + j) * o.stride] = (uint32_t)v[j];
     }
+}
+
+// Dark subtraction and threshold of one raw value (n-scaled integers: kept iff
+// n*raw - sum(dark) > max(floor(n*threshold), 0)).
+__device__ __forceinline__ uint32_t dense_value(uint32_t raw, int32_t nscale, int32_t nds, int32_t th)
+{
+    const int32_t y = __mul24((int32_t)raw, nscale) + nds;
+    return y > th ? (uint32_t)y : 0u;
 }
 
 template <int DPL, int K, bool RAW>
 __global__ void __launch_bounds__(256)
-k_mt_dense_pass(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg,
-                const int32_t* __restrict__ pix2c, uint32_t npix_det, const int64_t* __restrict__ dark_sum,
-                const int64_t* __restrict__ thr, int64_t nscale,
-                const uint32_t* __restrict__ series_in, int32_t P, int32_t n_in,
-                const LevelParam* __restrict__ levels, int32_t L0, int32_t nlev_pass,
-                const int4* __restrict__ bnd, int32_t nbnd,
-                int32_t T, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
-                uint64_t* __restrict__ pix_total, uint32_t* __restrict__ series_out, int32_t lf_out)
+k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg,
+               const int32_t* __restrict__ pix2c, uint32_t npix, const int64_t* __restrict__ dark_sum,
+               const int64_t* __restrict__ thr, int32_t nscale, const uint32_t* __restrict__ series_in,
+               uint32_t n_in, const LevelParam* __restrict__ levels, int32_t L0, int32_t nlev_pass,
+               const int4* __restrict__ bnd, int32_t nbnd, int32_t T, int64_t* __restrict__ G2,
+               int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
+               uint32_t* __restrict__ series_out, uint32_t lf_out, uint32_t* __restrict__ redo_count,
+               int32_t* __restrict__ redo_list)
 {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t c;
-    if constexpr (RAW) {
-        if (gid >= npix_det) return;
-        c = pix2c[gid];
-        if (c < 0) return;
-    } else {
-        if (gid >= P) return;
-        c = gid;
-    }
-    StreamState<DPL, K> s;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= npix) return;
+    const int32_t c = pix2c[gid];
+    if (c < 0) return;
+    FState<DPL, K> s;
 #pragma unroll
     for (int l = 0; l < K; l++) {
-        s.pend[l] = 0u;
+        s.pend[l] = 0.0;
 #pragma unroll
         for (int j = 0; j < 2 * DPL; j++) {
-            s.win[l][j] = 0u;
-            s.acc[l][j] = 0ull;
+            s.win[l][j] = 0.0;
+            s.acc[l][j] = 0.0;
         }
     }
-    int nlag[K], lo[K];
-#pragma unroll
-    for (int l = 0; l < K; l++) {
-        if (l < nlev_pass) {
-            nlag[l] = levels[L0 + l].count;
-            lo[l] = levels[L0 + l].lo;
-        } else {
-            nlag[l] = 0;
-            lo[l] = 1;
-        }
-    }
-    uint32_t cnt[K + 1];
-#pragma unroll
-    for (int l = 0; l <= K; l++) cnt[l] = 0u;
-    uint32_t* out_next = (series_out && nlev_pass == K) ? series_out + c * (int64_t)lf_out : nullptr;
-
-    uint64_t S = 0;
-    int bi = 0;
-    int64_t ds = 0, th = 0;
+    const SeriesOut o{(series_out && nlev_pass == K) ? series_out + gid : nullptr, (int64_t)npix, lf_out};
+    int32_t nds = 0, th = 0;
     if constexpr (RAW) {
-        ds = dark_sum ? dark_sum[c] : 0;
-        th = thr ? thr[c] : 0;
+        nds = dark_sum ? -(int32_t)dark_sum[c] : 0;
+        const int64_t t = thr ? thr[c] : 0;
+        th = t < 0 ? 0 : (t > INT32_MAX ? INT32_MAX : (int32_t)t);
     }
-    const uint32_t* in_row = RAW ? nullptr : series_in + c * (int64_t)n_in;
-    // Frames are fetched kPrefetch at a time so that many independent loads
-    // are in flight per thread; processing order stays frame order.
-    constexpr int kPrefetch = 16;
-    for (int t0 = 0; t0 < n_in; t0 += kPrefetch) {
-        uint32_t xs[kPrefetch];
+    const int64_t rowT = (int64_t)c * T;
+    uint32_t S = 0;
+    int bi = 0;
+    if constexpr (RAW) {
+        while (bi < nbnd && bnd[bi].x == 0) bi++;
+    }
+    int nextb = (RAW && bi < nbnd) ? bnd[bi].x : INT_MAX;
+    const uint16_t* col16 = reinterpret_cast<const uint16_t*>(imm) + gid;
+    const uint32_t* col32 = RAW ? nullptr : series_in + gid;
+
+    // loads of group t0 (frame indices clamped into range; the tail zeroes them)
+    auto load_raw = [&](uint32_t (&r)[kGroup], uint32_t t0) {
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) {
+            const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
+            if constexpr (RAW) r[u] = *reinterpret_cast<const uint16_t*>(
+                                          reinterpret_cast<const uint8_t*>(col16) + sel_off[t]);
+            else r[u] = col32[(int64_t)t * npix];
+        }
+    };
+    auto values = [&](uint32_t (&x)[kGroup], const uint32_t (&r)[kGroup], uint32_t t0) {
         if constexpr (RAW) {
             if (avg == 1) {
-                uint16_t raw[kPrefetch];
 #pragma unroll
-                for (int u = 0; u < kPrefetch; u++) {
-                    const int t = min(t0 + u, n_in - 1);
-                    raw[u] = reinterpret_cast<const uint16_t*>(imm + sel_off[t])[gid];
-                }
-#pragma unroll
-                for (int u = 0; u < kPrefetch; u++) {
-                    const int64_t y = nscale * (int64_t)raw[u] - ds;
-                    xs[u] = (y > th && y > 0) ? (uint32_t)y : 0u;
-                }
+                for (int u = 0; u < kGroup; u++) x[u] = dense_value(r[u], nscale, nds, th);
             } else {
 #pragma unroll
-                for (int u = 0; u < kPrefetch; u++) {
-                    const int t = min(t0 + u, n_in - 1);
-                    uint32_t acc = 0;
+                for (int u = 0; u < kGroup; u++) {
+                    const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
+                    uint32_t a_sum = 0;
                     for (int a = 0; a < avg; a++) {
-                        const uint16_t* fr = reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)t * avg + a]);
-                        const int64_t y = nscale * (int64_t)fr[gid] - ds;
-                        if (y > th && y > 0) acc += (uint32_t)y;
+                        const uint32_t raw = *reinterpret_cast<const uint16_t*>(
+                            reinterpret_cast<const uint8_t*>(col16) + sel_off[(int64_t)t * avg + a]);
+                        a_sum += dense_value(raw, nscale, nds, th);
                     }
-                    xs[u] = acc;
+                    x[u] = a_sum;
                 }
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < kPrefetch; u++) xs[u] = in_row[min(t0 + u, n_in - 1)];
+            for (int u = 0; u < kGroup; u++) x[u] = r[u];
         }
-        const int nu = min(kPrefetch, n_in - t0);
-        for (int u = 0; u < nu; u++) {
-            const uint32_t x = xs[u];
-            if constexpr (RAW) {
-                const int t = t0 + u;
-                S += x;
-                // boundary captures after frame t (running sum covers frames <= t)
-                while (bi < nbnd && bnd[bi].x == t + 1) {
-                    const int4 b = bnd[bi];
-                    if (b.y == 0) IP[c * T + b.z] = (int64_t)S;
-                    else if (b.y == 1) IF[c * T + b.z] = (int64_t)S;
-                    else IF[c * T + b.z] = (int64_t)S - IF[c * T + b.z];
-                    bi++;
-                }
+    };
+    // boundary captures inside group t0 (wave-uniform test), then the running sum
+    auto captures = [&](const uint32_t (&x)[kGroup], uint32_t t0) {
+        if constexpr (RAW) {
+            while (nextb <= (int)(t0 + kGroup)) {
+                const int4 b = bnd[bi];
+                const int n = b.x - (int)t0;
+                uint32_t part = S;
+#pragma unroll
+                for (int u = 0; u < kGroup; u++) part += (u < n) ? x[u] : 0u;
+                if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
+                else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
+                else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                bi++;
+                nextb = bi < nbnd ? bnd[bi].x : INT_MAX;
             }
-            stream_push<DPL, K, RAW, 0>(s, x, cnt, nlag, lo, out_next, lf_out);
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) S += x[u];
         }
+    };
+
+    const uint32_t nfull = n_in / kGroup * kGroup;
+    uint32_t rcur[kGroup];
+    if (RAW && avg != 1) {
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) rcur[u] = 0u;
+    } else {
+        load_raw(rcur, 0);
+    }
+    for (uint32_t t0 = 0; t0 < nfull; t0 += kGroup) {
+        // next group's loads are issued before this group is processed
+        uint32_t rnext[kGroup];
+        if (RAW && avg != 1) {
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) rnext[u] = 0u;
+        } else {
+            load_raw(rnext, t0 + kGroup);
+        }
+        uint32_t x[kGroup];
+        values(x, rcur, t0);
+        captures(x, t0);
+        double v[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
+        push_block<DPL, K, RAW, false, 0, kGroup>(s, v, t0, n_in, o);
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) rcur[u] = rnext[u];
+    }
+    if (nfull < n_in) {
+        uint32_t x[kGroup];
+        values(x, rcur, nfull);
+#pragma unroll
+        for (int u = 0; u < kGroup; u++)
+            if (nfull + (uint32_t)u >= n_in) x[u] = 0u;
+        captures(x, nfull);
+        double v[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
+        push_block<DPL, K, RAW, true, 0, kGroup>(s, v, nfull, n_in, o);
     }
     if constexpr (RAW) {
         pix_total[c] = S;
-        // boundaries at frame 0 (b == 0) cover no frames
+        // boundaries at frame 0 cover no frames
         for (int i = 0; i < nbnd; i++) {
             const int4 b = bnd[i];
             if (b.x != 0) break;
-            if (b.y == 0) IP[c * T + b.z] = 0;
-            else if (b.y == 1) IF[c * T + b.z] = 0;
+            if (b.y == 0) IP[rowT + b.z] = 0;
+            else if (b.y == 1) IF[rowT + b.z] = 0;
         }
+        if (S > kExactF64Total) redo_list[atomicAdd(redo_count, 1u)] = (int32_t)gid;
     }
 #pragma unroll
     for (int l = 0; l < K; l++) {
         if (l < nlev_pass) {
             const LevelParam lp = levels[L0 + l];
-            constexpr int dlo_any = 1;
+            const int dlo = (RAW && l == 0) ? 1 : DPL + 1;
 #pragma unroll
-            for (int d = dlo_any; d <= 2 * DPL; d++) {
+            for (int d = 1; d <= 2 * DPL; d++) {
                 const int rel = d - lp.lo;
-                if ((RAW && l == 0 ? d >= 1 : d >= DPL + 1) && rel >= 0 && rel < lp.count)
-                    G2[c * T + lp.first + rel] = (int64_t)s.acc[l][d - 1];
+                if (d >= dlo && rel >= 0 && rel < lp.count) G2[rowT + lp.first + rel] = (int64_t)s.acc[l][d - 1];
+            }
+        }
+    }
+}
+
+// ---- exact 64-bit integer variant for the listed pixels ---------------------
+
+template <int DPL, int K>
+struct XState {
+    uint32_t win[K][2 * DPL];
+    uint32_t pend[K];
+    uint64_t acc[K][2 * DPL];
+};
+
+template <int DPL, int K, bool FIRST, int l>
+__device__ __forceinline__ void push_exact(XState<DPL, K>& s, uint32_t x, uint32_t (&cnt)[K])
+{
+    if constexpr (l < K) {
+        constexpr int dlo = lag_lo<DPL, FIRST, l>();
+#pragma unroll
+        for (int d = dlo; d <= 2 * DPL; d++) s.acc[l][d - 1] += (uint64_t)x * (uint64_t)s.win[l][d - 1];
+#pragma unroll
+        for (int j = 2 * DPL - 1; j > 0; j--) s.win[l][j] = s.win[l][j - 1];
+        s.win[l][0] = x;
+        const uint32_t k = cnt[l]++;
+        if (k & 1u) push_exact<DPL, K, FIRST, l + 1>(s, s.pend[l] + x, cnt);
+        else s.pend[l] = x;
+    }
+}
+
+// Lag sums of the pixels listed by k_mt_dense_f64 (totals too large for exact
+// FP64), one thread per listed pixel, 64-bit integer accumulation.  Only G2
+// is rewritten: IP, IF and the series are integer results of the FP64 pass.
+template <int DPL, int K, bool RAW>
+__global__ void __launch_bounds__(256)
+k_mt_dense_exact(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg,
+                 const int32_t* __restrict__ pix2c, uint32_t npix, const int64_t* __restrict__ dark_sum,
+                 const int64_t* __restrict__ thr, int32_t nscale, const uint32_t* __restrict__ series_in,
+                 uint32_t n_in, const LevelParam* __restrict__ levels, int32_t L0, int32_t nlev_pass, int32_t T,
+                 int64_t* __restrict__ G2, const uint32_t* __restrict__ redo_count,
+                 const int32_t* __restrict__ redo_list)
+{
+    const uint32_t n = *redo_count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t gid = (uint32_t)redo_list[i];
+        const int32_t c = pix2c[gid];
+        XState<DPL, K> s;
+        uint32_t cnt[K];
+#pragma unroll
+        for (int l = 0; l < K; l++) {
+            s.pend[l] = 0u;
+            cnt[l] = 0u;
+#pragma unroll
+            for (int j = 0; j < 2 * DPL; j++) {
+                s.win[l][j] = 0u;
+                s.acc[l][j] = 0ull;
+            }
+        }
+        int32_t nds = 0, th = 0;
+        if constexpr (RAW) {
+            nds = dark_sum ? -(int32_t)dark_sum[c] : 0;
+            const int64_t t = thr ? thr[c] : 0;
+            th = t < 0 ? 0 : (t > INT32_MAX ? INT32_MAX : (int32_t)t);
+        }
+        for (uint32_t t = 0; t < n_in; t++) {
+            uint32_t x;
+            if constexpr (RAW) {
+                x = 0;
+                for (int a = 0; a < avg; a++) {
+                    const uint32_t raw = reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)t * avg + a])[gid];
+                    x += dense_value(raw, nscale, nds, th);
+                }
+            } else {
+                x = series_in[(int64_t)t * npix + gid];
+            }
+            push_exact<DPL, K, RAW, 0>(s, x, cnt);
+        }
+        const int64_t rowT = (int64_t)c * T;
+#pragma unroll
+        for (int l = 0; l < K; l++) {
+            if (l < nlev_pass) {
+                const LevelParam lp = levels[L0 + l];
+                const int dlo = (RAW && l == 0) ? 1 : DPL + 1;
+#pragma unroll
+                for (int d = 1; d <= 2 * DPL; d++) {
+                    const int rel = d - lp.lo;
+                    if (d >= dlo && rel >= 0 && rel < lp.count)
+                        G2[rowT + lp.first + rel] = (int64_t)s.acc[l][d - 1];
+                }
             }
         }
     }
@@ -233,41 +415,60 @@ k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
 }
 
 // ---- host side --------------------------------------------------------------------
-constexpr int kDenseK = 6;
 
 bool dense_stream_supported(const HostTables& h)
 {
     return h.dpl == 4 || h.dpl == 8;
 }
 
+size_t dense_series_elems(const HostTables& h)
+{
+    const size_t K = h.dpl == 4 ? dense_levels_per_pass<4>() : dense_levels_per_pass<8>();
+    return h.levels.size() > K ? (size_t)h.levels[K].lf * (size_t)h.W * (size_t)h.H : 1;
+}
+
 template <int DPL>
 static void run_passes(xpcs_mt_plan* p, hipStream_t s)
 {
+    constexpr int K = dense_levels_per_pass<DPL>();
     const HostTables& h = p->h;
     const int nlev = (int)h.levels.size();
-    const int npd = h.W * h.H;
+    const uint32_t npd = (uint32_t)(h.W * h.H);
+    const unsigned grid = (npd + 255) / 256;
+    uint32_t* redo_count = p->d_counters + 3;
     int L0 = 0;
-    int n_in = h.F;
+    uint32_t n_in = (uint32_t)h.F;
     int pass = 0;
     while (L0 < nlev) {
-        const int k = std::min(kDenseK, nlev - L0);
+        const int k = std::min(K, nlev - L0);
         const bool more = L0 + k < nlev;
-        const int lf_out = more ? h.levels[L0 + k].lf : 0;
+        const uint32_t lf_out = more ? (uint32_t)h.levels[L0 + k].lf : 0u;
         uint32_t* out = more ? (pass % 2 == 0 ? p->d_series_a : p->d_series_b) : nullptr;
         const uint32_t* in = pass == 0 ? nullptr : (pass % 2 == 1 ? p->d_series_a : p->d_series_b);
         if (pass == 0) {
             timer_begin(p, s, "k_mt_dense_raw");
-            k_mt_dense_pass<DPL, kDenseK, true><<<(npd + 255) / 256, 256, 0, s>>>(
-                p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, (uint32_t)npd, p->d_dark_sum, p->d_thr, p->nscale,
-                nullptr, h.P, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2, p->d_IP, p->d_IF,
-                p->d_pix_total, out, lf_out);
+            k_mt_dense_f64<DPL, K, true><<<grid, 256, 0, s>>>(
+                p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
+                nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2, p->d_IP, p->d_IF,
+                p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
+            timer_end(p, s);
+            timer_begin(p, s, "k_mt_dense_exact");
+            k_mt_dense_exact<DPL, K, true><<<256, 256, 0, s>>>(
+                p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
+                nullptr, n_in, p->d_levels, L0, k, h.T, p->d_G2, redo_count, p->d_long_list);
+            timer_end(p, s);
         } else {
             timer_begin(p, s, "k_mt_dense_series");
-            k_mt_dense_pass<DPL, kDenseK, false><<<(h.P + 255) / 256, 256, 0, s>>>(
-                nullptr, nullptr, 1, nullptr, 0, nullptr, nullptr, 1, in, h.P, n_in, p->d_levels, L0, k,
-                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out);
+            k_mt_dense_f64<DPL, K, false><<<grid, 256, 0, s>>>(
+                nullptr, nullptr, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k,
+                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr);
+            timer_end(p, s);
+            timer_begin(p, s, "k_mt_dense_exact");
+            k_mt_dense_exact<DPL, K, false><<<256, 256, 0, s>>>(
+                nullptr, nullptr, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k, h.T,
+                p->d_G2, redo_count, p->d_long_list);
+            timer_end(p, s);
         }
-        timer_end(p, s);
         n_in = lf_out;
         L0 += k;
         pass++;

@@ -53,7 +53,7 @@ class Desc(C.Structure):
 class Info(C.Structure):
     _fields_ = [("taus", C.c_int32), ("max_level", C.c_int32), ("dynamic_bins", C.c_int32),
                 ("static_bins", C.c_int32), ("pixels", C.c_int32), ("mapped_pixels", C.c_int32),
-                ("groups", C.c_int32), ("pad", C.c_int32), ("events", C.c_int64),
+                ("groups", C.c_int32), ("exact_pixels", C.c_int32), ("events", C.c_int64),
                 ("long_pixels", C.c_int64)]
 
 

@@ -33,6 +33,8 @@ WORKLOADS = {
     "plain-256": dict(W=256, H=256, F=1000, dpl=4, mu=0.012, tau=(20.0, 20.0), S=18, Q=6, seed=1001),
     "dense-512": dict(W=512, H=512, F=20000, dpl=4, mu=5.0, tau=(50.0, 50.0), S=27, Q=9, seed=1003,
                       dense=True, darks=2),
+    "dense-512-dpl8": dict(W=512, H=512, F=20000, dpl=8, mu=5.0, tau=(50.0, 50.0), S=27, Q=9, seed=1003,
+                           dense=True, darks=2),
 }
 
 
@@ -62,6 +64,41 @@ def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel, F=0, dense_raw=0):
     }.get(name, 0)
 
 
+def dense_pass_bytes(P, F, avg, dpl, levels):
+    """Algorithmic bytes of the dense streaming passes (raw pass, mean series pass).
+
+    Raw pass: 2-byte values of the P mapped pixels for every frame in, IP and IF
+    for every tau and G2 for the taus of its levels out (8 bytes each), level-K
+    series out (4 bytes).  Series pass: its input series in, its G2 out, the
+    next series out."""
+    K = 4 if dpl == 4 else 3
+    nlev = int(levels.max()) + 1
+    lf = [F >> L for L in range(nlev + 1)]
+    t_of = np.bincount(levels, minlength=nlev)
+    T = len(levels)
+    raw = 2 * P * F * avg + 16 * P * T + 8 * P * int(t_of[:K].sum()) + (4 * P * lf[K] if nlev > K else 0)
+    series = []
+    L0 = K
+    while L0 < nlev:
+        L1 = min(L0 + K, nlev)
+        series.append(4 * P * lf[L0] + 8 * P * int(t_of[L0:L1].sum()) + (4 * P * lf[L1] if L1 < nlev else 0))
+        L0 = L1
+    return {"k_mt_dense_raw": raw, "k_mt_dense_series": int(np.mean(series)) if series else 0}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic.txt), or None."""
+    path = ROOT / "profiles" / "hbm_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 5 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return float(f[4])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +108,9 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2000,
                     help="frames of the CPU baseline sample (full detector)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB elementwise copy (read 1 GiB, write 1 GiB) three times, "
+                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the pixel-sharded multi-GPU design is "
@@ -80,6 +120,17 @@ def main():
     import torch
     import synth
     import xpcs_mt
+
+    if args.copy_calibration:
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 1, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel": "elementwise add int32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "launches": 3}))
+        return
 
     wl = WORKLOADS[args.workload]
     W, H, F, dpl = wl["W"], wl["H"], wl["F"], wl["dpl"]
@@ -120,19 +171,23 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     frames_per_s = F * args.steps / elapsed
 
+    levels, _ = plan.schedule()
+    dbytes = dense_pass_bytes(P, F, wl.get("avg", 1), dpl, np.asarray(levels)) if dense else {}
     kern = {}
     for k, (ms, n) in ktimes.items():
         avg_ms = ms / max(n, 1)
-        b = algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen), F, int(dlen[darks:].sum()) * wl.get('avg', 1) if dense else 0)
+        b = dbytes.get(k) or algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen), F,
+                                               int(dlen[darks:].sum()) * wl.get('avg', 1) if dense else 0)
         kern[k] = dict(avg_ms=avg_ms, launches=n, alg_bytes=b,
                        gbs=(b / (avg_ms * 1e-3) / 1e9) if b and avg_ms > 0 else None)
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     d = kern[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None, "traffic": None,
+                "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None,
+                "traffic": measured_traffic(args.workload, dom), "traffic_source": "profiles/hbm_traffic.txt",
                 "alg_bytes_per_launch": d["alg_bytes"], "avg_ms": d["avg_ms"]}
-    job_bytes = sum(v["alg_bytes"] for v in kern.values())
-    job_kernel_ms = sum(v["avg_ms"] for v in kern.values())
+    job_bytes = sum(v["alg_bytes"] * v["launches"] for v in kern.values()) / args.steps
+    job_kernel_ms = sum(v["avg_ms"] * v["launches"] for v in kern.values()) / args.steps
 
     # correctness spot check of this run (cheap invariants)
     g2, se = plan.g2()

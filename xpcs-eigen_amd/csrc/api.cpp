@@ -377,6 +377,25 @@ xpcs_status finish_selection(xpcs_mt_plan* p, const std::vector<int64_t>& sel_of
     return XPCS_OK;
 }
 
+// Tile loads of the dense streaming kernel: one output frame per raw frame,
+// frames evenly spaced and 16-byte aligned, and every frame readable up to the
+// 64-pixel boundary after the detector's last pixel.
+void dense_tiling(xpcs_mt_plan* p, const std::vector<int64_t>& so, const uint8_t* base, size_t buf_bytes)
+{
+    p->tile_off0 = -1;
+    p->tile_step = 0;
+    const HostTables& h = p->h;
+    const int64_t npd = (int64_t)h.W * h.H;
+    if (h.avg != 1 || so.empty() || npd % 8 != 0) return;
+    const int64_t step = so.size() > 1 ? so[1] - so[0] : 16;
+    if (step <= 0 || step % 16 != 0 || ((uintptr_t)base + (uintptr_t)so[0]) % 16 != 0) return;
+    for (size_t k = 0; k < so.size(); k++)
+        if (so[k] != so[0] + (int64_t)k * step) return;
+    if (so.back() + 2 * ((npd + 63) / 64 * 64) > (int64_t)buf_bytes) return;
+    p->tile_off0 = so[0];
+    p->tile_step = step;
+}
+
 }  // namespace
 
 extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size_t nbytes)
@@ -428,10 +447,12 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
     const size_t lo = (size_t)off[(size_t)lo_frame] - 1024;
     const size_t hi = (size_t)off[(size_t)hi_frame] + (size_t)dl[(size_t)hi_frame] * (comp ? 6 : 2);
     const size_t len = hi - lo;
-    if (len > p->imm_cap) {
-        st = dalloc(&p->d_imm_owned, len);
+    // the tail pad keeps tile loads of the last frame inside the allocation
+    const size_t pad = 256;
+    if (len + pad > p->imm_cap) {
+        st = dalloc(&p->d_imm_owned, len + pad);
         if (st) return st;
-        p->imm_cap = len;
+        p->imm_cap = len + pad;
     }
     HIP_TRY(hipMemcpyAsync(p->d_imm_owned, b + lo, len, hipMemcpyHostToDevice, p->stream));
     p->d_imm = p->d_imm_owned;
@@ -441,6 +462,7 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
         so[k] = off[(size_t)sel[k]] - (int64_t)lo;
         sd[k] = dl[(size_t)sel[k]];
     }
+    if (!comp) dense_tiling(p, so, p->d_imm, p->imm_cap);
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = comp ? 1 : 2;
@@ -489,6 +511,7 @@ extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* de
         so[k] = payload_off[sel[k]];
         sd[k] = dlen[sel[k]];
     }
+    if (!compressed) dense_tiling(p, so, p->d_imm, nbytes);
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = compressed ? 1 : 2;

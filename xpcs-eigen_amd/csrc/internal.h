@@ -113,6 +113,11 @@ struct xpcs_mt_plan {
     const uint8_t* d_imm = nullptr;      // raw IMM bytes (owned when own_imm)
     uint8_t* d_imm_owned = nullptr;
     size_t imm_cap = 0;
+    // dense frames evenly spaced in d_imm (payload of frame t at tile_off0 +
+    // t * tile_step, 16-byte aligned, readable up to the next 64-pixel
+    // boundary): the streaming kernel loads them as tiles; -1 otherwise
+    int64_t tile_off0 = -1;
+    int64_t tile_step = 0;
     int64_t nsel = 0;                    // selected raw frames
     int64_t* d_sel_off = nullptr;        // payload byte offset per selected raw frame
     uint32_t* d_sel_dlen = nullptr;

@@ -38,10 +38,13 @@ namespace xpcs {
 constexpr uint32_t kExactF64Total = 94906265u;
 constexpr int kGroup = 16;
 
+// Levels per pass (register budget): dpl 4 at 4 levels keeps two waves per
+// SIMD; dpl 8 at 3 levels runs one wave per SIMD and is as fast as 2 levels at
+// two waves, with fewer passes over the coarsened series.
 template <int DPL>
 constexpr int dense_levels_per_pass()
 {
-    return DPL == 4 ? 6 : 4;
+    return DPL == 4 ? 4 : 3;
 }
 
 template <int DPL, bool FIRST, int l>
@@ -136,21 +139,39 @@ __device__ __forceinline__ uint32_t dense_value(uint32_t raw, int32_t nscale, in
     return y > th ? (uint32_t)y : 0u;
 }
 
-template <int DPL, int K, bool RAW>
+// Input of a pass: the level-K series of the previous pass, the raw frames
+// one 2-byte load per lane and frame, or the raw frames staged through LDS
+// (evenly spaced frames, avg 1): a wave fetches a 16-frame x 64-pixel tile
+// with two 16-byte loads per lane, DEPTH tiles ahead of use.
+enum DenseInput { kSeries = 0, kRawLanes = 1, kRawTiles = 2 };
+constexpr int kTileDepth = 3;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int DPL, int K, int MODE>
 __global__ void __launch_bounds__(256)
-k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg,
-               const int32_t* __restrict__ pix2c, uint32_t npix, const int64_t* __restrict__ dark_sum,
-               const int64_t* __restrict__ thr, int32_t nscale, const uint32_t* __restrict__ series_in,
-               uint32_t n_in, const LevelParam* __restrict__ levels, int32_t L0, int32_t nlev_pass,
-               const int4* __restrict__ bnd, int32_t nbnd, int32_t T, int64_t* __restrict__ G2,
-               int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
-               uint32_t* __restrict__ series_out, uint32_t lf_out, uint32_t* __restrict__ redo_count,
-               int32_t* __restrict__ redo_list)
+k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int64_t tile_off0,
+               int64_t tile_step, int avg, const int32_t* __restrict__ pix2c, uint32_t npix,
+               const int64_t* __restrict__ dark_sum, const int64_t* __restrict__ thr, int32_t nscale,
+               const uint32_t* __restrict__ series_in, uint32_t n_in, const LevelParam* __restrict__ levels,
+               int32_t L0, int32_t nlev_pass, const int4* __restrict__ bnd, int32_t nbnd, int32_t T,
+               int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
+               uint64_t* __restrict__ pix_total, uint32_t* __restrict__ series_out, uint32_t lf_out,
+               uint32_t* __restrict__ redo_count, int32_t* __restrict__ redo_list)
 {
+    constexpr bool RAW = MODE != kSeries;
+    constexpr bool TILES = MODE == kRawTiles;
+    __shared__ u32x4 s_tile[TILES ? 4 : 1][2][64];
+
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= npix) return;
-    const int32_t c = pix2c[gid];
-    if (c < 0) return;
+    const int32_t c = gid < npix ? pix2c[gid] : -1;
+    const bool active = c >= 0;
+    if constexpr (TILES) {
+        // every lane of a wave takes part in the tile loads; a wave without a
+        // mapped pixel has nothing to do
+        if (__ballot(active) == 0ull) return;
+    } else {
+        if (!active) return;
+    }
     FState<DPL, K> s;
 #pragma unroll
     for (int l = 0; l < K; l++) {
@@ -161,56 +182,24 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
             s.acc[l][j] = 0.0;
         }
     }
-    const SeriesOut o{(series_out && nlev_pass == K) ? series_out + gid : nullptr, (int64_t)npix, lf_out};
+    const SeriesOut o{(active && series_out && nlev_pass == K) ? series_out + gid : nullptr, (int64_t)npix,
+                      lf_out};
     int32_t nds = 0, th = 0;
     if constexpr (RAW) {
-        nds = dark_sum ? -(int32_t)dark_sum[c] : 0;
-        const int64_t t = thr ? thr[c] : 0;
-        th = t < 0 ? 0 : (t > INT32_MAX ? INT32_MAX : (int32_t)t);
+        if (active) {
+            nds = dark_sum ? -(int32_t)dark_sum[c] : 0;
+            const int64_t t = thr ? thr[c] : 0;
+            th = t < 0 ? 0 : (t > INT32_MAX ? INT32_MAX : (int32_t)t);
+        }
     }
-    const int64_t rowT = (int64_t)c * T;
+    const int64_t rowT = (int64_t)(active ? c : 0) * T;
     uint32_t S = 0;
     int bi = 0;
     if constexpr (RAW) {
         while (bi < nbnd && bnd[bi].x == 0) bi++;
     }
     int nextb = (RAW && bi < nbnd) ? bnd[bi].x : INT_MAX;
-    const uint16_t* col16 = reinterpret_cast<const uint16_t*>(imm) + gid;
-    const uint32_t* col32 = RAW ? nullptr : series_in + gid;
 
-    // loads of group t0 (frame indices clamped into range; the tail zeroes them)
-    auto load_raw = [&](uint32_t (&r)[kGroup], uint32_t t0) {
-#pragma unroll
-        for (int u = 0; u < kGroup; u++) {
-            const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
-            if constexpr (RAW) r[u] = *reinterpret_cast<const uint16_t*>(
-                                          reinterpret_cast<const uint8_t*>(col16) + sel_off[t]);
-            else r[u] = col32[(int64_t)t * npix];
-        }
-    };
-    auto values = [&](uint32_t (&x)[kGroup], const uint32_t (&r)[kGroup], uint32_t t0) {
-        if constexpr (RAW) {
-            if (avg == 1) {
-#pragma unroll
-                for (int u = 0; u < kGroup; u++) x[u] = dense_value(r[u], nscale, nds, th);
-            } else {
-#pragma unroll
-                for (int u = 0; u < kGroup; u++) {
-                    const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
-                    uint32_t a_sum = 0;
-                    for (int a = 0; a < avg; a++) {
-                        const uint32_t raw = *reinterpret_cast<const uint16_t*>(
-                            reinterpret_cast<const uint8_t*>(col16) + sel_off[(int64_t)t * avg + a]);
-                        a_sum += dense_value(raw, nscale, nds, th);
-                    }
-                    x[u] = a_sum;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < kGroup; u++) x[u] = r[u];
-        }
-    };
     // boundary captures inside group t0 (wave-uniform test), then the running sum
     auto captures = [&](const uint32_t (&x)[kGroup], uint32_t t0) {
         if constexpr (RAW) {
@@ -220,9 +209,11 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
                 uint32_t part = S;
 #pragma unroll
                 for (int u = 0; u < kGroup; u++) part += (u < n) ? x[u] : 0u;
-                if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
-                else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
-                else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                if (active) {
+                    if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
+                    else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
+                    else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                }
                 bi++;
                 nextb = bi < nbnd ? bnd[bi].x : INT_MAX;
             }
@@ -230,46 +221,134 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
             for (int u = 0; u < kGroup; u++) S += x[u];
         }
     };
-
-    const uint32_t nfull = n_in / kGroup * kGroup;
-    uint32_t rcur[kGroup];
-    if (RAW && avg != 1) {
-#pragma unroll
-        for (int u = 0; u < kGroup; u++) rcur[u] = 0u;
-    } else {
-        load_raw(rcur, 0);
-    }
-    for (uint32_t t0 = 0; t0 < nfull; t0 += kGroup) {
-        // next group's loads are issued before this group is processed
-        uint32_t rnext[kGroup];
-        if (RAW && avg != 1) {
-#pragma unroll
-            for (int u = 0; u < kGroup; u++) rnext[u] = 0u;
-        } else {
-            load_raw(rnext, t0 + kGroup);
-        }
-        uint32_t x[kGroup];
-        values(x, rcur, t0);
+    auto group = [&](uint32_t (&x)[kGroup], uint32_t t0) {
         captures(x, t0);
         double v[kGroup];
 #pragma unroll
         for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
         push_block<DPL, K, RAW, false, 0, kGroup>(s, v, t0, n_in, o);
-#pragma unroll
-        for (int u = 0; u < kGroup; u++) rcur[u] = rnext[u];
-    }
-    if (nfull < n_in) {
-        uint32_t x[kGroup];
-        values(x, rcur, nfull);
+    };
+    auto tail_group = [&](uint32_t (&x)[kGroup], uint32_t t0) {
 #pragma unroll
         for (int u = 0; u < kGroup; u++)
-            if (nfull + (uint32_t)u >= n_in) x[u] = 0u;
-        captures(x, nfull);
+            if (t0 + (uint32_t)u >= n_in) x[u] = 0u;
+        captures(x, t0);
         double v[kGroup];
 #pragma unroll
         for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
-        push_block<DPL, K, RAW, true, 0, kGroup>(s, v, nfull, n_in, o);
+        push_block<DPL, K, RAW, true, 0, kGroup>(s, v, t0, n_in, o);
+    };
+    const uint32_t nfull = n_in / kGroup * kGroup;
+
+    if constexpr (TILES) {
+        // lane = 8 * f + j loads frame t0 + 8 * i + f, pixels 8 j .. 8 j + 7 of
+        // the wave, for i = 0, 1; in LDS the tile is [16 frames][64 pixels]
+        const uint32_t lane = threadIdx.x & 63u;
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        const uint8_t* wsrc = imm + tile_off0 + 2 * (int64_t)(gid - lane) + 16 * (lane & 7u);
+        auto issue = [&](u32x4 (&r)[2], uint32_t t0) {
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint32_t t = min(t0 + 8u * i + (lane >> 3), n_in - 1);
+                r[i] = *reinterpret_cast<const u32x4*>(wsrc + (int64_t)t * tile_step);
+            }
+        };
+        auto stage = [&](const u32x4 (&r)[2], uint32_t (&x)[kGroup]) {
+            s_tile[w][0][lane] = r[0];
+            s_tile[w][1][lane] = r[1];
+            wave_sync();
+            const uint16_t* t16 = reinterpret_cast<const uint16_t*>(&s_tile[w][0][0]);
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) x[u] = dense_value(t16[u * 64 + lane], nscale, nds, th);
+            wave_sync();
+        };
+        u32x4 ring[kTileDepth][2];
+#pragma unroll
+        for (int d = 0; d < kTileDepth; d++) issue(ring[d], (uint32_t)(d * kGroup));
+        for (uint32_t g0 = 0; g0 < nfull; g0 += kTileDepth * kGroup) {
+#pragma unroll
+            for (int d = 0; d < kTileDepth; d++) {
+                const uint32_t t0 = g0 + (uint32_t)(d * kGroup);
+                if (t0 < nfull) {
+                    uint32_t x[kGroup];
+                    stage(ring[d], x);
+                    issue(ring[d], t0 + kTileDepth * kGroup);
+                    group(x, t0);
+                }
+            }
+        }
+        if (nfull < n_in) {
+            u32x4 r[2];
+            issue(r, nfull);
+            uint32_t x[kGroup];
+            stage(r, x);
+            tail_group(x, nfull);
+        }
+    } else {
+        const uint16_t* col16 = reinterpret_cast<const uint16_t*>(imm) + gid;
+        const uint32_t* col32 = RAW ? nullptr : series_in + gid;
+        // loads of group t0 (frame indices clamped into range; the tail zeroes them)
+        auto load = [&](uint32_t (&r)[kGroup], uint32_t t0) {
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) {
+                const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
+                if constexpr (RAW) r[u] = *reinterpret_cast<const uint16_t*>(
+                                              reinterpret_cast<const uint8_t*>(col16) + sel_off[t]);
+                else r[u] = col32[(int64_t)t * npix];
+            }
+        };
+        auto values = [&](uint32_t (&x)[kGroup], const uint32_t (&r)[kGroup], uint32_t t0) {
+            if constexpr (RAW) {
+                if (avg == 1) {
+#pragma unroll
+                    for (int u = 0; u < kGroup; u++) x[u] = dense_value(r[u], nscale, nds, th);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kGroup; u++) {
+                        const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
+                        uint32_t a_sum = 0;
+                        for (int a = 0; a < avg; a++) {
+                            const uint32_t raw = *reinterpret_cast<const uint16_t*>(
+                                reinterpret_cast<const uint8_t*>(col16) + sel_off[(int64_t)t * avg + a]);
+                            a_sum += dense_value(raw, nscale, nds, th);
+                        }
+                        x[u] = a_sum;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kGroup; u++) x[u] = r[u];
+            }
+        };
+        uint32_t rcur[kGroup];
+        if (RAW && avg != 1) {
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) rcur[u] = 0u;
+        } else {
+            load(rcur, 0);
+        }
+        for (uint32_t t0 = 0; t0 < nfull; t0 += kGroup) {
+            // next group's loads are issued before this group is processed
+            uint32_t rnext[kGroup];
+            if (RAW && avg != 1) {
+#pragma unroll
+                for (int u = 0; u < kGroup; u++) rnext[u] = 0u;
+            } else {
+                load(rnext, t0 + kGroup);
+            }
+            uint32_t x[kGroup];
+            values(x, rcur, t0);
+            group(x, t0);
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) rcur[u] = rnext[u];
+        }
+        if (nfull < n_in) {
+            uint32_t x[kGroup];
+            values(x, rcur, nfull);
+            tail_group(x, nfull);
+        }
     }
+    if (!active) return;
     if constexpr (RAW) {
         pix_total[c] = S;
         // boundaries at frame 0 cover no frames
@@ -427,10 +506,9 @@ size_t dense_series_elems(const HostTables& h)
     return h.levels.size() > K ? (size_t)h.levels[K].lf * (size_t)h.W * (size_t)h.H : 1;
 }
 
-template <int DPL>
+template <int DPL, int K>
 static void run_passes(xpcs_mt_plan* p, hipStream_t s)
 {
-    constexpr int K = dense_levels_per_pass<DPL>();
     const HostTables& h = p->h;
     const int nlev = (int)h.levels.size();
     const uint32_t npd = (uint32_t)(h.W * h.H);
@@ -447,10 +525,16 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
         const uint32_t* in = pass == 0 ? nullptr : (pass % 2 == 1 ? p->d_series_a : p->d_series_b);
         if (pass == 0) {
             timer_begin(p, s, "k_mt_dense_raw");
-            k_mt_dense_f64<DPL, K, true><<<grid, 256, 0, s>>>(
-                p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
-                nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2, p->d_IP, p->d_IF,
-                p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
+            if (p->tile_off0 >= 0)
+                k_mt_dense_f64<DPL, K, kRawTiles><<<grid, 256, 0, s>>>(
+                    p->d_imm, p->d_sel_off, p->tile_off0, p->tile_step, h.avg, p->d_pix2c, npd, p->d_dark_sum,
+                    p->d_thr, (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T,
+                    p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
+            else
+                k_mt_dense_f64<DPL, K, kRawLanes><<<grid, 256, 0, s>>>(
+                    p->d_imm, p->d_sel_off, 0, 0, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr,
+                    (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2,
+                    p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
             timer_end(p, s);
             timer_begin(p, s, "k_mt_dense_exact");
             k_mt_dense_exact<DPL, K, true><<<256, 256, 0, s>>>(
@@ -459,8 +543,8 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
             timer_end(p, s);
         } else {
             timer_begin(p, s, "k_mt_dense_series");
-            k_mt_dense_f64<DPL, K, false><<<grid, 256, 0, s>>>(
-                nullptr, nullptr, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k,
+            k_mt_dense_f64<DPL, K, kSeries><<<grid, 256, 0, s>>>(
+                nullptr, nullptr, 0, 0, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k,
                 nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr);
             timer_end(p, s);
             timer_begin(p, s, "k_mt_dense_exact");
@@ -477,8 +561,8 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
 
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s)
 {
-    if (p->h.dpl == 4) run_passes<4>(p, s);
-    else run_passes<8>(p, s);
+    if (p->h.dpl == 4) run_passes<4, dense_levels_per_pass<4>()>(p, s);
+    else run_passes<8, dense_levels_per_pass<8>()>(p, s);
 }
 
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s)

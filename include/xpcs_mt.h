@@ -227,6 +227,61 @@ xpcs_status xpcs_mt_kernel_times(xpcs_mt_plan* plan, char* names, size_t cap, do
                                  int64_t* launches, int32_t max_kernels, int32_t* n_kernels);
 xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* plan);
 
+/*
+ * Launch shapes of a plan (no reference counterpart: the reference's only knob
+ * is the OpenMP schedule of multiTau2, src/xpcs/corr.cpp:329).  0 = automatic.
+ *   tr_bucket_shift  pixels per transpose bucket = 2^shift (8..12)
+ *   tr_chunk_frames  raw frames per transpose workgroup (>= 1)
+ *   reg_blocks       workgroups of the register-resident row kernel (1..65535)
+ *   lds_blocks       workgroups of the LDS row kernel (1..65535)
+ *   dense_levels     levels per pass of the dense streaming kernel, i.e. the
+ *                    depth of its register-resident lag window
+ *                    (dpl 4: 3, 4, 5; dpl 8: 2, 3)
+ */
+typedef struct xpcs_mt_tuning {
+    int32_t tr_bucket_shift;
+    int32_t tr_chunk_frames;
+    int32_t reg_blocks;
+    int32_t lds_blocks;
+    int32_t dense_levels;
+    int32_t reserved[3];
+} xpcs_mt_tuning;
+
+xpcs_status xpcs_mt_set_tuning(xpcs_mt_plan* plan, const xpcs_mt_tuning* tuning);
+/* The shapes the next execute uses (automatic values resolved). */
+xpcs_status xpcs_mt_get_tuning(xpcs_mt_plan* plan, xpcs_mt_tuning* tuning);
+
+/*
+ * Autotuner keyed on occupancy: the plan's key is (ingest mode, dpl,
+ * log2 of events per pixel and frame in half octaves, log2 pixels, log2
+ * frames).  A key seen before in this process takes the cached shapes;
+ * otherwise candidate shapes are timed on the plan's own data (coordinate
+ * descent, HIP events, best of 2 runs each) and the fastest is cached.  The
+ * chosen shapes are set on the plan and written to *best (may be NULL).
+ * Runs jobs on the plan's internal stream; results of the last run are valid.
+ */
+xpcs_status xpcs_mt_autotune(xpcs_mt_plan* plan, xpcs_mt_tuning* best);
+
+/*
+ * Batched execution (no reference counterpart; the reference runs one job per
+ * process).  Plans already ingested are added to a batch; the first execute
+ * (re)captures every job into one hipGraph -- the jobs spread over `branches`
+ * parallel graph branches, longest first -- and every execute is one graph
+ * launch on `stream`.  Re-ingesting or re-tuning a plan makes the next
+ * execute recapture.  Results are read from each plan as after
+ * xpcs_mt_execute.  flags: XPCS_BATCH_AUTOTUNE tunes every plan before the
+ * capture.  Plans must outlive the batch and not be executed on their own
+ * while a batch launch is in flight.
+ */
+typedef struct xpcs_mt_batch xpcs_mt_batch;
+#define XPCS_BATCH_AUTOTUNE 0x1u
+
+xpcs_status xpcs_mt_batch_create(int32_t branches, xpcs_mt_batch** out);
+xpcs_status xpcs_mt_batch_add(xpcs_mt_batch* batch, xpcs_mt_plan* plan, int32_t* index);
+xpcs_status xpcs_mt_batch_execute(xpcs_mt_batch* batch, void* stream, uint32_t flags);
+xpcs_status xpcs_mt_batch_synchronize(xpcs_mt_batch* batch);
+xpcs_status xpcs_mt_batch_destroy(xpcs_mt_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,7 +59,7 @@ static void dfree(T*& p)
 
 void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name)
 {
-    if (!(p->h.flags & XPCS_MT_TIMING)) return;
+    if (!(p->h.flags & XPCS_MT_TIMING) || p->capturing) return;
     KernelTimer t;
     t.name = name;
     if (p->event_pool.size() >= 2) {
@@ -75,7 +75,7 @@ void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name)
 
 void timer_end(xpcs_mt_plan* p, hipStream_t s)
 {
-    if (!(p->h.flags & XPCS_MT_TIMING)) return;
+    if (!(p->h.flags & XPCS_MT_TIMING) || p->capturing) return;
     (void)hipEventRecord(p->pending.back().b, s);
 }
 
@@ -467,6 +467,7 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
     if (st) return st;
     p->mode = comp ? 1 : 2;
     p->executed = false;
+    p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
     return XPCS_OK;
 }
@@ -516,6 +517,7 @@ extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* de
     if (st) return st;
     p->mode = compressed ? 1 : 2;
     p->executed = false;
+    p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
     return XPCS_OK;
 }
@@ -551,17 +553,28 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     p->nscale = 1;
     p->mode = 3;
     p->executed = false;
+    p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
     return XPCS_OK;
 }
 
-extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
+namespace xpcs {
+
+// Dense frames go through the thread-per-pixel streaming kernels when level
+// values and pixel totals fit 32 bits and n-scaled raw values fit signed 32 bits.
+static bool dense_stream_ok(const xpcs_mt_plan* p)
 {
-    if (!p) return fail(XPCS_ERR_INVALID_ARG, "execute: NULL plan");
+    const HostTables& h = p->h;
+    return dense_stream_supported(h) &&
+           (double)h.F * 65535.0 * (double)p->nscale * (double)h.avg < 4294967295.0 &&
+           65535.0 * (double)p->nscale < 2147483647.0;
+}
+
+// Device buffers that depend on the ingested data and the tuning; no launches.
+xpcs_status prepare_execute(xpcs_mt_plan* p)
+{
     if (p->mode == 0) return fail(XPCS_ERR_STATE, "execute: nothing ingested");
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
     HostTables& h = p->h;
-    HIP_TRY(hipMemsetAsync(p->d_counters, 0, 16, s));
     if (p->mode == 1) {
         if (p->raw_events > p->ev_cap) {
             xpcs_status st = dalloc(&p->d_ev, (size_t)p->raw_events);
@@ -569,7 +582,7 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             if (st) return st;
             p->ev_cap = p->raw_events;
         }
-        p->tshape = transpose_shape(h.P, p->nsel);
+        p->tshape = transpose_shape(h.P, p->nsel, p->tune.tr_bucket_shift, p->tune.tr_chunk_frames);
         const int64_t nh = (int64_t)p->tshape.nb * p->tshape.nblk;
         if (nh + 1 > p->hist_cap) {
             xpcs_status st = dalloc(&p->d_hist, (size_t)nh);
@@ -583,31 +596,14 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             if (st) return st;
             p->bsum_cap = nbs;
         }
-        launch_transpose(p, s);
-        if (p->d_fs) {
-            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
-            launch_side(p, s);
-        }
-    }
-    // streaming path: level values and pixel totals in 32 bits, n-scaled raw
-    // values in signed 32 bits
-    const bool stream_ok = dense_stream_supported(h) &&
-                           (double)h.F * 65535.0 * (double)p->nscale * (double)h.avg < 4294967295.0 &&
-                           65535.0 * (double)p->nscale < 2147483647.0;
-    if (p->mode == 2 && stream_ok) {
-        // thread-per-pixel streaming multi-tau over the raw frames
-        const size_t need = dense_series_elems(h);
+    } else if (p->mode == 2 && dense_stream_ok(p)) {
+        const size_t need = dense_series_elems(p);
         if (need > p->series_cap) {
             xpcs_status st = dalloc(&p->d_series_a, need);
             if (!st) st = dalloc(&p->d_series_b, need);
             if (st) return st;
             p->series_cap = need;
         }
-        if (p->d_fs) {
-            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
-            launch_side_dense_raw(p, s);
-        }
-        launch_mt_dense(p, s);
     } else if (p->mode == 2) {
         const size_t need = (size_t)h.P * (size_t)h.F;
         if (need > p->rows_cap) {
@@ -615,17 +611,47 @@ extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
             if (st) return st;
             p->rows_cap = need;
         }
+    }
+    return XPCS_OK;
+}
+
+// Every launch of one job on stream s: no allocation, no host synchronisation
+// and no memset nodes (counters and side sums are cleared by a fill kernel),
+// so the sequence can be captured into a graph.
+void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
+{
+    HostTables& h = p->h;
+    launch_fill_u32(p->d_counters, 4, s);
+    if (p->d_fs) launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
+    if (p->mode == 1) {
+        launch_transpose(p, s);
+        if (p->d_fs) launch_side(p, s);
+        launch_mt_sparse(p, s);
+        launch_mt_long(p, s, false);
+    } else if (p->mode == 2 && dense_stream_ok(p)) {
+        // thread-per-pixel streaming multi-tau over the raw frames
+        if (p->d_fs) launch_side_dense_raw(p, s);
+        launch_mt_dense(p, s);
+    } else if (p->mode == 2) {
         launch_dense_transpose(p, s);
-        if (p->d_fs) {
-            HIP_TRY(hipMemsetAsync(p->d_fs, 0, sizeof(double) * (size_t)h.F * h.ns, s));
-            launch_side(p, s);
-        }
+        if (p->d_fs) launch_side(p, s);
         launch_mt_long(p, s, true);
     } else {
         launch_mt_sparse(p, s);
         launch_mt_long(p, s, false);
     }
     launch_normalize(p, s);
+}
+
+}  // namespace xpcs
+
+extern "C" xpcs_status xpcs_mt_execute(xpcs_mt_plan* p, void* stream)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "execute: NULL plan");
+    xpcs_status st = prepare_execute(p);
+    if (st) return st;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    enqueue_execute(p, s);
     HIP_TRY(hipGetLastError());
     p->stream_last = s;
     p->executed = true;

@@ -116,6 +116,11 @@ struct xpcs_mt_plan {
     // dense frames evenly spaced in d_imm (payload of frame t at tile_off0 +
     // t * tile_step, 16-byte aligned, readable up to the next 64-pixel
     // boundary): the streaming kernel loads them as tiles; -1 otherwise
+    // launch shapes (0 = automatic) and a counter bumped whenever the data or
+    // the shapes change (batches recapture their graph then)
+    xpcs_mt_tuning tune{};
+    uint64_t generation = 0;
+    bool capturing = false;              // inside a graph capture: no timer events
     int64_t tile_off0 = -1;
     int64_t tile_step = 0;
     int64_t nsel = 0;                    // selected raw frames
@@ -189,14 +194,22 @@ void set_error(const char* fmt, ...);
 xpcs_status fail(xpcs_status st, const char* fmt, ...);
 
 // kernel launchers (kernels_*.hip); all asynchronous on `s`
-TransposeShape transpose_shape(int64_t P, int64_t nsel);
+TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames);
 void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_side(xpcs_mt_plan* p, hipStream_t s);
 bool dense_stream_supported(const HostTables& h);
-size_t dense_series_elems(const HostTables& h);
+size_t dense_series_elems(const xpcs_mt_plan* p);
+int dense_levels(const xpcs_mt_plan* p);
+bool dense_levels_allowed(int dpl, int k);
+int reg_blocks(const xpcs_mt_plan* p);
+int lds_blocks(const xpcs_mt_plan* p);
+void launch_fill_u32(uint32_t* d, size_t n, hipStream_t s);
+void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
+xpcs_status prepare_execute(xpcs_mt_plan* p);
+void enqueue_execute(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s);

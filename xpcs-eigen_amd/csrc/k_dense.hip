@@ -500,9 +500,22 @@ bool dense_stream_supported(const HostTables& h)
     return h.dpl == 4 || h.dpl == 8;
 }
 
-size_t dense_series_elems(const HostTables& h)
+bool dense_levels_allowed(int dpl, int k)
 {
-    const size_t K = h.dpl == 4 ? dense_levels_per_pass<4>() : dense_levels_per_pass<8>();
+    return dpl == 4 ? (k >= 3 && k <= 5) : (dpl == 8 && (k == 2 || k == 3));
+}
+
+int dense_levels(const xpcs_mt_plan* p)
+{
+    const int k = p->tune.dense_levels;
+    if (k > 0 && dense_levels_allowed(p->h.dpl, k)) return k;
+    return p->h.dpl == 4 ? dense_levels_per_pass<4>() : dense_levels_per_pass<8>();
+}
+
+size_t dense_series_elems(const xpcs_mt_plan* p)
+{
+    const HostTables& h = p->h;
+    const size_t K = (size_t)dense_levels(p);
     return h.levels.size() > K ? (size_t)h.levels[K].lf * (size_t)h.W * (size_t)h.H : 1;
 }
 
@@ -561,8 +574,15 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
 
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s)
 {
-    if (p->h.dpl == 4) run_passes<4, dense_levels_per_pass<4>()>(p, s);
-    else run_passes<8, dense_levels_per_pass<8>()>(p, s);
+    const int k = dense_levels(p);
+    if (p->h.dpl == 4) {
+        if (k == 3) run_passes<4, 3>(p, s);
+        else if (k == 5) run_passes<4, 5>(p, s);
+        else run_passes<4, 4>(p, s);
+    } else {
+        if (k == 2) run_passes<8, 2>(p, s);
+        else run_passes<8, 3>(p, s);
+    }
 }
 
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s)

@@ -539,15 +539,13 @@ constexpr int kRegWaves = 8;
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
 {
     const HostTables& h = p->h;
-    int64_t blocks = ((int64_t)h.P + kRegWaves - 1) / kRegWaves;
-    if (blocks > 16384) blocks = 16384;
     timer_begin(p, s, "k_mt_reg");
-    k_mt_reg<kRegWaves><<<(int)blocks, kWave * kRegWaves, 0, s>>>(
+    k_mt_reg<kRegWaves><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
         p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
-    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<1024, kWave * kSparseWaves, 0, s>>>(
+    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list, p->d_counters, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total);
     timer_end(p, s);
@@ -566,5 +564,19 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 }
 
 int sparse_cap() { return kSparseCapUsed; }
+
+// Workgroups of the row kernels: one row per wave up to 16384 workgroups by
+// default (both kernels loop over their rows), or the tuned count.
+int reg_blocks(const xpcs_mt_plan* p)
+{
+    if (p->tune.reg_blocks > 0) return p->tune.reg_blocks;
+    const int64_t b = ((int64_t)p->h.P + kRegWaves - 1) / kRegWaves;
+    return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+int lds_blocks(const xpcs_mt_plan* p)
+{
+    return p->tune.lds_blocks > 0 ? p->tune.lds_blocks : 1024;
+}
 
 }  // namespace xpcs

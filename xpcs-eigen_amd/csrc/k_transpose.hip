@@ -410,13 +410,18 @@ static void scan_u32(const uint32_t* in, int64_t n, int64_t* out, int64_t* bsum,
     k_scan_total<<<1, 64, 0, s>>>(in, n, out);
 }
 
-TransposeShape transpose_shape(int64_t P, int64_t nsel)
+// Buckets of 2^bshift pixels (at least 256: k_tr_rows scans a bucket with 256
+// threads; at most 4096 buckets: the per-chunk LDS histogram), cf raw frames
+// per chunk workgroup (default: at most 1024 chunks).  bucket_shift and
+// chunk_frames override the defaults when non-zero.
+TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames)
 {
     TransposeShape t;
-    t.bshift = 8;
+    t.bshift = bucket_shift > 8 ? bucket_shift : 8;
     while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 4096) t.bshift++;
     t.nb = (int32_t)((P + (int64_t(1) << t.bshift) - 1) >> t.bshift);
     t.cf = nsel > 1024 ? (nsel + 1023) / 1024 : 1;
+    if (chunk_frames > 0) t.cf = chunk_frames;
     t.nblk = (int32_t)((nsel + t.cf - 1) / t.cf);
     if (t.nblk < 1) t.nblk = 1;
     return t;

@@ -137,30 +137,83 @@ def ring_tau(dqmap: np.ndarray, lo: float, hi: float) -> np.ndarray:
 # Device-side generator for the large workloads (torch, any device).  Same
 # model, different random stream from the numpy generator.
 # ---------------------------------------------------------------------------
-def ou_events_torch(width, height, frames, mu, tau_c, seed, device="cuda", chunk=64):
-    """Yield (t0, counts[chunk, npix] uint8/int16 tensor) blocks on `device`."""
+def _headers(compression, rows, cols, dlens, ts, exposure=1.0e-3):
+    """IMM headers of many frames at once: uint8 [n, 1024] (fields as _header)."""
+    ts = np.asarray(ts, np.int64)
+    n = ts.size
+    h = np.zeros((n, HEADER_BYTES), dtype=np.uint8)
+
+    def put(off, vals, dt):
+        a = np.empty((n, 1), dtype=dt)
+        a[:, 0] = vals
+        h[:, off:off + a.itemsize] = a.view(np.uint8).reshape(n, a.itemsize)
+    put(0, 2 if compression else 0, np.int32)
+    put(4, compression, np.int32)
+    put(108, rows, np.int32)
+    put(112, cols, np.int32)
+    put(116, 2, np.int32)
+    put(128, ts * exposure, np.float64)
+    put(136, exposure, np.float64)
+    put(152, np.asarray(dlens, np.int64), np.uint32)
+    put(156, 1, np.int32)
+    put(160, ts, np.int32)
+    put(616, 12, np.int32)
+    put(620, ts, np.int32)
+    return h
+
+
+def ou_events_torch(width, height, frames, mu, tau_c, seed, device="cuda", chunk=256):
+    """Yield (t0, counts[n, npix] int32 tensor) blocks of up to `chunk` frames.
+
+    Within a block the OU recursion is one lower-triangular matrix product per
+    distinct correlation time: E[t0+k] = a^(k+1) E[t0-1] + sum_j a^(k-j) c_j xi[t0+j]
+    with c_j = sqrt(1-a^2) (c = 1 for the very first frame, so E[0] = xi[0])."""
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     npix = width * height
-    tau = torch.as_tensor(np.broadcast_to(np.asarray(tau_c, np.float64), (npix,)).copy(),
-                          dtype=torch.float32, device=device)
-    a = torch.exp(-1.0 / tau)
-    b = torch.sqrt(1.0 - a * a)
+    tau = np.broadcast_to(np.asarray(tau_c, np.float64), (npix,))
+    uniq, inv = np.unique(tau, return_inverse=True)
+    groups = []
+    k = torch.arange(chunk, device=device, dtype=torch.float64)
+    for gi, tc in enumerate(uniq):
+        a = float(np.exp(-1.0 / tc))
+        b = float(np.sqrt(1.0 - a * a))
+        d = (k[:, None] - k[None, :])
+        M = torch.where(d >= 0, torch.pow(torch.tensor(a, device=device, dtype=torch.float64), d.clamp(min=0)),
+                        torch.zeros((), device=device, dtype=torch.float64)).to(torch.float32)
+        carry = torch.pow(torch.tensor(a, device=device, dtype=torch.float64), k + 1).to(torch.float32)
+        idx = None if uniq.size == 1 else torch.as_tensor(np.flatnonzero(inv == gi), device=device)
+        groups.append((M, carry, b, idx))
     s = float(np.sqrt(0.5))
-    er = torch.randn(npix, generator=g, device=device) * s
-    ei = torch.randn(npix, generator=g, device=device) * s
+    er = torch.zeros(npix, device=device)
+    ei = torch.zeros(npix, device=device)
     t = 0
     while t < frames:
         n = min(chunk, frames - t)
-        out = torch.empty((n, npix), dtype=torch.int32, device=device)
-        for k in range(n):
-            if t + k > 0:
-                er = a * er + b * (torch.randn(npix, generator=g, device=device) * s)
-                ei = a * ei + b * (torch.randn(npix, generator=g, device=device) * s)
-            lam = mu * (er * er + ei * ei)
-            out[k] = torch.poisson(lam, generator=g).clamp_(max=65535).to(torch.int32)
-        yield t, out
+        xr = torch.randn((n, npix), generator=g, device=device) * s
+        xi = torch.randn((n, npix), generator=g, device=device) * s
+        nr = torch.empty_like(xr)
+        ni = torch.empty_like(xi)
+        for M, carry, b, idx in groups:
+            cr = (xr if idx is None else xr[:, idx]) * b
+            ci = (xi if idx is None else xi[:, idx]) * b
+            if t == 0:
+                cr[0] /= b
+                ci[0] /= b
+            pr = er if idx is None else er[idx]
+            pi = ei if idx is None else ei[idx]
+            vr = M[:n, :n] @ cr + carry[:n, None] * pr[None, :]
+            vi = M[:n, :n] @ ci + carry[:n, None] * pi[None, :]
+            if idx is None:
+                nr, ni = vr, vi
+            else:
+                nr[:, idx] = vr
+                ni[:, idx] = vi
+        er, ei = nr[-1].clone(), ni[-1].clone()
+        lam = mu * (nr * nr + ni * ni)
+        counts = torch.poisson(lam, generator=g).clamp_(max=65535).to(torch.int32)
+        yield t, counts
         t += n
 
 
@@ -172,23 +225,35 @@ def imm_device_sparse(width, height, frames, mu, tau_c, seed, device="cuda"):
     elapsed = t*1e-3.
     """
     import torch
-    chunks = []
+    parts = []
     offs, dls = [], []
     pos = 0
     for t0, c in ou_events_torch(width, height, frames, mu, tau_c, seed, device):
-        for k in range(c.shape[0]):
-            t = t0 + k
-            idx = torch.nonzero(c[k], as_tuple=False).reshape(-1).to(torch.int32)
-            val = c[k][idx].to(torch.int16)
-            hdr = torch.from_numpy(_header(6, height, width, int(idx.numel()), t)).to(device)
-            chunks.append(hdr)
-            chunks.append(idx.view(torch.uint8))
-            chunks.append(val.view(torch.uint8))
-            n = int(idx.numel())
-            offs.append(pos + HEADER_BYTES)
-            dls.append(n)
-            pos += HEADER_BYTES + 6 * n
-    return torch.cat(chunks), np.array(offs, np.int64), np.array(dls, np.uint32)
+        n = c.shape[0]
+        nz = torch.nonzero(c, as_tuple=False)           # row-major: frame, then pixel
+        fr, px = nz[:, 0], nz[:, 1]
+        val = c[fr, px].to(torch.int16)
+        dl = torch.bincount(fr, minlength=n)
+        dl_h = dl.cpu().numpy().astype(np.int64)
+        rec = HEADER_BYTES + 6 * dl_h
+        start = np.concatenate([[0], np.cumsum(rec)[:-1]])          # frame record starts in the block
+        total = int(rec.sum())
+        buf = torch.empty(total, dtype=torch.uint8, device=device)
+        hdr = torch.from_numpy(_headers(6, height, width, dl_h, np.arange(t0, t0 + n))).to(device)
+        st = torch.as_tensor(start, device=device)
+        buf[(st[:, None] + torch.arange(HEADER_BYTES, device=device)[None, :]).reshape(-1)] = hdr.reshape(-1)
+        first = torch.cumsum(dl, 0) - dl                                 # first event of each frame
+        rank = torch.arange(fr.numel(), device=device) - first[fr]
+        ib = st[fr] + HEADER_BYTES + 4 * rank
+        buf[(ib[:, None] + torch.arange(4, device=device)[None, :]).reshape(-1)] = \
+            px.to(torch.int32).contiguous().view(torch.uint8)
+        vb = st[fr] + HEADER_BYTES + 4 * dl[fr] + 2 * rank
+        buf[(vb[:, None] + torch.arange(2, device=device)[None, :]).reshape(-1)] = val.contiguous().view(torch.uint8)
+        parts.append(buf)
+        offs.append(pos + start + HEADER_BYTES)
+        dls.append(dl_h)
+        pos += total
+    return torch.cat(parts), np.concatenate(offs).astype(np.int64), np.concatenate(dls).astype(np.uint32)
 
 
 def imm_device_dense(width, height, frames, mu, tau_c, seed, darks=2, dark_mean=2.0, dark_sd=0.5,
@@ -209,12 +274,11 @@ def imm_device_dense(width, height, frames, mu, tau_c, seed, darks=2, dark_mean=
     for t in range(darks):
         view[t, :HEADER_BYTES] = torch.from_numpy(_header(0, height, width, npix, t)).to(device)
         view[t, HEADER_BYTES:] = d16.to(torch.int16).view(torch.uint8)
-    t = darks
     for t0, c in ou_events_torch(width, height, frames, mu, tau_c, seed, device):
-        for k in range(c.shape[0]):
-            row = view[t]
-            row[:HEADER_BYTES] = torch.from_numpy(_header(0, height, width, npix, t)).to(device)
-            row[HEADER_BYTES:] = (c[k] + d16).clamp_(max=65535).to(torch.int32).to(torch.int16).view(torch.uint8)
-            t += 1
+        n = c.shape[0]
+        t = darks + t0
+        hdr = _headers(0, height, width, np.full(n, npix), np.arange(t, t + n))
+        view[t:t + n, :HEADER_BYTES] = torch.from_numpy(hdr).to(device)
+        view[t:t + n, HEADER_BYTES:] = (c + d16[None, :]).clamp_(max=65535).to(torch.int16).view(torch.uint8)
     offs = np.arange(total, dtype=np.int64) * rec + HEADER_BYTES
     return imm, offs, np.full(total, npix, np.uint32)

@@ -32,8 +32,11 @@ EXPORTS = [
     "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_execute", "xpcs_mt_synchronize",
     "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
     "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
-    "xpcs_mt_read_dark",
+    "xpcs_mt_read_dark", "xpcs_mt_set_tuning", "xpcs_mt_get_tuning", "xpcs_mt_autotune",
+    "xpcs_mt_batch_create", "xpcs_mt_batch_add", "xpcs_mt_batch_execute", "xpcs_mt_batch_synchronize",
+    "xpcs_mt_batch_destroy",
 ]
+XPCS_BATCH_AUTOTUNE = 0x1
 
 
 class XpcsError(RuntimeError):
@@ -95,6 +98,14 @@ def lib():
             "xpcs_mt_kernel_times_reset": ([vp], i32),
             "xpcs_build_qmap": ([i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp], i32),
             "xpcs_mt_read_dark": ([vp, vp, vp], i32),
+            "xpcs_mt_set_tuning": ([vp, vp], i32),
+            "xpcs_mt_get_tuning": ([vp, vp], i32),
+            "xpcs_mt_autotune": ([vp, vp], i32),
+            "xpcs_mt_batch_create": ([i32, vp], i32),
+            "xpcs_mt_batch_add": ([vp, vp, vp], i32),
+            "xpcs_mt_batch_execute": ([vp, vp, u32], i32),
+            "xpcs_mt_batch_synchronize": ([vp], i32),
+            "xpcs_mt_batch_destroy": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -156,6 +167,15 @@ def build_qmap(dqmap, sqmap):
                                 _p(g[3]), ng, _p(counts)))
     return dict(order=order[:P], n_mapped=nm, ngroups=ng, group_q=g[0][:ng], group_s=g[1][:ng],
                 group_begin=g[2][:ng], group_end=g[3][:ng], nq=int(counts[3]), ns=int(counts[4]))
+
+
+class Tuning(C.Structure):
+    """Launch shapes of a plan (include/xpcs_mt.h, 0 = automatic)."""
+    _fields_ = [("tr_bucket_shift", C.c_int32), ("tr_chunk_frames", C.c_int32), ("reg_blocks", C.c_int32),
+                ("lds_blocks", C.c_int32), ("dense_levels", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class Plan:
@@ -231,6 +251,22 @@ class Plan:
     def synchronize(self):
         check(lib().xpcs_mt_synchronize(self._h))
 
+    def set_tuning(self, **shapes):
+        t = Tuning()
+        for k, v in shapes.items():
+            setattr(t, k, int(v))
+        check(lib().xpcs_mt_set_tuning(self._h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().xpcs_mt_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self):
+        t = Tuning()
+        check(lib().xpcs_mt_autotune(self._h, C.byref(t)))
+        return t.as_dict()
+
     def sums(self):
         G2 = np.zeros((self.P, self.T), np.int64)
         IP = np.zeros((self.P, self.T), np.int64)
@@ -276,3 +312,42 @@ class Plan:
 
     def kernel_times_reset(self):
         check(lib().xpcs_mt_kernel_times_reset(self._h))
+
+
+class Batch:
+    """Many plans run as one captured hipGraph (xpcs_mt_batch_*)."""
+
+    def __init__(self, branches=8):
+        self._h = C.c_void_p()
+        self._plans = []
+        check(lib().xpcs_mt_batch_create(branches, C.byref(self._h)))
+
+    def add(self, plan):
+        i = C.c_int32(0)
+        check(lib().xpcs_mt_batch_add(self._h, plan._h, C.byref(i)))
+        self._plans.append(plan)
+        return i.value
+
+    def execute(self, stream=None, autotune=False):
+        check(lib().xpcs_mt_batch_execute(self._h, None if stream is None else C.c_void_p(stream),
+                                          XPCS_BATCH_AUTOTUNE if autotune else 0))
+
+    def synchronize(self):
+        check(lib().xpcs_mt_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            lib().xpcs_mt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -282,6 +282,40 @@ xpcs_status xpcs_mt_batch_execute(xpcs_mt_batch* batch, void* stream, uint32_t f
 xpcs_status xpcs_mt_batch_synchronize(xpcs_mt_batch* batch);
 xpcs_status xpcs_mt_batch_destroy(xpcs_mt_batch* batch);
 
+/*
+ * Streams (no reference counterpart: the reference correlates a finished IMM
+ * file, src/xpcs/main.cpp:222-268 then Corr::multiTau2 / Corr::normalizeG2s).
+ * Frames arrive in pieces; the per-pixel level state (last and first 2*dpl
+ * values of every level, 64-bit lag sums, pending pair values, totals) stays
+ * in HBM, and a snapshot after n frames gives g2/stderr and sums identical to
+ * a plan of n frames.  Compressed (sparse) IMM frames only; dpl 4 or 8; no
+ * stride/average/skip/dark frames/flags.  Each stream has its own HIP stream,
+ * so work of different streams overlaps on the device.
+ */
+typedef struct xpcs_mt_stream xpcs_mt_stream;
+
+/* Device bytes a stream of at most max_frames frames holds (admission). */
+xpcs_status xpcs_mt_stream_bytes(const xpcs_mt_desc* desc, int32_t max_frames, uint64_t* bytes);
+xpcs_status xpcs_mt_stream_open(const xpcs_mt_desc* desc, int32_t max_frames, xpcs_mt_stream** out);
+/* Whole compressed IMM frames in host memory (free again on return). */
+xpcs_status xpcs_mt_stream_push_imm(xpcs_mt_stream* stream, const void* imm, size_t nbytes);
+/* Frames already in device memory: payload offsets / event counts per frame
+ * (host arrays); the device bytes must stay valid until the next snapshot
+ * has finished. */
+xpcs_status xpcs_mt_stream_push_imm_device(xpcs_mt_stream* stream, const void* dev_imm, size_t nbytes,
+                                           const int64_t* payload_off, const uint32_t* dlen, int32_t nframes);
+/* Frames pushed so far and the taus of their schedule. */
+xpcs_status xpcs_mt_stream_frames(xpcs_mt_stream* stream, int32_t* frames, int32_t* taus);
+/* Enqueue g2 for the frames pushed so far (asynchronous). */
+xpcs_status xpcs_mt_stream_snapshot(xpcs_mt_stream* stream);
+/* Frames covered by the newest snapshot known to have finished (non-blocking). */
+xpcs_status xpcs_mt_stream_poll(xpcs_mt_stream* stream, int32_t* frames_done);
+/* Results of the newest snapshot (waits for it): g2/stderr [T_n][Q],
+ * sums [P][T_n] in the plan's pixel order. */
+xpcs_status xpcs_mt_stream_read_g2(xpcs_mt_stream* stream, double* g2, double* stderr_, int32_t* taus);
+xpcs_status xpcs_mt_stream_read_sums_i64(xpcs_mt_stream* stream, int64_t* G2, int64_t* IP, int64_t* IF);
+xpcs_status xpcs_mt_stream_close(xpcs_mt_stream* stream);
+
 #ifdef __cplusplus
 }
 #endif

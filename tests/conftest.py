@@ -15,6 +15,15 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def xmt():
+    # torch ships its own HIP runtime; tests that hand torch device memory to
+    # the library need torch's device initialised before the library's first
+    # HIP call (the order bench.py uses)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    except ImportError:
+        pass
     import xpcs_mt
     xpcs_mt.lib()
     return xpcs_mt

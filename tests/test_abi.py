@@ -34,3 +34,20 @@ def test_no_device_is_an_error_not_a_crash(xmt):
         xmt.Plan(4, 4, 10, 2, dq, dq)
     except xmt.XpcsError as e:
         assert e.status == 8
+
+
+def test_stream_bytes_and_unsupported_streams(xmt):
+    """Admission sizes are host-only arithmetic; streams refuse what they do not do."""
+    import numpy as np
+    import pytest
+    import synth
+    dq, sq = synth.annular_qmaps(64, 64, 6, 3)
+    P = int(np.count_nonzero((dq > 0) & (sq > 0)))
+    b4 = xmt.stream_bytes(64, 64, 5000, 4, dq, sq)
+    b8 = xmt.stream_bytes(64, 64, 5000, 8, dq, sq)
+    b8_long = xmt.stream_bytes(64, 64, 50000, 8, dq, sq)
+    # level state alone is P * levels * 2dpl * 16 bytes
+    assert b8 > b4 > P * 10 * 8 * 16
+    assert b8_long > b8
+    with pytest.raises(xmt.XpcsError, match="UNSUPPORTED"):
+        xmt.stream_bytes(64, 64, 5000, 2, dq, sq)

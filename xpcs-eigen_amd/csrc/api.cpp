@@ -180,13 +180,12 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
         }
         TRY(dalloc(&p->d_fs, (size_t)h.F * h.ns));
     }
-    // dense-row scratch: one row of F 64-bit values per resident workgroup
+    // dense-row scratch (allocated by the first execute that needs it): one row
+    // of F 64-bit values per resident workgroup
     {
         size_t per = (size_t)h.F * sizeof(uint64_t);
         size_t budget = (size_t)1 << 30;
-        int blocks = (int)std::min<size_t>(512, std::max<size_t>(1, budget / per));
-        p->long_blocks = blocks;
-        TRY(dalloc(&p->d_scratch, (size_t)blocks * h.F));
+        p->long_blocks = (int)std::min<size_t>(512, std::max<size_t>(1, budget / per));
     }
     if (hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -575,6 +574,11 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
 {
     if (p->mode == 0) return fail(XPCS_ERR_STATE, "execute: nothing ingested");
     HostTables& h = p->h;
+    const bool long_rows = p->mode != 2 || !dense_stream_ok(p);
+    if (long_rows && !p->d_scratch) {
+        xpcs_status st = dalloc(&p->d_scratch, (size_t)p->long_blocks * h.F);
+        if (st) return st;
+    }
     if (p->mode == 1) {
         if (p->raw_events > p->ev_cap) {
             xpcs_status st = dalloc(&p->d_ev, (size_t)p->raw_events);

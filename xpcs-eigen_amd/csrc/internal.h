@@ -209,6 +209,33 @@ int lds_blocks(const xpcs_mt_plan* p);
 void launch_fill_u32(uint32_t* d, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
 xpcs_status prepare_execute(xpcs_mt_plan* p);
+void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus);
+
+// online multi-tau (k_stream.hip)
+constexpr int kStreamChunk = 64;   // frames per update launch
+struct StreamLaunch {
+    const uint8_t* imm;            // device IMM bytes of the pushed frames
+    const int64_t* off;            // [nb] payload offsets
+    const uint32_t* dlen;          // [nb] events per frame
+    int32_t nb;                    // frames of this update (<= kStreamChunk)
+    const int32_t* pix2c;
+    uint32_t npix_det;
+    int32_t P;
+    uint32_t* tile;                // [kStreamChunk][P]
+    uint32_t* counters;
+    int32_t d2;                    // 2 * dpl (8 or 16)
+    int32_t n0;                    // frames before this update
+    int32_t nlev;                  // levels with state
+    uint32_t* win;
+    uint32_t* head;
+    uint64_t* acc;
+    uint32_t* pend;
+    uint64_t* total;
+};
+void launch_stream_update(const StreamLaunch& a, hipStream_t s);
+void launch_stream_snapshot(const StreamLaunch& a, int32_t n, const LevelParam* levels, int32_t nlev,
+                            const TauParam* taus_max, int32_t T, TauParam* taus, int64_t* G2, int64_t* IP,
+                            int64_t* IF, hipStream_t s);
 void enqueue_execute(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);

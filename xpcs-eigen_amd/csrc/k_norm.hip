@@ -172,25 +172,31 @@ k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const 
     oIF[o] = (float)((double)IF[c * T + i] * s / (lv * N));
 }
 
-void launch_normalize(xpcs_mt_plan* p, hipStream_t s)
+// Normalisation of the first T taus; sums are [P][T] rows, taus gives N per tau.
+void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus)
 {
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
     timer_begin(p, s, "k_norm");
-    if (nchunks > 0)
+    if (nchunks > 0 && T > 0)
         k_norm_partial<<<nchunks < 65536 ? nchunks : 65536, 128, 0, s>>>(
-            p->d_chunks, nchunks, p->d_taus, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_weight, p->d_part);
-    const int64_t ng = (int64_t)h.ngroups * h.T;
+            p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF, p->d_weight, p->d_part);
+    const int64_t ng = (int64_t)h.ngroups * T;
     if (ng > 0)
         k_norm_group<<<(int)((ng + 255) / 256), 256, 0, s>>>(p->d_part, p->d_group_chunk_first,
-                                                             p->d_group_size, h.ngroups, p->d_taus, h.T,
+                                                             p->d_group_size, h.ngroups, taus, T,
                                                              p->d_weight ? 1 : 0, p->d_g2s);
-    const int64_t nqt = (int64_t)h.nq * h.T;
+    const int64_t nqt = (int64_t)h.nq * T;
     if (nqt > 0)
         k_norm_q<<<(int)((nqt + 255) / 256), 256, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group,
-                                                          p->d_q_chunk_first, h.nq, h.T, p->d_g2,
+                                                          p->d_q_chunk_first, h.nq, T, p->d_g2,
                                                           p->d_se);
     timer_end(p, s);
+}
+
+void launch_normalize(xpcs_mt_plan* p, hipStream_t s)
+{
+    launch_normalize_with(p, s, p->h.T, p->d_taus);
 }
 
 void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF)

@@ -34,7 +34,9 @@ EXPORTS = [
     "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
     "xpcs_mt_read_dark", "xpcs_mt_set_tuning", "xpcs_mt_get_tuning", "xpcs_mt_autotune",
     "xpcs_mt_batch_create", "xpcs_mt_batch_add", "xpcs_mt_batch_execute", "xpcs_mt_batch_synchronize",
-    "xpcs_mt_batch_destroy",
+    "xpcs_mt_batch_destroy", "xpcs_mt_stream_bytes", "xpcs_mt_stream_open", "xpcs_mt_stream_push_imm",
+    "xpcs_mt_stream_push_imm_device", "xpcs_mt_stream_frames", "xpcs_mt_stream_snapshot",
+    "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
 ]
 XPCS_BATCH_AUTOTUNE = 0x1
 
@@ -106,6 +108,16 @@ def lib():
             "xpcs_mt_batch_execute": ([vp, vp, u32], i32),
             "xpcs_mt_batch_synchronize": ([vp], i32),
             "xpcs_mt_batch_destroy": ([vp], i32),
+            "xpcs_mt_stream_bytes": ([vp, i32, vp], i32),
+            "xpcs_mt_stream_open": ([vp, i32, vp], i32),
+            "xpcs_mt_stream_push_imm": ([vp, vp, C.c_size_t], i32),
+            "xpcs_mt_stream_push_imm_device": ([vp, vp, C.c_size_t, vp, vp, i32], i32),
+            "xpcs_mt_stream_frames": ([vp, vp, vp], i32),
+            "xpcs_mt_stream_snapshot": ([vp], i32),
+            "xpcs_mt_stream_poll": ([vp, vp], i32),
+            "xpcs_mt_stream_read_g2": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_stream_read_sums_i64": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_stream_close": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -351,3 +363,83 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+def _stream_desc(width, height, dpl, dq, sq, flatfield):
+    return Desc(width, height, 0, dpl, 1, 1, 0, 0, 0.0, 0.0, _p(dq), _p(sq), _p(flatfield), 0)
+
+
+def stream_bytes(width, height, max_frames, dpl, dqmap, sqmap, flatfield=None):
+    """Device bytes a stream of at most max_frames frames holds (host-only)."""
+    dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+    sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+    ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+    b = C.c_uint64(0)
+    check(lib().xpcs_mt_stream_bytes(C.byref(_stream_desc(width, height, dpl, dq, sq, ff)), max_frames,
+                                     C.byref(b)))
+    return b.value
+
+
+class Stream:
+    """Online multi-tau over frames pushed in pieces (xpcs_mt_stream_*)."""
+
+    def __init__(self, width, height, max_frames, dpl, dqmap, sqmap, flatfield=None):
+        self._dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+        self._sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+        self._ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
+        self._h = C.c_void_p()
+        check(lib().xpcs_mt_stream_open(C.byref(_stream_desc(width, height, dpl, self._dq, self._sq, self._ff)),
+                                        max_frames, C.byref(self._h)))
+        self.width, self.height, self.max_frames, self.dpl = width, height, max_frames, dpl
+        self.Q = int(self._dq.max())
+        self.P = int(np.count_nonzero((self._dq > 0) & (self._sq > 0)))
+
+    def push_imm(self, buf):
+        b = np.frombuffer(buf, np.uint8) if not isinstance(buf, np.ndarray) else np.ascontiguousarray(buf)
+        check(lib().xpcs_mt_stream_push_imm(self._h, _p(b), b.size))
+
+    def push_imm_device(self, ptr, nbytes, payload_off, dlen):
+        off = np.ascontiguousarray(payload_off, np.int64)
+        dl = np.ascontiguousarray(dlen, np.uint32)
+        check(lib().xpcs_mt_stream_push_imm_device(self._h, C.c_void_p(ptr), nbytes, _p(off), _p(dl), off.size))
+
+    def frames(self):
+        n, t = C.c_int32(0), C.c_int32(0)
+        check(lib().xpcs_mt_stream_frames(self._h, C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def snapshot(self):
+        check(lib().xpcs_mt_stream_snapshot(self._h))
+
+    def poll(self):
+        n = C.c_int32(0)
+        check(lib().xpcs_mt_stream_poll(self._h, C.byref(n)))
+        return n.value
+
+    def g2(self):
+        _, T = self.frames()
+        T = max(T, 1)
+        g2 = np.zeros((T, self.Q), np.float64)
+        se = np.zeros((T, self.Q), np.float64)
+        t = C.c_int32(0)
+        check(lib().xpcs_mt_stream_read_g2(self._h, _p(g2), _p(se), C.byref(t)))
+        return g2[: t.value], se[: t.value]
+
+    def sums(self):
+        _, T = self.frames()
+        G2 = np.zeros((self.P, T), np.int64)
+        IP = np.zeros((self.P, T), np.int64)
+        IF = np.zeros((self.P, T), np.int64)
+        check(lib().xpcs_mt_stream_read_sums_i64(self._h, _p(G2), _p(IP), _p(IF)))
+        return G2, IP, IF
+
+    def close(self):
+        if self._h:
+            lib().xpcs_mt_stream_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -182,6 +182,114 @@ def run_batched(dev, n_jobs, reps):
             "autotune_and_capture_s": tune_s, "distinct_tuned_shapes": len(tunes), "setup_s": gen_s}
 
 
+class DeviceFrames:
+    """Frames of one synthetic detector stream, already in device memory,
+    arriving at `fps` from `start` (wall clock of the serving loop)."""
+
+    def __init__(self, imm, off, dlen, total_frames, start, fps):
+        self.imm, self.off, self.dlen = imm, off, dlen
+        self.frames = total_frames          # the stream's full length (admission size)
+        self.generated = len(off)           # frames that exist for this run
+        self.start, self.fps = start, fps
+
+    def arrival(self, t):
+        return self.start + t / self.fps
+
+    def available(self, now):
+        if now < self.start:
+            return 0
+        return int(min(self.generated, np.floor((now - self.start) * self.fps) + 1))
+
+    def push(self, stream, a, b):
+        stream.push_imm_device(self.imm.data_ptr(), self.imm.numel(), self.off[a:b], self.dlen[a:b])
+
+
+def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
+    """Streaming service: 16 concurrent synthetic detector streams at 200 frames/s
+    (stream s: seed 5000 + s, square detector U[256, 1024], occupancy U[0.2 %, 3 %],
+    length U[2000, 50000] frames, OU correlation time log-U[1, 1000]), more
+    streams arriving as a Poisson process (mean one per 10 s) and queued until a
+    slot and HBM budget are free.  Runs `seconds` of wall time; frames beyond the
+    window are not generated.  Reports p50/p99 frame-arrival -> covering-g2
+    latency and, for comparison, the plain path's cost of recomputing each live
+    stream's g2 from scratch at its final frame count."""
+    import torch
+    import serve
+    import synth
+    import xpcs_mt
+    rng = np.random.default_rng(5000)
+    t_arr = [0.0] * n_streams
+    t = 0.0
+    while True:
+        t += float(rng.exponential(10.0))
+        if t >= seconds * 0.8:
+            break
+        t_arr.append(t)
+    specs = []
+    t0 = time.time()
+    for k, ta in enumerate(t_arr):
+        r = np.random.default_rng(5000 + k)
+        W = int(r.integers(256, 1025))
+        occ = float(r.uniform(0.002, 0.03))
+        length = int(r.integers(2000, 50001))
+        tau = float(np.exp(r.uniform(0.0, np.log(1000.0))))
+        dpl = int(r.choice([4, 8]))
+        Q = int(r.integers(2, 13))
+        n_gen = int(min(length, np.ceil((seconds - ta) * fps) + fps))
+        imm, off, dl = synth.imm_device_sparse(W, W, n_gen, occ / (1 - occ), tau, 5000 + k, device=dev)
+        dq, sq = synth.annular_qmaps(W, W, 3 * Q, Q)
+        specs.append(dict(W=W, dpl=dpl, dq=dq, sq=sq, imm=imm, off=off, dl=dl, length=length, start=ta))
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    svc = serve.StreamService(int(budget_gib * 2**30), max_active=n_streams)
+    start = time.perf_counter()
+    pending = list(range(len(specs)))
+    steps = 0
+    while True:
+        now = time.perf_counter() - start
+        while pending and specs[pending[0]]["start"] <= now:
+            k = pending.pop(0)
+            sp = specs[k]
+            svc.submit(k, sp["W"], sp["W"], sp["dpl"], sp["dq"], sp["sq"],
+                       DeviceFrames(sp["imm"], sp["off"], sp["dl"], sp["length"], sp["start"], fps), now=now)
+        svc.step(now)
+        steps += 1
+        if now >= seconds:
+            break
+    torch.cuda.synchronize()
+    pc = svc.percentiles()
+    live = [(k, e) for k, e in svc.active.items()]
+    # plain path: recompute g2 from scratch for each live stream at its frame count
+    plain_ms = []
+    for k, e in live:
+        sp = specs[k]
+        n = e.pushed
+        if n < 16:
+            continue
+        p = xpcs_mt.Plan(sp["W"], sp["W"], n, sp["dpl"], sp["dq"], sp["sq"])
+        p.ingest_imm_device(sp["imm"].data_ptr(), sp["imm"].numel(), sp["off"][:n], sp["dl"][:n], compressed=True)
+        p.execute()
+        p.synchronize()
+        t1 = time.perf_counter()
+        p.execute()
+        p.synchronize()
+        plain_ms.append(1e3 * (time.perf_counter() - t1))
+        p.close()
+    for k, e in live:
+        e.stream.close()
+    out = {"streams_submitted": len(specs), "streams_admitted": len(specs) - len(svc.queue),
+           "streams_finished": len(svc.results), "fps_per_stream": fps, "window_s": seconds,
+           "frames_covered": svc.frames_done, "frames_per_s": svc.frames_done / seconds,
+           "latency": pc, "budget_gib": budget_gib, "peak_state_gib": svc.peak_used / 2**30,
+           "loop_turns": steps, "admission_wait_s_max": max(svc.admit_wait) if svc.admit_wait else 0.0,
+           "plain_recompute_ms": {"p50": float(np.percentile(plain_ms, 50)), "max": float(max(plain_ms)),
+                                  "streams": len(plain_ms)} if plain_ms else None,
+           "setup_s": gen_s}
+    del specs
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +300,8 @@ def main():
                     help="frames of the CPU baseline sample (full detector)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
+    ap.add_argument("--stream-seconds", type=float, default=20.0,
+                    help="wall time of the streaming leg (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB elementwise copy (read 1 GiB, write 1 GiB) three times, "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -309,6 +419,7 @@ def main():
     plan.close()
     plan2 = None
     batched = run_batched(dev, args.batched_jobs, 3) if args.batched_jobs > 0 else None
+    streaming = run_streaming(dev, args.stream_seconds) if args.stream_seconds > 0 else None
 
     out = {
         "metric": "sparse 1Mpx frames/s + HBM GB/s; batched jobs/s vs plain; p99 stream latency",
@@ -330,6 +441,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "streaming": streaming,
         "kernels": kern,
         "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
         "end_to_end_pcie": e2e,

@@ -213,10 +213,13 @@ void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauP
 
 // online multi-tau (k_stream.hip)
 constexpr int kStreamChunk = 64;   // frames per update launch
+struct StreamFrames {             // passed by value: no upload per update
+    int64_t off[kStreamChunk];     // payload offsets
+    uint32_t dlen[kStreamChunk];   // events per frame
+};
 struct StreamLaunch {
     const uint8_t* imm;            // device IMM bytes of the pushed frames
-    const int64_t* off;            // [nb] payload offsets
-    const uint32_t* dlen;          // [nb] events per frame
+    StreamFrames fr;
     int32_t nb;                    // frames of this update (<= kStreamChunk)
     const int32_t* pix2c;
     uint32_t npix_det;

@@ -23,13 +23,13 @@ namespace xpcs {
 
 // Frames of a push: events into a dense [nb][P] value tile (summed per pixel).
 __global__ void __launch_bounds__(256)
-k_stream_scatter(const uint8_t* __restrict__ imm, const int64_t* __restrict__ off,
-                 const uint32_t* __restrict__ dlen, int32_t nb, const int32_t* __restrict__ pix2c,
-                 uint32_t npix_det, int32_t P, uint32_t* __restrict__ tile, uint32_t* __restrict__ counters)
+k_stream_scatter(const uint8_t* __restrict__ imm, const StreamFrames fr, int32_t nb,
+                 const int32_t* __restrict__ pix2c, uint32_t npix_det, int32_t P, uint32_t* __restrict__ tile,
+                 uint32_t* __restrict__ counters)
 {
     for (int f = blockIdx.x; f < nb; f += gridDim.x) {
-        const uint8_t* base = imm + off[f];
-        const uint32_t n = dlen[f];
+        const uint8_t* base = imm + fr.off[f];
+        const uint32_t n = fr.dlen[f];
         for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
             const uint32_t idx = load_u32_a2(base + 4 * (size_t)j);
             const uint32_t v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)n)[j];
@@ -153,8 +153,7 @@ void launch_stream_update(const StreamLaunch& a, hipStream_t s)
 {
     const unsigned gp = (unsigned)((a.P + 255) / 256);
     launch_fill_u32(a.tile, (size_t)a.nb * a.P, s);
-    k_stream_scatter<<<a.nb, 256, 0, s>>>(a.imm, a.off, a.dlen, a.nb, a.pix2c, a.npix_det, a.P, a.tile,
-                                          a.counters);
+    k_stream_scatter<<<a.nb, 256, 0, s>>>(a.imm, a.fr, a.nb, a.pix2c, a.npix_det, a.P, a.tile, a.counters);
     if (a.d2 == 8)
         k_stream_update<8><<<gp, 256, 0, s>>>(a.tile, a.nb, a.P, a.n0, a.nlev, a.win, a.head, a.acc, a.pend,
                                               a.total);

@@ -30,8 +30,6 @@ struct xpcs_mt_stream {
     uint32_t* pend = nullptr;
     uint64_t* total = nullptr;
     uint32_t* tile = nullptr;
-    int64_t* d_off = nullptr;    // [max_frames] payload offsets of the pushed frames
-    uint32_t* d_dlen = nullptr;
     uint8_t* d_imm = nullptr;    // copies of host-pushed frames
     size_t imm_cap = 0;
     uint8_t* h_imm = nullptr;    // pinned staging of host-pushed frames
@@ -87,16 +85,18 @@ StreamLaunch launch_args(xpcs_mt_stream* s)
     return a;
 }
 
-// Frames [n, n + nf) whose payloads are at imm + d_off[n ..]: updates of at
-// most kStreamChunk frames each.
-void enqueue_updates(xpcs_mt_stream* s, const uint8_t* imm, int32_t nf)
+// Frames [n, n + nf) with payloads at imm + off[i]: updates of at most
+// kStreamChunk frames each, their frame tables passed as kernel arguments.
+void enqueue_updates(xpcs_mt_stream* s, const uint8_t* imm, const int64_t* off, const uint32_t* dlen, int32_t nf)
 {
     StreamLaunch a = launch_args(s);
     a.imm = imm;
     for (int32_t k = 0; k < nf; k += kStreamChunk) {
         a.nb = std::min(kStreamChunk, nf - k);
-        a.off = s->d_off + s->n;
-        a.dlen = s->d_dlen + s->n;
+        for (int32_t i = 0; i < a.nb; i++) {
+            a.fr.off[i] = off[k + i];
+            a.fr.dlen[i] = dlen[k + i];
+        }
         a.n0 = s->n;
         launch_stream_update(a, s->plan->stream);
         s->n += a.nb;
@@ -118,7 +118,7 @@ extern "C" xpcs_status xpcs_mt_stream_bytes(const xpcs_mt_desc* desc, int32_t ma
     const size_t P = (size_t)h.P, T = (size_t)h.T;
     size_t b = state_bytes(h.P, (int32_t)h.levels.size(), 2 * h.dpl);
     b += P * T * 24 + P * 8 + h.chunks.size() * T * sizeof(NormPartial) + (size_t)h.ngroups * T * 8 +
-         2 * (size_t)h.nq * T * 8 + (size_t)h.W * h.H * 4 + P * 24 + (size_t)max_frames * 12;
+         2 * (size_t)h.nq * T * 8 + (size_t)h.W * h.H * 4 + P * 24;
     *bytes = b;
     return XPCS_OK;
 }
@@ -148,8 +148,6 @@ extern "C" xpcs_status xpcs_mt_stream_open(const xpcs_mt_desc* desc, int32_t max
     if (!st) st = salloc(&s->pend, L * P);
     if (!st) st = salloc(&s->total, P);
     if (!st) st = salloc(&s->tile, (size_t)kStreamChunk * P);
-    if (!st) st = salloc(&s->d_off, (size_t)max_frames);
-    if (!st) st = salloc(&s->d_dlen, (size_t)max_frames);
     if (!st) st = salloc(&s->d_taus_snap, (size_t)h.T);
     hipStream_t q = s->plan->stream;
     if (!st) {
@@ -185,11 +183,7 @@ extern "C" xpcs_status xpcs_mt_stream_push_imm_device(xpcs_mt_stream* s, const v
     for (int32_t f = 0; f < nframes; f++)
         if (payload_off[f] < 0 || (size_t)payload_off[f] + (size_t)dlen[f] * 6 > nbytes)
             return fail(XPCS_ERR_FORMAT, "stream_push: frame %d lies outside the buffer", f);
-    // fresh slots of the append-only frame table: a synchronous copy cannot
-    // race with updates in flight, and the caller's arrays are free on return
-    HIP_TRY(hipMemcpy(s->d_off + s->n, payload_off, sizeof(int64_t) * nframes, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->d_dlen + s->n, dlen, sizeof(uint32_t) * nframes, hipMemcpyHostToDevice));
-    enqueue_updates(s, static_cast<const uint8_t*>(dev_imm), nframes);
+    enqueue_updates(s, static_cast<const uint8_t*>(dev_imm), payload_off, dlen, nframes);
     HIP_TRY(hipGetLastError());
     return XPCS_OK;
 }
@@ -332,7 +326,7 @@ extern "C" xpcs_status xpcs_mt_stream_close(xpcs_mt_stream* s)
     if (!s) return XPCS_OK;
     if (s->plan && s->plan->stream) (void)hipStreamSynchronize(s->plan->stream);
     for (void* p : {(void*)s->win, (void*)s->head, (void*)s->acc, (void*)s->pend, (void*)s->total,
-                    (void*)s->tile, (void*)s->d_off, (void*)s->d_dlen, (void*)s->d_imm, (void*)s->d_taus_snap})
+                    (void*)s->tile, (void*)s->d_imm, (void*)s->d_taus_snap})
         if (p) (void)hipFree(p);
     if (s->h_imm) (void)hipHostFree(s->h_imm);
     if (s->done) (void)hipEventDestroy(s->done);

@@ -299,6 +299,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2000,
                     help="frames of the CPU baseline sample (full detector)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes for the main workload")
     ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
     ap.add_argument("--stream-seconds", type=float, default=20.0,
                     help="wall time of the streaming leg (0 = skip)")
@@ -346,6 +347,9 @@ def main():
     darks = wl.get("darks", 0)
     plan = xpcs_mt.Plan(W, H, F, dpl, dq, sq, dark_frames=darks, timing=True)
     plan.ingest_imm_device(imm.data_ptr(), nbytes, payload_off, dlen, compressed=not dense)
+    # launch shapes from the occupancy-keyed autotuner (untimed, as a serving
+    # process would after its first job of this shape)
+    tuned = plan.autotune() if not args.no_autotune else None
     for _ in range(args.warmup):
         plan.execute()
         plan.synchronize()
@@ -386,6 +390,7 @@ def main():
     # correctness spot check of this run (cheap invariants)
     g2, se = plan.g2()
     finite = float(np.isfinite(g2).mean())
+    plan_shapes = plan.get_tuning()
 
     # PCIe-inclusive end-to-end (host IMM bytes -> g2 on host), one pass
     host_imm = imm.cpu().numpy()
@@ -437,7 +442,8 @@ def main():
         "config": {"workload": f"{args.workload}: {W}x{H}, {F} frames, dpl={dpl}, mu={wl['mu']}, "
                                f"{wl['S']} static / {wl['Q']} dynamic rings",
                    "frames": F, "taus": T, "pixels": P, "events_level0": E0,
-                   "raw_events": int(dlen.sum()), "parallelism": "single GPU"},
+                   "raw_events": int(dlen.sum()), "parallelism": "single GPU",
+                   "launch_shapes": plan_shapes, "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

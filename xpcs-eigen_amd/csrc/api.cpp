@@ -154,7 +154,14 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     TRY(dalloc(&p->d_g2s, std::max<size_t>(1, (size_t)h.ngroups * h.T)));
     TRY(dalloc(&p->d_g2, std::max<size_t>(1, (size_t)h.nq * h.T)));
     TRY(dalloc(&p->d_se, std::max<size_t>(1, (size_t)h.nq * h.T)));
-    TRY(dalloc(&p->d_off, (size_t)h.P + 1));
+    TRY(dalloc(&p->d_off, (size_t)std::max<int64_t>(h.P, (int64_t)h.W * h.H) + 1));
+    {
+        const int64_t npd = (int64_t)h.W * h.H;
+        std::vector<uint32_t> bits((size_t)(npd / 32 + 1), 0u);
+        for (int64_t i = 0; i < npd; i++)
+            if (h.pix2c[(size_t)i] >= 0) bits[(size_t)(i >> 5)] |= 1u << (i & 31);
+        TRY(upload(&p->d_mapped, bits, s));
+    }
     TRY(dalloc(&p->d_long_list, (size_t)h.P));
     TRY(dalloc(&p->d_mid_list, (size_t)h.P));
     TRY(dalloc(&p->d_counters, 4));
@@ -201,7 +208,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
 {
     if (!p) return XPCS_OK;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    dfree(p->d_pix2c); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_chunks);
+    dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_chunks);
     dfree(p->d_group_chunk_first); dfree(p->d_group_q); dfree(p->d_group_size);
     dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
     dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
@@ -465,6 +472,8 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = comp ? 1 : 2;
+    p->nrows = (int64_t)h.W * h.H;
+    p->d_rowmap = p->d_c2pix;
     p->executed = false;
     p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
@@ -515,6 +524,8 @@ extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* de
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = compressed ? 1 : 2;
+    p->nrows = (int64_t)h.W * h.H;
+    p->d_rowmap = p->d_c2pix;
     p->executed = false;
     p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
@@ -551,6 +562,8 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     p->events = off[(size_t)h.P];
     p->nscale = 1;
     p->mode = 3;
+    p->nrows = h.P;
+    p->d_rowmap = nullptr;
     p->executed = false;
     p->generation++;
     HIP_TRY(hipStreamSynchronize(p->stream));
@@ -586,7 +599,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
             if (st) return st;
             p->ev_cap = p->raw_events;
         }
-        p->tshape = transpose_shape(h.P, p->nsel, p->tune.tr_bucket_shift, p->tune.tr_chunk_frames);
+        p->tshape = transpose_shape(p->nrows, p->nsel, p->tune.tr_bucket_shift, p->tune.tr_chunk_frames);
         const int64_t nh = (int64_t)p->tshape.nb * p->tshape.nblk;
         if (nh + 1 > p->hist_cap) {
             xpcs_status st = dalloc(&p->d_hist, (size_t)nh);
@@ -672,7 +685,7 @@ static xpcs_status sync_and_check(xpcs_mt_plan* p)
     if (cnt[1] & 1u) return fail(XPCS_ERR_RANGE, "execute: an IMM pixel index lies outside the detector");
     if (p->mode == 1) {
         int64_t e = 0;
-        HIP_TRY(hipMemcpy(&e, p->d_off + p->h.P, sizeof(e), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&e, p->d_off + p->nrows, sizeof(e), hipMemcpyDeviceToHost));
         p->events = e;
     }
     return XPCS_OK;

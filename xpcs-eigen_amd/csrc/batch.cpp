@@ -86,8 +86,8 @@ extern "C" xpcs_status xpcs_mt_get_tuning(xpcs_mt_plan* p, xpcs_mt_tuning* t)
 {
     if (!p || !t) return fail(XPCS_ERR_INVALID_ARG, "get_tuning: NULL argument");
     *t = xpcs_mt_tuning{};
-    const TransposeShape sh = transpose_shape(p->h.P, std::max<int64_t>(p->nsel, 1), p->tune.tr_bucket_shift,
-                                              p->tune.tr_chunk_frames);
+    const TransposeShape sh = transpose_shape(std::max<int64_t>(p->nrows, 1), std::max<int64_t>(p->nsel, 1),
+                                              p->tune.tr_bucket_shift, p->tune.tr_chunk_frames);
     t->tr_bucket_shift = sh.bshift;
     t->tr_chunk_frames = (int32_t)sh.cf;
     t->reg_blocks = reg_blocks(p);
@@ -131,7 +131,7 @@ extern "C" xpcs_status xpcs_mt_autotune(xpcs_mt_plan* p, xpcs_mt_tuning* best_ou
     std::vector<std::pair<int32_t xpcs_mt_tuning::*, std::vector<int32_t>>> axes;
     if (sparse_rows) {
         if (p->mode == 1) {
-            const TransposeShape sh = transpose_shape(p->h.P, std::max<int64_t>(p->nsel, 1), 0, 0);
+            const TransposeShape sh = transpose_shape(std::max<int64_t>(p->nrows, 1), std::max<int64_t>(p->nsel, 1), 0, 0);
             const int32_t cf = (int32_t)sh.cf;
             axes.push_back({&xpcs_mt_tuning::tr_bucket_shift, {8, 9, 10}});
             axes.push_back({&xpcs_mt_tuning::tr_chunk_frames, {std::max(1, cf / 2), cf * 2, cf * 4}});

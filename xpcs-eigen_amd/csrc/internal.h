@@ -121,6 +121,12 @@ struct xpcs_mt_plan {
     xpcs_mt_tuning tune{};
     uint64_t generation = 0;
     bool capturing = false;              // inside a graph capture: no timer events
+    // pixel-major event rows (CSR): nrows rows; pixel c's row is d_rowmap[c]
+    // (its raster index, rows of the IMM transpose) or c (d_rowmap == NULL,
+    // rows handed in by xpcs_mt_ingest_rows)
+    int64_t nrows = 0;
+    const int32_t* d_rowmap = nullptr;
+    uint32_t* d_mapped = nullptr;        // bit per detector pixel: dqmap, sqmap >= 1
     int64_t tile_off0 = -1;
     int64_t tile_step = 0;
     int64_t nsel = 0;                    // selected raw frames

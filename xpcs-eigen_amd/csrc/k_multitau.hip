@@ -65,6 +65,14 @@ __device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uin
 // ---- register path: rows of at most 64 events, one event per lane ------------------
 constexpr int kRegMaxTaus = 128;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+// Lag accumulators are kept in kAccCopies interleaved copies per tau; lane l
+// adds into copy l % kAccCopies, so the lanes of a wave that hit the same lag
+// spread over banks instead of serialising on one address.
+#ifndef XPCS_ACC_COPIES
+#define XPCS_ACC_COPIES 4
+#endif
+constexpr int kAccCopies = XPCS_ACC_COPIES;
+static_assert(kAccCopies >= 1 && (kAccCopies & (kAccCopies - 1)) == 0, "copies: a power of two");
 
 template <int J>
 __device__ __forceinline__ void bitonic_step(uint32_t& k, uint32_t& v, int kk)
@@ -100,6 +108,48 @@ __device__ __forceinline__ void bitonic_sort64(uint32_t& k, uint32_t& v, int m)
         bitonic_step<32>(k, v, 64); bitonic_step<16>(k, v, 64); bitonic_step<8>(k, v, 64);
         bitonic_step<4>(k, v, 64); bitonic_step<2>(k, v, 64); bitonic_step<1>(k, v, 64);
     }
+}
+
+// Rows leave the transpose in frame order up to local shuffles (events of one
+// pixel placed concurrently).  A few odd-even transposition rounds sort such a
+// row; the bitonic network handles whatever is still out of order.  Lanes
+// past the row hold kNoKey, so the order test needs no bound.
+__device__ __forceinline__ bool reg_unsorted(uint32_t k)
+{
+    return __ballot(next_lane_u32(k, kNoKey) < k) != 0ull;
+}
+
+__device__ __forceinline__ void reg_sort(uint32_t& k, uint32_t& v, int n)
+{
+    if (!reg_unsorted(k)) return;
+    const int lane = lane_id();
+    for (int r = 0; r < 3; r++) {
+        // even phase: (2j, 2j+1)
+        {
+            const uint32_t pk = xor_lane_u32<1>(k);
+            const uint32_t pv = xor_lane_u32<1>(v);
+            const bool lower = (lane & 1) == 0;
+            if (lower ? (pk < k) : (pk > k)) {
+                k = pk;
+                v = pv;
+            }
+        }
+        // odd phase: (2j+1, 2j+2)
+        {
+            const uint32_t nk = next_lane_u32(k, kNoKey), nv = next_lane_u32(v, 0u);
+            const uint32_t bk = prev_lane_u32(k, 0u), bv = prev_lane_u32(v, 0u);
+            const bool odd = (lane & 1) != 0;
+            if (odd && nk < k) {
+                k = nk;
+                v = nv;
+            } else if (!odd && lane > 0 && bk > k) {
+                k = bk;
+                v = bv;
+            }
+        }
+        if (!reg_unsorted(k)) return;
+    }
+    bitonic_sort64(k, v, next_pow2(n));
 }
 
 // One binning step in registers: keys >> shift, keys >= lf dropped, equal keys
@@ -176,9 +226,10 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
          const TauParam* __restrict__ taus, const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
          uint64_t* __restrict__ pix_total, int32_t* __restrict__ mid_list,
-         int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap)
+         int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap,
+         const int32_t* __restrict__ rowmap)
 {
-    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus];
+    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus * kAccCopies];
     const int lane = lane_id();
     const int w = wave_id();
     unsigned long long* acc = s_acc[w];
@@ -186,8 +237,9 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
     const bool reg_ok = T <= kRegMaxTaus;
 
     for (int64_t p = (int64_t)blockIdx.x * WAVES + w; p < P; p += nw) {
-        const int64_t base = off[p];
-        const int64_t n64 = off[p + 1] - base;
+        const int64_t row = rowmap ? (int64_t)rowmap[p] : p;   // CSR row of pixel p
+        const int64_t base = off[row];
+        const int64_t n64 = off[row + 1] - base;
         if (n64 > kWave || (!reg_ok && n64 > 0)) {
             if (lane == 0) {
                 if (n64 <= mid_cap) {
@@ -213,29 +265,31 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (lane == 0) pix_total[p] = 0;
             continue;
         }
-        for (int i = lane; i < T; i += kWave) acc[i] = 0ull;
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = 0ull;
         uint32_t k = kNoKey, v = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
             k = (uint32_t)(e >> 32);
             v = (uint32_t)e;
         }
-        bitonic_sort64(k, v, next_pow2(n));
+        reg_sort(k, v, n);
         int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
         // IP and IF of every tau from the level-0 prefix sums:
         // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau')
         {
             const uint32_t pre = wave_incl_scan_u32(v);
             if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
+            // S0(lf_L 2^L) depends on the level only: lane L holds it
+            const uint32_t s_lvl = reg_prefix_below(
+                k, pre, nl, lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
             for (int i0 = 0; i0 < T; i0 += kWave) {
                 const int i = i0 + lane;
                 const TauParam tp = taus[i < T ? i : T - 1];
                 const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
                 const uint32_t b_if = (uint32_t)tp.tp << tp.level;
-                const uint32_t b_end = (uint32_t)tp.lf << tp.level;
                 const uint32_t s_ip = reg_prefix_below(k, pre, nl, b_ip);
                 const uint32_t s_if = reg_prefix_below(k, pre, nl, b_if);
-                const uint32_t s_end = reg_prefix_below(k, pre, nl, b_end);
+                const uint32_t s_end = (uint32_t)__shfl((int)s_lvl, tp.level, 64);
                 if (i < T) {
                     iprow[i] = (int64_t)s_ip;
                     ifrow[i] = (int64_t)(s_end - s_if);
@@ -247,20 +301,29 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (L > 0) nl = reg_level(k, v, nl, 1, (uint32_t)lp.lf, true);
             if (nl < 2) break;
             // lag products: partner of lane i is lane i+m, m = 1..hi
-            const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
+            const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
+            // lanes past nl hold kNoKey, so a valid lane's partner beyond the
+            // row has d > hi; only the lanes past nl need masking
+            const bool valid = lane < nl;
+            const int cp = lane & (kAccCopies - 1);
             uint32_t kj = k, vj = v;
             for (uint32_t m = 1; m <= hi; m++) {
                 kj = next_lane_u32(kj, kNoKey);
                 vj = next_lane_u32(vj, 0u);
                 const uint32_t d = kj - k;
-                const bool active = lane < nl && kj != kNoKey && d <= hi;
-                if (__ballot(active) == 0ull) break;
-                if (active && d >= lo)
-                    atomicAdd(&acc[lp.first + (int)(d - lo)], (unsigned long long)v * (unsigned long long)vj);
+                if (__ballot(valid && d <= hi) == 0ull) break;
+                if (valid && d - lo <= span)
+                    atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp],
+                              (unsigned long long)v * (unsigned long long)vj);
             }
         }
         wave_sync();
-        for (int i = lane; i < T; i += kWave) g2row[i] = (int64_t)acc[i];
+        for (int i = lane; i < T; i += kWave) {
+            unsigned long long g = 0ull;
+#pragma unroll
+            for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
+            g2row[i] = (int64_t)g;
+        }
         wave_sync();
     }
 }
@@ -271,7 +334,7 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const int32_t* __restrict__ rows, const uint32_t* __restrict__ counters,
             const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
             int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
-            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total)
+            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap)
 {
     __shared__ uint32_t s_key[WAVES][CAP];
     __shared__ uint32_t s_val[WAVES][CAP];
@@ -286,8 +349,9 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
     const int64_t nrows = (int64_t)counters[2];
     for (int64_t r = (int64_t)blockIdx.x * WAVES + w; r < nrows; r += nw) {
         const int64_t p = rows[r];
-        const int64_t base = off[p];
-        const int64_t n64 = off[p + 1] - base;
+        const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+        const int64_t base = off[row];
+        const int64_t n64 = off[row + 1] - base;
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
@@ -376,7 +440,9 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         }
         if (lane == 0) pix_total[p] = carry;
         wave_sync();
-        for (int i = lane; i < T; i += kWave) aux[i] = 0ull;
+        // lag accumulators: nc copies per tau while T * nc fits the CAP slots
+        const int nc = T * kAccCopies <= CAP ? kAccCopies : 1;
+        for (int i = lane; i < T * nc; i += kWave) aux[i] = 0ull;
         wave_sync();
         // ---- levels: bin in place, then lag products ----
         int nl = n0;
@@ -391,12 +457,18 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 for (int j = i + 1; j < nl; j++) {
                     const uint32_t d = key[j] - ki;
                     if (d > hi) break;
-                    if (d >= lo) atomicAdd(&aux[lp.first + (int)(d - lo)], (unsigned long long)(vi * val[j]));
+                    if (d >= lo)
+                        atomicAdd(&aux[(lp.first + (int)(d - lo)) * nc + (lane & (nc - 1))],
+                                  (unsigned long long)(vi * val[j]));
                 }
             }
             wave_sync();
         }
-        for (int i = lane; i < T; i += kWave) g2row[i] = (int64_t)aux[i];
+        for (int i = lane; i < T; i += kWave) {
+            unsigned long long g = 0ull;
+            for (int c = 0; c < nc; c++) g += aux[i * nc + c];
+            g2row[i] = (int64_t)g;
+        }
         wave_sync();
     }
 }
@@ -487,7 +559,7 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F,
           const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense,
           uint64_t* __restrict__ scratch, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
-          int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total)
+          int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap)
 {
     __shared__ unsigned long long s_red[kLongBlock / 64];
     __shared__ unsigned long long s_lag[kMaxLagBlock];
@@ -511,7 +583,8 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         } else {
             for (int t = threadIdx.x; t < F; t += blockDim.x) x[t] = 0;
             __syncthreads();
-            const int64_t b = off[p], e = off[p + 1];
+            const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+            const int64_t b = off[row], e = off[row + 1];
             for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
                 const uint64_t v = ev[i];
                 atomicAdd(reinterpret_cast<unsigned long long*>(&x[v >> 32]),
@@ -542,12 +615,13 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
     timer_begin(p, s, "k_mt_reg");
     k_mt_reg<kRegWaves><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
-        p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed);
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed,
+        p->d_rowmap);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
     k_mt_sparse<kSparseCapUsed, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list, p->d_counters, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
     timer_end(p, s);
 }
 
@@ -559,7 +633,7 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
                                                     p->d_levels, (int32_t)h.levels.size(), h.F,
                                                     p->d_long_list, p->d_counters, dense_rows ? 1 : 0,
                                                     p->d_scratch, p->d_G2, p->d_IP, p->d_IF,
-                                                    p->d_pix_total);
+                                                    p->d_pix_total, p->d_rowmap);
     timer_end(p, s);
 }
 

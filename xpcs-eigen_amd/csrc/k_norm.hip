@@ -16,6 +16,8 @@
 
 namespace xpcs {
 
+constexpr int kNormBatch = 16;
+
 __global__ void __launch_bounds__(256)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
                int32_t T, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
@@ -29,16 +31,17 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
             int64_t sg = 0, sp = 0, sf = 0;
             double wg = 0.0, wp = 0.0, wf = 0.0;
             double n = 0.0, mean = 0.0, m2 = 0.0;
-            for (int p0 = ch.begin; p0 < ch.end; p0 += 4) {
-              int64_t gg[4], aa[4], bb[4];
+            // kNormBatch pixels' loads in flight per thread
+            for (int p0 = ch.begin; p0 < ch.end; p0 += kNormBatch) {
+              int64_t gg[kNormBatch], aa[kNormBatch], bb[kNormBatch];
 #pragma unroll
-              for (int u = 0; u < 4; u++) {
+              for (int u = 0; u < kNormBatch; u++) {
                   const int pu = min(p0 + u, ch.end - 1);
                   gg[u] = G2[(int64_t)pu * T + i];
                   aa[u] = IP[(int64_t)pu * T + i];
                   bb[u] = IF[(int64_t)pu * T + i];
               }
-              for (int u = 0; u < 4 && p0 + u < ch.end; u++) {
+              for (int u = 0; u < kNormBatch && p0 + u < ch.end; u++) {
                 const int p = p0 + u;
                 const int64_t g = gg[u];
                 const int64_t a = aa[u];
@@ -93,6 +96,7 @@ k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ g
     double sg, sp, sf;
     if (weighted) {
         sg = sp = sf = 0.0;
+#pragma unroll 8
         for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
             const NormPartial& r = part[(int64_t)c * T + i];
             sg += __longlong_as_double(r.g2);
@@ -101,6 +105,7 @@ k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ g
         }
     } else {
         int64_t ig = 0, ip = 0, iff = 0;
+#pragma unroll 8
         for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
             const NormPartial& r = part[(int64_t)c * T + i];
             ig += r.g2;
@@ -133,20 +138,32 @@ k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
     }
     g2[(int64_t)i * nq + q] = acc / cnt;
     double n = 0.0, mean = 0.0, m2 = 0.0;
-    for (int c = q_chunk_first[q]; c < q_chunk_first[q + 1]; c++) {
-        const NormPartial& r = part[(int64_t)c * T + i];
-        if (r.n == 0.0) continue;
-        if (n == 0.0) {
-            n = r.n;
-            mean = r.mean;
-            m2 = r.m2;
-            continue;
+    const int c1 = q_chunk_first[q + 1];
+    for (int c0 = q_chunk_first[q]; c0 < c1; c0 += kNormBatch) {
+        // the chunk states are loaded kNormBatch at a time, then merged in order
+        double rn[kNormBatch], rmean[kNormBatch], rm2[kNormBatch];
+#pragma unroll
+        for (int u = 0; u < kNormBatch; u++) {
+            const NormPartial& r = part[(int64_t)min(c0 + u, c1 - 1) * T + i];
+            rn[u] = c0 + u < c1 ? r.n : 0.0;
+            rmean[u] = r.mean;
+            rm2[u] = r.m2;
         }
-        const double nt = n + r.n;
-        const double d = r.mean - mean;
-        mean += d * (r.n / nt);
-        m2 += r.m2 + d * d * (n * r.n / nt);
-        n = nt;
+#pragma unroll
+        for (int u = 0; u < kNormBatch; u++) {
+            if (rn[u] == 0.0) continue;
+            if (n == 0.0) {
+                n = rn[u];
+                mean = rmean[u];
+                m2 = rm2[u];
+                continue;
+            }
+            const double nt = n + rn[u];
+            const double d = rmean[u] - mean;
+            mean += d * (rn[u] / nt);
+            m2 += rm2[u] + d * d * (n * rn[u] / nt);
+            n = nt;
+        }
     }
     se[(int64_t)i * nq + q] = sqrt(1.0 / n) * sqrt(m2 / n);
 }

@@ -46,10 +46,27 @@ __device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, boo
     return c >= 0 && v != 0;
 }
 
+// An event of the transpose: raster pixel r, count v; kept when the pixel is
+// mapped (bit r of `mapped`, 128 KB for a megapixel: cache resident, unlike a
+// per-pixel id table) and v != 0.  Rows are per raster pixel.
+__device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j,
+                                             const uint32_t* __restrict__ mapped, uint32_t npix_det,
+                                             uint32_t* __restrict__ counters, uint32_t& r, uint32_t& v)
+{
+    const uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
+    v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen)[j];
+    if (idx >= npix_det) {
+        if (counters) atomicOr(&counters[1], 1u);
+        return false;
+    }
+    r = idx;
+    return v != 0 && ((mapped[idx >> 5] >> (idx & 31u)) & 1u);
+}
+
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf,
-           const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
+           const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
            int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters)
 {
     extern __shared__ uint32_t s_h[];
@@ -63,13 +80,13 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
         const uint32_t bd = blockDim.x;
         for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
-            int32_t c[4];
+            uint32_t c[4];
             uint32_t v[4];
             bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event(base, dlen, al4, j, pix2c, npix_det, counters, c[u], v[u]);
+                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, mapped, npix_det, counters, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -83,7 +100,7 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
             const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
-            int64_t cf, const int32_t* __restrict__ pix2c, uint32_t npix_det, int bshift, int32_t nb,
+            int64_t cf, const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
             int32_t nblk, const int64_t* __restrict__ hoff, uint64_t* __restrict__ b1)
 {
     extern __shared__ unsigned long long s_base[];   // [nb] segment starts of this chunk
@@ -103,14 +120,14 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
         const uint32_t bd = blockDim.x;
         for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
-            int32_t c[4];
+            uint32_t c[4];
             uint32_t v[4];
             bool ok[4];
             uint32_t slot[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c[u], v[u]);
+                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, mapped, npix_det, nullptr, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -410,14 +427,18 @@ static void scan_u32(const uint32_t* in, int64_t n, int64_t* out, int64_t* bsum,
     k_scan_total<<<1, 64, 0, s>>>(in, n, out);
 }
 
-// Buckets of 2^bshift pixels (at least 256: k_tr_rows scans a bucket with 256
-// threads; at most 4096 buckets: the per-chunk LDS histogram), cf raw frames
+// Buckets of 2^bshift pixel rows (at least 256: k_tr_rows scans a bucket with
+// 256 threads; at most 4096 buckets: the per-chunk LDS histogram; by default
+// about 1024, which keeps each chunk's bucket segments long), cf raw frames
 // per chunk workgroup (default: at most 1024 chunks).  bucket_shift and
 // chunk_frames override the defaults when non-zero.
 TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames)
 {
     TransposeShape t;
-    t.bshift = bucket_shift > 8 ? bucket_shift : 8;
+    t.bshift = 8;
+    if (bucket_shift > 8) t.bshift = bucket_shift;
+    else if (bucket_shift == 0)
+        while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 1024) t.bshift++;
     while (((P + (int64_t(1) << t.bshift) - 1) >> t.bshift) > 4096) t.bshift++;
     t.nb = (int32_t)((P + (int64_t(1) << t.bshift) - 1) >> t.bshift);
     t.cf = nsel > 1024 ? (nsel + 1023) / 1024 : 1;
@@ -434,7 +455,7 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
     const int64_t nh = (int64_t)t.nb * t.nblk;
     timer_begin(p, s, "k_tr_count");
     k_tr_count<<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, p->d_pix2c, npd, t.bshift, t.nb, t.nblk,
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, p->d_mapped, npd, t.bshift, t.nb, t.nblk,
         p->d_hist, p->d_counters);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_scan");
@@ -442,12 +463,12 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
     timer_end(p, s);
     timer_begin(p, s, "k_tr_bucket");
     k_tr_bucket<<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, p->d_pix2c, npd, t.bshift,
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, p->d_mapped, npd, t.bshift,
         t.nb, t.nblk, p->d_hoff, p->d_b1);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_rows");
     k_tr_rows<<<t.nb < 4096 ? t.nb : 4096, kTrBlock, (size_t)sizeof(unsigned long long) << t.bshift, s>>>(
-        p->d_b1, p->d_hoff, t.nblk, t.bshift, t.nb, p->h.P, p->d_off, p->d_ev);
+        p->d_b1, p->d_hoff, t.nblk, t.bshift, t.nb, (int32_t)p->nrows, p->d_off, p->d_ev);
     timer_end(p, s);
 }
 

@@ -68,10 +68,7 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 // Lag accumulators are kept in kAccCopies interleaved copies per tau; lane l
 // adds into copy l % kAccCopies, so the lanes of a wave that hit the same lag
 // spread over banks instead of serialising on one address.
-#ifndef XPCS_ACC_COPIES
-#define XPCS_ACC_COPIES 4
-#endif
-constexpr int kAccCopies = XPCS_ACC_COPIES;
+constexpr int kAccCopies = 4;
 static_assert(kAccCopies >= 1 && (kAccCopies & (kAccCopies - 1)) == 0, "copies: a power of two");
 
 template <int J>
@@ -311,7 +308,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
                 kj = next_lane_u32(kj, kNoKey);
                 vj = next_lane_u32(vj, 0u);
                 const uint32_t d = kj - k;
-                if (__ballot(valid && d <= hi) == 0ull) break;
+                if (__builtin_amdgcn_ballot_w64(valid && d <= hi) == 0ull) break;
                 if (valid && d - lo <= span)
                     atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp],
                               (unsigned long long)v * (unsigned long long)vj);
@@ -326,6 +323,37 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         }
         wave_sync();
     }
+}
+
+// LDS rows of m (a power of two > 64) entries, kNoKey-padded: up to 3
+// odd-even transposition rounds; true when the row ends up sorted.
+__device__ bool lds_sort_rounds(uint32_t* key, uint32_t* val, int m)
+{
+    const int lane = lane_id();
+    auto sorted = [&]() {
+        bool bad = false;
+        for (int i = lane; i + 1 < m; i += kWave) bad |= key[i] > key[i + 1];
+        return __builtin_amdgcn_ballot_w64(bad) == 0ull;
+    };
+    if (sorted()) return true;
+    for (int r = 0; r < 3; r++) {
+        for (int ph = 0; ph < 2; ph++) {
+            for (int t = lane; 2 * t + ph + 1 < m; t += kWave) {
+                const int i = 2 * t + ph;
+                const uint32_t a = key[i], b = key[i + 1];
+                if (b < a) {
+                    key[i] = b;
+                    key[i + 1] = a;
+                    const uint32_t va = val[i];
+                    val[i] = val[i + 1];
+                    val[i + 1] = va;
+                }
+            }
+            wave_sync();
+        }
+        if (sorted()) return true;
+    }
+    return false;
 }
 
 template <int CAP, int WAVES>
@@ -357,6 +385,7 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         int64_t* ifrow = IF + p * T;
         const int n = (int)n64;         // 0 < n <= CAP by construction of the list
         const int m = next_pow2(n);
+        bool m_sorted = false;
         // ---- load and sort by frame ----
         if (m <= kWave) {
             uint32_t k = 0xFFFFFFFFu, v = 0;
@@ -365,19 +394,7 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 k = (uint32_t)(e >> 32);
                 v = (uint32_t)e;
             }
-            for (int kk = 2; kk <= m; kk <<= 1) {
-                for (int j = kk >> 1; j > 0; j >>= 1) {
-                    uint32_t pk = __shfl_xor(k, j, 64);
-                    uint32_t pv = __shfl_xor(v, j, 64);
-                    const bool up = (lane & kk) == 0;
-                    const bool lower = (lane & j) == 0;
-                    const bool take = (lower == up) ? (pk < k) : (pk > k);
-                    if (take) {
-                        k = pk;
-                        v = pv;
-                    }
-                }
-            }
+            reg_sort(k, v, n);
             key[lane] = k;
             val[lane] = v;
         } else {
@@ -392,6 +409,10 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 val[i] = v;
             }
             wave_sync();
+            // nearly sorted rows: odd-even transposition rounds, the bitonic
+            // network only for what is still out of order
+            if (lds_sort_rounds(key, val, m)) m_sorted = true;
+            if (!m_sorted)
             for (int kk = 2; kk <= m; kk <<= 1) {
                 for (int j = kk >> 1; j > 0; j >>= 1) {
                     for (int i = lane; i < m; i += kWave) {

@@ -141,6 +141,8 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
     }
 }
 
+// Launched with kTrBlock threads (2^bshift >= kTrBlock row counters: a whole
+// number per thread).
 __global__ void __launch_bounds__(kTrBlock)
 k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int32_t nblk, int bshift,
           int32_t nb, int32_t P, int64_t* __restrict__ off, uint64_t* __restrict__ ev)
@@ -467,7 +469,10 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
         t.nb, t.nblk, p->d_hoff, p->d_b1);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_rows");
-    k_tr_rows<<<t.nb < 4096 ? t.nb : 4096, kTrBlock, (size_t)sizeof(unsigned long long) << t.bshift, s>>>(
+    // 256 threads: placement tiles of 1024 events hold about one event per
+    // pixel, so rows come out nearly in frame order (cheap to sort downstream)
+    const int rows_threads = kTrBlock;
+    k_tr_rows<<<t.nb < 4096 ? t.nb : 4096, rows_threads, (size_t)sizeof(unsigned long long) << t.bshift, s>>>(
         p->d_b1, p->d_hoff, t.nblk, t.bshift, t.nb, (int32_t)p->nrows, p->d_off, p->d_ev);
     timer_end(p, s);
 }

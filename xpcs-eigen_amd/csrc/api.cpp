@@ -4,7 +4,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "internal.h"
 
@@ -383,6 +386,73 @@ xpcs_status finish_selection(xpcs_mt_plan* p, const std::vector<int64_t>& sel_of
     return XPCS_OK;
 }
 
+// Host bytes to device memory through two pinned staging buffers: while the
+// DMA engine drains one, host threads fill the other (pageable memory cannot
+// be handed to the DMA engine directly, and one thread's memcpy is slower than
+// the PCIe link).  The buffers are allocated once per process and shared by
+// all plans (one staged copy at a time).  Returns when every piece has been
+// copied; the caller's buffer is not read afterwards.
+struct StagePool {
+    std::mutex mu;
+    uint8_t* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+};
+
+StagePool& stage_pool()
+{
+    static StagePool pool;   // lives until the process ends
+    return pool;
+}
+
+xpcs_status h2d_staged(uint8_t* dst, const uint8_t* src, size_t len, hipStream_t s)
+{
+    constexpr size_t kStage = size_t(32) << 20;
+    if (len == 0) return XPCS_OK;
+    StagePool& sp = stage_pool();
+    std::lock_guard<std::mutex> lk(sp.mu);
+    if (!sp.buf[0]) {
+        for (int i = 0; i < 2; i++) {
+            if (hipHostMalloc(reinterpret_cast<void**>(&sp.buf[i]), kStage, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&sp.done[i], hipEventDisableTiming) != hipSuccess) {
+                for (int j = 0; j <= i; j++) {
+                    if (sp.buf[j]) (void)hipHostFree(sp.buf[j]);
+                    sp.buf[j] = nullptr;
+                }
+                return fail(XPCS_ERR_NOMEM, "ingest: pinned staging of %zu bytes failed", kStage);
+            }
+        }
+        sp.bytes = kStage;
+    }
+    const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    int b = 0;
+    xpcs_status st = XPCS_OK;
+    bool issued[2] = {false, false};
+    for (size_t off = 0; off < len && !st; off += sp.bytes, b ^= 1) {
+        const size_t n = std::min(sp.bytes, len - off);
+        if (issued[b] && hipEventSynchronize(sp.done[b]) != hipSuccess) {
+            st = fail(XPCS_ERR_HIP, "ingest: staging copy failed");
+            break;
+        }
+        // parallel host copy into the pinned buffer
+        uint8_t* stage = sp.buf[b];
+        const size_t per = (n + nthreads - 1) / nthreads;
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nthreads && t * per < n; t++)
+            pool.emplace_back([=] { std::memcpy(stage + t * per, src + off + t * per, std::min(per, n - t * per)); });
+        std::memcpy(stage, src + off, std::min(per, n));
+        for (auto& th : pool) th.join();
+        if (hipMemcpyAsync(dst + off, stage, n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(sp.done[b], s) != hipSuccess)
+            st = fail(XPCS_ERR_HIP, "ingest: host-to-device copy failed");
+        issued[b] = true;
+    }
+    // the pool is free for the next caller only when its copies have finished
+    for (int i = 0; i < 2; i++)
+        if (issued[i]) (void)hipEventSynchronize(sp.done[i]);
+    return st;
+}
+
 // Tile loads of the dense streaming kernel: one output frame per raw frame,
 // frames evenly spaced and 16-byte aligned, and every frame readable up to the
 // 64-pixel boundary after the detector's last pixel.
@@ -460,7 +530,8 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
         if (st) return st;
         p->imm_cap = len + pad;
     }
-    HIP_TRY(hipMemcpyAsync(p->d_imm_owned, b + lo, len, hipMemcpyHostToDevice, p->stream));
+    st = h2d_staged(p->d_imm_owned, b + lo, len, p->stream);
+    if (st) return st;
     p->d_imm = p->d_imm_owned;
     std::vector<int64_t> so(sel.size());
     std::vector<uint32_t> sd(sel.size());

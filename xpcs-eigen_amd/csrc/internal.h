@@ -94,6 +94,7 @@ struct xpcs_mt_plan {
     hipStream_t stream_last = nullptr;   // stream of the last execute
     bool own_stream = false;
 
+
     // device tables
     int32_t* d_pix2c = nullptr;
     xpcs::TauParam* d_taus = nullptr;

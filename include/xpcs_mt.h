@@ -283,6 +283,42 @@ xpcs_status xpcs_mt_batch_synchronize(xpcs_mt_batch* batch);
 xpcs_status xpcs_mt_batch_destroy(xpcs_mt_batch* batch);
 
 /*
+ * Two-time analysis (Corr::twotime, src/xpcs/corr.cpp:562-924; configuration
+ * keys analysis_type "Twotime", qphi_bin_to_process, smoothing_method,
+ * smoothing_filter, twotime2onetime_window_size, configuration.cpp:151,216-232).
+ * For every listed dynamic bin present in the q-map (ascending):
+ *   C2T[f0][f1] = sum_p x[p,f0] x[p,f1] / P_q  for f0 <= f1, 0 below,
+ *   g2full[f][i] = mean of diagonal f, g2partials[f][w][i] (f < window,
+ *   w < (frames - window) / window) = sum of diagonal f over window w / window,
+ * with x the pixel values (counts / avg_frames, dense values per dark scale,
+ * times the flat field) after smoothing: smoothing 1 = "StaticMap" (divide by
+ * the mean of the pixel's static bin, per frame or, with average, its mean
+ * over frames), 2 = "symmetric" (the same per dynamic bin), 0 = none.  FP64 on
+ * the device (integer counts give exact Gram sums).  A q-bin's values are
+ * built in pixel chunks of at most chunk_bytes device bytes (0: 4 GiB), so
+ * HBM holds the F x F results plus one chunk.  Results stay on the device
+ * until xpcs_mt_twotime runs again or the plan is destroyed.
+ */
+typedef struct xpcs_tt_desc {
+    const int32_t* qbins;       /* qphi_bin_to_process (1-based dynamic bins)  */
+    int32_t nq;
+    int32_t smoothing;          /* 0 none, 1 StaticMap, 2 symmetric             */
+    int32_t average;            /* smoothing_filter == "Average"                */
+    int32_t window;             /* twotime2onetime_window_size (>= 1)           */
+    int64_t chunk_bytes;        /* value chunk size; 0 = default (4 GiB)        */
+} xpcs_tt_desc;
+
+xpcs_status xpcs_mt_twotime(xpcs_mt_plan* plan, const xpcs_tt_desc* desc, void* stream);
+/* Processed q-bins (ascending), rows/columns of sg, windows of g2partials. */
+xpcs_status xpcs_mt_twotime_info(xpcs_mt_plan* plan, int32_t* nq, int32_t* qbins, int32_t cap, int32_t* sg_rows,
+                                 int32_t* sg_cols, int32_t* total_partials);
+/* C2T of the i-th processed q-bin, [frames][frames]. */
+xpcs_status xpcs_mt_read_twotime(xpcs_mt_plan* plan, int32_t i, double* c2t);
+/* g2full [frames][nq], g2partials [window][total_partials][nq] (the
+ * reference's layouts, corr.cpp:882-912), sg [sg_rows][sg_cols]; any may be NULL. */
+xpcs_status xpcs_mt_read_twotime_1d(xpcs_mt_plan* plan, double* g2full, double* g2partials, double* sg);
+
+/*
  * Streams (no reference counterpart: the reference correlates a finished IMM
  * file, src/xpcs/main.cpp:222-268 then Corr::multiTau2 / Corr::normalizeG2s).
  * Frames arrive in pieces; the per-pixel level state (last and first 2*dpl

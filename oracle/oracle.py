@@ -293,3 +293,104 @@ def run_exact(buf, dqmap, sqmap, frames, dpl, darks=0, frame_from=0, stride=1, a
         weight = np.asarray(flatfield, np.float64).reshape(-1)[qm["order"]]
     g2, se = normalize_f64(G2, IP, IF, Nv, qm, weight)
     return dict(qm=qm, csr=csr, level=lev, tau=tau, N=Nv, G2=G2, IP=IP, IF=IF, g2=g2, stderr=se)
+
+
+# ---- two-time (Corr::twotime, src/xpcs/corr.cpp:562-924) -------------------------
+def _twotime_bins(qm, qbins):
+    """Listed q-bins present in m_mapping (ascending) and their groups, as
+    Configuration::QbinPixelList builds them (src/xpcs/configuration.cpp:359-380)."""
+    listed = {int(q) for q in qbins}
+    out = []
+    for q in sorted({int(x) for x in qm["group_q"]}):
+        if q not in listed:
+            continue
+        gs = [g for g in range(qm["ngroups"]) if int(qm["group_q"][g]) == q]
+        out.append((q, gs))
+    return out
+
+
+def _twotime_values(csr, qm, frames, weight=None, value_scale=1.0):
+    """Dense pixel x frame values (float64) from the pixel-major rows."""
+    P = qm["npix"]
+    X = np.zeros((P, frames), np.float64)
+    off = csr["off"]
+    for c in range(P):
+        b, e = int(off[c]), int(off[c + 1])
+        if e > b:
+            w = 1.0 if weight is None else float(weight[c])
+            X[c, csr["frame"][b:e]] += csr["cnt"][b:e].astype(np.float64) * w * value_scale
+    return X
+
+
+def twotime_f64(csr, qm, frames, qbins, smoothing=0, average=False, wsize=1, weight=None, value_scale=1.0):
+    """FP64 two-time: smoothing (0 none, 1 StaticMap corr.cpp:433-494 with
+    ComputeSGStaticMap :1228-1305, 2 symmetric :496-544 with ComputeSGSymmetric
+    :1166-1226; average = smoothing_filter "Average"), then per listed q-bin
+    C2T[f0][f1] = sum_p x[p,f0] x[p,f1] / P_q for f0 <= f1 (0 below the
+    diagonal, :800-840), g2full[ff] = mean of diagonal ff, g2partials[ff][w] =
+    sum of diagonal ff over window w / wsize (:842-870).  The Gram matrix is a
+    float64 BLAS product (the textbook form of the reference's pair loops)."""
+    F = frames
+    X = _twotime_values(csr, qm, F, weight, value_scale)
+    bins = _twotime_bins(qm, qbins)
+    gb, ge = qm["group_begin"], qm["group_end"]
+    sg = None
+    if smoothing == 1:
+        groups = [g for _, gs in bins for g in gs]
+        sg = np.zeros((len(groups), F))
+        for i, g in enumerate(groups):
+            sg[i] = X[gb[g]:ge[g]].sum(0) / (ge[g] - gb[g])
+        if average:
+            sg = sg.sum(1, keepdims=True) / F
+        for i, g in enumerate(groups):
+            blk = X[gb[g]:ge[g]]
+            d = np.broadcast_to(sg[i], (blk.shape[0], F))
+            X[gb[g]:ge[g]] = np.where(blk != 0, blk / np.where(d != 0, d, 1.0), 0.0)
+    elif smoothing == 2:
+        sg = np.zeros((len(bins), F))
+        for i, (q, gs) in enumerate(bins):
+            rows = np.concatenate([np.arange(gb[g], ge[g]) for g in gs])
+            sg[i] = X[rows].sum(0) / rows.size
+        if average:
+            sg = sg.sum(1, keepdims=True) / F
+        for i, (q, gs) in enumerate(bins):
+            for g in gs:
+                blk = X[gb[g]:ge[g]]
+                d = np.broadcast_to(sg[i], (blk.shape[0], F))
+                X[gb[g]:ge[g]] = np.where(blk != 0, blk / np.where(d != 0, d, 1.0), 0.0)
+    total = (F - wsize) // wsize
+    nqb = len(bins)
+    c2t = []
+    g2full = np.zeros((F, nqb))
+    g2part = np.zeros((wsize, max(total, 0), nqb))
+    for i, (q, gs) in enumerate(bins):
+        rows = np.concatenate([np.arange(gb[g], ge[g]) for g in gs])
+        Xq = X[rows]
+        G = np.triu(Xq.T @ Xq) / rows.size
+        c2t.append(G)
+        for ff in range(F):
+            d = np.diagonal(G, ff)
+            g2full[ff, i] = d.mean()
+            if ff < wsize:
+                for w in range(total):
+                    g2part[ff, w, i] = d[w * wsize:min((w + 1) * wsize, d.size)].sum() / wsize
+    return dict(qbins=[q for q, _ in bins], c2t=c2t, g2full=g2full, g2partials=g2part, sg=sg)
+
+
+def twotime_pairs(csr, qm, frames, q, value_scale=1.0):
+    """C2T of one q-bin by the reference's own loop order (corr.cpp:816-835:
+    for each pixel, every event pair j <= k adds val_j val_k at (f_j, f_k)),
+    for small cases."""
+    bins = dict(_twotime_bins(qm, [q]))
+    gb, ge = qm["group_begin"], qm["group_end"]
+    G = np.zeros((frames, frames))
+    n = 0
+    for g in bins.get(q, []):
+        for c in range(gb[g], ge[g]):
+            b, e = int(csr["off"][c]), int(csr["off"][c + 1])
+            fr, v = csr["frame"][b:e], csr["cnt"][b:e].astype(np.float64) * value_scale
+            for j in range(e - b):
+                for k in range(j, e - b):
+                    G[fr[j], fr[k]] += v[j] * v[k]
+            n += 1
+    return G / max(n, 1)

@@ -221,6 +221,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs); dfree(p->d_series_a); dfree(p->d_series_b); dfree(p->d_bnd);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : p->event_pool) (void)hipEventDestroy(e);
+    twotime_release(p);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return XPCS_OK;
@@ -610,12 +611,22 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     if ((!frame || !count) && row_off[(int64_t)p->h.W * p->h.H] > 0)
         return fail(XPCS_ERR_INVALID_ARG, "ingest_rows: NULL frame/count arrays");
     HostTables& h = p->h;
+    const int64_t npd = (int64_t)h.W * h.H;
+    if (row_off[0] < 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_rows: row_off[0] < 0");
+    for (int64_t j = 0; j < npd; j++)
+        if (row_off[j + 1] < row_off[j])
+            return fail(XPCS_ERR_INVALID_ARG, "ingest_rows: row_off decreases at pixel %lld", (long long)j);
     std::vector<int64_t> off((size_t)h.P + 1, 0);
     for (int c = 0; c < h.P; c++) {
         const int64_t j = h.order[(size_t)c];
         off[(size_t)c + 1] = off[(size_t)c] + (row_off[j + 1] - row_off[j]);
     }
-    std::vector<uint64_t> ev((size_t)off[(size_t)h.P]);
+    std::vector<uint64_t> ev;
+    try {
+        ev.resize((size_t)off[(size_t)h.P]);
+    } catch (const std::bad_alloc&) {
+        return fail(XPCS_ERR_NOMEM, "ingest_rows: %lld events do not fit host memory", (long long)off[(size_t)h.P]);
+    }
     for (int c = 0; c < h.P; c++) {
         const int64_t j = h.order[(size_t)c];
         int64_t o = off[(size_t)c];

@@ -157,6 +157,13 @@ struct xpcs_mt_plan {
     int64_t events = 0;                  // known after execute (host copy)
     // dense rows (fallback path)
     uint32_t* d_rows = nullptr;          // [P][F]
+    // two-time results (per processed q-bin, device FP64)
+    std::vector<int32_t> tt_q;
+    std::vector<double*> tt_c2t, tt_full, tt_part, tt_sg;
+    std::vector<int32_t> tt_sg_rows;
+    int32_t tt_sg_cols = 0, tt_total = 0, tt_wsize = 0;
+    double* tt_x = nullptr;              // scratch: values of one pixel chunk [rows][F]
+    size_t tt_x_cap = 0;
     size_t rows_cap = 0;
     // dense streaming path: coarsened series between passes, boundary events
     uint32_t* d_series_a = nullptr;
@@ -217,6 +224,18 @@ void launch_fill_u32(uint32_t* d, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
 xpcs_status prepare_execute(xpcs_mt_plan* p);
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus);
+// two-time (k_twotime.hip)
+void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, double* X, hipStream_t s);
+void launch_tt_colsum(const double* X, int32_t F, int32_t lo, int32_t hi, const int32_t* d_bin_begin,
+                      const int32_t* d_bin_end, int32_t nbins, double* d_sgsum, hipStream_t s);
+void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins,
+                       bool average, double* d_sga, hipStream_t s);
+void launch_tt_divide(double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row, const double* d_div,
+                      bool per_frame, hipStream_t s);
+void launch_tt_gram(const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s);
+void launch_tt_diag(const double* C, int32_t F, int32_t wsize, int32_t total, double* g2full, double* g2part,
+                    hipStream_t s);
+void twotime_release(xpcs_mt_plan* p);
 
 // online multi-tau (k_stream.hip)
 constexpr int kStreamChunk = 64;   // frames per update launch

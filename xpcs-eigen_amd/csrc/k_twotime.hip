@@ -1,0 +1,248 @@
+// Two-time correlation (Corr::twotime, src/xpcs/corr.cpp:562-924): per listed
+// q-bin, C2T[f0][f1] = sum_p x[p,f0] x[p,f1] / P_q for f0 <= f1, the Gram
+// matrix of the q-bin's pixel x frame values -- the one dense contraction of
+// the reference, computed here with FP64 matrix-core instructions
+// (v_mfma_f64_16x16x4f64): integer counts give exact sums (every partial sum
+// an integer below 2^53), smoothed values are FP64 throughout.
+//
+//   k_tt_values    dense X[i][f] (P_q x F, FP64, pixel-major) from the rows
+//   k_tt_colsum    per (bin, frame) sums of X over a bin's pixels (smoothing)
+//   k_tt_divide    X[i][f] /= sg (non-zero entries only, as the reference
+//                  divides the events it holds)
+//   k_tt_gram      upper-triangle 64x64 tiles of X^T X / P_q; each wave owns a
+//                  32x32 quarter (2x2 MFMA blocks), pixels staged through LDS
+//   k_tt_diag      g2full and g2partials from the diagonals
+#include "device_util.h"
+#include "internal.h"
+
+namespace xpcs {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// One row of X per thread block: row i is compact pixel c0 + i.
+__global__ void __launch_bounds__(256)
+k_tt_values(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
+            const uint32_t* __restrict__ dense_rows, int32_t c0, int32_t n, int32_t F,
+            const double* __restrict__ weight, double scale, double* __restrict__ X)
+{
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int64_t c = c0 + i;
+        const double w = (weight ? weight[c] : 1.0) * scale;
+        double* xr = X + i * (int64_t)F;
+        if (dense_rows) {
+            const uint32_t* r = dense_rows + c * (int64_t)F;
+            for (int f = threadIdx.x; f < F; f += blockDim.x) xr[f] = (double)r[f] * w;
+        } else {
+            const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
+            const int64_t b = off[row], e = off[row + 1];
+            for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+                const uint64_t v = ev[k];
+                const uint32_t f = (uint32_t)(v >> 32);
+                // averaged raw frames share an output frame: several events may add here
+                if (f < (uint32_t)F) atomicAdd(&xr[f], (double)(uint32_t)v * w);
+            }
+        }
+    }
+}
+
+// sgsum[b][f] += X over the bin's pixels that lie in the chunk [lo, hi) (rows
+// of X are chunk-relative).  One thread per (bin, frame) adds the pixels in
+// order, so the sum does not depend on how the q-bin is chunked.
+__global__ void __launch_bounds__(256)
+k_tt_colsum(const double* __restrict__ X, int32_t F, int32_t lo, int32_t hi, const int32_t* __restrict__ bin_begin,
+            const int32_t* __restrict__ bin_end, int32_t nbins, double* __restrict__ sgsum)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int b = blockIdx.y; b < nbins; b += gridDim.y) {
+        const int i0 = max(bin_begin[b], lo), i1 = min(bin_end[b], hi);
+        if (i0 >= i1) continue;
+        double s = sgsum[(int64_t)b * F + f];
+        for (int i = i0; i < i1; i++) s += X[(int64_t)(i - lo) * F + f];
+        sgsum[(int64_t)b * F + f] = s;
+    }
+}
+
+// sg[b][f] = sgsum[b][f] / pixels of bin b (in place)
+__global__ void __launch_bounds__(256)
+k_tt_sgfinal(double* __restrict__ sg, int32_t F, const int32_t* __restrict__ bin_begin,
+             const int32_t* __restrict__ bin_end, int32_t nbins)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int b = blockIdx.y; b < nbins; b += gridDim.y) sg[(int64_t)b * F + f] /= (double)(bin_end[b] - bin_begin[b]);
+}
+
+// Mean over frames of each bin's sg row (smoothing filter "Average").
+__global__ void __launch_bounds__(256)
+k_tt_sgavg(const double* __restrict__ sg, int32_t F, int32_t nbins, double* __restrict__ sga)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    double s = 0.0;
+    for (int f = 0; f < F; f++) s += sg[(int64_t)b * F + f];
+    sga[b] = s / (double)F;
+}
+
+// X[i][f] /= sg of the pixel's bin (chunk row i is q-bin pixel lo + i); only
+// non-zero values are divided, as the reference divides the events it holds.
+__global__ void __launch_bounds__(256)
+k_tt_divide(double* __restrict__ X, int32_t F, int32_t lo, int32_t n, const int32_t* __restrict__ bin_of_row,
+            const double* __restrict__ sg, int32_t per_frame)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    for (int64_t i = blockIdx.y; i < n; i += gridDim.y) {
+        const int b = bin_of_row[lo + i];
+        const double d = per_frame ? sg[(int64_t)b * F + f] : sg[b];
+        double& x = X[i * F + f];
+        if (x != 0.0) x = x / d;
+    }
+}
+
+constexpr int kGramTile = 64;   // output tile (frames x frames)
+constexpr int kGramK = 16;      // pixels per LDS stage
+
+// Upper-triangle tiles (ti <= tj) of C += X^T X * inv_pq over one chunk of
+// pixels; the lower triangle is left as the caller cleared it.  256 threads: wave w owns rows
+// 32*(w/2)..+31 and columns 32*(w%2)..+31 of the tile as 2x2 MFMA blocks.
+// MFMA operand layout (16x16x4 FP64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; its result register r holds D[l/16 + 4r][l%16], r = 0..3.
+__global__ void __launch_bounds__(256)
+k_tt_gram(const double* __restrict__ X, int32_t n, int32_t F, int32_t ntile, double inv_pq,
+          double* __restrict__ C)
+{
+    __shared__ double sA[kGramK][kGramTile];
+    __shared__ double sB[kGramK][kGramTile];
+    // blockIdx.x enumerates the upper-triangle tiles row by row
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) {
+        t -= ntile - ti;
+        ti++;
+    }
+    const int tj = ti + t;
+    const int f0 = ti * kGramTile, f1 = tj * kGramTile;
+    const int lane = lane_id();
+    const int w = wave_id();
+    const int rb0 = (w >> 1) * 32, cb0 = (w & 1) * 32;
+    v4d acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int p0 = 0; p0 < n; p0 += kGramK) {
+        // stage kGramK pixel rows of both frame ranges (zero outside X)
+        for (int e = threadIdx.x; e < kGramK * kGramTile; e += blockDim.x) {
+            const int k = e / kGramTile, j = e % kGramTile;
+            const int p = p0 + k;
+            const bool pin = p < n;
+            sA[k][j] = (pin && f0 + j < F) ? X[(int64_t)p * F + f0 + j] : 0.0;
+            sB[k][j] = (pin && f1 + j < F) ? X[(int64_t)p * F + f1 + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < kGramK; k4 += 4) {
+            const int kk = k4 + (lane >> 4);
+            double av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) av[a] = sA[kk][rb0 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; b++) bv[b] = sB[kk][cb0 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = f0 + rb0 + a * 16 + (lane >> 4) + 4 * r;
+                const int col = f1 + cb0 + b * 16 + (lane & 15);
+                if (row < F && col < F && row <= col) C[(int64_t)row * F + col] += acc[a][b][r] * inv_pq;
+            }
+}
+
+// g2full[ff] = mean over fx < F-ff of C[fx][fx+ff]; for ff < wsize,
+// g2partials[ff][wn] = sum over fx in window wn (fx / wsize == wn) / wsize.
+// Thread per diagonal (sequential in fx, as the reference sums).
+__global__ void __launch_bounds__(256)
+k_tt_diag(const double* __restrict__ C, int32_t F, int32_t wsize, int32_t total, double* __restrict__ g2full,
+          double* __restrict__ g2part)
+{
+    const int ff = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ff >= F) return;
+    double s = 0.0;
+    double part = 0.0;
+    int wn = 0;
+    for (int fx = 0; fx < F - ff; fx++) {
+        const double v = C[(int64_t)fx * F + fx + ff];
+        s += v;
+        if (ff < wsize && total > 0) {
+            const int wcur = fx / wsize;
+            if (wcur != wn) {
+                if (wn < total) g2part[(int64_t)ff * total + wn] = part / (double)wsize;
+                part = 0.0;
+                wn = wcur;
+            }
+            if (wn < total) part += v;
+        }
+    }
+    g2full[ff] = s / (double)(F - ff);
+    if (ff < wsize && total > 0) {
+        if (wn < total) g2part[(int64_t)ff * total + wn] = part / (double)wsize;
+        for (int k = wn + 1; k < total; k++) g2part[(int64_t)ff * total + k] = 0.0;
+    }
+}
+
+// ---- host side --------------------------------------------------------------------
+void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, double* X, hipStream_t s)
+{
+    const int32_t F = p->h.F;
+    launch_fill_u64(reinterpret_cast<uint64_t*>(X), (size_t)n * F, s);
+    const unsigned g = (unsigned)std::min<int64_t>(std::max<int32_t>(n, 1), 65535);
+    k_tt_values<<<g, 256, 0, s>>>(p->mode == 2 ? nullptr : p->d_ev, p->d_off, p->d_rowmap,
+                                  p->mode == 2 ? p->d_rows : nullptr, c0, n, F, p->d_weight, scale, X);
+}
+
+void launch_tt_colsum(const double* X, int32_t F, int32_t lo, int32_t hi, const int32_t* d_bin_begin,
+                      const int32_t* d_bin_end, int32_t nbins, double* d_sgsum, hipStream_t s)
+{
+    dim3 g((F + 255) / 256, std::min(nbins, 65535));
+    k_tt_colsum<<<g, 256, 0, s>>>(X, F, lo, hi, d_bin_begin, d_bin_end, nbins, d_sgsum);
+}
+
+void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins,
+                       bool average, double* d_sga, hipStream_t s)
+{
+    dim3 g((F + 255) / 256, std::min(nbins, 65535));
+    k_tt_sgfinal<<<g, 256, 0, s>>>(d_sg, F, d_bin_begin, d_bin_end, nbins);
+    if (average) k_tt_sgavg<<<(nbins + 255) / 256, 256, 0, s>>>(d_sg, F, nbins, d_sga);
+}
+
+void launch_tt_divide(double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row, const double* d_div,
+                      bool per_frame, hipStream_t s)
+{
+    dim3 g((F + 255) / 256, std::min(std::max(n, 1), 65535));
+    k_tt_divide<<<g, 256, 0, s>>>(X, F, lo, n, d_bin_of_row, d_div, per_frame ? 1 : 0);
+}
+
+void launch_tt_gram(const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s)
+{
+    const int nt = (F + kGramTile - 1) / kGramTile;
+    const int ntiles = nt * (nt + 1) / 2;
+    k_tt_gram<<<ntiles, 256, 0, s>>>(X, n, F, nt, inv_pq, C);
+}
+
+void launch_tt_diag(const double* C, int32_t F, int32_t wsize, int32_t total, double* g2full, double* g2part,
+                    hipStream_t s)
+{
+    k_tt_diag<<<(F + 255) / 256, 256, 0, s>>>(C, F, wsize, total, g2full, g2part);
+}
+
+}  // namespace xpcs

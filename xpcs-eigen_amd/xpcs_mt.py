@@ -37,6 +37,7 @@ EXPORTS = [
     "xpcs_mt_batch_destroy", "xpcs_mt_stream_bytes", "xpcs_mt_stream_open", "xpcs_mt_stream_push_imm",
     "xpcs_mt_stream_push_imm_device", "xpcs_mt_stream_frames", "xpcs_mt_stream_snapshot",
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
+    "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
 ]
 XPCS_BATCH_AUTOTUNE = 0x1
 
@@ -118,6 +119,10 @@ def lib():
             "xpcs_mt_stream_read_g2": ([vp, vp, vp, vp], i32),
             "xpcs_mt_stream_read_sums_i64": ([vp, vp, vp, vp], i32),
             "xpcs_mt_stream_close": ([vp], i32),
+            "xpcs_mt_twotime": ([vp, vp, vp], i32),
+            "xpcs_mt_twotime_info": ([vp, vp, vp, i32, vp, vp, vp], i32),
+            "xpcs_mt_read_twotime": ([vp, i32, vp], i32),
+            "xpcs_mt_read_twotime_1d": ([vp, vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -179,6 +184,11 @@ def build_qmap(dqmap, sqmap):
                                 _p(g[3]), ng, _p(counts)))
     return dict(order=order[:P], n_mapped=nm, ngroups=ng, group_q=g[0][:ng], group_s=g[1][:ng],
                 group_begin=g[2][:ng], group_end=g[3][:ng], nq=int(counts[3]), ns=int(counts[4]))
+
+
+class TTDesc(C.Structure):
+    _fields_ = [("qbins", C.c_void_p), ("nq", C.c_int32), ("smoothing", C.c_int32), ("average", C.c_int32),
+                ("window", C.c_int32), ("chunk_bytes", C.c_int64)]
 
 
 class Tuning(C.Structure):
@@ -252,9 +262,13 @@ class Plan:
                                               off.size, 1 if compressed else 0))
 
     def ingest_rows(self, row_off, frame, count):
-        ro = np.ascontiguousarray(row_off, np.int64)
-        fr = np.ascontiguousarray(frame, np.int32)
-        ct = np.ascontiguousarray(count, np.uint32)
+        ro = np.ascontiguousarray(row_off, np.int64).reshape(-1)
+        fr = np.ascontiguousarray(frame, np.int32).reshape(-1)
+        ct = np.ascontiguousarray(count, np.uint32).reshape(-1)
+        if ro.size != self.width * self.height + 1:
+            raise ValueError(f"row_off needs width*height+1 = {self.width * self.height + 1} entries, got {ro.size}")
+        if fr.size < ro[-1] or ct.size < ro[-1]:
+            raise ValueError(f"frame/count hold fewer than row_off[-1] = {ro[-1]} events")
         check(lib().xpcs_mt_ingest_rows(self._h, _p(ro), _p(fr), _p(ct)))
 
     def execute(self, stream=None):
@@ -262,6 +276,28 @@ class Plan:
 
     def synchronize(self):
         check(lib().xpcs_mt_synchronize(self._h))
+
+    def twotime(self, qbins, smoothing=0, average=False, window=1, chunk_bytes=0):
+        """Two-time analysis (xpcs_mt_twotime); returns dict(qbins, c2t list,
+        g2full [F, nq], g2partials [window, total, nq], sg)."""
+        qb = np.ascontiguousarray(qbins, np.int32).reshape(-1)
+        d = TTDesc(_p(qb), qb.size, int(smoothing), 1 if average else 0, int(window), int(chunk_bytes))
+        check(lib().xpcs_mt_twotime(self._h, C.byref(d), None))
+        nq, rows, cols, tot = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        qs = np.zeros(max(qb.size, 1), np.int32)
+        check(lib().xpcs_mt_twotime_info(self._h, C.byref(nq), _p(qs), qs.size, C.byref(rows), C.byref(cols),
+                                         C.byref(tot)))
+        F, n = self.frames, nq.value
+        c2t = []
+        for i in range(n):
+            m = np.zeros((F, F), np.float64)
+            check(lib().xpcs_mt_read_twotime(self._h, i, _p(m)))
+            c2t.append(m)
+        full = np.zeros((F, n), np.float64)
+        part = np.zeros((int(window), tot.value, n), np.float64)
+        sg = np.zeros((rows.value, cols.value), np.float64) if rows.value else None
+        check(lib().xpcs_mt_read_twotime_1d(self._h, _p(full), _p(part) if part.size else None, _p(sg)))
+        return dict(qbins=qs[:n].tolist(), c2t=c2t, g2full=full, g2partials=part, sg=sg)
 
     def set_tuning(self, **shapes):
         t = Tuning()

@@ -43,12 +43,20 @@ def test_missing_key_is_reported(tmp_path):
     assert r.returncode == 1 and "configuration key missing" in r.stderr
 
 
-def test_twotime_is_refused(tmp_path):
+def test_twotime_unknown_smoothing_is_refused(tmp_path):
+    """main.cpp:161-164: two-time with a smoothing method other than StaticMap
+    or symmetric (any case) stops with status 1 before any work."""
+    job = make_job(tmp_path, F=50, analysis_type="Twotime", smoothing_method="Gaussian")
+    r = run(job["conf"])
+    assert r.returncode == 1 and "Smoothing method is not valid" in r.stderr
+
+
+def test_twotime_keys_are_required(tmp_path):
     job = make_job(tmp_path, F=50)
     with xpcs_h5.H5(job["conf"]) as h:
         h.write_string("/xpcs/analysis_type", "Twotime")
     r = run(job["conf"])
-    assert r.returncode == 2 and "Twotime" in r.stderr
+    assert r.returncode == 1 and "configuration key missing: /xpcs/smoothing_method" in r.stderr
 
 
 def test_usage():
@@ -119,3 +127,36 @@ def test_cli_exchange_and_positional_imm(tmp_path, orc):
         assert np.allclose(g2[ok], ref["g2"][ok], rtol=1e-6)
     bad = run(job["conf"], tmp_path / "nope.imm")
     assert bad.returncode == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,filt", [("StaticMap", "Average"), ("SYMMETRIC", "None")])
+def test_cli_twotime_matches_oracle(tmp_path, orc, method, filt):
+    """analysis_type Twotime: C2T_all/g2_%05d, g2full, g2partials and sg
+    (float, corr.cpp:433-560, :781-924) against the FP64 oracle; no tau/g2."""
+    job = make_job(tmp_path, F=120, analysis_type="Twotime", smoothing_method=method, smoothing_filter=filt,
+                   qphi_bins=[3, 1], twotime_window=4, static_window=30)
+    print("corr twotime job", method, filt, flush=True)
+    r = run(job["conf"])
+    assert r.returncode == 0, r.stderr
+    ex = orc.run_exact(job["imm"].read_bytes(), job["dq"], job["sq"], job["F"], 2)
+    smoothing = 1 if method == "StaticMap" else 2
+    want = orc.twotime_f64(ex["csr"], ex["qm"], job["F"], [3, 1], smoothing, filt == "Average", 4)
+    F = job["F"]
+    with xpcs_h5.H5(job["conf"]) as h:
+        for i, q in enumerate(want["qbins"]):
+            c2t = h.read(f"/exchange/C2T_all/g2_{q:05d}", np.float32)
+            assert c2t.shape == (F, F)
+            np.testing.assert_allclose(c2t, want["c2t"][i], rtol=1e-6, atol=0)
+        full = h.read("/exchange/g2full", np.float32)
+        assert full.shape == (F, 2)
+        np.testing.assert_allclose(full, want["g2full"], rtol=1e-6, atol=0)
+        part = h.read("/exchange/g2partials", np.float32)
+        assert part.shape == (4, (F - 4) // 4, 2)
+        np.testing.assert_allclose(part, want["g2partials"], rtol=1e-6, atol=0)
+        sg = h.read("/exchange/sg", np.float32)
+        assert sg.shape == want["sg"].shape
+        np.testing.assert_allclose(sg, want["sg"], rtol=1e-6, atol=0)
+        assert h.dims("/exchange/pixelSum") == (job["H"], job["W"])
+        with pytest.raises(Exception):
+            h.dims("/exchange/tau")

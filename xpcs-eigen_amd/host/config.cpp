@@ -2,6 +2,11 @@
 // Configuration::init, src/xpcs/configuration.cpp:80-242).
 #include "config.h"
 
+#include <algorithm>
+#include <cctype>
+#include <cstddef>
+#include <cstring>
+
 #include "h5lite.h"
 
 namespace xpcs {
@@ -58,6 +63,15 @@ struct Reader {
 };
 
 }  // namespace
+
+static bool iequal(const std::string& a, const char* b)
+{
+    const size_t n = std::strlen(b);
+    if (a.size() != n) return false;
+    for (size_t i = 0; i < n; i++)
+        if (std::toupper((unsigned char)a[i]) != std::toupper((unsigned char)b[i])) return false;
+    return true;
+}
 
 bool load_config(const std::string& path, const std::string& entry, JobConfig* c, std::string* err)
 {
@@ -138,7 +152,28 @@ bool load_config(const std::string& path, const std::string& entry, JobConfig* c
             return false;
         }
     }
+    if (f.exists(E + "/twotime2onetime_window_size"))
+        c->twotime_window = (int32_t)r.i64(E + "/twotime2onetime_window_size");
     c->analysis_type = r.str_opt(E + "/analysis_type");
+    // Two-time keys (configuration.cpp:216-232); string matches ignore case
+    // (Configuration::SmoothingMethod/SmoothingFilter, :784-806).
+    c->twotime = c->analysis_type == "Twotime";
+    if (c->twotime) {
+        const std::string m = r.str(E + "/smoothing_method"), flt = r.str(E + "/smoothing_filter");
+        c->smoothing = iequal(m, "StaticMap") ? 1 : iequal(m, "symmetric") ? 2 : 0;
+        c->smoothing_average = iequal(flt, "Average");
+        if (r.need(E + "/qphi_bin_to_process")) {
+            std::vector<h5::hsize_t> d;
+            std::vector<int32_t> v;
+            if (!f.read_i32(E + "/qphi_bin_to_process", &v, &d)) {
+                *err = "cannot read qphi_bin_to_process";
+                return false;
+            }
+            // the reference takes the first dimension's count of entries
+            const size_t n = std::min(v.size(), d.empty() ? (size_t)1 : (size_t)d[0]);
+            c->qphi_bins.assign(v.begin(), v.begin() + (ptrdiff_t)n);
+        }
+    }
     c->imm_path = r.str(E + "/input_file_local");
     if (!r.ok) return false;
     if (c->delays_per_level < 1 || c->stride < 1 || c->average < 1 || c->data_begin_todo < 1 ||

@@ -33,7 +33,12 @@ struct JobConfig {
     bool flatfield_enabled = false;
     std::vector<double> flatfield;   // /measurement/instrument/detector/flatfield
     float norm_factor = 1.f;         // partition_norm_factor (computed in float as the reference)
-    std::string analysis_type;       // "Twotime" selects the (unsupported) two-time analysis
+    std::string analysis_type;       // "Twotime" selects the two-time analysis
+    int32_t twotime_window = 1;      // twotime2onetime_window_size (optional, default 1)
+    bool twotime = false;            // analysis_type == "Twotime"
+    int32_t smoothing = 0;           // smoothing_method: 1 "StaticMap", 2 "symmetric" (any case), 0 other
+    bool smoothing_average = false;  // smoothing_filter == "Average" (any case)
+    std::vector<int32_t> qphi_bins;  // qphi_bin_to_process
 
     int32_t frames() const { return (int32_t)((data_end_todo - data_begin_todo + 1) / (stride * average)); }
     int32_t real_frames() const { return (int32_t)(data_end_todo - data_begin_todo + 1); }

@@ -1,20 +1,24 @@
-// `corr` -- drop-in replacement of the reference's multi-tau command line
-// (src/xpcs/main.cpp:86-479) running the correlator on the GPU through the C ABI.
+// `corr` -- drop-in replacement of the reference's command line
+// (src/xpcs/main.cpp:86-479): multi-tau, or two-time when <entry>/analysis_type
+// is "Twotime", running the correlator on the GPU through the C ABI.
 //
 //   corr <config.hdf5> [data.imm] [--imm=PATH] [--entry=/xpcs] [--exchange=GROUP]
-//        [--inpath=P --outpath=Q] [--g2out] [--darkout] [--quiet]
+//        [--inpath=P --outpath=Q] [--g2out] [--darkout] [--frame_threading] [--quiet]
 //
 // Flags use the gflags syntax of the reference (--name=value, --name value,
 // --bool / --nobool).  The IMM path comes from --imm, else the optional second
 // positional argument (as README.md documents), else <entry>/input_file_local,
 // with the --inpath prefix replaced by --outpath.  Results are written into the
 // configuration file under <entry>/output_data (or --exchange) with the
-// reference's dataset names, shapes and dtypes.
+// reference's dataset names, shapes and dtypes.  --frame_threading selects the
+// reference's other two-time loop order (corr.cpp:574-779); both compute the
+// same C2T, so it changes nothing here.
 //
-// Exit status: 0 ok, 1 usage or configuration error, 2 option outside this
-// build (two-time analysis, UFXC/Rigaku/HDF5 readers, --frameout,
-// normalize_by_framesum), 3 unreadable or malformed IMM, 4 GPU error,
-// 5 cannot write results.  (The reference exits 0 on most failures.)
+// Exit status: 0 ok, 1 usage or configuration error (including an unknown
+// two-time smoothing method), 2 option outside this build (UFXC/Rigaku/HDF5
+// readers, --frameout, normalize_by_framesum), 3 unreadable or malformed IMM,
+// 4 GPU error, 5 cannot write results.  (The reference exits 0 on most failures.)
+
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -170,7 +174,9 @@ int main(int argc, char** argv)
         Scope s("Configuration Total");
         if (!xpcs::load_config(fl.pos[0], entry, &cfg, &why)) return die(1, why);
     }
-    if (cfg.analysis_type == "Twotime") return die(2, "analysis_type Twotime (two-time correlation) is not built in this tool");
+    // main.cpp:161-164: two-time needs a known smoothing method
+    if (cfg.twotime && cfg.smoothing == 0) return die(1, "Smoothing method is not valid");
+    if (cfg.twotime && cfg.twotime_window < 1) return die(1, "twotime2onetime_window_size must be >= 1");
     if (cfg.normalize_by_framesum)
         return die(2, "normalize_by_framesum rescales counts by frame sums; the exact integer path does not build it");
     if (!flag_str(fl, "imm").empty()) cfg.imm_path = flag_str(fl, "imm");
@@ -249,6 +255,25 @@ int main(int argc, char** argv)
     std::vector<int32_t> lev((size_t)T), tau((size_t)T);
     xpcs_mt_schedule(plan, lev.data(), tau.data());
 
+    // two-time (Corr::twotime, corr.cpp:562-924): C2T per listed q-bin,
+    // g2full, g2partials and the smoothing image sg
+    int32_t tt_nq = 0, tt_rows = 0, tt_cols = 0, tt_total = 0;
+    std::vector<int32_t> tt_q;
+    if (cfg.twotime) {
+        Scope s("Computing G2 TwoTimes");
+        xpcs_tt_desc td{};
+        td.qbins = cfg.qphi_bins.data();
+        td.nq = (int32_t)cfg.qphi_bins.size();
+        td.smoothing = cfg.smoothing;
+        td.average = cfg.smoothing_average ? 1 : 0;
+        td.window = cfg.twotime_window;
+        st = xpcs_mt_twotime(plan, &td, nullptr);
+        tt_q.resize(std::max<size_t>(cfg.qphi_bins.size(), 1));
+        if (!st)
+            st = xpcs_mt_twotime_info(plan, &tt_nq, tt_q.data(), (int32_t)tt_q.size(), &tt_rows, &tt_cols, &tt_total);
+        if (st) return die(st == XPCS_ERR_RANGE ? 3 : 4, xpcs_last_error());
+    }
+
     // timestamps of the data frames (row 0: 1-based index, row 1: header value)
     const int32_t FR = cfg.real_frames();
     std::vector<double> clk((size_t)2 * FR, 0.0), tck((size_t)2 * FR, 0.0);
@@ -290,11 +315,38 @@ int main(int argc, char** argv)
                   out.write_f32(P("partition-mean-partial"), ppart.data(), V{(unsigned long long)std::max(nwin, 0), (unsigned long long)S}) &&
                   out.write_f32(P("partition_norm_factor"), &cfg.norm_factor, V{1, 1}) &&
                   out.write_f64(P("timestamp_clock"), clk.data(), V{2, (unsigned long long)FR}) &&
-                  out.write_f64(P("timestamp_tick"), tck.data(), V{2, (unsigned long long)FR}) &&
-                  out.write_f32(P("tau"), tauf.data(), V{1, (unsigned long long)T}) &&
-                  out.write_f32(P("norm-0-g2"), g2f.data(), V{(unsigned long long)T, (unsigned long long)Q}) &&
-                  out.write_f32(P("norm-0-stderr"), sef.data(), V{(unsigned long long)T, (unsigned long long)Q});
-        if (ok && flag_true(fl, "g2out")) {
+                  out.write_f64(P("timestamp_tick"), tck.data(), V{2, (unsigned long long)FR});
+        if (ok && !cfg.twotime)
+            ok = out.write_f32(P("tau"), tauf.data(), V{1, (unsigned long long)T}) &&
+                 out.write_f32(P("norm-0-g2"), g2f.data(), V{(unsigned long long)T, (unsigned long long)Q}) &&
+                 out.write_f32(P("norm-0-stderr"), sef.data(), V{(unsigned long long)T, (unsigned long long)Q});
+        if (ok && cfg.twotime) {
+            // C2T_all/g2_%05d [frames][frames] (chunked, deflate 6), g2full
+            // [frames][q], g2partials [window][partials][q], sg [rows][cols]
+            const size_t FF = (size_t)F * (size_t)F;
+            std::vector<double> c2t(FF);
+            std::vector<float> c2tf(FF);
+            for (int32_t i = 0; ok && i < tt_nq; i++) {
+                st = xpcs_mt_read_twotime(plan, i, c2t.data());
+                if (st) return die(4, xpcs_last_error());
+                for (size_t k = 0; k < FF; k++) c2tf[k] = (float)c2t[k];
+                char name[32];
+                std::snprintf(name, sizeof(name), "C2T_all/g2_%05d", tt_q[(size_t)i]);
+                ok = out.write_f32_deflate(P(name), c2tf.data(), V{(unsigned long long)F, (unsigned long long)F}, 6);
+            }
+            const size_t W = (size_t)cfg.twotime_window, tot = (size_t)tt_total, nq = (size_t)tt_nq;
+            std::vector<double> full((size_t)F * nq), part(W * tot * nq), sg((size_t)tt_rows * tt_cols);
+            if (ok) {
+                st = xpcs_mt_read_twotime_1d(plan, full.data(), part.data(), sg.data());
+                if (st) return die(4, xpcs_last_error());
+                auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+                const std::vector<float> fullf = f32(full), partf = f32(part), sgf = f32(sg);
+                ok = out.write_f32(P("g2full"), fullf.data(), V{(unsigned long long)F, nq}) &&
+                     out.write_f32(P("g2partials"), partf.data(), V{W, tot, nq}) &&
+                     out.write_f32(P("sg"), sgf.data(), V{(unsigned long long)tt_rows, (unsigned long long)tt_cols});
+            }
+        }
+        if (ok && flag_true(fl, "g2out") && !cfg.twotime) {
             const size_t n = (size_t)cfg.width * cfg.height * T;
             std::vector<float> a(n), b(n), c(n);
             st = xpcs_mt_read_ref_f32(plan, a.data(), b.data(), c.data());

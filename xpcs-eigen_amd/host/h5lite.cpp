@@ -47,6 +47,10 @@ struct Api {
     hid_t (*Pcreate)(hid_t);
     herr_t (*Pclose)(hid_t);
     herr_t (*Pset_create_intermediate_group)(hid_t, unsigned);
+    // optional (deflate-compressed datasets): absent symbols mean plain datasets
+    herr_t (*Pset_chunk)(hid_t, int, const hsize_t*) = nullptr;
+    herr_t (*Pset_deflate)(hid_t, unsigned) = nullptr;
+    hid_t* P_CLS_DATASET_CREATE = nullptr;
     hid_t* T_NATIVE_INT;
     hid_t* T_NATIVE_LLONG;
     hid_t* T_NATIVE_FLOAT;
@@ -101,6 +105,9 @@ void load()
               bind(g.T_NATIVE_FLOAT, "H5T_NATIVE_FLOAT_g") && bind(g.T_NATIVE_DOUBLE, "H5T_NATIVE_DOUBLE_g") &&
               bind(g.T_C_S1, "H5T_C_S1_g") && bind(g.P_CLS_LINK_CREATE, "H5P_CLS_LINK_CREATE_ID_g");
     if (!ok) return;
+    g.Pset_chunk = reinterpret_cast<decltype(g.Pset_chunk)>(dlsym(g.so, "H5Pset_chunk"));
+    g.Pset_deflate = reinterpret_cast<decltype(g.Pset_deflate)>(dlsym(g.so, "H5Pset_deflate"));
+    g.P_CLS_DATASET_CREATE = reinterpret_cast<hid_t*>(dlsym(g.so, "H5P_CLS_DATASET_CREATE_ID_g"));
     if (g.open() < 0) {
         g.why = "H5open failed";
         return;
@@ -279,19 +286,38 @@ bool File::read_string(const std::string& path, std::string* v) const
 }
 
 bool File::write_any(const std::string& path, hid_t memtype, hid_t filetype, const void* data,
-                     const std::vector<hsize_t>& d)
+                     const std::vector<hsize_t>& d, int deflate)
 {
     if (fid_ < 0) return false;
     if (exists(path)) g.Ldelete(fid_, path.c_str(), 0);
     hid_t lcpl = g.Pcreate(*g.P_CLS_LINK_CREATE);
     g.Pset_create_intermediate_group(lcpl, 1);
     hid_t sp = d.empty() ? g.Screate(0 /* H5S_SCALAR */) : g.Screate_simple((int)d.size(), d.data(), nullptr);
-    hid_t ds = g.Dcreate2(fid_, path.c_str(), filetype, sp, lcpl, 0, 0);
+    // chunked + deflate when asked for and available (chunks of at most 128 per dimension)
+    hid_t dcpl = 0;
+    bool nonzero = !d.empty();
+    for (hsize_t x : d) nonzero = nonzero && x > 0;
+    if (deflate >= 0 && nonzero && g.Pset_chunk && g.Pset_deflate && g.P_CLS_DATASET_CREATE) {
+        std::vector<hsize_t> cd(d.size());
+        for (size_t i = 0; i < d.size(); i++) cd[i] = d[i] < 128 ? d[i] : 128;
+        dcpl = g.Pcreate(*g.P_CLS_DATASET_CREATE);
+        if (dcpl < 0 || g.Pset_chunk(dcpl, (int)cd.size(), cd.data()) < 0 || g.Pset_deflate(dcpl, (unsigned)deflate) < 0) {
+            if (dcpl > 0) g.Pclose(dcpl);
+            dcpl = 0;
+        }
+    }
+    hid_t ds = g.Dcreate2(fid_, path.c_str(), filetype, sp, lcpl, dcpl, 0);
     bool ok = ds >= 0 && g.Dwrite(ds, memtype, 0, 0, 0, data) >= 0;
     if (ds >= 0) g.Dclose(ds);
+    if (dcpl > 0) g.Pclose(dcpl);
     g.Sclose(sp);
     g.Pclose(lcpl);
     return ok;
+}
+
+bool File::write_f32_deflate(const std::string& path, const float* data, const std::vector<hsize_t>& d, int level)
+{
+    return write_any(path, *g.T_NATIVE_FLOAT, *g.T_NATIVE_FLOAT, data, d, level);
 }
 
 bool File::write_f32(const std::string& path, const float* data, const std::vector<hsize_t>& d)

@@ -46,11 +46,13 @@ public:
     bool write_i32(const std::string& path, const int32_t* data, const std::vector<hsize_t>& d);
     bool write_i64(const std::string& path, const int64_t* data, const std::vector<hsize_t>& d);
     bool write_string(const std::string& path, const std::string& s);
+    // Chunked, deflate-compressed float dataset (plain when libhdf5 lacks the filter API).
+    bool write_f32_deflate(const std::string& path, const float* data, const std::vector<hsize_t>& d, int level);
     hid_t id() const { return fid_; }
 
 private:
     bool write_any(const std::string& path, hid_t memtype, hid_t filetype, const void* data,
-                   const std::vector<hsize_t>& d);
+                   const std::vector<hsize_t>& d, int deflate = -1);
     bool read_any(const std::string& path, hid_t memtype, void* buf, size_t elem, size_t cap,
                   std::vector<hsize_t>* d) const;
     hid_t fid_ = -1;

@@ -91,8 +91,11 @@ class H5:
 def write_config(path, *, width, height, dqmap, sqmap, imm_path, data_begin_todo=1, data_end_todo,
                  dpl, entry="/xpcs", output="/exchange", compression=True, stride=1, avg=1,
                  dark_begin_todo=0, dark_end_todo=0, lld=0.0, sigma=0.0, static_window=100,
-                 flatfield=None, int_dtype=np.int32, float_dtype=np.float32):
-    """Writes every key Configuration::init reads (src/xpcs/configuration.cpp:80-242)."""
+                 flatfield=None, int_dtype=np.int32, float_dtype=np.float32, analysis_type="Multitau",
+                 smoothing_method="StaticMap", smoothing_filter="None", qphi_bins=(1,), twotime_window=1):
+    """Writes every key Configuration::init reads (src/xpcs/configuration.cpp:80-242);
+    the two-time keys (smoothing_method, smoothing_filter, qphi_bin_to_process)
+    only with analysis_type "Twotime"."""
     I = lambda v: np.array([v], int_dtype)
     Fv = lambda v: np.array([v], float_dtype)
     det = "/measurement/instrument/detector/"
@@ -110,7 +113,7 @@ def write_config(path, *, width, height, dqmap, sqmap, imm_path, data_begin_todo
         h.write(entry + "/delays_per_level", I(dpl))
         h.write(entry + "/dark_begin_todo", I(dark_begin_todo))
         h.write(entry + "/dark_end_todo", I(dark_end_todo))
-        h.write(entry + "/twotime2onetime_window_size", I(1))
+        h.write(entry + "/twotime2onetime_window_size", I(twotime_window))
         h.write(entry + "/stride_frames", np.array([stride], np.int64))
         h.write(entry + "/avg_frames", np.array([avg], np.int64))
         h.write(entry + "/normalize_by_framesum", I(0))
@@ -125,5 +128,9 @@ def write_config(path, *, width, height, dqmap, sqmap, imm_path, data_begin_todo
         h.write_string(entry + "/flatfield_enabled", "ENABLED" if flatfield is not None else "DISABLED")
         if flatfield is not None:
             h.write(det + "flatfield", np.asarray(flatfield, np.float64).reshape(height, width))
-        h.write_string(entry + "/analysis_type", "Multitau")
+        h.write_string(entry + "/analysis_type", analysis_type)
+        if analysis_type == "Twotime":
+            h.write_string(entry + "/smoothing_method", smoothing_method)
+            h.write_string(entry + "/smoothing_filter", smoothing_filter)
+            h.write(entry + "/qphi_bin_to_process", np.asarray(qphi_bins, np.int64).reshape(-1, 1))
         h.write_string(entry + "/input_file_local", str(imm_path))

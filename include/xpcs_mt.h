@@ -294,9 +294,14 @@ xpcs_status xpcs_mt_batch_destroy(xpcs_mt_batch* batch);
  * times the flat field) after smoothing: smoothing 1 = "StaticMap" (divide by
  * the mean of the pixel's static bin, per frame or, with average, its mean
  * over frames), 2 = "symmetric" (the same per dynamic bin), 0 = none.  FP64 on
- * the device (integer counts give exact Gram sums).  A q-bin's values are
- * built in pixel chunks of at most chunk_bytes device bytes (0: 4 GiB), so
- * HBM holds the F x F results plus one chunk.  Results stay on the device
+ * the device; integer counts give exact sums while every C2T sum stays below
+ * 2^53.  The sums are formed per q-bin either from the event pairs of every
+ * pixel (the reference's loop, Sum_p n_p^2 / 2 products, FP64 atomics) or as
+ * a dense Gram product on the FP64 matrix cores (P_q x F^2 products);
+ * method 0 picks the cheaper one from the row lengths, 1 forces the Gram
+ * product, 2 the event pairs (sparse ingests only: dense frames always use
+ * the Gram product).  The Gram product builds a q-bin's values in pixel chunks
+ * of at most chunk_bytes device bytes (0: 4 GiB).  Results stay on the device
  * until xpcs_mt_twotime runs again or the plan is destroyed.
  */
 typedef struct xpcs_tt_desc {
@@ -305,13 +310,15 @@ typedef struct xpcs_tt_desc {
     int32_t smoothing;          /* 0 none, 1 StaticMap, 2 symmetric             */
     int32_t average;            /* smoothing_filter == "Average"                */
     int32_t window;             /* twotime2onetime_window_size (>= 1)           */
-    int64_t chunk_bytes;        /* value chunk size; 0 = default (4 GiB)        */
+    int32_t method;             /* 0 auto, 1 Gram product, 2 event pairs        */
+    int64_t chunk_bytes;        /* Gram value chunk size; 0 = default (4 GiB)   */
 } xpcs_tt_desc;
 
 xpcs_status xpcs_mt_twotime(xpcs_mt_plan* plan, const xpcs_tt_desc* desc, void* stream);
-/* Processed q-bins (ascending), rows/columns of sg, windows of g2partials. */
-xpcs_status xpcs_mt_twotime_info(xpcs_mt_plan* plan, int32_t* nq, int32_t* qbins, int32_t cap, int32_t* sg_rows,
-                                 int32_t* sg_cols, int32_t* total_partials);
+/* Processed q-bins (ascending) and the method each used (1 Gram, 2 event
+ * pairs; either array may be NULL), rows/columns of sg, windows of g2partials. */
+xpcs_status xpcs_mt_twotime_info(xpcs_mt_plan* plan, int32_t* nq, int32_t* qbins, int32_t* methods, int32_t cap,
+                                 int32_t* sg_rows, int32_t* sg_cols, int32_t* total_partials);
 /* C2T of the i-th processed q-bin, [frames][frames]. */
 xpcs_status xpcs_mt_read_twotime(xpcs_mt_plan* plan, int32_t i, double* c2t);
 /* g2full [frames][nq], g2partials [window][total_partials][nq] (the

@@ -162,6 +162,7 @@ struct xpcs_mt_plan {
     std::vector<double*> tt_c2t, tt_full, tt_part, tt_sg;
     std::vector<int32_t> tt_sg_rows;
     int32_t tt_sg_cols = 0, tt_total = 0, tt_wsize = 0;
+    std::vector<int32_t> tt_method;      // per processed q-bin: 1 Gram, 2 event pairs
     double* tt_x = nullptr;              // scratch: values of one pixel chunk [rows][F]
     size_t tt_x_cap = 0;
     size_t rows_cap = 0;
@@ -233,6 +234,11 @@ void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, cons
 void launch_tt_divide(double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row, const double* d_div,
                       bool per_frame, hipStream_t s);
 void launch_tt_gram(const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s);
+void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
+                     const double* d_div, bool per_frame, double* C, hipStream_t s);
+void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
+                         double* d_sgsum, hipStream_t s);
+void launch_tt_scale(double* C, int64_t n, double sc, hipStream_t s);
 void launch_tt_diag(const double* C, int32_t F, int32_t wsize, int32_t total, double* g2full, double* g2part,
                     hipStream_t s);
 void twotime_release(xpcs_mt_plan* p);

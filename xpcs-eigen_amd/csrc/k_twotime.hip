@@ -5,6 +5,16 @@
 // (v_mfma_f64_16x16x4f64): integer counts give exact sums (every partial sum
 // an integer below 2^53), smoothed values are FP64 throughout.
 //
+// Two ways to form the sums, chosen per q-bin by their cost:
+//   event pairs  the reference's own loop (every pixel's event pairs, corr.cpp
+//                :816-835), Sum_p n_p^2 / 2 products scattered into C with
+//                FP64 atomics -- far less work for sparse (photon-counting) data;
+//   Gram         P_q x F^2 dense FP64 products on the matrix cores -- for
+//                dense data, where nearly every (pixel, frame) holds a value.
+//
+//   k_tt_pairs     one wave per pixel, all event pairs -> C (unscaled)
+//   k_tt_sg_sparse per (bin, frame) sums of the values from the events
+//   k_tt_scale     C *= 1 / P_q after the pairs
 //   k_tt_values    dense X[i][f] (P_q x F, FP64, pixel-major) from the rows
 //   k_tt_colsum    per (bin, frame) sums of X over a bin's pixels (smoothing)
 //   k_tt_divide    X[i][f] /= sg (non-zero entries only, as the reference
@@ -100,6 +110,120 @@ k_tt_divide(double* __restrict__ X, int32_t F, int32_t lo, int32_t n, const int3
     }
 }
 
+// ---- event pairs ------------------------------------------------------------------
+__device__ __forceinline__ double read_lane_f64(double x, int l)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Event k of the row [b, b + n): output frame and value (count x flat field x
+// scale, divided by the smoothing image when `dv` is set).  Lanes past the row
+// get value 0.
+__device__ __forceinline__ void tt_event(const uint64_t* __restrict__ ev, int64_t b, int64_t n, int64_t k,
+                                         int32_t F, double w, const double* __restrict__ dv, int32_t per_frame,
+                                         uint32_t& f, double& v)
+{
+    f = 0u;
+    v = 0.0;
+    if (k < n) {
+        const uint64_t e = ev[b + k];
+        const uint32_t fe = (uint32_t)(e >> 32);
+        if (fe < (uint32_t)F) {
+            f = fe;
+            v = (double)(uint32_t)e * w;
+            if (dv) v = v / (per_frame ? dv[fe] : dv[0]);
+        }
+    }
+}
+
+// C[f_a][f_b] += v_a v_b over the unordered event pairs {a, b} of every pixel
+// (f_a <= f_b; a == b gives v_a^2 on the diagonal).  Two raw events of one
+// output frame (averaged frames) are one merged value in the reference, whose
+// square holds their cross term twice: such a pair adds 2 v_a v_b.  One wave
+// per pixel; events in blocks of 64, the first of each pair broadcast from a
+// lane, the second held by every lane.
+__global__ void __launch_bounds__(256)
+k_tt_pairs(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
+           int32_t c0, int32_t Pq, int32_t F, const double* __restrict__ weight, double scale,
+           const int32_t* __restrict__ bin_of_row, const double* __restrict__ div, int32_t per_frame,
+           double* __restrict__ C)
+{
+    const int lane = lane_id();
+    const int wpb = (int)(blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * wpb + wave_id(); i < Pq; i += (int64_t)gridDim.x * wpb) {
+        const int64_t c = c0 + i;
+        const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
+        const int64_t b = off[row];
+        const int64_t n = off[row + 1] - b;
+        if (n <= 0) continue;
+        const double w = (weight ? weight[c] : 1.0) * scale;
+        const double* dv = nullptr;
+        if (div) dv = per_frame ? div + (int64_t)bin_of_row[i] * F : div + bin_of_row[i];
+        for (int64_t I = 0; I < n; I += 64) {
+            uint32_t fa;
+            double va;
+            tt_event(ev, b, n, I + lane, F, w, dv, per_frame, fa, va);
+            const int na = (int)min<int64_t>(64, n - I);
+            for (int64_t J = I; J < n; J += 64) {
+                uint32_t fb;
+                double vb;
+                tt_event(ev, b, n, J + lane, F, w, dv, per_frame, fb, vb);
+                const bool live = J + lane < n;
+                for (int t = 0; t < na; t++) {
+                    const uint32_t f1 = read_lane_u32(fa, t);
+                    const double v1 = read_lane_f64(va, t);
+                    if (!live || (J == I && lane < t)) continue;
+                    double prod = v1 * vb;
+                    uint32_t r = f1, col = fb;
+                    if (J != I || lane != t) {
+                        if (r == col) prod += prod;
+                        else if (r > col) {
+                            r = fb;
+                            col = f1;
+                        }
+                    }
+                    unsafeAtomicAdd(&C[(int64_t)r * F + col], prod);
+                }
+            }
+        }
+    }
+}
+
+// sgsum[bin][f] += value of every event of the q-bin's pixels
+__global__ void __launch_bounds__(256)
+k_tt_sg_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
+               int32_t c0, int32_t Pq, int32_t F, const double* __restrict__ weight, double scale,
+               const int32_t* __restrict__ bin_of_row, double* __restrict__ sgsum)
+{
+    const int lane = lane_id();
+    const int wpb = (int)(blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * wpb + wave_id(); i < Pq; i += (int64_t)gridDim.x * wpb) {
+        const int64_t c = c0 + i;
+        const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
+        const int64_t b = off[row];
+        const int64_t n = off[row + 1] - b;
+        const double w = (weight ? weight[c] : 1.0) * scale;
+        double* dst = sgsum + (int64_t)bin_of_row[i] * F;
+        for (int64_t k = lane; k < n; k += 64) {
+            uint32_t f;
+            double v;
+            tt_event(ev, b, n, k, F, w, nullptr, 0, f, v);
+            if (v != 0.0) unsafeAtomicAdd(&dst[f], v);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_tt_scale(double* __restrict__ C, int64_t n, double s)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        C[i] *= s;
+}
+
+// ---- Gram ---------------------------------------------------------------------------
 constexpr int kGramTile = 64;   // output tile (frames x frames)
 constexpr int kGramK = 16;      // pixels per LDS stage
 
@@ -208,6 +332,30 @@ void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, doub
     const unsigned g = (unsigned)std::min<int64_t>(std::max<int32_t>(n, 1), 65535);
     k_tt_values<<<g, 256, 0, s>>>(p->mode == 2 ? nullptr : p->d_ev, p->d_off, p->d_rowmap,
                                   p->mode == 2 ? p->d_rows : nullptr, c0, n, F, p->d_weight, scale, X);
+}
+
+static unsigned wave_grid(int64_t items)
+{
+    return (unsigned)std::min<int64_t>(std::max<int64_t>((items + 3) / 4, 1), 65536);
+}
+
+void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
+                     const double* d_div, bool per_frame, double* C, hipStream_t s)
+{
+    k_tt_pairs<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight, scale,
+                                             d_bin_of_row, d_div, per_frame ? 1 : 0, C);
+}
+
+void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
+                         double* d_sgsum, hipStream_t s)
+{
+    k_tt_sg_sparse<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight,
+                                                 scale, d_bin_of_row, d_sgsum);
+}
+
+void launch_tt_scale(double* C, int64_t n, double sc, hipStream_t s)
+{
+    k_tt_scale<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(C, n, sc);
 }
 
 void launch_tt_colsum(const double* X, int32_t F, int32_t lo, int32_t hi, const int32_t* d_bin_begin,

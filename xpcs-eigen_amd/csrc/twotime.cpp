@@ -31,6 +31,7 @@ void twotime_release(xpcs_mt_plan* p)
     p->tt_sg.clear();
     p->tt_q.clear();
     p->tt_sg_rows.clear();
+    p->tt_method.clear();
     if (p->tt_x) (void)hipFree(p->tt_x);
     p->tt_x = nullptr;
     p->tt_x_cap = 0;
@@ -77,6 +78,9 @@ struct Scratch {
 };
 
 constexpr size_t kTtDefaultChunkBytes = (size_t)4 << 30;
+// Event pairs are taken when pairs * kPairVsGram < P_q F^2: one scattered
+// FP64 atomic add costs about as much as this many matrix-core products.
+constexpr double kPairVsGram = 256.0;
 
 }  // namespace
 
@@ -87,6 +91,9 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     if (d->window < 1) return fail(XPCS_ERR_INVALID_ARG, "twotime: window size must be >= 1");
     if (d->smoothing < 0 || d->smoothing > 2) return fail(XPCS_ERR_INVALID_ARG, "twotime: smoothing must be 0, 1 or 2");
     if (d->chunk_bytes < 0) return fail(XPCS_ERR_INVALID_ARG, "twotime: chunk_bytes must be >= 0");
+    if (d->method < 0 || d->method > 2) return fail(XPCS_ERR_INVALID_ARG, "twotime: method must be 0, 1 or 2");
+    if (d->method == 2 && p->mode == 2)
+        return fail(XPCS_ERR_UNSUPPORTED, "twotime: dense frames use the Gram product (method 0 or 1)");
     HostTables& h = p->h;
     const int32_t F = h.F;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
@@ -129,10 +136,29 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     p->tt_wsize = d->window;
     p->tt_total = total;
     p->tt_sg_cols = d->average ? 1 : F;
+    // method per q-bin: event pairs cost Sum_p n_p (n_p + 1) / 2 scattered
+    // atomic adds, the Gram product P_q F^2 matrix-core products
+    std::vector<int32_t> method(qs.size(), p->mode == 2 ? 1 : d->method);
+    if (p->mode != 2 && d->method == 0 && !qs.empty()) {
+        std::vector<int64_t> hoff((size_t)p->nrows + 1);
+        HIP_TRY(hipMemcpyAsync(hoff.data(), p->d_off, hoff.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (size_t k = 0; k < qs.size(); k++) {
+            const int g0 = h.q_first_group[(size_t)qs[k] - 1], g1 = h.q_first_group[(size_t)qs[k]];
+            const int32_t c0 = h.group_begin[(size_t)g0], c1 = h.group_end[(size_t)g1 - 1];
+            double pairs = 0.0;
+            for (int32_t c = c0; c < c1; c++) {
+                const size_t r = p->mode == 1 ? (size_t)h.order[(size_t)c] : (size_t)c;
+                const double n = (double)(hoff[r + 1] - hoff[r]);
+                pairs += n * (n + 1.0) * 0.5;
+            }
+            method[k] = pairs * kPairVsGram < (double)(c1 - c0) * (double)F * (double)F ? 2 : 1;
+        }
+    }
     int32_t maxpq = 0;
-    for (int32_t q : qs) {
-        const int g0 = h.q_first_group[(size_t)q - 1], g1 = h.q_first_group[(size_t)q];
-        maxpq = std::max(maxpq, h.group_end[(size_t)g1 - 1] - h.group_begin[(size_t)g0]);
+    for (size_t k = 0; k < qs.size(); k++) {
+        const int g0 = h.q_first_group[(size_t)qs[k] - 1], g1 = h.q_first_group[(size_t)qs[k]];
+        if (method[k] == 1) maxpq = std::max(maxpq, h.group_end[(size_t)g1 - 1] - h.group_begin[(size_t)g0]);
     }
     const size_t row_bytes = (size_t)F * sizeof(double);
     const size_t cb = d->chunk_bytes > 0 ? (size_t)d->chunk_bytes : kTtDefaultChunkBytes;
@@ -150,7 +176,9 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
         twotime_release(p);
         return st;
     };
-    for (int32_t q : qs) {
+    for (size_t k = 0; k < qs.size(); k++) {
+        const int32_t q = qs[k];
+        const bool pairs = method[k] == 2;
         const int g0 = h.q_first_group[(size_t)q - 1], g1 = h.q_first_group[(size_t)q];
         const int32_t c0 = h.group_begin[(size_t)g0];
         const int32_t Pq = h.group_end[(size_t)g1 - 1] - c0;
@@ -162,6 +190,7 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
         p->tt_full.push_back(full);
         p->tt_part.push_back(part);
         p->tt_q.push_back(q);
+        p->tt_method.push_back(method[k]);
         if (st) return bail(st);
         launch_fill_u64(reinterpret_cast<uint64_t*>(C), (size_t)F * F, s);
         const bool one_chunk = Pq <= chunk;
@@ -199,20 +228,29 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
             if (st) return bail(st);
             p->tt_sg_rows.push_back(nb);
             launch_fill_u64(reinterpret_cast<uint64_t*>(sg), (size_t)nb * F, s);
-            for (int32_t lo = 0; lo < Pq; lo += chunk) {
-                const int32_t n = std::min(chunk, Pq - lo);
-                launch_tt_values(p, c0 + lo, n, scale, X, s);
-                launch_tt_colsum(X, F, lo, lo + n, dbb, dbe, nb, sg, s);
+            if (pairs) {
+                launch_tt_sg_sparse(p, c0, Pq, scale, dbor, sg, s);
+            } else {
+                for (int32_t lo = 0; lo < Pq; lo += chunk) {
+                    const int32_t n = std::min(chunk, Pq - lo);
+                    launch_tt_values(p, c0 + lo, n, scale, X, s);
+                    launch_tt_colsum(X, F, lo, lo + n, dbb, dbe, nb, sg, s);
+                }
             }
             launch_tt_sgfinal(sg, F, dbb, dbe, nb, d->average != 0, sga, s);
             div = d->average ? sga : sg;
         }
-        for (int32_t lo = 0; lo < Pq; lo += chunk) {
-            const int32_t n = std::min(chunk, Pq - lo);
-            // with smoothing and one chunk the values are already in place
-            if (!(div && one_chunk)) launch_tt_values(p, c0 + lo, n, scale, X, s);
-            if (div) launch_tt_divide(X, F, lo, n, dbor, div, d->average == 0, s);
-            launch_tt_gram(X, n, F, 1.0 / (double)Pq, C, s);
+        if (pairs) {
+            launch_tt_pairs(p, c0, Pq, scale, dbor, div, d->average == 0, C, s);
+            launch_tt_scale(C, (int64_t)F * F, 1.0 / (double)Pq, s);
+        } else {
+            for (int32_t lo = 0; lo < Pq; lo += chunk) {
+                const int32_t n = std::min(chunk, Pq - lo);
+                // with smoothing and one chunk the values are already in place
+                if (!(div && one_chunk)) launch_tt_values(p, c0 + lo, n, scale, X, s);
+                if (div) launch_tt_divide(X, F, lo, n, dbor, div, d->average == 0, s);
+                launch_tt_gram(X, n, F, 1.0 / (double)Pq, C, s);
+            }
         }
         launch_tt_diag(C, F, d->window, total, full, part, s);
         const hipError_t e = hipGetLastError();
@@ -226,12 +264,15 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     return XPCS_OK;
 }
 
-extern "C" xpcs_status xpcs_mt_twotime_info(xpcs_mt_plan* p, int32_t* nq, int32_t* qbins, int32_t cap,
-                                            int32_t* sg_rows, int32_t* sg_cols, int32_t* total_partials)
+extern "C" xpcs_status xpcs_mt_twotime_info(xpcs_mt_plan* p, int32_t* nq, int32_t* qbins, int32_t* methods,
+                                            int32_t cap, int32_t* sg_rows, int32_t* sg_cols, int32_t* total_partials)
 {
     if (!p) return fail(XPCS_ERR_INVALID_ARG, "twotime_info: NULL plan");
     if (nq) *nq = (int32_t)p->tt_q.size();
-    for (int32_t i = 0; qbins && i < cap && i < (int32_t)p->tt_q.size(); i++) qbins[i] = p->tt_q[(size_t)i];
+    for (int32_t i = 0; i < cap && i < (int32_t)p->tt_q.size(); i++) {
+        if (qbins) qbins[i] = p->tt_q[(size_t)i];
+        if (methods) methods[i] = p->tt_method[(size_t)i];
+    }
     int32_t rows = 0;
     for (int32_t r : p->tt_sg_rows) rows += r;
     if (sg_rows) *sg_rows = rows;

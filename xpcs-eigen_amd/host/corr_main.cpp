@@ -270,7 +270,8 @@ int main(int argc, char** argv)
         st = xpcs_mt_twotime(plan, &td, nullptr);
         tt_q.resize(std::max<size_t>(cfg.qphi_bins.size(), 1));
         if (!st)
-            st = xpcs_mt_twotime_info(plan, &tt_nq, tt_q.data(), (int32_t)tt_q.size(), &tt_rows, &tt_cols, &tt_total);
+            st = xpcs_mt_twotime_info(plan, &tt_nq, tt_q.data(), nullptr, (int32_t)tt_q.size(), &tt_rows, &tt_cols,
+                                      &tt_total);
         if (st) return die(st == XPCS_ERR_RANGE ? 3 : 4, xpcs_last_error());
     }
 

@@ -120,7 +120,7 @@ def lib():
             "xpcs_mt_stream_read_sums_i64": ([vp, vp, vp, vp], i32),
             "xpcs_mt_stream_close": ([vp], i32),
             "xpcs_mt_twotime": ([vp, vp, vp], i32),
-            "xpcs_mt_twotime_info": ([vp, vp, vp, i32, vp, vp, vp], i32),
+            "xpcs_mt_twotime_info": ([vp, vp, vp, vp, i32, vp, vp, vp], i32),
             "xpcs_mt_read_twotime": ([vp, i32, vp], i32),
             "xpcs_mt_read_twotime_1d": ([vp, vp, vp, vp], i32),
         }
@@ -188,7 +188,7 @@ def build_qmap(dqmap, sqmap):
 
 class TTDesc(C.Structure):
     _fields_ = [("qbins", C.c_void_p), ("nq", C.c_int32), ("smoothing", C.c_int32), ("average", C.c_int32),
-                ("window", C.c_int32), ("chunk_bytes", C.c_int64)]
+                ("window", C.c_int32), ("method", C.c_int32), ("chunk_bytes", C.c_int64)]
 
 
 class Tuning(C.Structure):
@@ -277,16 +277,18 @@ class Plan:
     def synchronize(self):
         check(lib().xpcs_mt_synchronize(self._h))
 
-    def twotime(self, qbins, smoothing=0, average=False, window=1, chunk_bytes=0):
-        """Two-time analysis (xpcs_mt_twotime); returns dict(qbins, c2t list,
-        g2full [F, nq], g2partials [window, total, nq], sg)."""
+    def twotime(self, qbins, smoothing=0, average=False, window=1, method=0, chunk_bytes=0):
+        """Two-time analysis (xpcs_mt_twotime); returns dict(qbins, methods
+        (1 Gram, 2 event pairs), c2t list, g2full [F, nq], g2partials
+        [window, total, nq], sg)."""
         qb = np.ascontiguousarray(qbins, np.int32).reshape(-1)
-        d = TTDesc(_p(qb), qb.size, int(smoothing), 1 if average else 0, int(window), int(chunk_bytes))
+        d = TTDesc(_p(qb), qb.size, int(smoothing), 1 if average else 0, int(window), int(method), int(chunk_bytes))
         check(lib().xpcs_mt_twotime(self._h, C.byref(d), None))
         nq, rows, cols, tot = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         qs = np.zeros(max(qb.size, 1), np.int32)
-        check(lib().xpcs_mt_twotime_info(self._h, C.byref(nq), _p(qs), qs.size, C.byref(rows), C.byref(cols),
-                                         C.byref(tot)))
+        ms = np.zeros(max(qb.size, 1), np.int32)
+        check(lib().xpcs_mt_twotime_info(self._h, C.byref(nq), _p(qs), _p(ms), qs.size, C.byref(rows),
+                                         C.byref(cols), C.byref(tot)))
         F, n = self.frames, nq.value
         c2t = []
         for i in range(n):
@@ -297,7 +299,7 @@ class Plan:
         part = np.zeros((int(window), tot.value, n), np.float64)
         sg = np.zeros((rows.value, cols.value), np.float64) if rows.value else None
         check(lib().xpcs_mt_read_twotime_1d(self._h, _p(full), _p(part) if part.size else None, _p(sg)))
-        return dict(qbins=qs[:n].tolist(), c2t=c2t, g2full=full, g2partials=part, sg=sg)
+        return dict(qbins=qs[:n].tolist(), methods=ms[:n].tolist(), c2t=c2t, g2full=full, g2partials=part, sg=sg)
 
     def set_tuning(self, **shapes):
         t = Tuning()

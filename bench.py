@@ -204,6 +204,93 @@ class DeviceFrames:
         stream.push_imm_device(self.imm.data_ptr(), self.imm.numel(), self.off[a:b], self.dlen[a:b])
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, frames, reps=3, cpu=True):
+    """Two-time leg (analysis_type Twotime, StaticMap smoothing, window 64):
+    the sparse 1 Mpx data (first `frames` frames, every dynamic bin listed; the
+    event-pair path) and a dense 512x512 detector (2048 frames, 9 bins; the
+    FP64 matrix-core Gram path).  Wall time per job includes the transpose and
+    every two-time kernel; results stay in HBM."""
+    import torch
+    import synth
+    import xpcs_mt
+    out = {}
+    qbins = list(range(1, int(dq.max()) + 1))
+    plan = xpcs_mt.Plan(W, H, frames, 8, dq, sq, timing=True)
+    plan.ingest_imm_device(imm.data_ptr(), imm.numel(), payload_off[:frames], dlen[:frames], compressed=True)
+    plan.twotime_run(qbins, 1, False, 64)          # warm-up: allocations, code load
+    plan.kernel_times_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = plan.twotime_run(qbins, 1, False, 64)
+    wall = (time.perf_counter() - t0) / reps
+    kt = {k: dict(avg_ms=v[0] / max(v[1], 1), launches=v[1] // reps, ms_per_job=v[0] / reps)
+          for k, v in plan.kernel_times().items()}
+    # event pairs of the listed bins, from the row lengths of a CPU transpose
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    host = imm[: int(payload_off[frames - 1]) + 6 * int(dlen[frames - 1])].cpu().numpy()
+    qm = oracle.build_qmap(dq, sq)
+    fr = oracle.imm_index(host)
+    sel, of = oracle.select_frames(fr.size, frames)
+    csr = oracle.sparse_transpose(host, fr, sel, of, qm)
+    n = np.diff(csr["off"]).astype(np.float64)
+    pairs = float((n * (n + 1) / 2).sum())
+    pk = kt.get("k_tt_pairs", {})
+    sparse = {"workload": f"{W}x{H} sparse, {frames} frames, {len(qbins)} dynamic bins, StaticMap, window 64",
+              "methods": info["methods"], "ms_per_job": 1e3 * wall, "jobs_per_s": 1.0 / wall,
+              "event_pairs": pairs, "kernels": kt,
+              "pairs_per_s": pairs / (pk["ms_per_job"] * 1e-3) if pk else None}
+    if cpu:
+        nth = min(len(os.sched_getaffinity(0)), 16, len(qbins))
+        t1 = time.perf_counter()
+        oracle.twotime_pairs_c(csr, qm, frames, qbins, nthreads=nth)
+        cpu_s = time.perf_counter() - t1
+        sparse["cpu_baseline"] = {
+            "value": 1.0 / cpu_s, "unit": "jobs/s", "cores": nth, "kind": "port", "seconds": cpu_s,
+            "sample": f"the whole job's C2T: reference pair loop (corr.cpp:816-840) in C, FP64, one OpenMP "
+                      f"thread per q-bin, {len(qbins)} bins x {frames}^2 matrices (zeroing, pairs, 1/P_q); "
+                      f"no smoothing, no diagonals"}
+    plan.close()
+    del csr, host
+    out["sparse"] = sparse
+
+    Wd = Hd = 512
+    Fd = 2048
+    dqd, sqd = synth.annular_qmaps(Wd, Hd, 27, 9)
+    immd, pod, dld = synth.imm_device_dense(Wd, Hd, Fd, 5.0, 50.0, 1003, darks=2, device=dev)
+    torch.cuda.synchronize()
+    pd = xpcs_mt.Plan(Wd, Hd, Fd, 4, dqd, sqd, dark_frames=2, timing=True)
+    pd.ingest_imm_device(immd.data_ptr(), immd.numel(), pod, dld, compressed=False)
+    qd = list(range(1, 10))
+    pd.twotime_run(qd, 1, False, 64)
+    pd.kernel_times_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = pd.twotime_run(qd, 1, False, 64)
+    wall = (time.perf_counter() - t0) / reps
+    kt = {k: dict(avg_ms=v[0] / max(v[1], 1), launches=v[1] // reps, ms_per_job=v[0] / reps)
+          for k, v in pd.kernel_times().items()}
+    P = pd.P
+    # algorithmic FP64 flops of the Gram products: 2 per product, upper
+    # triangle with its diagonal, every pixel of the listed bins
+    flops = 2.0 * P * Fd * (Fd + 1) / 2
+    g = kt["k_tt_gram"]
+    tf = flops / (g["ms_per_job"] * 1e-3) / 1e12
+    out["dense"] = {"workload": f"{Wd}x{Hd} dense (2 dark frames), {Fd} frames, 9 dynamic bins, StaticMap, window 64",
+                    "methods": info["methods"], "ms_per_job": 1e3 * wall, "jobs_per_s": 1.0 / wall,
+                    "kernels": kt, "pixels": P,
+                    "roofline": {"bound": "mfma", "kernel": "k_tt_gram", "achieved": tf,
+                                 "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": tf / FP64_MFMA_PEAK_TFLOPS, "flops_per_job": flops}}
+    pd.close()
+    del immd
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
     """Streaming service: 16 concurrent synthetic detector streams at 200 frames/s
     (stream s: seed 5000 + s, square detector U[256, 1024], occupancy U[0.2 %, 3 %],
@@ -303,6 +390,8 @@ def main():
     ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
     ap.add_argument("--stream-seconds", type=float, default=20.0,
                     help="wall time of the streaming leg (0 = skip)")
+    ap.add_argument("--twotime-frames", type=int, default=8192,
+                    help="frames of the two-time leg on the sparse data (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB elementwise copy (read 1 GiB, write 1 GiB) three times, "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -429,6 +518,10 @@ def main():
 
     plan.close()
     plan2 = None
+    twotime = None
+    if args.twotime_frames > 0 and not dense:
+        twotime = run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, min(args.twotime_frames, F),
+                              cpu=not args.no_cpu)
     batched = run_batched(dev, args.batched_jobs, 3) if args.batched_jobs > 0 else None
     streaming = run_streaming(dev, args.stream_seconds) if args.stream_seconds > 0 else None
 
@@ -454,6 +547,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "streaming": streaming,
+        "twotime": twotime,
         "kernels": kern,
         "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
         "end_to_end_pcie": e2e,

@@ -81,6 +81,10 @@ def lib():
         L.xo_cpu_pipeline.argtypes = [u8p, C.c_int64, C.c_int, C.c_int, i32p, i32p, C.c_int,
                                       C.c_int, C.c_int, f32p, f32p, f64p]
         L.xo_cpu_pipeline.restype = C.c_int
+        L.xo_twotime_pairs.argtypes = [C.c_int64, C.c_int64, i64p, i32p, f64p, C.c_int, f64p]
+        L.xo_twotime_pairs.restype = None
+        L.xo_twotime_bins.argtypes = [C.c_int, i64p, i64p, i64p, i32p, f64p, C.c_int, f64p, C.c_int]
+        L.xo_twotime_bins.restype = None
         _LIB = L
     return _LIB
 
@@ -394,3 +398,21 @@ def twotime_pairs(csr, qm, frames, q, value_scale=1.0):
                     G[fr[j], fr[k]] += v[j] * v[k]
             n += 1
     return G / max(n, 1)
+
+
+def twotime_pairs_c(csr, qm, frames, qbins, value_scale=1.0, weight=None, nthreads=0):
+    """C2T of the listed q-bins by the reference's pair loop in C (FP64);
+    rows must hold one value per frame, frames ascending (the oracle's
+    transposes produce such rows)."""
+    bins = _twotime_bins(qm, qbins)
+    gb, ge = qm["group_begin"], qm["group_end"]
+    bb = np.array([gb[gs[0]] for _, gs in bins], np.int64)
+    be = np.array([ge[gs[-1]] for _, gs in bins], np.int64)
+    val = csr["cnt"].astype(np.float64) * value_scale
+    if weight is not None:
+        val *= np.repeat(np.asarray(weight, np.float64), np.diff(csr["off"]))
+    C = np.zeros((max(len(bins), 1), frames, frames))
+    lib().xo_twotime_bins(len(bins), bb, be, np.ascontiguousarray(csr["off"], np.int64),
+                          np.ascontiguousarray(csr["frame"], np.int32), np.ascontiguousarray(val), frames, C,
+                          nthreads)
+    return [q for q, _ in bins], C[:len(bins)]

@@ -180,6 +180,16 @@ int xo_cpu_pipeline(const uint8_t* bytes, int64_t nbytes, int width, int height,
                     const int32_t* dqmap, const int32_t* sqmap, int frames, int dpl,
                     int nthreads, float* g2_out, float* stderr_out, double* stage_seconds);
 
+/* ---- two-time (Corr::twotimeQBinThreading, corr.cpp:781-840) ------------ */
+/* C (F x F, FP64) of the compact pixels [c0, c1): every event pair j <= k of a
+ * row adds val_j val_k at C[f_j][f_k], then C /= (c1 - c0).  Rows hold one
+ * value per frame, frames ascending. */
+void xo_twotime_pairs(int64_t c0, int64_t c1, const int64_t* off, const int32_t* frame, const double* val,
+                      int frames, double* C);
+/* nbins q-bins (compact ranges) at once into C[nbins][F][F], OpenMP over bins. */
+void xo_twotime_bins(int nbins, const int64_t* bin_begin, const int64_t* bin_end, const int64_t* off,
+                     const int32_t* frame, const double* val, int frames, double* C, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

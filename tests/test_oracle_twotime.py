@@ -97,3 +97,16 @@ def test_unlisted_and_absent_qbins_are_skipped(orc):
     r = orc.twotime_f64(csr, qm, c["F"], [3, 1, 99])
     assert r["qbins"] == [1, 3]
     assert r["g2full"].shape == (40, 2)
+
+
+def test_c_pair_loop_matches_gram(orc):
+    """The C restatement of the reference pair loop (CPU baseline of the
+    two-time measurement) == the BLAS Gram form, with flat-field weights."""
+    c = cases.sparse_case(W=12, H=12, F=80, mu=0.3, tau_c=6.0, seed=23, S=6, Q=3)
+    ff = np.random.default_rng(24).uniform(0.5, 1.5, (12, 12))
+    csr, qm, vs, w = cases.twotime_inputs(orc, c, flatfield=ff)
+    r = orc.twotime_f64(csr, qm, c["F"], [1, 3], weight=w, value_scale=vs)
+    qs, Cc = orc.twotime_pairs_c(csr, qm, c["F"], [1, 3], value_scale=vs, weight=w, nthreads=2)
+    assert qs == r["qbins"]
+    for i in range(len(qs)):
+        np.testing.assert_allclose(Cc[i], r["c2t"][i], rtol=1e-13, atol=0)

@@ -82,7 +82,7 @@ void timer_end(xpcs_mt_plan* p, hipStream_t s)
     (void)hipEventRecord(p->pending.back().b, s);
 }
 
-static void collect_timers(xpcs_mt_plan* p)
+void collect_timers(xpcs_mt_plan* p)
 {
     for (auto& t : p->pending) {
         float ms = 0.f;

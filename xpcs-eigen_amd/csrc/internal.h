@@ -165,6 +165,10 @@ struct xpcs_mt_plan {
     std::vector<int32_t> tt_method;      // per processed q-bin: 1 Gram, 2 event pairs
     double* tt_x = nullptr;              // scratch: values of one pixel chunk [rows][F]
     size_t tt_x_cap = 0;
+    double* tt_dpart = nullptr;          // scratch: diagonal segment sums
+    size_t tt_dpart_cap = 0;
+    // result buffers of the previous call, reused by size by the next one
+    std::vector<std::pair<double*, size_t>> tt_pool;
     size_t rows_cap = 0;
     // dense streaming path: coarsened series between passes, boundary events
     uint32_t* d_series_a = nullptr;
@@ -227,20 +231,23 @@ xpcs_status prepare_execute(xpcs_mt_plan* p);
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus);
 // two-time (k_twotime.hip)
 void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, double* X, hipStream_t s);
-void launch_tt_colsum(const double* X, int32_t F, int32_t lo, int32_t hi, const int32_t* d_bin_begin,
-                      const int32_t* d_bin_end, int32_t nbins, double* d_sgsum, hipStream_t s);
+void launch_tt_colsum(xpcs_mt_plan* p, const double* X, int32_t F, int32_t lo, int32_t hi,
+                      const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins, double* d_sgsum,
+                      hipStream_t s);
 void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins,
                        bool average, double* d_sga, hipStream_t s);
-void launch_tt_divide(double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row, const double* d_div,
-                      bool per_frame, hipStream_t s);
-void launch_tt_gram(const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s);
+void launch_tt_divide(xpcs_mt_plan* p, double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row,
+                      const double* d_div, bool per_frame, hipStream_t s);
+void launch_tt_gram(xpcs_mt_plan* p, const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s);
 void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
                      const double* d_div, bool per_frame, double* C, hipStream_t s);
+void launch_tt_pair_count(xpcs_mt_plan* p, const int32_t* d_c_begin, const int32_t* d_c_end, int32_t nq, double* d_out,
+                          hipStream_t s);
 void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
                          double* d_sgsum, hipStream_t s);
-void launch_tt_scale(double* C, int64_t n, double sc, hipStream_t s);
-void launch_tt_diag(const double* C, int32_t F, int32_t wsize, int32_t total, double* g2full, double* g2part,
-                    hipStream_t s);
+int tt_diag_segments(int32_t F);
+void launch_tt_diag(xpcs_mt_plan* p, double* C, int32_t F, int32_t wsize, int32_t total, double rescale,
+                    double* partial, double* g2full, double* g2part, hipStream_t s);
 void twotime_release(xpcs_mt_plan* p);
 
 // online multi-tau (k_stream.hip)
@@ -280,5 +287,6 @@ void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float*
 // timing helpers (api.cpp)
 void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name);
 void timer_end(xpcs_mt_plan* p, hipStream_t s);
+void collect_timers(xpcs_mt_plan* p);   // after the stream has drained
 
 }  // namespace xpcs

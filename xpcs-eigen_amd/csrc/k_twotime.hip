@@ -14,14 +14,14 @@
 //
 //   k_tt_pairs     one wave per pixel, all event pairs -> C (unscaled)
 //   k_tt_sg_sparse per (bin, frame) sums of the values from the events
-//   k_tt_scale     C *= 1 / P_q after the pairs
 //   k_tt_values    dense X[i][f] (P_q x F, FP64, pixel-major) from the rows
 //   k_tt_colsum    per (bin, frame) sums of X over a bin's pixels (smoothing)
 //   k_tt_divide    X[i][f] /= sg (non-zero entries only, as the reference
 //                  divides the events it holds)
 //   k_tt_gram      upper-triangle 64x64 tiles of X^T X / P_q; each wave owns a
 //                  32x32 quarter (2x2 MFMA blocks), pixels staged through LDS
-//   k_tt_diag      g2full and g2partials from the diagonals
+//   k_tt_diag_*    g2full from segmented diagonal sums (scaling the event-pair
+//                  sums by 1 / P_q in the same pass), k_tt_partials g2partials
 #include "device_util.h"
 #include "internal.h"
 
@@ -56,20 +56,24 @@ k_tt_values(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, co
 }
 
 // sgsum[b][f] += X over the bin's pixels that lie in the chunk [lo, hi) (rows
-// of X are chunk-relative).  One thread per (bin, frame) adds the pixels in
-// order, so the sum does not depend on how the q-bin is chunked.
+// of X are chunk-relative); blockIdx.z takes a segment of kColSeg rows, whose
+// partial sum is added atomically.
+constexpr int kColSeg = 64;
 __global__ void __launch_bounds__(256)
 k_tt_colsum(const double* __restrict__ X, int32_t F, int32_t lo, int32_t hi, const int32_t* __restrict__ bin_begin,
             const int32_t* __restrict__ bin_end, int32_t nbins, double* __restrict__ sgsum)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
-    for (int b = blockIdx.y; b < nbins; b += gridDim.y) {
-        const int i0 = max(bin_begin[b], lo), i1 = min(bin_end[b], hi);
-        if (i0 >= i1) continue;
-        double s = sgsum[(int64_t)b * F + f];
-        for (int i = i0; i < i1; i++) s += X[(int64_t)(i - lo) * F + f];
-        sgsum[(int64_t)b * F + f] = s;
+    for (int64_t r0 = lo + (int64_t)blockIdx.z * kColSeg; r0 < hi; r0 += (int64_t)gridDim.z * kColSeg) {
+        const int r1 = (int)min<int64_t>(r0 + kColSeg, hi);
+        for (int b = blockIdx.y; b < nbins; b += gridDim.y) {
+            const int i0 = max(bin_begin[b], (int)r0), i1 = min(bin_end[b], r1);
+            if (i0 >= i1) continue;
+            double s = 0.0;
+            for (int i = i0; i < i1; i++) s += X[(int64_t)(i - lo) * F + f];
+            unsafeAtomicAdd(&sgsum[(int64_t)b * F + f], s);
+        }
     }
 }
 
@@ -192,6 +196,29 @@ k_tt_pairs(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, con
     }
 }
 
+// Event pairs Sum n (n + 1) / 2 of the compact pixels [c_begin[q], c_end[q]),
+// one block per q-bin (the cost estimate that picks the method).
+__global__ void __launch_bounds__(256)
+k_tt_pair_count(const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
+                const int32_t* __restrict__ c_begin, const int32_t* __restrict__ c_end, double* __restrict__ out)
+{
+    __shared__ double s_red[256];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int c = c_begin[q] + (int)threadIdx.x; c < c_end[q]; c += blockDim.x) {
+        const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
+        const double n = (double)(off[row + 1] - off[row]);
+        s += n * (n + 1.0) * 0.5;
+    }
+    s_red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = s_red[0];
+}
+
 // sgsum[bin][f] += value of every event of the q-bin's pixels
 __global__ void __launch_bounds__(256)
 k_tt_sg_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
@@ -216,24 +243,21 @@ k_tt_sg_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
     }
 }
 
-__global__ void __launch_bounds__(256)
-k_tt_scale(double* __restrict__ C, int64_t n, double s)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        C[i] *= s;
-}
-
 // ---- Gram ---------------------------------------------------------------------------
-constexpr int kGramTile = 64;   // output tile (frames x frames)
+constexpr int kGramTile = 128;  // output tile (frames x frames)
 constexpr int kGramK = 16;      // pixels per LDS stage
+constexpr int kGramLoads = kGramK * kGramTile / 256;   // doubles per thread and operand per stage
 
-// Upper-triangle tiles (ti <= tj) of C += X^T X * inv_pq over one chunk of
-// pixels; the lower triangle is left as the caller cleared it.  256 threads: wave w owns rows
-// 32*(w/2)..+31 and columns 32*(w%2)..+31 of the tile as 2x2 MFMA blocks.
+// Upper-triangle tiles (ti <= tj) of C += X^T X * inv_pq over the pixels
+// [p_begin, p_end) of split blockIdx.y; with several splits the tile sums are
+// added atomically.  256 threads: wave w owns rows 64*(w/2)..+63 and columns
+// 64*(w%2)..+63 of the tile as 4x4 MFMA blocks (on a diagonal tile the wave
+// below the diagonal only loads).  The next stage's pixels are loaded into
+// registers while the current one is multiplied from LDS.
 // MFMA operand layout (16x16x4 FP64): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; its result register r holds D[l/16 + 4r][l%16], r = 0..3.
 __global__ void __launch_bounds__(256)
-k_tt_gram(const double* __restrict__ X, int32_t n, int32_t F, int32_t ntile, double inv_pq,
+k_tt_gram(const double* __restrict__ X, int32_t n, int32_t F, int32_t ntile, int32_t ksplit, double inv_pq,
           double* __restrict__ C)
 {
     __shared__ double sA[kGramK][kGramTile];
@@ -245,82 +269,127 @@ k_tt_gram(const double* __restrict__ X, int32_t n, int32_t F, int32_t ntile, dou
         ti++;
     }
     const int tj = ti + t;
+    const bool diag = ti == tj;
     const int f0 = ti * kGramTile, f1 = tj * kGramTile;
+    const int p_begin = (int)((int64_t)n * blockIdx.y / ksplit);
+    const int p_end = (int)((int64_t)n * (blockIdx.y + 1) / ksplit);
     const int lane = lane_id();
     const int w = wave_id();
-    const int rb0 = (w >> 1) * 32, cb0 = (w & 1) * 32;
-    v4d acc[2][2];
+    const int rb0 = (w >> 1) * 64, cb0 = (w & 1) * 64;
+    const bool idle = diag && rb0 > cb0;   // block entirely below the diagonal
+    const double(*sBr)[kGramTile] = diag ? sA : sB;
+    v4d acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int p0 = 0; p0 < n; p0 += kGramK) {
-        // stage kGramK pixel rows of both frame ranges (zero outside X)
-        for (int e = threadIdx.x; e < kGramK * kGramTile; e += blockDim.x) {
+        for (int b = 0; b < 4; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    double ra[kGramLoads], rb[kGramLoads];
+    auto load = [&](int p0) {
+#pragma unroll
+        for (int i = 0; i < kGramLoads; i++) {
+            const int e = (int)threadIdx.x + i * 256;
             const int k = e / kGramTile, j = e % kGramTile;
             const int p = p0 + k;
-            const bool pin = p < n;
-            sA[k][j] = (pin && f0 + j < F) ? X[(int64_t)p * F + f0 + j] : 0.0;
-            sB[k][j] = (pin && f1 + j < F) ? X[(int64_t)p * F + f1 + j] : 0.0;
+            const bool pin = p < p_end;
+            ra[i] = (pin && f0 + j < F) ? X[(int64_t)p * F + f0 + j] : 0.0;
+            rb[i] = (!diag && pin && f1 + j < F) ? X[(int64_t)p * F + f1 + j] : 0.0;
+        }
+    };
+    load(p_begin);
+    for (int p0 = p_begin; p0 < p_end; p0 += kGramK) {
+#pragma unroll
+        for (int i = 0; i < kGramLoads; i++) {
+            const int e = (int)threadIdx.x + i * 256;
+            sA[e / kGramTile][e % kGramTile] = ra[i];
+            if (!diag) sB[e / kGramTile][e % kGramTile] = rb[i];
         }
         __syncthreads();
+        if (p0 + kGramK < p_end) load(p0 + kGramK);
+        if (!idle) {
 #pragma unroll
-        for (int k4 = 0; k4 < kGramK; k4 += 4) {
-            const int kk = k4 + (lane >> 4);
-            double av[2], bv[2];
+            for (int k4 = 0; k4 < kGramK; k4 += 4) {
+                const int kk = k4 + (lane >> 4);
+                double av[4], bv[4];
 #pragma unroll
-            for (int a = 0; a < 2; a++) av[a] = sA[kk][rb0 + a * 16 + (lane & 15)];
+                for (int a = 0; a < 4; a++) av[a] = sA[kk][rb0 + a * 16 + (lane & 15)];
 #pragma unroll
-            for (int b = 0; b < 2; b++) bv[b] = sB[kk][cb0 + b * 16 + (lane & 15)];
+                for (int b = 0; b < 4; b++) bv[b] = sBr[kk][cb0 + b * 16 + (lane & 15)];
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+                for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    for (int b = 0; b < 4; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
+    if (idle) return;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < 4; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = f0 + rb0 + a * 16 + (lane >> 4) + 4 * r;
                 const int col = f1 + cb0 + b * 16 + (lane & 15);
-                if (row < F && col < F && row <= col) C[(int64_t)row * F + col] += acc[a][b][r] * inv_pq;
+                if (row < F && col < F && row <= col) {
+                    double* c = C + (int64_t)row * F + col;
+                    const double v = acc[a][b][r] * inv_pq;
+                    if (ksplit > 1)
+                        unsafeAtomicAdd(c, v);
+                    else
+                        *c += v;
+                }
             }
 }
 
-// g2full[ff] = mean over fx < F-ff of C[fx][fx+ff]; for ff < wsize,
-// g2partials[ff][wn] = sum over fx in window wn (fx / wsize == wn) / wsize.
-// Thread per diagonal (sequential in fx, as the reference sums).
+// Diagonal sums, pass 1: thread (ff, segment) sums C[fx][fx + ff] over the
+// segment's fx (threads of a block read consecutive columns of one row).  With
+// `scale` != 1 every upper-triangle element is scaled in the same pass (each is
+// visited exactly once), which finishes the event-pair sums.
 __global__ void __launch_bounds__(256)
-k_tt_diag(const double* __restrict__ C, int32_t F, int32_t wsize, int32_t total, double* __restrict__ g2full,
-          double* __restrict__ g2part)
+k_tt_diag_seg(double* __restrict__ C, int32_t F, int32_t seg_len, double scale, int32_t rescale,
+              double* __restrict__ partial)
+{
+    const int ff = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ff >= F) return;
+    const int fx0 = blockIdx.y * seg_len;
+    const int fx1 = min(fx0 + seg_len, F - ff);
+    double s = 0.0;
+    for (int fx = fx0; fx < fx1; fx++) {
+        double* c = C + (int64_t)fx * F + fx + ff;
+        double x = *c;
+        if (rescale) {
+            x *= scale;
+            *c = x;
+        }
+        s += x;
+    }
+    partial[(int64_t)blockIdx.y * F + ff] = s;
+}
+
+// pass 2: g2full[ff] = sum of the segment sums / (F - ff)
+__global__ void __launch_bounds__(256)
+k_tt_diag_final(const double* __restrict__ partial, int32_t nseg, int32_t F, double* __restrict__ g2full)
 {
     const int ff = blockIdx.x * blockDim.x + threadIdx.x;
     if (ff >= F) return;
     double s = 0.0;
-    double part = 0.0;
-    int wn = 0;
-    for (int fx = 0; fx < F - ff; fx++) {
-        const double v = C[(int64_t)fx * F + fx + ff];
-        s += v;
-        if (ff < wsize && total > 0) {
-            const int wcur = fx / wsize;
-            if (wcur != wn) {
-                if (wn < total) g2part[(int64_t)ff * total + wn] = part / (double)wsize;
-                part = 0.0;
-                wn = wcur;
-            }
-            if (wn < total) part += v;
-        }
-    }
+    for (int k = 0; k < nseg; k++) s += partial[(int64_t)k * F + ff];
     g2full[ff] = s / (double)(F - ff);
-    if (ff < wsize && total > 0) {
-        if (wn < total) g2part[(int64_t)ff * total + wn] = part / (double)wsize;
-        for (int k = wn + 1; k < total; k++) g2part[(int64_t)ff * total + k] = 0.0;
+}
+
+// g2partials[ff][w] = sum of C[fx][fx + ff] over window w (fx / wsize == w) / wsize,
+// ff < wsize; one block per window.
+__global__ void __launch_bounds__(256)
+k_tt_partials(const double* __restrict__ C, int32_t F, int32_t wsize, int32_t total, double* __restrict__ part)
+{
+    const int w = blockIdx.x;
+    if (w >= total) return;
+    for (int ff = threadIdx.x; ff < wsize; ff += blockDim.x) {
+        double s = 0.0;
+        for (int fx = w * wsize; fx < (w + 1) * wsize; fx++) s += C[(int64_t)fx * F + fx + ff];
+        part[(int64_t)ff * total + w] = s / (double)wsize;
     }
 }
 
@@ -328,10 +397,12 @@ k_tt_diag(const double* __restrict__ C, int32_t F, int32_t wsize, int32_t total,
 void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, double* X, hipStream_t s)
 {
     const int32_t F = p->h.F;
+    timer_begin(p, s, "k_tt_values");
     launch_fill_u64(reinterpret_cast<uint64_t*>(X), (size_t)n * F, s);
     const unsigned g = (unsigned)std::min<int64_t>(std::max<int32_t>(n, 1), 65535);
     k_tt_values<<<g, 256, 0, s>>>(p->mode == 2 ? nullptr : p->d_ev, p->d_off, p->d_rowmap,
                                   p->mode == 2 ? p->d_rows : nullptr, c0, n, F, p->d_weight, scale, X);
+    timer_end(p, s);
 }
 
 static unsigned wave_grid(int64_t items)
@@ -342,27 +413,36 @@ static unsigned wave_grid(int64_t items)
 void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
                      const double* d_div, bool per_frame, double* C, hipStream_t s)
 {
+    timer_begin(p, s, "k_tt_pairs");
     k_tt_pairs<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight, scale,
                                              d_bin_of_row, d_div, per_frame ? 1 : 0, C);
+    timer_end(p, s);
+}
+
+void launch_tt_pair_count(xpcs_mt_plan* p, const int32_t* d_c_begin, const int32_t* d_c_end, int32_t nq, double* d_out,
+                          hipStream_t s)
+{
+    k_tt_pair_count<<<nq, 256, 0, s>>>(p->d_off, p->d_rowmap, d_c_begin, d_c_end, d_out);
 }
 
 void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,
                          double* d_sgsum, hipStream_t s)
 {
+    timer_begin(p, s, "k_tt_sg_sparse");
     k_tt_sg_sparse<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight,
                                                  scale, d_bin_of_row, d_sgsum);
+    timer_end(p, s);
 }
 
-void launch_tt_scale(double* C, int64_t n, double sc, hipStream_t s)
+void launch_tt_colsum(xpcs_mt_plan* p, const double* X, int32_t F, int32_t lo, int32_t hi,
+                      const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins, double* d_sgsum,
+                      hipStream_t s)
 {
-    k_tt_scale<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(C, n, sc);
-}
-
-void launch_tt_colsum(const double* X, int32_t F, int32_t lo, int32_t hi, const int32_t* d_bin_begin,
-                      const int32_t* d_bin_end, int32_t nbins, double* d_sgsum, hipStream_t s)
-{
-    dim3 g((F + 255) / 256, std::min(nbins, 65535));
+    dim3 g((F + 255) / 256, std::min(nbins, 65535),
+           (unsigned)std::min<int64_t>(((int64_t)hi - lo + kColSeg - 1) / kColSeg, 65535));
+    timer_begin(p, s, "k_tt_colsum");
     k_tt_colsum<<<g, 256, 0, s>>>(X, F, lo, hi, d_bin_begin, d_bin_end, nbins, d_sgsum);
+    timer_end(p, s);
 }
 
 void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, const int32_t* d_bin_end, int32_t nbins,
@@ -373,24 +453,43 @@ void launch_tt_sgfinal(double* d_sg, int32_t F, const int32_t* d_bin_begin, cons
     if (average) k_tt_sgavg<<<(nbins + 255) / 256, 256, 0, s>>>(d_sg, F, nbins, d_sga);
 }
 
-void launch_tt_divide(double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row, const double* d_div,
-                      bool per_frame, hipStream_t s)
+void launch_tt_divide(xpcs_mt_plan* p, double* X, int32_t F, int32_t lo, int32_t n, const int32_t* d_bin_of_row,
+                      const double* d_div, bool per_frame, hipStream_t s)
 {
     dim3 g((F + 255) / 256, std::min(std::max(n, 1), 65535));
+    timer_begin(p, s, "k_tt_divide");
     k_tt_divide<<<g, 256, 0, s>>>(X, F, lo, n, d_bin_of_row, d_div, per_frame ? 1 : 0);
+    timer_end(p, s);
 }
 
-void launch_tt_gram(const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s)
+void launch_tt_gram(xpcs_mt_plan* p, const double* X, int32_t n, int32_t F, double inv_pq, double* C, hipStream_t s)
 {
     const int nt = (F + kGramTile - 1) / kGramTile;
     const int ntiles = nt * (nt + 1) / 2;
-    k_tt_gram<<<ntiles, 256, 0, s>>>(X, n, F, nt, inv_pq, C);
+    // split the pixels when the tiles alone cannot fill the chip (>= 256 pixels a split)
+    const int ksplit = std::max(1, std::min((1024 + ntiles - 1) / ntiles, n / 256));
+    timer_begin(p, s, "k_tt_gram");
+    k_tt_gram<<<dim3(ntiles, ksplit), 256, 0, s>>>(X, n, F, nt, ksplit, inv_pq, C);
+    timer_end(p, s);
 }
 
-void launch_tt_diag(const double* C, int32_t F, int32_t wsize, int32_t total, double* g2full, double* g2part,
-                    hipStream_t s)
+int tt_diag_segments(int32_t F)
 {
-    k_tt_diag<<<(F + 255) / 256, 256, 0, s>>>(C, F, wsize, total, g2full, g2part);
+    const int len = std::max(16, (F + 127) / 128);
+    return (F + len - 1) / len;
+}
+
+void launch_tt_diag(xpcs_mt_plan* p, double* C, int32_t F, int32_t wsize, int32_t total, double rescale,
+                    double* partial, double* g2full, double* g2part, hipStream_t s)
+{
+    const int nseg = tt_diag_segments(F);
+    const int len = (F + nseg - 1) / nseg;
+    timer_begin(p, s, "k_tt_diag");
+    dim3 g((F + 255) / 256, nseg);
+    k_tt_diag_seg<<<g, 256, 0, s>>>(C, F, len, rescale, rescale != 1.0 ? 1 : 0, partial);
+    k_tt_diag_final<<<(F + 255) / 256, 256, 0, s>>>(partial, nseg, F, g2full);
+    if (total > 0) k_tt_partials<<<total, 256, 0, s>>>(C, F, wsize, total, g2part);
+    timer_end(p, s);
 }
 
 }  // namespace xpcs

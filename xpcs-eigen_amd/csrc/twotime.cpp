@@ -25,6 +25,8 @@ void twotime_release(xpcs_mt_plan* p)
     for (auto* v : {&p->tt_c2t, &p->tt_full, &p->tt_part, &p->tt_sg})
         for (double* d : *v)
             if (d) (void)hipFree(d);
+    for (auto& b : p->tt_pool) (void)hipFree(b.first);
+    p->tt_pool.clear();
     p->tt_c2t.clear();
     p->tt_full.clear();
     p->tt_part.clear();
@@ -35,6 +37,9 @@ void twotime_release(xpcs_mt_plan* p)
     if (p->tt_x) (void)hipFree(p->tt_x);
     p->tt_x = nullptr;
     p->tt_x_cap = 0;
+    if (p->tt_dpart) (void)hipFree(p->tt_dpart);
+    p->tt_dpart = nullptr;
+    p->tt_dpart_cap = 0;
 }
 
 }  // namespace xpcs
@@ -48,6 +53,56 @@ xpcs_status talloc(T** d, size_t n)
     if (hipMalloc(reinterpret_cast<void**>(d), std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
         return fail(XPCS_ERR_NOMEM, "twotime: hipMalloc of %zu bytes failed", n * sizeof(T));
     return XPCS_OK;
+}
+
+// The previous call's results become the pool of this one.
+void recycle(xpcs_mt_plan* p)
+{
+    auto keep = [&](std::vector<double*>& v, auto size_of) {
+        for (size_t i = 0; i < v.size(); i++)
+            if (v[i]) p->tt_pool.emplace_back(v[i], size_of(i));
+        v.clear();
+    };
+    const size_t F = (size_t)p->h.F, W = (size_t)std::max(p->tt_wsize, 0), tot = (size_t)std::max(p->tt_total, 0);
+    keep(p->tt_c2t, [&](size_t) { return F * F; });
+    keep(p->tt_full, [&](size_t) { return F; });
+    keep(p->tt_part, [&](size_t) { return std::max<size_t>(W * tot, 1); });
+    keep(p->tt_sg, [&](size_t i) {
+        return std::max<size_t>((size_t)p->tt_sg_rows[i] * (p->tt_sg_cols == 1 ? 1 : F), 1);
+    });
+    p->tt_q.clear();
+    p->tt_sg_rows.clear();
+    p->tt_method.clear();
+}
+
+// A pooled buffer of at least n doubles (the smallest that fits, if it wastes
+// less than half), else a new one.
+xpcs_status take(xpcs_mt_plan* p, double** d, size_t n)
+{
+    n = std::max<size_t>(n, 1);
+    size_t best = p->tt_pool.size();
+    for (size_t i = 0; i < p->tt_pool.size(); i++) {
+        const size_t c = p->tt_pool[i].second;
+        if (c >= n && c <= 2 * n && (best == p->tt_pool.size() || c < p->tt_pool[best].second)) best = i;
+    }
+    if (best < p->tt_pool.size()) {
+        *d = p->tt_pool[best].first;
+        p->tt_pool.erase(p->tt_pool.begin() + (ptrdiff_t)best);
+        return XPCS_OK;
+    }
+    return talloc(d, n);
+}
+
+// A plan-owned scratch buffer of at least n doubles.
+xpcs_status grow(double** d, size_t* cap, size_t n)
+{
+    if (*cap >= n && *d) return XPCS_OK;
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    *cap = 0;
+    xpcs_status st = talloc(d, n);
+    if (!st) *cap = n;
+    return st;
 }
 
 // Device buffers that live for one xpcs_mt_twotime call: released once the
@@ -97,7 +152,9 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     HostTables& h = p->h;
     const int32_t F = h.F;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : p->stream;
-    twotime_release(p);
+    // the previous results may still be read by work queued on another stream
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    recycle(p);
     // pixel-major values of the ingested data
     if (p->mode == 1) {
         xpcs_status st = prepare_execute(p);
@@ -139,21 +196,25 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     // method per q-bin: event pairs cost Sum_p n_p (n_p + 1) / 2 scattered
     // atomic adds, the Gram product P_q F^2 matrix-core products
     std::vector<int32_t> method(qs.size(), p->mode == 2 ? 1 : d->method);
+    Scratch tmp{s, {}};
     if (p->mode != 2 && d->method == 0 && !qs.empty()) {
-        std::vector<int64_t> hoff((size_t)p->nrows + 1);
-        HIP_TRY(hipMemcpyAsync(hoff.data(), p->d_off, hoff.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<int32_t> cb(qs.size()), ce(qs.size());
         for (size_t k = 0; k < qs.size(); k++) {
-            const int g0 = h.q_first_group[(size_t)qs[k] - 1], g1 = h.q_first_group[(size_t)qs[k]];
-            const int32_t c0 = h.group_begin[(size_t)g0], c1 = h.group_end[(size_t)g1 - 1];
-            double pairs = 0.0;
-            for (int32_t c = c0; c < c1; c++) {
-                const size_t r = p->mode == 1 ? (size_t)h.order[(size_t)c] : (size_t)c;
-                const double n = (double)(hoff[r + 1] - hoff[r]);
-                pairs += n * (n + 1.0) * 0.5;
-            }
-            method[k] = pairs * kPairVsGram < (double)(c1 - c0) * (double)F * (double)F ? 2 : 1;
+            cb[k] = h.group_begin[(size_t)h.q_first_group[(size_t)qs[k] - 1]];
+            ce[k] = h.group_end[(size_t)h.q_first_group[(size_t)qs[k]] - 1];
         }
+        int32_t *dcb = nullptr, *dce = nullptr;
+        double* dpairs = nullptr;
+        xpcs_status st = tmp.upload(&dcb, cb);
+        if (!st) st = tmp.upload(&dce, ce);
+        if (!st) st = tmp.alloc(&dpairs, qs.size());
+        if (st) return st;
+        launch_tt_pair_count(p, dcb, dce, (int32_t)qs.size(), dpairs, s);
+        std::vector<double> pairs(qs.size());
+        HIP_TRY(hipMemcpyAsync(pairs.data(), dpairs, sizeof(double) * qs.size(), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (size_t k = 0; k < qs.size(); k++)
+            method[k] = pairs[k] * kPairVsGram < (double)(ce[k] - cb[k]) * (double)F * (double)F ? 2 : 1;
     }
     int32_t maxpq = 0;
     for (size_t k = 0; k < qs.size(); k++) {
@@ -164,12 +225,15 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     const size_t cb = d->chunk_bytes > 0 ? (size_t)d->chunk_bytes : kTtDefaultChunkBytes;
     const int32_t chunk = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)maxpq, cb / row_bytes));
     if (maxpq > 0) {
-        xpcs_status st = talloc(&p->tt_x, (size_t)chunk * F);
+        xpcs_status st = grow(&p->tt_x, &p->tt_x_cap, (size_t)chunk * F);
         if (st) return st;
-        p->tt_x_cap = (size_t)chunk * F;
     }
-    Scratch tmp{s, {}};
     double* X = p->tt_x;
+    if (!qs.empty()) {
+        xpcs_status st = grow(&p->tt_dpart, &p->tt_dpart_cap, (size_t)tt_diag_segments(F) * F);
+        if (st) return st;
+    }
+    double* dpart = p->tt_dpart;   // diagonal segment sums
     // an error leaves no partial results behind
     auto bail = [&](xpcs_status st) {
         (void)hipStreamSynchronize(s);
@@ -183,9 +247,9 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
         const int32_t c0 = h.group_begin[(size_t)g0];
         const int32_t Pq = h.group_end[(size_t)g1 - 1] - c0;
         double *C = nullptr, *full = nullptr, *part = nullptr;
-        xpcs_status st = talloc(&C, (size_t)F * F);
-        if (!st) st = talloc(&full, (size_t)F);
-        if (!st) st = talloc(&part, (size_t)d->window * (size_t)std::max(total, 0));
+        xpcs_status st = take(p, &C, (size_t)F * F);
+        if (!st) st = take(p, &full, (size_t)F);
+        if (!st) st = take(p, &part, (size_t)d->window * (size_t)std::max(total, 0));
         p->tt_c2t.push_back(C);
         p->tt_full.push_back(full);
         p->tt_part.push_back(part);
@@ -216,8 +280,8 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
             st = tmp.upload(&dbb, bb);
             if (!st) st = tmp.upload(&dbe, be);
             if (!st) st = tmp.upload(&dbor, bor);
-            if (!st) st = talloc(&sg, (size_t)nb * F);
-            if (!st && d->average) st = talloc(&sga, (size_t)nb);
+            if (!st) st = take(p, &sg, (size_t)nb * F);
+            if (!st && d->average) st = take(p, &sga, (size_t)nb);
             // the result keeps the mean image, or with average its frame mean
             if (d->average) {
                 if (sg) tmp.bufs.push_back(sg);
@@ -234,7 +298,7 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
                 for (int32_t lo = 0; lo < Pq; lo += chunk) {
                     const int32_t n = std::min(chunk, Pq - lo);
                     launch_tt_values(p, c0 + lo, n, scale, X, s);
-                    launch_tt_colsum(X, F, lo, lo + n, dbb, dbe, nb, sg, s);
+                    launch_tt_colsum(p, X, F, lo, lo + n, dbb, dbe, nb, sg, s);
                 }
             }
             launch_tt_sgfinal(sg, F, dbb, dbe, nb, d->average != 0, sga, s);
@@ -242,21 +306,25 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
         }
         if (pairs) {
             launch_tt_pairs(p, c0, Pq, scale, dbor, div, d->average == 0, C, s);
-            launch_tt_scale(C, (int64_t)F * F, 1.0 / (double)Pq, s);
         } else {
             for (int32_t lo = 0; lo < Pq; lo += chunk) {
                 const int32_t n = std::min(chunk, Pq - lo);
                 // with smoothing and one chunk the values are already in place
                 if (!(div && one_chunk)) launch_tt_values(p, c0 + lo, n, scale, X, s);
-                if (div) launch_tt_divide(X, F, lo, n, dbor, div, d->average == 0, s);
-                launch_tt_gram(X, n, F, 1.0 / (double)Pq, C, s);
+                if (div) launch_tt_divide(p, X, F, lo, n, dbor, div, d->average == 0, s);
+                launch_tt_gram(p, X, n, F, 1.0 / (double)Pq, C, s);
             }
         }
-        launch_tt_diag(C, F, d->window, total, full, part, s);
+        // the event-pair sums take their 1 / P_q in the diagonal pass
+        launch_tt_diag(p, C, F, d->window, total, pairs ? 1.0 / (double)Pq : 1.0, dpart, full, part, s);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bail(fail(XPCS_ERR_HIP, "twotime: launch failed: %s", hipGetErrorString(e)));
     }
     if (hipStreamSynchronize(s) != hipSuccess) return bail(fail(XPCS_ERR_HIP, "twotime: device error"));
+    // pooled buffers this call did not reuse
+    for (auto& b : p->tt_pool) (void)hipFree(b.first);
+    p->tt_pool.clear();
+    collect_timers(p);
     uint32_t cnt[2] = {0, 0};
     HIP_TRY(hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     if (cnt[1] & 1u)

@@ -281,6 +281,22 @@ class Plan:
         """Two-time analysis (xpcs_mt_twotime); returns dict(qbins, methods
         (1 Gram, 2 event pairs), c2t list, g2full [F, nq], g2partials
         [window, total, nq], sg)."""
+        info = self.twotime_run(qbins, smoothing, average, window, method, chunk_bytes)
+        rows, cols, tot = info["sg_rows"], info["sg_cols"], info["total_partials"]
+        F, n = self.frames, len(info["qbins"])
+        c2t = []
+        for i in range(n):
+            m = np.zeros((F, F), np.float64)
+            check(lib().xpcs_mt_read_twotime(self._h, i, _p(m)))
+            c2t.append(m)
+        full = np.zeros((F, n), np.float64)
+        part = np.zeros((int(window), tot, n), np.float64)
+        sg = np.zeros((rows, cols), np.float64) if rows else None
+        check(lib().xpcs_mt_read_twotime_1d(self._h, _p(full), _p(part) if part.size else None, _p(sg)))
+        return dict(qbins=info["qbins"], methods=info["methods"], c2t=c2t, g2full=full, g2partials=part, sg=sg)
+
+    def twotime_run(self, qbins, smoothing=0, average=False, window=1, method=0, chunk_bytes=0):
+        """Runs xpcs_mt_twotime and returns its info (results stay on the device)."""
         qb = np.ascontiguousarray(qbins, np.int32).reshape(-1)
         d = TTDesc(_p(qb), qb.size, int(smoothing), 1 if average else 0, int(window), int(method), int(chunk_bytes))
         check(lib().xpcs_mt_twotime(self._h, C.byref(d), None))
@@ -289,17 +305,9 @@ class Plan:
         ms = np.zeros(max(qb.size, 1), np.int32)
         check(lib().xpcs_mt_twotime_info(self._h, C.byref(nq), _p(qs), _p(ms), qs.size, C.byref(rows),
                                          C.byref(cols), C.byref(tot)))
-        F, n = self.frames, nq.value
-        c2t = []
-        for i in range(n):
-            m = np.zeros((F, F), np.float64)
-            check(lib().xpcs_mt_read_twotime(self._h, i, _p(m)))
-            c2t.append(m)
-        full = np.zeros((F, n), np.float64)
-        part = np.zeros((int(window), tot.value, n), np.float64)
-        sg = np.zeros((rows.value, cols.value), np.float64) if rows.value else None
-        check(lib().xpcs_mt_read_twotime_1d(self._h, _p(full), _p(part) if part.size else None, _p(sg)))
-        return dict(qbins=qs[:n].tolist(), methods=ms[:n].tolist(), c2t=c2t, g2full=full, g2partials=part, sg=sg)
+        n = nq.value
+        return dict(qbins=qs[:n].tolist(), methods=ms[:n].tolist(), sg_rows=rows.value, sg_cols=cols.value,
+                    total_partials=tot.value)
 
     def set_tuning(self, **shapes):
         t = Tuning()

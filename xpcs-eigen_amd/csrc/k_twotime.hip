@@ -196,8 +196,9 @@ k_tt_pairs(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, con
     }
 }
 
-// Event pairs Sum n (n + 1) / 2 of the compact pixels [c_begin[q], c_end[q]),
-// one block per q-bin (the cost estimate that picks the method).
+// Event pairs Sum n (n + 1) / 2 of the compact pixels [c_begin[q], c_end[q])
+// added into out[q] (cleared by the caller): blockIdx.x is the q-bin,
+// blockIdx.y a stride of its pixels (the cost estimate that picks the method).
 __global__ void __launch_bounds__(256)
 k_tt_pair_count(const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
                 const int32_t* __restrict__ c_begin, const int32_t* __restrict__ c_end, double* __restrict__ out)
@@ -205,7 +206,8 @@ k_tt_pair_count(const int64_t* __restrict__ off, const int32_t* __restrict__ row
     __shared__ double s_red[256];
     const int q = blockIdx.x;
     double s = 0.0;
-    for (int c = c_begin[q] + (int)threadIdx.x; c < c_end[q]; c += blockDim.x) {
+    const int stride = (int)(blockDim.x * gridDim.y);
+    for (int c = c_begin[q] + (int)(blockIdx.y * blockDim.x + threadIdx.x); c < c_end[q]; c += stride) {
         const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
         const double n = (double)(off[row + 1] - off[row]);
         s += n * (n + 1.0) * 0.5;
@@ -216,7 +218,7 @@ k_tt_pair_count(const int64_t* __restrict__ off, const int32_t* __restrict__ row
         if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[q] = s_red[0];
+    if (threadIdx.x == 0) unsafeAtomicAdd(&out[q], s_red[0]);
 }
 
 // sgsum[bin][f] += value of every event of the q-bin's pixels
@@ -422,7 +424,8 @@ void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, cons
 void launch_tt_pair_count(xpcs_mt_plan* p, const int32_t* d_c_begin, const int32_t* d_c_end, int32_t nq, double* d_out,
                           hipStream_t s)
 {
-    k_tt_pair_count<<<nq, 256, 0, s>>>(p->d_off, p->d_rowmap, d_c_begin, d_c_end, d_out);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(d_out), (size_t)nq, s);
+    k_tt_pair_count<<<dim3(nq, 64), 256, 0, s>>>(p->d_off, p->d_rowmap, d_c_begin, d_c_end, d_out);
 }
 
 void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, const int32_t* d_bin_of_row,

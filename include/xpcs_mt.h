@@ -158,6 +158,43 @@ xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* plan, const void* dev_imm, s
                                       int64_t raw_frames, int32_t compressed);
 
 /*
+ * Packed detector words (the readers of src/xpcs/io/ufxc.cpp:59-168 and
+ * src/xpcs/io/rigaku.cpp:63-273, selected by the reference's --ufxc / --rigaku):
+ *   XPCS_FMT_UFXC    u32 words: 11-bit wrapping frame counter (w >> 21),
+ *                    2-bit value ((w >> 15) & 3), column-major pixel (w & 0x7fff)
+ *   XPCS_FMT_RIGAKU  u64 words: frame (w >> 40), column-major pixel
+ *                    ((w >> 16) & 0xfffff), 11-bit value (w & 0x7ff)
+ * Column-major pixel p is detector pixel (p % height) * width + p / height.
+ * The frame logic of each reader (UFXC: counter unwrap, frames from 0 since
+ * its SkipFrames does nothing; Rigaku: the flush/average state machine that
+ * makes no output frame for a file frame without words, stops at `frames`
+ * output frames and keeps the last one only with more than one pixel) runs on
+ * the host; the words are copied to the device unchanged and decoded there.
+ * Both formats are sparse: dark_frames of the plan must be 0.
+ */
+#define XPCS_FMT_UFXC 1
+#define XPCS_FMT_RIGAKU 2
+xpcs_status xpcs_mt_ingest_packed(xpcs_mt_plan* plan, const void* words, size_t nbytes, int32_t format);
+
+/*
+ * Raw frames (the HDF5 reader, src/xpcs/io/hdf5.cpp:62-221: dataset
+ * /entry/data/data, frames x height x width): nframes frames of width*height
+ * values, 2 or 4 bytes each, unsigned, frame-major.  Non-zero values are the
+ * events (the reader keeps them, then the sparse filter); frames are skipped to
+ * frame_from, no dark frames (the plan's dark_frames must be 0).  Values must
+ * be below 65536.
+ */
+xpcs_status xpcs_mt_ingest_frames(xpcs_mt_plan* plan, const void* frames, int64_t nframes, int32_t bytes_per_value);
+
+/*
+ * Host-only: the frame records xpcs_mt_ingest_packed would build for `desc`
+ * (run of words first_word[i] .. + count[i] - 1 in output frame frame[i]).
+ * Writes up to cap records, total in *n.
+ */
+xpcs_status xpcs_packed_records(const xpcs_mt_desc* desc, const void* words, size_t nbytes, int32_t format,
+                                int64_t* first_word, uint32_t* count, int32_t* frame, int64_t cap, int64_t* n);
+
+/*
  * Index the headers of an IMM file in host memory (io::Header fields of
  * src/xpcs/io/imm.h:63-144).  Writes up to cap entries, total in *frames.
  */

@@ -416,3 +416,105 @@ def twotime_pairs_c(csr, qm, frames, qbins, value_scale=1.0, weight=None, nthrea
                           np.ascontiguousarray(csr["frame"], np.int32), np.ascontiguousarray(val), frames, C,
                           nthreads)
     return [q for q, _ in bins], C[:len(bins)]
+
+
+# ---- packed detector formats (src/xpcs/io/ufxc.cpp, src/xpcs/io/rigaku.cpp) ---------
+def _raster(col_major, width, height):
+    """Detector pixel of a column-major id (ufxc.cpp:142-146, rigaku.cpp:201-203)."""
+    col_major = np.asarray(col_major, np.int64)
+    return (col_major % height) * width + col_major // height
+
+
+def ufxc_decode(words, width, height):
+    """Raw frame (unwrapped counter, ufxc.cpp:75-97), raster pixel and value
+    ((w >> 15) & 3, ufxc.cpp:140-148) of every UFXC word."""
+    w = np.asarray(words, np.uint32)
+    n = w.size
+    ff = np.zeros(n, np.int64)
+    if n:
+        value = int(w[0] >> 21)
+        f0 = value
+        bf = 0
+        for i in range(1, n):
+            c = int(w[i] >> 21)
+            diff = c - value
+            if diff < -2000:
+                bf += 2048
+            elif diff > 2000:
+                bf -= 2048
+            ff[i] = c + bf - f0
+            value = c
+    return ff, _raster(w & 0x7FFF, width, height), ((w >> 15) & 3).astype(np.int64)
+
+
+def ufxc_imm(words, width, height, raw_frames):
+    """The UFXC words as an equivalent compressed IMM of raw_frames frames (the
+    reader's frames 0, 1, ...; words of a frame in file order; empty frames
+    where the file has none)."""
+    ff, pix, val = ufxc_decode(words, width, height)
+    ev = []
+    for f in range(raw_frames):
+        m = ff == f
+        ev.append((pix[m].astype(np.int32), val[m].astype(np.uint16)))
+    import synth
+    return synth.imm_bytes_events(ev, width, height)
+
+
+def rigaku_assign(words, width, height, mapped, start, frames, stride=1, avg=1):
+    """Output frame of every Rigaku word (-1: not used) by the reader's state
+    machine (rigaku.cpp:128-255): skip frames <= start and, with stride > 1,
+    frames not divisible by it; flush the pending frame when a word's frame
+    differs from the previous flush's (avg 1) or passes the next expected frame
+    (avg > 1; it then advances by the read-in count); stop at `frames` output
+    frames; keep the last pending frame only with more than one pixel.  Masked
+    pixels are not pending pixels."""
+    w = np.asarray(words, np.uint64)
+    read_in = stride if stride > 1 else avg
+    if stride > 1 and avg > 1:
+        read_in = stride * avg
+    real = 0
+    next_expected = start + read_in
+    previous = start + 1
+    out = np.full(w.size, -1, np.int64)
+    pending = {}
+    for i in range(w.size):
+        frame = int(w[i] >> 40)
+        if frame <= start:
+            continue
+        if real >= frames:
+            break
+        if stride > 1 and frame != 0 and frame % stride != 0:
+            continue
+        if (avg > 1 and frame > next_expected) or (avg == 1 and frame != previous):
+            for k in pending.values():
+                out[k] = real
+            pending = {}
+            previous = frame
+            real += 1
+            next_expected += read_in
+        pix = int(_raster([(int(w[i]) >> 16) & 0xFFFFF], width, height)[0])
+        if not mapped[pix]:
+            continue
+        pending.setdefault(pix, []).append(i)
+    if len(pending) > 1 and real < frames:
+        for k in pending.values():
+            out[k] = real
+    return out
+
+
+def rigaku_imm(words, width, height, mapped, start, frames, stride=1, avg=1):
+    """The Rigaku words as an equivalent compressed IMM of `frames` output
+    frames (summed counts of each output frame's words)."""
+    w = np.asarray(words, np.uint64)
+    of = rigaku_assign(w, width, height, mapped, start, frames, stride, avg)
+    pix = _raster((w >> np.uint64(16)) & np.uint64(0xFFFFF), width, height)
+    val = (w & np.uint64(0x7FF)).astype(np.int64)
+    ev = []
+    for f in range(frames):
+        m = of == f
+        acc = np.zeros(width * height, np.int64)
+        np.add.at(acc, pix[m], val[m])
+        idx = np.flatnonzero(acc)
+        ev.append((idx.astype(np.int32), acc[idx].astype(np.uint16)))
+    import synth
+    return synth.imm_bytes_events(ev, width, height)

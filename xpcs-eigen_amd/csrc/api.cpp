@@ -544,6 +544,8 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = comp ? 1 : 2;
+    p->fmt = kFmtImm;
+    if (comp) p->nscale = 1;
     p->nrows = (int64_t)h.W * h.H;
     p->d_rowmap = p->d_c2pix;
     p->executed = false;
@@ -596,7 +598,117 @@ extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* de
     st = finish_selection(p, so, sd, ofr);
     if (st) return st;
     p->mode = compressed ? 1 : 2;
+    p->fmt = kFmtImm;
+    if (compressed) p->nscale = 1;
     p->nrows = (int64_t)h.W * h.H;
+    p->d_rowmap = p->d_c2pix;
+    p->executed = false;
+    p->generation++;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_ingest_packed(xpcs_mt_plan* p, const void* words, size_t nbytes, int32_t format)
+{
+    if (!p || (!words && nbytes)) return fail(XPCS_ERR_INVALID_ARG, "ingest_packed: NULL argument");
+    HostTables& h = p->h;
+    if (h.darks > 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_packed: packed formats have no dark frames");
+    std::vector<PackedRecord> rec;
+    xpcs_status st = packed_records(words, nbytes, format, h, rec);
+    if (st) return st;
+    const int64_t wb = format == XPCS_FMT_UFXC ? 4 : 8;
+    // copy the word range the records cover
+    int64_t lo = 0, hi = 0;
+    if (!rec.empty()) {
+        lo = rec.front().word;
+        for (const auto& r : rec) {
+            lo = std::min(lo, r.word);
+            hi = std::max(hi, r.word + (int64_t)r.count);
+        }
+    }
+    const size_t len = (size_t)(hi - lo) * (size_t)wb;
+    if (len + 256 > p->imm_cap) {
+        st = dalloc(&p->d_imm_owned, len + 256);
+        if (st) return st;
+        p->imm_cap = len + 256;
+    }
+    st = h2d_staged(p->d_imm_owned, static_cast<const uint8_t*>(words) + (size_t)lo * (size_t)wb, len, p->stream);
+    if (st) return st;
+    p->d_imm = p->d_imm_owned;
+    std::vector<int64_t> so(rec.size());
+    std::vector<uint32_t> sd(rec.size());
+    std::vector<int32_t> ofr(rec.size());
+    for (size_t k = 0; k < rec.size(); k++) {
+        so[k] = (rec[k].word - lo) * wb;
+        sd[k] = rec[k].count;
+        ofr[k] = rec[k].frame;
+    }
+    st = finish_selection(p, so, sd, ofr);
+    if (st) return st;
+    p->clock.clear();
+    p->ticks.clear();
+    p->nscale = 1;
+    p->mode = 1;
+    p->fmt = format;
+    p->nrows = (int64_t)h.W * h.H;
+    p->d_rowmap = p->d_c2pix;
+    p->executed = false;
+    p->generation++;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_ingest_frames(xpcs_mt_plan* p, const void* frames, int64_t nframes,
+                                             int32_t bytes_per_value)
+{
+    if (!p || !frames || nframes <= 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_frames: bad arguments");
+    if (bytes_per_value != 2 && bytes_per_value != 4)
+        return fail(XPCS_ERR_INVALID_ARG, "ingest_frames: values are 2 or 4 bytes, not %d", bytes_per_value);
+    HostTables& h = p->h;
+    if (h.darks > 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_frames: raw frames have no dark frames");
+    std::vector<int64_t> sel;
+    std::vector<int32_t> ofr;
+    int64_t first = 0;
+    // the HDF5 reader skips to frame_from and has no darks (it reports compression)
+    xpcs_status st = select_frames(h, nframes, true, &first, sel, ofr);
+    if (st) return st;
+    const int64_t npd = (int64_t)h.W * h.H;
+    const int64_t f_lo = sel.front(), f_hi = sel.back() + 1;
+    const size_t len = (size_t)(f_hi - f_lo) * (size_t)npd * 2;
+    if (len + 256 > p->imm_cap) {
+        st = dalloc(&p->d_imm_owned, len + 256);
+        if (st) return st;
+        p->imm_cap = len + 256;
+    }
+    if (bytes_per_value == 2) {
+        st = h2d_staged(p->d_imm_owned, static_cast<const uint8_t*>(frames) + (size_t)f_lo * npd * 2, len, p->stream);
+    } else {
+        // 32-bit values narrowed on the host (the kernels read 16-bit values)
+        const uint32_t* src = static_cast<const uint32_t*>(frames) + (size_t)f_lo * npd;
+        std::vector<uint16_t> nar((size_t)(f_hi - f_lo) * (size_t)npd);
+        for (size_t i = 0; i < nar.size(); i++) {
+            if (src[i] > 65535u)
+                return fail(XPCS_ERR_UNSUPPORTED, "ingest_frames: value %u at frame %lld exceeds 65535", src[i],
+                            (long long)(f_lo + (int64_t)(i / (size_t)npd)));
+            nar[i] = (uint16_t)src[i];
+        }
+        st = h2d_staged(p->d_imm_owned, reinterpret_cast<const uint8_t*>(nar.data()), len, p->stream);
+    }
+    if (st) return st;
+    p->d_imm = p->d_imm_owned;
+    st = dense_prepare(p, nullptr);
+    if (st) return st;
+    std::vector<int64_t> so(sel.size());
+    std::vector<uint32_t> sd(sel.size(), (uint32_t)npd);
+    for (size_t k = 0; k < sel.size(); k++) so[k] = (sel[k] - f_lo) * npd * 2;
+    dense_tiling(p, so, p->d_imm, p->imm_cap);
+    st = finish_selection(p, so, sd, ofr);
+    if (st) return st;
+    p->clock.clear();
+    p->ticks.clear();
+    p->mode = 2;
+    p->fmt = kFmtImm;
+    p->nrows = npd;
     p->d_rowmap = p->d_c2pix;
     p->executed = false;
     p->generation++;
@@ -643,6 +755,7 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     HIP_TRY(hipMemcpyAsync(p->d_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, p->stream));
     p->events = off[(size_t)h.P];
     p->nscale = 1;
+    p->fmt = kFmtImm;
     p->mode = 3;
     p->nrows = h.P;
     p->d_rowmap = nullptr;

@@ -85,6 +85,15 @@ struct KernelTimer {
     hipEvent_t a, b;
 };
 
+// Word formats of sparse frame records (the transpose decodes them on the device).
+constexpr int32_t kFmtImm = 0;
+constexpr int32_t kFmtUfxc = 1;
+constexpr int32_t kFmtRigaku = 2;
+struct EvFormat {
+    int32_t kind;
+    uint32_t W, H;   // detector width and height (column-major pixel ids of UFXC / Rigaku)
+};
+
 }  // namespace xpcs
 
 struct xpcs_mt_plan {
@@ -111,6 +120,7 @@ struct xpcs_mt_plan {
 
     // ingest state: 0 none, 1 sparse IMM, 2 dense IMM, 3 rows
     int mode = 0;
+    int32_t fmt = 0;                     // word format of the sparse frame records (kFmt*)
     const uint8_t* d_imm = nullptr;      // raw IMM bytes (owned when own_imm)
     uint8_t* d_imm_owned = nullptr;
     size_t imm_cap = 0;
@@ -208,6 +218,16 @@ namespace xpcs {
 
 // host_tables.cpp
 xpcs_status build_tables(const xpcs_mt_desc& d, HostTables& h);
+// formats.cpp: frame records of packed detector words
+struct PackedRecord {
+    int64_t word;     // first word of the run
+    uint32_t count;   // words in the run
+    int32_t frame;    // output frame
+};
+xpcs_status ufxc_records(const uint32_t* w, int64_t n, const HostTables& h, std::vector<PackedRecord>& out);
+xpcs_status rigaku_records(const uint64_t* w, int64_t n, const HostTables& h, std::vector<PackedRecord>& out);
+xpcs_status packed_records(const void* words, size_t nbytes, int32_t format, const HostTables& h,
+                           std::vector<PackedRecord>& rec);
 xpcs_status build_qmap(const int32_t* dq, const int32_t* sq, HostTables& h);
 void set_error(const char* fmt, ...);
 xpcs_status fail(xpcs_status st, const char* fmt, ...);
@@ -215,6 +235,7 @@ xpcs_status fail(xpcs_status st, const char* fmt, ...);
 // kernel launchers (kernels_*.hip); all asynchronous on `s`
 TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames);
 void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
+EvFormat ev_format(const xpcs_mt_plan* p);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);

@@ -32,12 +32,38 @@ constexpr int kTrBlock = 256;
 constexpr int kTrChunkBlock = 1024;   // workgroup of the per-chunk kernels
 __device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total);
 
-__device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j,
+// Element j of a frame record: raster pixel index and count, by word format.
+//   IMM      int32 index[dlen] then int16 value[dlen] (src/xpcs/io/imm.cpp:87-99)
+//   UFXC     u32 words: value (w >> 15) & 3, column-major pixel w & 0x7fff
+//            (src/xpcs/io/ufxc.cpp:140-148)
+//   Rigaku   u64 words: value w & 0x7ff, column-major pixel (w >> 16) & 0xfffff
+//            (src/xpcs/io/rigaku.cpp:201-208)
+// Column-major pixels p become (p % H) * W + p / H.
+__device__ __forceinline__ void tr_decode(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j, EvFormat fm,
+                                          uint32_t& idx, uint32_t& v)
+{
+    if (fm.kind == kFmtImm) {
+        idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
+        v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen)[j];
+    } else if (fm.kind == kFmtUfxc) {
+        const uint32_t w = reinterpret_cast<const uint32_t*>(base)[j];
+        const uint32_t pix = w & 0x7fffu;
+        idx = (pix % fm.H) * fm.W + pix / fm.H;
+        v = (w >> 15) & 3u;
+    } else {
+        const uint64_t w = reinterpret_cast<const uint64_t*>(base)[j];
+        const uint32_t pix = (uint32_t)(w >> 16) & 0xfffffu;
+        idx = (pix % fm.H) * fm.W + pix / fm.H;
+        v = (uint32_t)w & 0x7ffu;
+    }
+}
+
+__device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j, EvFormat fm,
                                          const int32_t* __restrict__ pix2c, uint32_t npix_det,
                                          uint32_t* __restrict__ counters, int32_t& c, uint32_t& v)
 {
-    const uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
-    v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen)[j];
+    uint32_t idx;
+    tr_decode(base, dlen, al4, j, fm, idx, v);
     if (idx >= npix_det) {
         if (counters) atomicOr(&counters[1], 1u);
         return false;
@@ -49,12 +75,12 @@ __device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, boo
 // An event of the transpose: raster pixel r, count v; kept when the pixel is
 // mapped (bit r of `mapped`, 128 KB for a megapixel: cache resident, unlike a
 // per-pixel id table) and v != 0.  Rows are per raster pixel.
-__device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j,
+__device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j, EvFormat fm,
                                              const uint32_t* __restrict__ mapped, uint32_t npix_det,
                                              uint32_t* __restrict__ counters, uint32_t& r, uint32_t& v)
 {
-    const uint32_t idx = al4 ? reinterpret_cast<const uint32_t*>(base)[j] : load_u32_a2(base + 4 * (size_t)j);
-    v = reinterpret_cast<const uint16_t*>(base + 4 * (size_t)dlen)[j];
+    uint32_t idx;
+    tr_decode(base, dlen, al4, j, fm, idx, v);
     if (idx >= npix_det) {
         if (counters) atomicOr(&counters[1], 1u);
         return false;
@@ -65,7 +91,7 @@ __device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen,
 
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
-           const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf,
+           const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf, EvFormat fm,
            const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
            int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters)
 {
@@ -86,7 +112,7 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, mapped, npix_det, counters, c[u], v[u]);
+                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, fm, mapped, npix_det, counters, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -100,7 +126,7 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
             const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
-            int64_t cf, const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
+            int64_t cf, EvFormat fm, const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
             int32_t nblk, const int64_t* __restrict__ hoff, uint64_t* __restrict__ b1)
 {
     extern __shared__ unsigned long long s_base[];   // [nb] segment starts of this chunk
@@ -127,7 +153,7 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, mapped, npix_det, nullptr, c[u], v[u]);
+                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, fm, mapped, npix_det, nullptr, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -347,7 +373,8 @@ constexpr int kSideMaxBins = 4096;
 __global__ void __launch_bounds__(256)
 k_side_sparse(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
               const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
-              const int32_t* __restrict__ pix2c, uint32_t npix_det, const int32_t* __restrict__ sbin_of_c,
+              EvFormat fm, const int32_t* __restrict__ pix2c, uint32_t npix_det,
+              const int32_t* __restrict__ sbin_of_c,
               const double* __restrict__ weight, int32_t S, double* __restrict__ fs)
 {
     __shared__ double s_acc[kSideMaxBins];
@@ -360,7 +387,7 @@ k_side_sparse(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_o
         for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
             int32_t c;
             uint32_t v;
-            if (!tr_event(base, dlen, al4, j, pix2c, npix_det, nullptr, c, v)) continue;
+            if (!tr_event(base, dlen, al4, j, fm, pix2c, npix_det, nullptr, c, v)) continue;
             atomicAdd(&s_acc[sbin_of_c[c] - 1], (double)v * (weight ? weight[c] : 1.0));
         }
         __syncthreads();
@@ -395,13 +422,15 @@ k_side_dense(const uint32_t* __restrict__ rows, int32_t P, int32_t F, const int3
     if (acc != 0.0) atomicAdd(&fs[(int64_t)f * S + cur - 1], acc);
 }
 
+EvFormat ev_format(const xpcs_mt_plan* p);
+
 void launch_side(xpcs_mt_plan* p, hipStream_t s)
 {
     const HostTables& h = p->h;
     timer_begin(p, s, "k_side");
     if (p->mode == 1) {
         k_side_sparse<<<p->nsel < 65536 ? (int)p->nsel : 65536, 256, 0, s>>>(
-            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, p->d_pix2c,
+            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, ev_format(p), p->d_pix2c,
             (uint32_t)((int64_t)h.W * h.H), p->d_sbin_of_c, p->d_weight, h.ns, p->d_fs);
     } else if (p->mode == 2) {
         const int chunk = 256;
@@ -413,6 +442,15 @@ void launch_side(xpcs_mt_plan* p, hipStream_t s)
 }
 
 // ---- launchers ------------------------------------------------------------------
+EvFormat ev_format(const xpcs_mt_plan* p)
+{
+    EvFormat f;
+    f.kind = p->fmt;
+    f.W = (uint32_t)p->h.W;
+    f.H = (uint32_t)p->h.H;
+    return f;
+}
+
 static int grid_for(int64_t n, int64_t cap)
 {
     if (n <= 0) return 1;
@@ -457,16 +495,16 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
     const int64_t nh = (int64_t)t.nb * t.nblk;
     timer_begin(p, s, "k_tr_count");
     k_tr_count<<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, p->d_mapped, npd, t.bshift, t.nb, t.nblk,
-        p->d_hist, p->d_counters);
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
+        t.nblk, p->d_hist, p->d_counters);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_scan");
     scan_u32(p->d_hist, nh, p->d_hoff, p->d_bsum, s);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_bucket");
     k_tr_bucket<<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, p->d_mapped, npd, t.bshift,
-        t.nb, t.nblk, p->d_hoff, p->d_b1);
+        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, ev_format(p), p->d_mapped, npd,
+        t.bshift, t.nb, t.nblk, p->d_hoff, p->d_b1);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_rows");
     // 256 threads: placement tiles of 1024 events hold about one event per

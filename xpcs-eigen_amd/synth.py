@@ -115,6 +115,60 @@ def imm_bytes_dense(counts: np.ndarray, width: int, height: int) -> bytes:
     return b"".join(parts)
 
 
+def _col_major(idx, width, height):
+    """Column-major pixel id of raster pixel idx (row * width + col): col * height + row."""
+    idx = np.asarray(idx, np.int64)
+    return (idx % width) * height + idx // width
+
+
+def ufxc_words(counts: np.ndarray, width: int, height: int, counter0: int = 0, split=()) -> np.ndarray:
+    """Packed UFXC u32 words of counts [frames, width*height] (values clipped to
+    the 2-bit range): frame counter (counter0 + f) mod 2048 << 21 | value << 15
+    | column-major pixel.  Frames in `split` are written in two halves with the
+    next frame's words between them (non-contiguous frames)."""
+    if width * height > (1 << 15):
+        raise ValueError("UFXC pixel ids are 15 bits")
+    per = []
+    for f in range(counts.shape[0]):
+        idx = np.flatnonzero(counts[f])
+        v = np.minimum(counts[f, idx], 3).astype(np.uint32)
+        w = ((np.uint32((counter0 + f) % 2048) << np.uint32(21)) | (v << np.uint32(15))
+             | _col_major(idx, width, height).astype(np.uint32))
+        per.append(w.astype(np.uint32))
+    out = []
+    f = 0
+    while f < len(per):
+        if f in split and f + 1 < len(per):
+            h = per[f].size // 2
+            out += [per[f][:h], per[f + 1], per[f][h:]]
+            f += 2
+        else:
+            out.append(per[f])
+            f += 1
+    return np.concatenate(out) if out else np.zeros(0, np.uint32)
+
+
+def rigaku_words(counts: np.ndarray, width: int, height: int, first_frame: int = 1, frame_numbers=None,
+                 extra=()) -> np.ndarray:
+    """Packed Rigaku u64 words of counts [frames, width*height] (values clipped to
+    11 bits): file frame << 40 | column-major pixel << 16 | value, file frames
+    first_frame + f (or frame_numbers[f]); frames without counts have no words.
+    `extra` appends (file frame, raster pixel, value) words at the end."""
+    if width * height > (1 << 20):
+        raise ValueError("Rigaku pixel ids are 20 bits")
+    out = []
+    for f in range(counts.shape[0]):
+        idx = np.flatnonzero(counts[f])
+        v = np.minimum(counts[f, idx], 2047).astype(np.uint64)
+        fn = np.uint64(frame_numbers[f] if frame_numbers is not None else first_frame + f)
+        out.append((fn << np.uint64(40)) | (_col_major(idx, width, height).astype(np.uint64) << np.uint64(16)) | v)
+    for fn, idx, v in extra:
+        out.append(np.array([(np.uint64(fn) << np.uint64(40))
+                             | (np.uint64(_col_major([idx], width, height)[0]) << np.uint64(16))
+                             | np.uint64(v)], np.uint64))
+    return np.concatenate(out).astype(np.uint64) if out else np.zeros(0, np.uint64)
+
+
 def dark_frames(width, height, n, mean=2.0, sd=0.5, seed=0):
     """n identical integer dark frames D = clip(round(N(mean, sd)), 0)."""
     rng = np.random.Generator(np.random.PCG64(seed))

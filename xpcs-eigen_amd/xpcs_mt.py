@@ -22,6 +22,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libxpcs_mt.so"
 
 XPCS_MT_TIMING = 0x1
+XPCS_MT_SIDE_STATS = 0x2
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "FORMAT", 4: "RANGE", 5: "UNSUPPORTED",
           6: "STATE", 7: "NOMEM", 8: "NO_DEVICE"}
 
@@ -38,7 +39,10 @@ EXPORTS = [
     "xpcs_mt_stream_push_imm_device", "xpcs_mt_stream_frames", "xpcs_mt_stream_snapshot",
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
     "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
+    "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records",
 ]
+FMT_UFXC = 1
+FMT_RIGAKU = 2
 XPCS_BATCH_AUTOTUNE = 0x1
 
 
@@ -123,6 +127,9 @@ def lib():
             "xpcs_mt_twotime_info": ([vp, vp, vp, vp, i32, vp, vp, vp], i32),
             "xpcs_mt_read_twotime": ([vp, i32, vp], i32),
             "xpcs_mt_read_twotime_1d": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_ingest_packed": ([vp, vp, C.c_size_t, i32], i32),
+            "xpcs_mt_ingest_frames": ([vp, vp, C.c_int64, i32], i32),
+            "xpcs_packed_records": ([vp, vp, C.c_size_t, i32, vp, vp, vp, C.c_int64, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -200,18 +207,44 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+def _desc(width, height, frames, dpl, dq, sq, stride=1, average=1, frame_from=0, dark_frames=0, lld=0.0,
+          sigma=0.0, ff=None, flags=0):
+    return Desc(width, height, frames, dpl, stride, average, frame_from, dark_frames, lld, sigma,
+                _p(dq), _p(sq), _p(ff), flags)
+
+
+def packed_records(words, fmt, width, height, frames, dqmap, sqmap, stride=1, average=1, frame_from=0):
+    """Frame records of packed UFXC / Rigaku words (host only): arrays of
+    (first word, word count, output frame)."""
+    dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
+    sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
+    w = np.frombuffer(words, np.uint8) if not isinstance(words, np.ndarray) else np.ascontiguousarray(words)
+    d = _desc(width, height, frames, 4, dq, sq, stride, average, frame_from)
+    n = C.c_int64(0)
+    check(lib().xpcs_packed_records(C.byref(d), _p(w), w.nbytes, fmt, None, None, None, 0, C.byref(n)))
+    fw = np.zeros(max(n.value, 1), np.int64)
+    cnt = np.zeros(max(n.value, 1), np.uint32)
+    fr = np.zeros(max(n.value, 1), np.int32)
+    check(lib().xpcs_packed_records(C.byref(d), _p(w), w.nbytes, fmt, _p(fw), _p(cnt), _p(fr), n.value,
+                                    C.byref(n)))
+    k = n.value
+    return fw[:k], cnt[:k], fr[:k]
+
+
 class Plan:
     """One multi-tau job on the current HIP device."""
 
     def __init__(self, width, height, frames, dpl, dqmap, sqmap, stride=1, average=1,
-                 frame_from=0, dark_frames=0, lld=0.0, sigma=0.0, flatfield=None, timing=False):
+                 frame_from=0, dark_frames=0, lld=0.0, sigma=0.0, flatfield=None, timing=False,
+                 side_stats=False):
         self._dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
         self._sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
         self._ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
         if self._dq.size != width * height or self._sq.size != width * height:
             raise ValueError("q-maps must have height*width entries")
         d = Desc(width, height, frames, dpl, stride, average, frame_from, dark_frames, lld, sigma,
-                 _p(self._dq), _p(self._sq), _p(self._ff), XPCS_MT_TIMING if timing else 0)
+                 _p(self._dq), _p(self._sq), _p(self._ff),
+                 (XPCS_MT_TIMING if timing else 0) | (XPCS_MT_SIDE_STATS if side_stats else 0))
         self._h = C.c_void_p()
         check(lib().xpcs_mt_plan_create(C.byref(d), C.byref(self._h)))
         info = self.info()
@@ -260,6 +293,21 @@ class Plan:
         dl = np.ascontiguousarray(dlen, np.uint32)
         check(lib().xpcs_mt_ingest_imm_device(self._h, C.c_void_p(dev_ptr), nbytes, _p(off), _p(dl),
                                               off.size, 1 if compressed else 0))
+
+    def ingest_packed(self, words, fmt):
+        """UFXC (FMT_UFXC, u32) or Rigaku (FMT_RIGAKU, u64) words, as bytes or an array."""
+        w = np.frombuffer(words, np.uint8) if not isinstance(words, np.ndarray) else np.ascontiguousarray(words)
+        check(lib().xpcs_mt_ingest_packed(self._h, _p(w), w.nbytes, int(fmt)))
+
+    def ingest_frames(self, frames):
+        """Raw frames [n, height, width] of uint16 or uint32 (the HDF5 reader's dataset)."""
+        a = np.ascontiguousarray(frames)
+        if a.dtype not in (np.uint16, np.uint32):
+            raise ValueError("frames must be uint16 or uint32")
+        n = a.shape[0] if a.ndim else 0
+        if a.size != n * self.width * self.height:
+            raise ValueError("frames must hold n * height * width values")
+        check(lib().xpcs_mt_ingest_frames(self._h, _p(a), n, a.dtype.itemsize))
 
     def ingest_rows(self, row_off, frame, count):
         ro = np.ascontiguousarray(row_off, np.int64).reshape(-1)

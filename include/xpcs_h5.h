@@ -30,7 +30,7 @@ int32_t xpcs_h5_read_f32(xpcs_h5_file* f, const char* path, float* out, int64_t 
 int32_t xpcs_h5_read_i32(xpcs_h5_file* f, const char* path, int32_t* out, int64_t cap);
 /* string dataset (fixed or variable length) into buf (cap bytes incl. NUL) */
 int32_t xpcs_h5_read_string(xpcs_h5_file* f, const char* path, char* buf, int64_t cap);
-/* writers: dtype 0 = int32, 1 = int64, 2 = float32, 3 = float64 */
+/* writers: dtype 0 = int32, 1 = int64, 2 = float32, 3 = float64, 4 = uint16, 5 = uint32 */
 int32_t xpcs_h5_write(xpcs_h5_file* f, const char* path, int32_t dtype, const void* data,
                       const uint64_t* dims, int32_t ndims);
 int32_t xpcs_h5_write_string(xpcs_h5_file* f, const char* path, const char* s);

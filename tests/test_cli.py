@@ -60,6 +60,7 @@ def test_twotime_keys_are_required(tmp_path):
 
 
 def test_usage():
+    """No configuration argument, or an unknown flag: status 1."""
     assert run().returncode == 1
     assert run("--bogus", "x").returncode == 1
 
@@ -160,3 +161,46 @@ def test_cli_twotime_matches_oracle(tmp_path, orc, method, filt):
         assert h.dims("/exchange/pixelSum") == (job["H"], job["W"])
         with pytest.raises(Exception):
             h.dims("/exchange/tau")
+
+
+def _g2_matches(conf, ref):
+    with xpcs_h5.H5(conf) as h:
+        g2 = h.read("/exchange/norm-0-g2", np.float32)
+    ok = ~np.isnan(ref["g2"])
+    assert g2.shape == ref["g2"].shape
+    assert np.allclose(g2[ok], ref["g2"][ok], rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", ["ufxc", "rigaku", "hdf5"])
+def test_cli_other_formats(tmp_path, orc, fmt):
+    """--ufxc / --rigaku / --hdf5 inputs (main.cpp:204-220) against the oracle's
+    restatement of each reader as an equivalent IMM."""
+    W, H, F = 24, 20, 240
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    counts = synth.ou_counts(W, H, F, 0.3, 8.0, 9)
+    data = tmp_path / f"data.{fmt}"
+    if fmt == "ufxc":
+        counts = np.minimum(counts, 3)
+        w = synth.ufxc_words(counts, W, H, counter0=2000)
+        data.write_bytes(w.tobytes())
+        imm = orc.ufxc_imm(w, W, H, F)
+    elif fmt == "rigaku":
+        w = synth.rigaku_words(counts, W, H, first_frame=1)
+        data.write_bytes(w.tobytes())
+        mapped = (dq.reshape(-1) > 0) & (sq.reshape(-1) > 0)
+        imm = orc.rigaku_imm(w, W, H, mapped, 0, F)
+    else:
+        with xpcs_h5.H5(data, create=True) as h:
+            h.write("/entry/data/data", counts.reshape(F, H, W).astype(np.uint16))
+        imm = synth.imm_bytes_sparse(counts, W, H)
+    conf = tmp_path / "cfg.h5"
+    xpcs_h5.write_config(conf, width=W, height=H, dqmap=dq, sqmap=sq, imm_path=data, data_end_todo=F, dpl=4,
+                         static_window=40)
+    print("corr", fmt, flush=True)
+    r = run(conf, f"--{fmt}")
+    assert r.returncode == 0, r.stderr
+    _g2_matches(conf, orc.run_exact(imm, dq, sq, F, 4))
+    with xpcs_h5.H5(conf) as h:
+        ts = h.read("/exchange/timestamp_clock", np.float64)
+        assert ts.shape == (2, F) and np.array_equal(ts[0], np.arange(1, F + 1))

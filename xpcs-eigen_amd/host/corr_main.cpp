@@ -3,7 +3,8 @@
 // is "Twotime", running the correlator on the GPU through the C ABI.
 //
 //   corr <config.hdf5> [data.imm] [--imm=PATH] [--entry=/xpcs] [--exchange=GROUP]
-//        [--inpath=P --outpath=Q] [--g2out] [--darkout] [--frame_threading] [--quiet]
+//        [--inpath=P --outpath=Q] [--g2out] [--darkout] [--frame_threading]
+//        [--ufxc | --rigaku | --hdf5 [--transposed]] [--quiet]
 //
 // Flags use the gflags syntax of the reference (--name=value, --name value,
 // --bool / --nobool).  The IMM path comes from --imm, else the optional second
@@ -14,10 +15,14 @@
 // reference's other two-time loop order (corr.cpp:574-779); both compute the
 // same C2T, so it changes nothing here.
 //
+// Input: IMM, or with --ufxc / --rigaku / --hdf5 (the reference's flags, same
+// precedence) UFXC or Rigaku packed words, or HDF5 frames (/entry/data/data);
+// --transposed is accepted (the reference's HDF5 reader does not use it).
+//
 // Exit status: 0 ok, 1 usage or configuration error (including an unknown
-// two-time smoothing method), 2 option outside this build (UFXC/Rigaku/HDF5
-// readers, --frameout, normalize_by_framesum), 3 unreadable or malformed IMM,
-// 4 GPU error, 5 cannot write results.  (The reference exits 0 on most failures.)
+// two-time smoothing method), 2 option outside this build (--frameout,
+// normalize_by_framesum), 3 unreadable or malformed input file, 4 GPU error,
+// 5 cannot write results.  (The reference exits 0 on most failures.)
 
 #include <fcntl.h>
 #include <sys/mman.h>
@@ -160,8 +165,11 @@ int main(int argc, char** argv)
     if (fl.pos.empty()) return die(1, "missing configuration file argument");
     g_quiet = flag_true(fl, "quiet");
     Scope total("Total");
-    if (flag_true(fl, "ufxc") || flag_true(fl, "rigaku") || flag_true(fl, "hdf5"))
-        return die(2, "only IMM input is built in this tool (--ufxc/--rigaku/--hdf5 readers are not)");
+    // input format (main.cpp:204-220: --ufxc, else --rigaku, else --hdf5, else IMM)
+    enum { kImm, kUfxc, kRigaku, kHdf5 } fmt = kImm;
+    if (flag_true(fl, "ufxc")) fmt = kUfxc;
+    else if (flag_true(fl, "rigaku")) fmt = kRigaku;
+    else if (flag_true(fl, "hdf5")) fmt = kHdf5;
     if (!flag_str(fl, "frameout").empty() && std::atoi(flag_str(fl, "frameout").c_str()) > 0)
         return die(2, "--frameout (debug frame dump) is not built in this tool");
 
@@ -189,9 +197,24 @@ int main(int argc, char** argv)
     if (!flag_str(fl, "exchange").empty()) cfg.output_group = flag_str(fl, "exchange");
     info("Processing IMM file at path %s..", cfg.imm_path.c_str());
 
-    // map the IMM file
+    // map the IMM / UFXC / Rigaku file, or read the HDF5 frames
     Mapped imm;
-    {
+    std::vector<uint16_t> h5_16;
+    std::vector<uint32_t> h5_32;
+    int64_t h5_frames = 0;
+    if (fmt == kHdf5) {
+        // Hdf5::Hdf5 / NextFrames (src/xpcs/io/hdf5.cpp:62-199): /entry/data/data,
+        // frames x (height x width) unsigned 16- or 32-bit values
+        Scope s("Reading HDF5 frames");
+        xpcs::h5::File in;
+        if (!in.open(cfg.imm_path, false, &why)) return die(3, "cannot open HDF5 file " + cfg.imm_path + ": " + why);
+        std::vector<xpcs::h5::hsize_t> dd;
+        if (!in.read_frames("/entry/data/data", &h5_16, &h5_32, &dd) || dd.size() != 3)
+            return die(3, "cannot read /entry/data/data (3-D, 16- or 32-bit unsigned) from " + cfg.imm_path);
+        if ((int64_t)dd[1] * (int64_t)dd[2] != (int64_t)cfg.width * cfg.height)
+            return die(3, "HDF5 frames do not hold x_dimension*y_dimension values");
+        h5_frames = (int64_t)dd[0];
+    } else {
         int fd = open(cfg.imm_path.c_str(), O_RDONLY);
         if (fd < 0) return die(3, "cannot open IMM file " + cfg.imm_path);
         struct stat st;
@@ -202,6 +225,8 @@ int main(int argc, char** argv)
         if (!imm.p || imm.p == MAP_FAILED) return die(3, "cannot map IMM file " + cfg.imm_path);
         info("File size %.3f MB", imm.n / 1048576.0);
     }
+    // UFXC, Rigaku and HDF5 readers report compressed data: no dark frames (main.cpp:227-239)
+    if (fmt != kImm) cfg.dark_frames = 0;
 
     const int32_t F = cfg.frames();
     info("Data frames=%d stride=%lld average=%lld", F, (long long)cfg.stride, (long long)cfg.average);
@@ -234,7 +259,12 @@ int main(int argc, char** argv)
 
     {
         Scope s("Loading data");
-        st = xpcs_mt_ingest_imm(plan, imm.p, imm.n);
+        if (fmt == kUfxc) st = xpcs_mt_ingest_packed(plan, imm.p, imm.n, XPCS_FMT_UFXC);
+        else if (fmt == kRigaku) st = xpcs_mt_ingest_packed(plan, imm.p, imm.n, XPCS_FMT_RIGAKU);
+        else if (fmt == kHdf5)
+            st = h5_32.empty() ? xpcs_mt_ingest_frames(plan, h5_16.data(), h5_frames, 2)
+                               : xpcs_mt_ingest_frames(plan, h5_32.data(), h5_frames, 4);
+        else st = xpcs_mt_ingest_imm(plan, imm.p, imm.n);
         if (st) return die(st == XPCS_ERR_UNSUPPORTED ? 2 : 3, xpcs_last_error());
     }
     {
@@ -275,10 +305,20 @@ int main(int argc, char** argv)
         if (st) return die(st == XPCS_ERR_RANGE ? 3 : 4, xpcs_last_error());
     }
 
-    // timestamps of the data frames (row 0: 1-based index, row 1: header value)
+    // timestamps of the data frames (row 0: 1-based index, row 1: header value;
+    // the other readers give frame indices: UFXC the raw frame from 0 (its
+    // SkipFrames does nothing), HDF5 the raw frame after the skip, Rigaku the
+    // output frame (and nothing past `frames`))
     const int32_t FR = cfg.real_frames();
     std::vector<double> clk((size_t)2 * FR, 0.0), tck((size_t)2 * FR, 0.0);
-    {
+    if (fmt != kImm) {
+        for (int32_t i = 0; i < FR; i++) {
+            clk[(size_t)i] = tck[(size_t)i] = (double)(i + 1);
+            double v = fmt == kUfxc ? (double)i : fmt == kHdf5 ? (double)(cfg.frame_from() + i) : (double)i;
+            if (fmt == kRigaku && i >= F) v = 0.0;
+            clk[(size_t)FR + i] = tck[(size_t)FR + i] = v;
+        }
+    } else {
         int64_t n = 0;
         int32_t comp = 0;
         xpcs_imm_index(imm.p, imm.n, nullptr, nullptr, nullptr, nullptr, 0, &n, &comp);

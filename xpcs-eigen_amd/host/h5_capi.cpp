@@ -94,6 +94,8 @@ extern "C" int32_t xpcs_h5_write(xpcs_h5_file* f, const char* path, int32_t dtyp
     case 1: ok = f->f.write_i64(path, static_cast<const int64_t*>(data), d); break;
     case 2: ok = f->f.write_f32(path, static_cast<const float*>(data), d); break;
     case 3: ok = f->f.write_f64(path, static_cast<const double*>(data), d); break;
+    case 4: ok = f->f.write_u16(path, static_cast<const uint16_t*>(data), d); break;
+    case 5: ok = f->f.write_u32(path, static_cast<const uint32_t*>(data), d); break;
     default: return err("xpcs_h5_write: unknown dtype");
     }
     return ok ? 0 : err(std::string("cannot write ") + path);

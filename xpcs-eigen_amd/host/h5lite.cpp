@@ -53,6 +53,8 @@ struct Api {
     hid_t* P_CLS_DATASET_CREATE = nullptr;
     hid_t* T_NATIVE_INT;
     hid_t* T_NATIVE_LLONG;
+    hid_t* T_NATIVE_USHORT;
+    hid_t* T_NATIVE_UINT;
     hid_t* T_NATIVE_FLOAT;
     hid_t* T_NATIVE_DOUBLE;
     hid_t* T_C_S1;
@@ -102,6 +104,7 @@ void load()
               bind(g.Ldelete, "H5Ldelete") && bind(g.Pcreate, "H5Pcreate") && bind(g.Pclose, "H5Pclose") &&
               bind(g.Pset_create_intermediate_group, "H5Pset_create_intermediate_group") &&
               bind(g.T_NATIVE_INT, "H5T_NATIVE_INT_g") && bind(g.T_NATIVE_LLONG, "H5T_NATIVE_LLONG_g") &&
+              bind(g.T_NATIVE_USHORT, "H5T_NATIVE_USHORT_g") && bind(g.T_NATIVE_UINT, "H5T_NATIVE_UINT_g") &&
               bind(g.T_NATIVE_FLOAT, "H5T_NATIVE_FLOAT_g") && bind(g.T_NATIVE_DOUBLE, "H5T_NATIVE_DOUBLE_g") &&
               bind(g.T_C_S1, "H5T_C_S1_g") && bind(g.P_CLS_LINK_CREATE, "H5P_CLS_LINK_CREATE_ID_g");
     if (!ok) return;
@@ -318,6 +321,45 @@ bool File::write_any(const std::string& path, hid_t memtype, hid_t filetype, con
 bool File::write_f32_deflate(const std::string& path, const float* data, const std::vector<hsize_t>& d, int level)
 {
     return write_any(path, *g.T_NATIVE_FLOAT, *g.T_NATIVE_FLOAT, data, d, level);
+}
+
+bool File::write_u16(const std::string& path, const uint16_t* data, const std::vector<hsize_t>& d)
+{
+    return write_any(path, *g.T_NATIVE_USHORT, *g.T_NATIVE_USHORT, data, d);
+}
+
+bool File::write_u32(const std::string& path, const uint32_t* data, const std::vector<hsize_t>& d)
+{
+    return write_any(path, *g.T_NATIVE_UINT, *g.T_NATIVE_UINT, data, d);
+}
+
+bool File::read_frames(const std::string& path, std::vector<uint16_t>* v16, std::vector<uint32_t>* v32,
+                       std::vector<hsize_t>* d) const
+{
+    if (!exists(path)) return false;
+    hid_t ds = g.Dopen2(fid_, path.c_str(), 0);
+    if (ds < 0) return false;
+    hid_t ft = g.Dget_type(ds);
+    const size_t sz = g.Tget_size(ft);
+    g.Tclose(ft);
+    hid_t sp = g.Dget_space(ds);
+    const long long n = g.Sget_simple_extent_npoints(sp);
+    const int nd = g.Sget_simple_extent_ndims(sp);
+    d->assign(nd > 0 ? (size_t)nd : 0, 0);
+    if (nd > 0) g.Sget_simple_extent_dims(sp, d->data(), nullptr);
+    bool ok = n >= 0 && (sz == 2 || sz == 4);
+    v16->clear();
+    v32->clear();
+    if (ok && sz == 2) {
+        v16->resize((size_t)n);
+        ok = g.Dread(ds, *g.T_NATIVE_USHORT, 0, 0, 0, v16->data()) >= 0;
+    } else if (ok) {
+        v32->resize((size_t)n);
+        ok = g.Dread(ds, *g.T_NATIVE_UINT, 0, 0, 0, v32->data()) >= 0;
+    }
+    g.Sclose(sp);
+    g.Dclose(ds);
+    return ok;
 }
 
 bool File::write_f32(const std::string& path, const float* data, const std::vector<hsize_t>& d)

@@ -40,11 +40,16 @@ public:
     bool read_i32(const std::string& path, std::vector<int32_t>* v, std::vector<hsize_t>* d) const;
     bool read_f64v(const std::string& path, std::vector<double>* v, std::vector<hsize_t>* d) const;
     bool read_f32v(const std::string& path, std::vector<float>* v, std::vector<hsize_t>* d) const;
+    // Unsigned 16- or 32-bit dataset (by its stored size) into v16 or v32.
+    bool read_frames(const std::string& path, std::vector<uint16_t>* v16, std::vector<uint32_t>* v32,
+                     std::vector<hsize_t>* d) const;
     // Writers: create or replace the dataset (and missing parent groups).
     bool write_f32(const std::string& path, const float* data, const std::vector<hsize_t>& d);
     bool write_f64(const std::string& path, const double* data, const std::vector<hsize_t>& d);
     bool write_i32(const std::string& path, const int32_t* data, const std::vector<hsize_t>& d);
     bool write_i64(const std::string& path, const int64_t* data, const std::vector<hsize_t>& d);
+    bool write_u16(const std::string& path, const uint16_t* data, const std::vector<hsize_t>& d);
+    bool write_u32(const std::string& path, const uint32_t* data, const std::vector<hsize_t>& d);
     bool write_string(const std::string& path, const std::string& s);
     // Chunked, deflate-compressed float dataset (plain when libhdf5 lacks the filter API).
     bool write_f32_deflate(const std::string& path, const float* data, const std::vector<hsize_t>& d, int level);

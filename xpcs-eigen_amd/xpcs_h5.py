@@ -80,7 +80,7 @@ class H5:
     def write(self, path, arr):
         a = np.ascontiguousarray(arr)
         code = {np.dtype(np.int32): 0, np.dtype(np.int64): 1, np.dtype(np.float32): 2,
-                np.dtype(np.float64): 3}[a.dtype]
+                np.dtype(np.float64): 3, np.dtype(np.uint16): 4, np.dtype(np.uint32): 5}[a.dtype]
         dims = (C.c_uint64 * max(a.ndim, 1))(*a.shape)
         _check(lib().xpcs_h5_write(self._f, path.encode(), code, a.ctypes.data_as(C.c_void_p), dims, a.ndim))
 

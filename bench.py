@@ -291,6 +291,81 @@ def run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, frames, reps=3, cpu=T
     return out
 
 
+def run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, g2_ref, reps=3):
+    """Other detector formats on the same sparse events: Rigaku u64 words of the
+    whole detector, UFXC u32 words of its 256x128 corner, and the corner's
+    counts as 16-bit raw frames (the HDF5 reader's dataset).  Per format: host
+    record pass, ingest (host bytes -> HBM), device job (execute), and the
+    PCIe-inclusive rate (ingest + execute + g2 read)."""
+    import xpcs_mt
+    fr = np.repeat(np.arange(F, dtype=np.int64), dlen[:F].astype(np.int64))
+    idx = np.concatenate([np.frombuffer(host_imm, np.uint32, int(dlen[f]), int(payload_off[f])) for f in range(F)])
+    val = np.concatenate([np.frombuffer(host_imm, np.uint16, int(dlen[f]), int(payload_off[f]) + 4 * int(dlen[f]))
+                          for f in range(F)]).astype(np.uint64)
+    idx = idx.astype(np.int64)
+    out = {}
+
+    def leg(name, plan_args, ingest, ref=None):
+        res = {}
+        plan = xpcs_mt.Plan(*plan_args)
+        t0 = time.perf_counter()
+        ingest(plan)
+        t1 = time.perf_counter()
+        plan.execute()
+        g2, _ = plan.g2()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            plan.execute()
+        plan.synchronize()
+        t3 = time.perf_counter()
+        res.update(ingest_s=t1 - t0, e2e_frames_per_s=plan_args[2] / (t2 - t0),
+                   device_ms_per_job=1e3 * (t3 - t2) / reps,
+                   device_frames_per_s=plan_args[2] * reps / (t3 - t2))
+        if ref is not None:
+            res["g2_equal_imm_path"] = bool(np.array_equal(np.nan_to_num(g2, nan=-1.0), np.nan_to_num(ref, nan=-1.0)))
+        plan.close()
+        return res
+
+    # Rigaku: frame f + 1, column-major pixel, 11-bit value (same events)
+    col = (idx % W) * H + idx // W
+    rw = ((fr + 1).astype(np.uint64) << np.uint64(40)) | (col.astype(np.uint64) << np.uint64(16)) | np.minimum(val, 2047)
+    t = time.perf_counter()
+    xpcs_mt.packed_records(rw, xpcs_mt.FMT_RIGAKU, W, H, F, dq, sq)
+    rec_s = time.perf_counter() - t
+    r = leg("rigaku", (W, H, F, dpl, dq, sq), lambda p: p.ingest_packed(rw, xpcs_mt.FMT_RIGAKU), g2_ref)
+    r.update(workload=f"{W}x{H} Rigaku words, {F} frames, {rw.size} words", host_records_s=rec_s)
+    out["rigaku"] = r
+    del rw
+    # UFXC and raw frames: the 256 x 128 corner (rows < 128, columns < 256)
+    Wc, Hc = 256, 128
+    m = (idx // W < Hc) & (idx % W < Wc)
+    ci = (idx[m] // W) * Wc + idx[m] % Wc
+    cf = fr[m]
+    cv = np.minimum(val[m], 3)
+    dqc, sqc = synth_corner(dq, sq, W, Wc, Hc)
+    uw = ((cf % 2048).astype(np.uint32) << np.uint32(21)) | (cv.astype(np.uint32) << np.uint32(15)) | \
+        ((ci % Wc) * Hc + ci // Wc).astype(np.uint32)
+    t = time.perf_counter()
+    xpcs_mt.packed_records(uw, xpcs_mt.FMT_UFXC, Wc, Hc, F, dqc, sqc)
+    rec_s = time.perf_counter() - t
+    u = leg("ufxc", (Wc, Hc, F, dpl, dqc, sqc), lambda p: p.ingest_packed(uw, xpcs_mt.FMT_UFXC))
+    u.update(workload=f"{Wc}x{Hc} UFXC words (2-bit values), {F} frames, {uw.size} words", host_records_s=rec_s)
+    out["ufxc"] = u
+    frames = np.zeros((F, Hc * Wc), np.uint16)
+    frames[cf, ci] = val[m].astype(np.uint16)
+    h = leg("frames", (Wc, Hc, F, dpl, dqc, sqc), lambda p: p.ingest_frames(frames.reshape(F, Hc, Wc)))
+    h.update(workload=f"{Wc}x{Hc} 16-bit raw frames (HDF5 reader dataset), {F} frames, {frames.nbytes} bytes")
+    out["frames"] = h
+    return out
+
+
+def synth_corner(dq, sq, W, Wc, Hc):
+    """q-maps of the detector's top-left Wc x Hc corner."""
+    dq2 = np.asarray(dq).reshape(-1, W)[:Hc, :Wc]
+    sq2 = np.asarray(sq).reshape(-1, W)[:Hc, :Wc]
+    return np.ascontiguousarray(dq2), np.ascontiguousarray(sq2)
+
+
 def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
     """Streaming service: 16 concurrent synthetic detector streams at 200 frames/s
     (stream s: seed 5000 + s, square detector U[256, 1024], occupancy U[0.2 %, 3 %],
@@ -390,6 +465,7 @@ def main():
     ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
     ap.add_argument("--stream-seconds", type=float, default=20.0,
                     help="wall time of the streaming leg (0 = skip)")
+    ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
     ap.add_argument("--twotime-frames", type=int, default=8192,
                     help="frames of the two-time leg on the sparse data (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -518,6 +594,9 @@ def main():
 
     plan.close()
     plan2 = None
+    formats = None
+    if not args.no_formats and not dense:
+        formats = run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, g2)
     twotime = None
     if args.twotime_frames > 0 and not dense:
         twotime = run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, min(args.twotime_frames, F),
@@ -548,6 +627,7 @@ def main():
         "batched": batched,
         "streaming": streaming,
         "twotime": twotime,
+        "formats": formats,
         "kernels": kern,
         "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
         "end_to_end_pcie": e2e,

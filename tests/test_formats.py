@@ -9,13 +9,23 @@ import synth
 import xpcs_mt
 
 
-def word_frames(records, n):
+def word_frames(records, n, keep=None):
+    """Output frame of every word from the records (-1: in no record); words
+    where `keep` is False (masked pixels) are reported as -1."""
     fw, cnt, fr = records
     out = np.full(n, -1, np.int64)
     for b, c, f in zip(fw, cnt, fr):
         assert np.all(out[b:b + c] == -1), "a word in two records"
         out[b:b + c] = f
+    if keep is not None:
+        out[~keep] = -1
     return out
+
+
+def rigaku_mapped_words(w, W, H, mapped):
+    import oracle
+    pix = oracle._raster((w >> np.uint64(16)) & np.uint64(0xFFFFF), W, H)
+    return mapped[pix]
 
 
 def counts(W, H, F, mu, seed):
@@ -81,7 +91,8 @@ def test_rigaku_records(stride, avg, start, F):
     mapped = (dq.reshape(-1) > 0) & (sq.reshape(-1) > 0)
     rec = xpcs_mt.packed_records(w, xpcs_mt.FMT_RIGAKU, W, H, F, dq, sq, stride, avg, start)
     want = oracle.rigaku_assign(w, W, H, mapped, start, F, stride, avg)
-    assert np.array_equal(word_frames(rec, w.size), want)
+    assert np.array_equal(word_frames(rec, w.size, rigaku_mapped_words(w, W, H, mapped)), want)
+    assert len(rec[0]) <= F * stride * avg + 1     # runs break only at frame changes
 
 
 def test_rigaku_last_frame_needs_two_pixels():
@@ -97,7 +108,7 @@ def test_rigaku_last_frame_needs_two_pixels():
                         ([(3, px[0], 5), (3, px[0], 1)], False)):
         w = synth.rigaku_words(c[:2], W, H, extra=extra)
         rec = xpcs_mt.packed_records(w, xpcs_mt.FMT_RIGAKU, W, H, 10, dq, sq)
-        got = word_frames(rec, w.size)
+        got = word_frames(rec, w.size, rigaku_mapped_words(w, W, H, mapped))
         assert np.array_equal(got, oracle.rigaku_assign(w, W, H, mapped, 0, 10))
         assert (got[-len(extra):] >= 0).all() == kept
 

@@ -77,8 +77,8 @@ xpcs_status ufxc_records(const uint32_t* w, int64_t n, const HostTables& h, std:
 // from the previous flush's (avg 1) or passes the next expected frame (avg > 1,
 // which then advances by stride*avg): output frames are counted in flushes, so
 // a file frame without words makes no output frame.  Words of masked pixels
-// are dropped (the transpose drops them on the device; here they only do not
-// count as pixels).  Reading stops at `frames` output frames.  The last pending
+// are not pending pixels (the transpose drops them on the device; here they
+// stay in their run).  Reading stops at `frames` output frames.  The last pending
 // frame is kept only when it holds more than one distinct pixel (:233).
 xpcs_status rigaku_records(const uint64_t* w, int64_t n, const HostTables& h, std::vector<PackedRecord>& out)
 {
@@ -113,9 +113,10 @@ xpcs_status rigaku_records(const uint64_t* w, int64_t n, const HostTables& h, st
         if (pix >= npd)
             return fail(XPCS_ERR_RANGE, "rigaku: word %lld names pixel %lld outside the %lldx%lld detector",
                         (long long)i, (long long)pix, (long long)W, (long long)H);
-        if (h.pix2c[(size_t)pix] < 0) continue;
         if (real >= h.F) continue;   // a word of the frame past the last one (never flushed)
-        if (seen[(size_t)pix] != real) {
+        // masked pixels are no pending pixels; their words stay in the run
+        // (the transpose drops them), so runs break only at frame changes
+        if (h.pix2c[(size_t)pix] >= 0 && seen[(size_t)pix] != real) {
             seen[(size_t)pix] = real;
             distinct++;
         }

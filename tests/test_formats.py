@@ -113,13 +113,14 @@ def test_rigaku_last_frame_needs_two_pixels():
         assert (got[-len(extra):] >= 0).all() == kept
 
 
-def test_rigaku_pixel_out_of_range():
+def test_rigaku_pixel_out_of_range_is_left_to_the_device():
+    """Pixels are decoded where the words are used (the transpose flags them);
+    the host pass only assigns frames."""
     W, H = 8, 8
     dq, sq = maps(W, H)
-    w = np.array([(1 << 40) | (2000 << 16) | 1], np.uint64)   # raster 250 of 64
-    with pytest.raises(xpcs_mt.XpcsError) as e:
-        xpcs_mt.packed_records(w, xpcs_mt.FMT_RIGAKU, W, H, 5, dq, sq)
-    assert e.value.status == 4
+    w = np.array([(1 << 40) | (2000 << 16) | 1, (2 << 40) | (3 << 16) | 1], np.uint64)   # raster 250 of 64
+    rec = xpcs_mt.packed_records(w, xpcs_mt.FMT_RIGAKU, W, H, 5, dq, sq)
+    assert word_frames(rec, w.size)[0] == 0
 
 
 def test_packed_sizes_and_format():

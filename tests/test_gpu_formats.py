@@ -124,6 +124,18 @@ def test_ufxc_pixel_out_of_range_is_an_error(xmt):
     assert e.value.status == 4
 
 
+def test_rigaku_pixel_out_of_range_is_an_error(xmt):
+    W, H = 8, 8
+    dq, sq = maps(W, H, 2, 1)
+    w = np.array([(1 << 40) | (2000 << 16) | 1, (2 << 40) | (3 << 16) | 1, (3 << 40) | (4 << 16) | 1], np.uint64)
+    plan = xmt.Plan(W, H, 5, 2, dq, sq)
+    plan.ingest_packed(w, xmt.FMT_RIGAKU)
+    plan.execute()
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.g2()
+    assert e.value.status == 4
+
+
 def test_packed_needs_no_darks(xmt):
     W, H = 8, 8
     dq, sq = maps(W, H, 2, 1)

@@ -87,49 +87,62 @@ xpcs_status rigaku_records(const uint64_t* w, int64_t n, const HostTables& h, st
     const int64_t start = h.frame_from, R = (int64_t)h.stride * h.avg;
     int64_t real = 0, next_expected = start + R, previous = start + 1;
     std::vector<PackedRecord> pending;
-    std::vector<int64_t> seen((size_t)npd, -1);   // flush index that last saw the pixel
-    int64_t distinct = 0;
-    int64_t run_b = -1, run_e = -1;
-    auto close_run = [&] {
-        if (run_b >= 0) push_run(pending, run_b, run_e, (int32_t)real);
-        run_b = run_e = -1;
+    // Only the frame field decides output frames, so the machine steps over
+    // runs of words with one frame number; inside a run each word can cause at
+    // most one flush, and once a word does not, no later word of the run does.
+    // Pixels are checked by the transpose on the device (words of records only,
+    // as the reference decodes only the words it keeps).
+    auto flush_due = [&](int64_t frame) {
+        return (h.avg > 1 && frame > next_expected) || (h.avg == 1 && frame != previous);
     };
-    for (int64_t i = 0; i < n; i++) {
-        const int64_t frame = (int64_t)(w[i] >> 40);
+    bool done = false;
+    for (int64_t b = 0; b < n && !done;) {
+        const uint64_t fw = w[b] >> 40;
+        int64_t e = b + 1;
+        while (e < n && (w[e] >> 40) == fw) e++;
+        const int64_t frame = (int64_t)fw;
+        int64_t i = b;
+        b = e;
         if (frame <= start) continue;
         if (real >= h.F) break;
         if (h.stride > 1 && frame != 0 && frame % h.stride != 0) continue;
-        if ((h.avg > 1 && frame > next_expected) || (h.avg == 1 && frame != previous)) {
-            close_run();
+        while (i < e && flush_due(frame)) {
+            // word i flushes the pending frame and opens frame real + 1
             out.insert(out.end(), pending.begin(), pending.end());
             pending.clear();
-            distinct = 0;
             previous = frame;
             real++;
             next_expected += R;
+            if (real >= h.F) {   // the next word stops the reader; frame `frames` is never kept
+                done = true;
+                break;
+            }
+            push_run(pending, i, i + 1, (int32_t)real);
+            i++;
         }
-        const uint32_t col_major = (uint32_t)(w[i] >> 16) & 0xfffffu;
-        const int64_t pix = (int64_t)(col_major % (uint32_t)H) * W + col_major / (uint32_t)H;
-        if (pix >= npd)
-            return fail(XPCS_ERR_RANGE, "rigaku: word %lld names pixel %lld outside the %lldx%lld detector",
-                        (long long)i, (long long)pix, (long long)W, (long long)H);
-        if (real >= h.F) continue;   // a word of the frame past the last one (never flushed)
-        // masked pixels are no pending pixels; their words stay in the run
-        // (the transpose drops them), so runs break only at frame changes
-        if (h.pix2c[(size_t)pix] >= 0 && seen[(size_t)pix] != real) {
-            seen[(size_t)pix] = real;
-            distinct++;
-        }
-        if (run_b >= 0 && run_e == i) {
-            run_e = i + 1;
+        if (done || i >= e) continue;
+        // the rest of the run joins the pending frame (merged with a one-word run just before it)
+        if (!pending.empty() && pending.back().word + (int64_t)pending.back().count == i &&
+            pending.back().frame == (int32_t)real && pending.back().count < (1u << 31)) {
+            const int64_t b0 = pending.back().word;
+            pending.pop_back();
+            push_run(pending, b0, e, (int32_t)real);
         } else {
-            close_run();
-            run_b = i;
-            run_e = i + 1;
+            push_run(pending, i, e, (int32_t)real);
         }
     }
-    close_run();
-    if (distinct > 1 && real < h.F) out.insert(out.end(), pending.begin(), pending.end());
+    if (!done && real < h.F && !pending.empty()) {
+        // the last pending frame is kept with more than one distinct mapped pixel
+        std::vector<int64_t> pix;
+        for (const auto& r : pending)
+            for (int64_t k = r.word; k < r.word + (int64_t)r.count; k++) {
+                const uint32_t cm = (uint32_t)(w[k] >> 16) & 0xfffffu;
+                const int64_t p = (int64_t)(cm % (uint32_t)H) * W + cm / (uint32_t)H;
+                if (p < npd && h.pix2c[(size_t)p] >= 0) pix.push_back(p);
+            }
+        std::sort(pix.begin(), pix.end());
+        if (std::unique(pix.begin(), pix.end()) - pix.begin() > 1) out.insert(out.end(), pending.begin(), pending.end());
+    }
     return XPCS_OK;
 }
 

@@ -52,6 +52,13 @@ typedef int32_t xpcs_status;
 #define XPCS_MT_TIMING 0x1u      /* record HIP events around every kernel */
 #define XPCS_MT_SIDE_STATS 0x2u  /* also sum counts per (frame, static bin) for frameSum /
                                     partition-mean-partial (xpcs_mt_read_side) */
+/* normalize_by_framesum (src/xpcs/main.cpp:313-336): every value of frame f is
+ * scaled by mean_f(frameSum) / frameSum[f] before the correlation, so G2/IP/IF
+ * are FP64 sums (xpcs_mt_read_sums_f64; xpcs_mt_read_sums_i64 refuses) formed
+ * by the per-pixel dense-series kernel; two-time uses the same values.  Side
+ * statistics stay those of the raw counts, as in the reference.  Not for rows
+ * ingests (no frame sums) or streams. */
+#define XPCS_MT_NORMALIZE_BY_FRAMESUM 0x4u
 
 typedef struct xpcs_mt_plan xpcs_mt_plan;
 
@@ -223,6 +230,9 @@ xpcs_status xpcs_mt_synchronize(xpcs_mt_plan* plan);
 
 /* Exact per-pixel sums, [P][T] int64 in compact pixel order (summed-bin units). */
 xpcs_status xpcs_mt_read_sums_i64(xpcs_mt_plan* plan, int64_t* G2, int64_t* IP, int64_t* IF);
+/* G2/IP/IF as FP64, [pixels][taus] (the normalize_by_framesum sums, or the
+ * exact integer sums converted). */
+xpcs_status xpcs_mt_read_sums_f64(xpcs_mt_plan* plan, double* G2, double* IP, double* IF);
 
 /*
  * Per-pixel sums in the reference's float units and layout [T][W*H]

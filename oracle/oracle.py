@@ -518,3 +518,98 @@ def rigaku_imm(words, width, height, mapped, start, frames, stride=1, avg=1):
         ev.append((idx.astype(np.int32), acc[idx].astype(np.uint16)))
     import synth
     return synth.imm_bytes_events(ev, width, height)
+
+
+# ---- normalize_by_framesum (src/xpcs/main.cpp:313-336) ------------------------------
+def frame_weights(csr, frames, weight=None):
+    """mean_f(S_f) / S_f with S_f the frame's summed values over mapped pixels
+    (flat-field weighted), 0 for a frame without counts; the reference's
+    1/(avg * pixels) factors of frameSum cancel in the ratio."""
+    S = np.zeros(frames)
+    cnt = csr["cnt"].astype(np.float64)
+    if weight is not None:
+        cnt = cnt * np.repeat(np.asarray(weight, np.float64), np.diff(csr["off"]))
+    np.add.at(S, csr["frame"], cnt)
+    m = S.mean()
+    return np.where(S > 0, m / np.where(S > 0, S, 1.0), 0.0)
+
+
+def multitau_f64(csr, frames, level, tau, frame_weight=None):
+    """FP64 G2/IP/IF [P][T] of the values count x frame_weight[frame]
+    (corr.cpp:315-431 in closed form: binned series x_L, N = lf_L - tau',
+    G2 = sum_{t<N} x_L[t] x_L[t+tau'], IP = sum_{t<N} x_L[t],
+    IF = sum_{tau'<=t<lf_L} x_L[t])."""
+    P = csr["off"].size - 1
+    X = np.zeros((P, frames))
+    rows = np.repeat(np.arange(P), np.diff(csr["off"]))
+    v = csr["cnt"].astype(np.float64)
+    if frame_weight is not None:
+        v = v * np.asarray(frame_weight)[csr["frame"]]
+    np.add.at(X, (rows, csr["frame"]), v)
+    T = level.size
+    G2, IP, IF = (np.zeros((P, T)) for _ in range(3))
+    series = {0: X}
+    for L in range(1, int(level.max()) + 1 if T else 1):
+        prev = series[L - 1]
+        lf = frames >> L
+        series[L] = prev[:, 0:2 * lf:2] + prev[:, 1:2 * lf:2]
+    for i in range(T):
+        L = int(level[i])
+        x = series[L]
+        lf = frames >> L
+        tp = int(tau[i]) >> L
+        N = lf - tp
+        G2[:, i] = (x[:, :N] * x[:, tp:tp + N]).sum(1)
+        IP[:, i] = x[:, :N].sum(1)
+        IF[:, i] = x[:, tp:lf].sum(1)
+    return G2, IP, IF
+
+
+def normalize_f64_sums(G2, IP, IF, Nvec, qm, weight=None):
+    """normalizeG2s (corr.cpp:927-1091) in FP64 over FP64 sums: the numpy form
+    of xo_normalize_f64 (static-bin g2s, NaN-skipping mean per q, stderr of the
+    per-pixel ratios with population variance)."""
+    G2, IP, IF = (np.asarray(a, np.float64) for a in (G2, IP, IF))
+    T = G2.shape[1]
+    nq = qm["nq"]
+    w = np.ones(G2.shape[0]) if weight is None else np.asarray(weight, np.float64)
+    N = np.asarray(Nvec, np.float64)
+    ng = qm["ngroups"]
+    g2s = np.zeros((ng, T))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for g in range(ng):
+            b, e = qm["group_begin"][g], qm["group_end"][g]
+            sg = (w[b:e, None] ** 2 * G2[b:e]).sum(0)
+            sp = (w[b:e, None] * IP[b:e]).sum(0)
+            sf = (w[b:e, None] * IF[b:e]).sum(0)
+            g2s[g] = (e - b) * N * sg / (sp * sf)
+        g2 = np.zeros((T, nq))
+        se = np.zeros((T, nq))
+        for q in range(1, nq + 1):
+            gs = [g for g in range(ng) if qm["group_q"][g] == q]
+            v = g2s[gs]
+            ok = ~np.isnan(v)
+            g2[:, q - 1] = np.where(ok, v, 0.0).sum(0) / ok.sum(0)
+            rows = np.concatenate([np.arange(qm["group_begin"][g], qm["group_end"][g]) for g in gs]) if gs else \
+                np.zeros(0, np.int64)
+            x = N[None, :] * G2[rows] / (IP[rows] * IF[rows])
+            x[w[rows] == 0.0] = np.nan
+            x = np.nan_to_num(x, nan=0.0, posinf=np.inf)
+            n = rows.size
+            se[:, q - 1] = np.sqrt(1.0 / n) * np.sqrt(x.var(0)) if n else np.nan
+    return g2, se
+
+
+def run_fsnorm(buf, dqmap, sqmap, frames, dpl, darks=0, frame_from=0, stride=1, avg=1, flatfield=None, lld=0.0,
+               sigma=0.0):
+    """Whole normalize_by_framesum path: the exact path's rows, frame weights,
+    FP64 multi-tau and normalisation."""
+    r = run_exact(buf, dqmap, sqmap, frames, dpl, darks, frame_from, stride, avg, flatfield, lld, sigma)
+    qm, csr = r["qm"], r["csr"]
+    weight = None
+    if flatfield is not None:
+        weight = np.asarray(flatfield, np.float64).reshape(-1)[qm["order"]]
+    fw = frame_weights(csr, frames, weight)
+    G2, IP, IF = multitau_f64(csr, frames, r["level"], r["tau"], fw)
+    g2, se = normalize_f64_sums(G2, IP, IF, r["N"], qm, weight)
+    return dict(r, fw=fw, G2f=G2, IPf=IP, IFf=IF, g2f=g2, stderrf=se)

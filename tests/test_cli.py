@@ -204,3 +204,16 @@ def test_cli_other_formats(tmp_path, orc, fmt):
     with xpcs_h5.H5(conf) as h:
         ts = h.read("/exchange/timestamp_clock", np.float64)
         assert ts.shape == (2, F) and np.array_equal(ts[0], np.arange(1, F + 1))
+
+
+@pytest.mark.gpu
+def test_cli_normalize_by_framesum(tmp_path, orc):
+    job = make_job(tmp_path, F=300, normalize_by_framesum=1)
+    print("corr normalize_by_framesum", flush=True)
+    r = run(job["conf"])
+    assert r.returncode == 0, r.stderr
+    ref = orc.run_fsnorm(job["imm"].read_bytes(), job["dq"], job["sq"], job["F"], job["dpl"])
+    with xpcs_h5.H5(job["conf"]) as h:
+        g2 = h.read("/exchange/norm-0-g2", np.float32)
+    ok = ~np.isnan(ref["g2f"])
+    assert np.allclose(g2[ok], ref["g2f"][ok], rtol=1e-6)

@@ -1,0 +1,98 @@
+"""normalize_by_framesum on the device (FP64 per-pixel series) vs the FP64
+oracle: sums within 1e-12, g2 within 1e-9 relative (FP64 both sides, summed in
+different orders)."""
+import numpy as np
+import pytest
+
+import cases
+import synth
+
+pytestmark = pytest.mark.gpu
+
+SUM_RTOL = 1e-12
+G2_RTOL = 1e-9
+
+
+def run_gpu(xmt, c, **kw):
+    print(f"fsnorm W={c['W']} H={c['H']} F={c['F']} kw={sorted(kw)}", flush=True)
+    plan = xmt.Plan(c["W"], c["H"], c["F"], c["dpl"], c["dq"], c["sq"], normalize_by_framesum=True, **kw)
+    plan.ingest_imm(c["imm"])
+    plan.execute()
+    return plan
+
+
+def check(plan, r):
+    G2, IP, IF = plan.sums_f64()
+    for got, want in ((G2, r["G2f"]), (IP, r["IPf"]), (IF, r["IFf"])):
+        np.testing.assert_allclose(got, want, rtol=SUM_RTOL, atol=1e-9)
+    g2, se = plan.g2()
+    ok = ~np.isnan(r["g2f"])
+    assert np.array_equal(np.isnan(g2), ~ok)
+    np.testing.assert_allclose(g2[ok], r["g2f"][ok], rtol=G2_RTOL)
+    np.testing.assert_allclose(se, r["stderrf"], rtol=1e-7, atol=1e-12)
+
+
+def test_sparse(xmt, orc):
+    c = cases.sparse_case(W=32, H=32, F=500, mu=0.1, tau_c=10.0, seed=601)
+    plan = run_gpu(xmt, c)
+    check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"]))
+    with pytest.raises(xmt.XpcsError):
+        plan.sums()                   # the int64 reader refuses FP64 sums
+
+
+def test_sparse_flatfield_stride_average(xmt, orc):
+    c = cases.sparse_case(W=24, H=24, F=900, mu=0.15, tau_c=8.0, seed=602)
+    c["F"] = 200
+    ff = np.random.default_rng(603).uniform(0.5, 1.5, (24, 24))
+    plan = run_gpu(xmt, c, stride=2, average=2, frame_from=3, flatfield=ff)
+    check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"], stride=2, avg=2, frame_from=3,
+                               flatfield=ff))
+
+
+def test_dense_with_darks(xmt, orc):
+    W = H = 16
+    F = 300
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    sig = synth.ou_counts(W, H, F, 4.0, 15.0, seed=604)
+    dark = synth.dark_frames(W, H, 2, seed=605)
+    c = dict(W=W, H=H, F=F, dpl=4, dq=dq, sq=sq,
+             imm=synth.imm_bytes_dense(np.concatenate([dark, sig + dark[0]], axis=0), W, H))
+    plan = run_gpu(xmt, c, dark_frames=2)
+    check(plan, orc.run_fsnorm(c["imm"], dq, sq, F, 4, darks=2))
+
+
+def test_side_stats_are_the_raw_counts(xmt):
+    c = cases.sparse_case(W=24, H=24, F=300, mu=0.2, tau_c=8.0, seed=606)
+    a = xmt.Plan(c["W"], c["H"], c["F"], 4, c["dq"], c["sq"], side_stats=True, normalize_by_framesum=True)
+    b = xmt.Plan(c["W"], c["H"], c["F"], 4, c["dq"], c["sq"], side_stats=True)
+    for p in (a, b):
+        p.ingest_imm(c["imm"])
+        p.execute()
+    sa, sb = a.side(window=30, frame_sum=True, partial=True), b.side(window=30, frame_sum=True, partial=True)
+    for k in sa:
+        np.testing.assert_allclose(sa[k], sb[k], rtol=1e-6)
+
+
+def test_twotime_uses_normalised_values(xmt, orc):
+    c = cases.sparse_case(W=24, H=24, F=200, mu=0.2, tau_c=8.0, seed=607, S=6, Q=3)
+    plan = xmt.Plan(c["W"], c["H"], c["F"], 4, c["dq"], c["sq"], normalize_by_framesum=True)
+    plan.ingest_imm(c["imm"])
+    r = orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], 4)
+    for method in (1, 2):
+        print("fsnorm twotime method", method, flush=True)
+        got = plan.twotime([1, 3], 1, False, 4, method)
+        csr = dict(r["csr"])
+        csr["cnt"] = csr["cnt"].astype(np.float64) * r["fw"][csr["frame"]]
+        want = orc.twotime_f64(csr, r["qm"], c["F"], [1, 3], 1, False, 4)
+        for x, y in zip(got["c2t"], want["c2t"]):
+            np.testing.assert_allclose(x, y, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(got["g2full"], want["g2full"], rtol=1e-10)
+
+
+def test_rows_ingest_is_refused(xmt):
+    c = cases.sparse_case(W=8, H=8, F=50, mu=0.3, tau_c=5.0, seed=608, S=2, Q=1)
+    plan = xmt.Plan(8, 8, 50, 4, c["dq"], c["sq"], normalize_by_framesum=True)
+    plan.ingest_rows(np.zeros(65, np.int64), np.zeros(0, np.int32), np.zeros(0, np.uint32))
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.execute()
+    assert e.value.status == 5

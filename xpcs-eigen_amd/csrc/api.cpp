@@ -183,13 +183,15 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
         p->nbnd = (int32_t)bnd.size();
         TRY(upload(&p->d_bnd, bnd, s));
     }
-    if (h.flags & XPCS_MT_SIDE_STATS) {
+    p->fsnorm = (h.flags & XPCS_MT_NORMALIZE_BY_FRAMESUM) != 0;
+    if (h.flags & (XPCS_MT_SIDE_STATS | XPCS_MT_NORMALIZE_BY_FRAMESUM)) {
         if (h.ns > 4096 || (int64_t)h.F * h.ns > (int64_t(1) << 26)) {
             xpcs_mt_plan_destroy(p);
             return fail(XPCS_ERR_UNSUPPORTED, "plan: side statistics need S <= 4096 and F*S <= 2^26");
         }
         TRY(dalloc(&p->d_fs, (size_t)h.F * h.ns));
     }
+    if (p->fsnorm) TRY(dalloc(&p->d_fw, (size_t)h.F));
     // dense-row scratch (allocated by the first execute that needs it): one row
     // of F 64-bit values per resident workgroup
     {
@@ -217,7 +219,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
     dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
-    dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total);
+    dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total); dfree(p->d_fw);
     dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs); dfree(p->d_series_a); dfree(p->d_series_b); dfree(p->d_bnd);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : p->event_pool) (void)hipEventDestroy(e);
@@ -781,8 +783,10 @@ static bool dense_stream_ok(const xpcs_mt_plan* p)
 xpcs_status prepare_execute(xpcs_mt_plan* p)
 {
     if (p->mode == 0) return fail(XPCS_ERR_STATE, "execute: nothing ingested");
+    if (p->fsnorm && p->mode == 3)
+        return fail(XPCS_ERR_UNSUPPORTED, "execute: normalize_by_framesum needs frame sums (IMM or packed ingest)");
     HostTables& h = p->h;
-    const bool long_rows = p->mode != 2 || !dense_stream_ok(p);
+    const bool long_rows = p->fsnorm || p->mode != 2 || !dense_stream_ok(p);
     if (long_rows && !p->d_scratch) {
         xpcs_status st = dalloc(&p->d_scratch, (size_t)p->long_blocks * h.F);
         if (st) return st;
@@ -808,7 +812,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
             if (st) return st;
             p->bsum_cap = nbs;
         }
-    } else if (p->mode == 2 && dense_stream_ok(p)) {
+    } else if (p->mode == 2 && dense_stream_ok(p) && !p->fsnorm) {
         const size_t need = dense_series_elems(p);
         if (need > p->series_cap) {
             xpcs_status st = dalloc(&p->d_series_a, need);
@@ -835,7 +839,14 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
     HostTables& h = p->h;
     launch_fill_u32(p->d_counters, 4, s);
     if (p->d_fs) launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
-    if (p->mode == 1) {
+    if (p->fsnorm) {
+        // normalize_by_framesum: frame sums first, then every pixel in FP64
+        if (p->mode == 1) launch_transpose(p, s);
+        else launch_dense_transpose(p, s);
+        launch_side(p, s);
+        launch_frame_weights(p, s);
+        launch_mt_fsnorm(p, s, p->mode == 2);
+    } else if (p->mode == 1) {
         launch_transpose(p, s);
         if (p->d_fs) launch_side(p, s);
         launch_mt_sparse(p, s);
@@ -893,10 +904,33 @@ extern "C" xpcs_status xpcs_mt_synchronize(xpcs_mt_plan* p)
     return sync_and_check(p);
 }
 
+extern "C" xpcs_status xpcs_mt_read_sums_f64(xpcs_mt_plan* p, double* G2, double* IP, double* IF)
+{
+    if (!p || !G2 || !IP || !IF) return fail(XPCS_ERR_INVALID_ARG, "read_sums_f64: NULL argument");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "read_sums_f64: execute first");
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    const size_t n = (size_t)p->h.P * p->h.T;
+    std::vector<int64_t> tmp(n);
+    double* out[3] = {G2, IP, IF};
+    const int64_t* src[3] = {p->d_G2, p->d_IP, p->d_IF};
+    for (int k = 0; k < 3; k++) {
+        HIP_TRY(hipMemcpy(tmp.data(), src[k], n * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            double v;
+            if (p->fsnorm) std::memcpy(&v, &tmp[i], sizeof(v));
+            else v = (double)tmp[i];
+            out[k][i] = v;
+        }
+    }
+    return XPCS_OK;
+}
+
 extern "C" xpcs_status xpcs_mt_read_sums_i64(xpcs_mt_plan* p, int64_t* G2, int64_t* IP, int64_t* IF)
 {
     if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_sums: NULL plan");
     if (!p->executed) return fail(XPCS_ERR_STATE, "read_sums: execute first");
+    if (p->fsnorm) return fail(XPCS_ERR_STATE, "read_sums: normalize_by_framesum sums are FP64 (read_sums_f64)");
     xpcs_status st = sync_and_check(p);
     if (st) return st;
     const size_t n = (size_t)p->h.P * p->h.T * sizeof(int64_t);

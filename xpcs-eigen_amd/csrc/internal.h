@@ -121,6 +121,8 @@ struct xpcs_mt_plan {
     // ingest state: 0 none, 1 sparse IMM, 2 dense IMM, 3 rows
     int mode = 0;
     int32_t fmt = 0;                     // word format of the sparse frame records (kFmt*)
+    bool fsnorm = false;                 // normalize_by_framesum: FP64 sums in the G2/IP/IF buffers
+    double* d_fw = nullptr;              // frame weights mean(frameSum) / frameSum[f]
     const uint8_t* d_imm = nullptr;      // raw IMM bytes (owned when own_imm)
     uint8_t* d_imm_owned = nullptr;
     size_t imm_cap = 0;
@@ -250,6 +252,8 @@ void launch_fill_u32(uint32_t* d, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
 xpcs_status prepare_execute(xpcs_mt_plan* p);
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus);
+void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
+void launch_frame_weights(xpcs_mt_plan* p, hipStream_t s);
 // two-time (k_twotime.hip)
 void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, double* X, hipStream_t s);
 void launch_tt_colsum(xpcs_mt_plan* p, const double* X, int32_t F, int32_t lo, int32_t hi,

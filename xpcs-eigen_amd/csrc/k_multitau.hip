@@ -511,115 +511,168 @@ __device__ uint64_t block_sum_u64(uint64_t x, unsigned long long* s_red)
     return t;
 }
 
+// FP64 block sum through the same 64-bit LDS slots (wave order fixed).
+__device__ double block_sum_f64(double x, unsigned long long* s_red)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane_id() == 0) s_red[w] = (unsigned long long)__double_as_longlong(x);
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += __longlong_as_double((long long)s_red[i]);
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ uint64_t block_sum(uint64_t x, unsigned long long* s_red) { return block_sum_u64(x, s_red); }
+__device__ __forceinline__ double block_sum(double x, unsigned long long* s_red) { return block_sum_f64(x, s_red); }
+// a sum as stored in the int64 [P][T] buffers: integers as such, FP64 as bits
+__device__ __forceinline__ int64_t sum_bits(uint64_t v) { return (int64_t)v; }
+__device__ __forceinline__ int64_t sum_bits(double v) { return (int64_t)__double_as_longlong(v); }
+__device__ __forceinline__ unsigned long long slot_bits(uint64_t v) { return v; }
+__device__ __forceinline__ unsigned long long slot_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+template <typename AccT>
+__device__ __forceinline__ AccT slot_value(unsigned long long v);
+template <>
+__device__ __forceinline__ uint64_t slot_value<uint64_t>(unsigned long long v) { return v; }
+template <>
+__device__ __forceinline__ double slot_value<double>(unsigned long long v) { return __longlong_as_double((long long)v); }
+
 // Lag sums of one level over a dense series src[0, lf) for lags lo..hi; the
-// series is read from `src` (level 0 of the dense path) or scratch.
-template <typename SrcT>
+// series is read from `src` (level 0 of the dense path) or scratch.  AccT is
+// uint64_t (exact integer sums) or double (frame-sum normalised values).
+template <typename AccT, typename SrcT>
 __device__ void long_level(const SrcT* __restrict__ src, int lf, const LevelParam& lp,
                            int64_t* __restrict__ g2row, int64_t* __restrict__ iprow,
                            int64_t* __restrict__ ifrow, unsigned long long* s_red,
                            unsigned long long* s_lag, uint64_t* total0)
 {
     // sum over the level
-    uint64_t s = 0;
-    for (int t = threadIdx.x; t < lf; t += blockDim.x) s += (uint64_t)src[t];
-    const uint64_t S = block_sum_u64(s, s_red);
-    if (total0 && threadIdx.x == 0) *total0 = S;
+    AccT s = 0;
+    for (int t = threadIdx.x; t < lf; t += blockDim.x) s += (AccT)src[t];
+    const AccT S = block_sum(s, s_red);
+    if (total0 && threadIdx.x == 0) *total0 = (uint64_t)sum_bits(S);
     // products, kMaxLagBlock lags at a time
     for (int l0 = lp.lo; l0 <= lp.hi; l0 += kMaxLagBlock) {
-        uint64_t acc[kMaxLagBlock];
+        AccT acc[kMaxLagBlock];
 #pragma unroll
         for (int d = 0; d < kMaxLagBlock; d++) acc[d] = 0;
         const int nlag = min(kMaxLagBlock, lp.hi - l0 + 1);
         for (int t = threadIdx.x; t < lf; t += blockDim.x) {
-            const uint64_t xt = (uint64_t)src[t];
+            const AccT xt = (AccT)src[t];
             if (xt == 0) continue;
 #pragma unroll
             for (int d = 0; d < kMaxLagBlock; d++) {
                 const int u = t + l0 + d;
-                if (d < nlag && u < lf) acc[d] += xt * (uint64_t)src[u];
+                if (d < nlag && u < lf) acc[d] += xt * (AccT)src[u];
             }
         }
 #pragma unroll
         for (int d = 0; d < kMaxLagBlock; d++) {
-            const uint64_t v = block_sum_u64(acc[d], s_red);
-            if (threadIdx.x == 0 && d < nlag) s_lag[d] = v;
+            const AccT v = block_sum(acc[d], s_red);
+            if (threadIdx.x == 0 && d < nlag) s_lag[d] = slot_bits(v);
         }
         __syncthreads();
-        for (int d = threadIdx.x; d < nlag; d += blockDim.x) g2row[lp.first + (l0 - lp.lo) + d] = (int64_t)s_lag[d];
+        for (int d = threadIdx.x; d < nlag; d += blockDim.x)
+            g2row[lp.first + (l0 - lp.lo) + d] = sum_bits(slot_value<AccT>(s_lag[d]));
         __syncthreads();
     }
     // IP = S - sum of the last tau' values, IF = S - sum of the first tau' values
     for (int d = threadIdx.x; d < lp.count; d += blockDim.x) {
         const int tp = lp.lo + d;
-        uint64_t head = 0, tail = 0;
-        for (int t = 0; t < tp && t < lf; t++) head += (uint64_t)src[t];
-        for (int t = max(lf - tp, 0); t < lf; t++) tail += (uint64_t)src[t];
-        iprow[lp.first + d] = (int64_t)(S - tail);
-        ifrow[lp.first + d] = (int64_t)(S - head);
+        AccT head = 0, tail = 0;
+        for (int t = 0; t < tp && t < lf; t++) head += (AccT)src[t];
+        for (int t = max(lf - tp, 0); t < lf; t++) tail += (AccT)src[t];
+        iprow[lp.first + d] = sum_bits((AccT)(S - tail));
+        ifrow[lp.first + d] = sum_bits((AccT)(S - head));
     }
     __syncthreads();
 }
 
 // x[k] = src[2k] + src[2k+1] for k < lf_out, chunked so it may run in place.
-template <typename SrcT>
-__device__ void long_bin(const SrcT* src, uint64_t* x, int lf_out)
+template <typename AccT, typename SrcT>
+__device__ void long_bin(const SrcT* src, AccT* x, int lf_out)
 {
     for (int k0 = 0; k0 < lf_out; k0 += blockDim.x) {
         const int k = k0 + threadIdx.x;
-        uint64_t v = 0;
-        if (k < lf_out) v = (uint64_t)src[2 * k] + (uint64_t)src[2 * k + 1];
+        AccT v = 0;
+        if (k < lf_out) v = (AccT)src[2 * k] + (AccT)src[2 * k + 1];
         __syncthreads();
         if (k < lf_out) x[k] = v;
         __syncthreads();
     }
 }
 
+// One workgroup per pixel row: the listed long rows (or, with all_rows, every
+// pixel) of the sparse events, or every dense row.  With frame weights fw the
+// level-0 value of frame f is count x fw[f] and the sums are FP64
+// (AccT = double, normalize_by_framesum, main.cpp:313-336).
+template <typename AccT>
 __global__ void __launch_bounds__(kLongBlock)
 k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
           const uint32_t* __restrict__ rows, int32_t P, int32_t T,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F,
-          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense,
-          uint64_t* __restrict__ scratch, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
-          int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap)
+          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense, int all_rows,
+          const double* __restrict__ fw, uint64_t* __restrict__ scratch, int64_t* __restrict__ G2,
+          int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
+          const int32_t* __restrict__ rowmap)
 {
     __shared__ unsigned long long s_red[kLongBlock / 64];
     __shared__ unsigned long long s_lag[kMaxLagBlock];
-    uint64_t* x = scratch + (size_t)blockIdx.x * (size_t)F;
-    const int64_t nrows = dense ? (int64_t)P : (int64_t)counters[0];
+    AccT* x = reinterpret_cast<AccT*>(scratch + (size_t)blockIdx.x * (size_t)F);
+    const int64_t nrows = (dense || all_rows) ? (int64_t)P : (int64_t)counters[0];
     for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-        const int64_t p = dense ? r : (int64_t)long_list[r];
+        const int64_t p = (dense || all_rows) ? r : (int64_t)long_list[r];
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
         for (int i = threadIdx.x; i < T; i += blockDim.x) {
-            g2row[i] = 0;
-            iprow[i] = 0;
-            ifrow[i] = 0;
+            g2row[i] = sum_bits((AccT)0);
+            iprow[i] = sum_bits((AccT)0);
+            ifrow[i] = sum_bits((AccT)0);
         }
         __syncthreads();
-        if (dense) {
+        if (dense && !fw) {
             const uint32_t* row = rows + p * (int64_t)F;
-            long_level<uint32_t>(row, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
-            if (nlev > 1) long_bin<uint32_t>(row, x, levels[1].lf);
+            long_level<AccT, uint32_t>(row, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
+            if (nlev > 1) long_bin<AccT, uint32_t>(row, x, levels[1].lf);
         } else {
-            for (int t = threadIdx.x; t < F; t += blockDim.x) x[t] = 0;
-            __syncthreads();
-            const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
-            const int64_t b = off[row], e = off[row + 1];
-            for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-                const uint64_t v = ev[i];
-                atomicAdd(reinterpret_cast<unsigned long long*>(&x[v >> 32]),
-                          (unsigned long long)(uint32_t)v);
+            uint64_t counts = 0;   // the pixel's raw count total (pixelSum precedes the normalisation)
+            if (dense) {
+                const uint32_t* row = rows + p * (int64_t)F;
+                for (int t = threadIdx.x; t < F; t += blockDim.x) {
+                    x[t] = (AccT)row[t] * (AccT)fw[t];
+                    counts += row[t];
+                }
+            } else {
+                for (int t = threadIdx.x; t < F; t += blockDim.x) x[t] = 0;
+                __syncthreads();
+                const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+                const int64_t b = off[row], e = off[row + 1];
+                for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+                    const uint64_t v = ev[i];
+                    const uint32_t f = (uint32_t)(v >> 32);
+                    if (fw)
+                        atomicAdd(reinterpret_cast<double*>(&x[f]), (double)(uint32_t)v * fw[f]);
+                    else
+                        atomicAdd(reinterpret_cast<unsigned long long*>(&x[f]), (unsigned long long)(uint32_t)v);
+                    counts += (uint32_t)v;
+                }
             }
             __threadfence_block();
             __syncthreads();
-            long_level<uint64_t>(x, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
-            if (nlev > 1) long_bin<uint64_t>(x, x, levels[1].lf);
+            if (fw) {
+                const uint64_t c = block_sum_u64(counts, s_red);
+                if (threadIdx.x == 0) pix_total[p] = c;
+            }
+            long_level<AccT, AccT>(x, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, fw ? nullptr : pix_total + p);
+            if (nlev > 1) long_bin<AccT, AccT>(x, x, levels[1].lf);
         }
         for (int L = 1; L < nlev; L++) {
             const LevelParam lp = levels[L];
-            long_level<uint64_t>(x, lp.lf, lp, g2row, iprow, ifrow, s_red, s_lag, nullptr);
-            if (L + 1 < nlev) long_bin<uint64_t>(x, x, levels[L + 1].lf);
+            long_level<AccT, AccT>(x, lp.lf, lp, g2row, iprow, ifrow, s_red, s_lag, nullptr);
+            if (L + 1 < nlev) long_bin<AccT, AccT>(x, x, levels[L + 1].lf);
         }
         __syncthreads();
     }
@@ -650,11 +703,22 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
     timer_begin(p, s, dense_rows ? "k_mt_long_dense" : "k_mt_long");
-    k_mt_long<<<p->long_blocks, kLongBlock, 0, s>>>(p->d_ev, p->d_off, p->d_rows, h.P, h.T,
-                                                    p->d_levels, (int32_t)h.levels.size(), h.F,
-                                                    p->d_long_list, p->d_counters, dense_rows ? 1 : 0,
-                                                    p->d_scratch, p->d_G2, p->d_IP, p->d_IF,
-                                                    p->d_pix_total, p->d_rowmap);
+    k_mt_long<uint64_t><<<p->long_blocks, kLongBlock, 0, s>>>(
+        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
+        p->d_counters, dense_rows ? 1 : 0, 0, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
+        p->d_rowmap);
+    timer_end(p, s);
+}
+
+// Every pixel in FP64 with the frame weights (normalize_by_framesum).
+void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
+{
+    const HostTables& h = p->h;
+    timer_begin(p, s, "k_mt_long_f64");
+    k_mt_long<double><<<p->long_blocks, kLongBlock, 0, s>>>(
+        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
+        p->d_counters, dense_rows ? 1 : 0, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
+        p->d_rowmap);
     timer_end(p, s);
 }
 

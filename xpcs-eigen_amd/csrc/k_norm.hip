@@ -18,6 +18,10 @@ namespace xpcs {
 
 constexpr int kNormBatch = 16;
 
+// FP: the sums are FP64 (frame-sum normalised values) stored in the same
+// 8-byte buffers; they take the weighted (FP64) route with weight 1 when the
+// plan has no flat field.
+template <bool FP>
 __global__ void __launch_bounds__(256)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
                int32_t T, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
@@ -47,7 +51,15 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
                 const int64_t a = aa[u];
                 const int64_t b = bb[u];
                 double x;
-                if (weight) {
+                if (FP) {
+                    const double w = weight ? weight[p] : 1.0;
+                    const double gd = __longlong_as_double(g), ad = __longlong_as_double(a),
+                                 bd = __longlong_as_double(b);
+                    wg += w * w * gd;
+                    wp += w * ad;
+                    wf += w * bd;
+                    x = (w == 0.0) ? 0.0 : N * gd / (ad * bd);
+                } else if (weight) {
                     const double w = weight[p];
                     wg += w * w * (double)g;
                     wp += w * (double)a;
@@ -67,7 +79,7 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
               }
             }
             NormPartial r;
-            if (weight) {
+            if (FP || weight) {
                 r.g2 = __double_as_longlong(wg);
                 r.ip = __double_as_longlong(wp);
                 r.ifs = __double_as_longlong(wf);
@@ -168,7 +180,11 @@ k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
     se[(int64_t)i * nq + q] = sqrt(1.0 / n) * sqrt(m2 / n);
 }
 
-// Per-pixel sums in the reference's float units, layout [T][W*H].
+// Per-pixel sums in the reference's float units, layout [T][W*H] (FP: the
+// sums are FP64 bit patterns).
+__device__ __forceinline__ double sum_value(int64_t v, bool fp) { return fp ? __longlong_as_double(v) : (double)v; }
+
+template <bool FP>
 __global__ void __launch_bounds__(256)
 k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const int64_t* __restrict__ IF,
           const int32_t* __restrict__ c2pix, int32_t P, const TauParam* __restrict__ taus, int32_t T,
@@ -184,9 +200,9 @@ k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const 
     const double lv = (double)(1ll << tp.level);
     const double N = (double)tp.N;
     const int64_t o = (int64_t)i * npix_det + c2pix[c];
-    oG2[o] = (float)((double)G2[c * T + i] * s * s / (lv * lv * N));
-    oIP[o] = (float)((double)IP[c * T + i] * s / (lv * N));
-    oIF[o] = (float)((double)IF[c * T + i] * s / (lv * N));
+    oG2[o] = (float)(sum_value(G2[c * T + i], FP) * s * s / (lv * lv * N));
+    oIP[o] = (float)(sum_value(IP[c * T + i], FP) * s / (lv * N));
+    oIF[o] = (float)(sum_value(IF[c * T + i], FP) * s / (lv * N));
 }
 
 // Normalisation of the first T taus; sums are [P][T] rows, taus gives N per tau.
@@ -195,14 +211,21 @@ void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauP
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
     timer_begin(p, s, "k_norm");
-    if (nchunks > 0 && T > 0)
-        k_norm_partial<<<nchunks < 65536 ? nchunks : 65536, 128, 0, s>>>(
-            p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF, p->d_weight, p->d_part);
+    const bool fp = p->fsnorm;
+    const unsigned nb = nchunks < 65536 ? nchunks : 65536;
+    if (nchunks > 0 && T > 0) {
+        if (fp)
+            k_norm_partial<true><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF,
+                                                    p->d_weight, p->d_part);
+        else
+            k_norm_partial<false><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF,
+                                                     p->d_weight, p->d_part);
+    }
     const int64_t ng = (int64_t)h.ngroups * T;
     if (ng > 0)
         k_norm_group<<<(int)((ng + 255) / 256), 256, 0, s>>>(p->d_part, p->d_group_chunk_first,
                                                              p->d_group_size, h.ngroups, taus, T,
-                                                             p->d_weight ? 1 : 0, p->d_g2s);
+                                                             (fp || p->d_weight) ? 1 : 0, p->d_g2s);
     const int64_t nqt = (int64_t)h.nq * T;
     if (nqt > 0)
         k_norm_q<<<(int)((nqt + 255) / 256), 256, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group,
@@ -221,9 +244,9 @@ void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float*
     const HostTables& h = p->h;
     const int64_t n = (int64_t)h.P * h.T;
     const double inv_scale = 1.0 / ((double)h.avg * (double)p->nscale);
-    k_ref_f32<<<(int)((n + 255) / 256), 256, 0, s>>>(p->d_G2, p->d_IP, p->d_IF, p->d_c2pix, h.P,
-                                                     p->d_taus, h.T, p->d_weight, inv_scale,
-                                                     (int64_t)h.W * h.H, G2, IP, IF);
+    auto k = p->fsnorm ? k_ref_f32<true> : k_ref_f32<false>;
+    k<<<(int)((n + 255) / 256), 256, 0, s>>>(p->d_G2, p->d_IP, p->d_IF, p->d_c2pix, h.P, p->d_taus, h.T,
+                                             p->d_weight, inv_scale, (int64_t)h.W * h.H, G2, IP, IF);
 }
 
 }  // namespace xpcs

@@ -441,6 +441,43 @@ void launch_side(xpcs_mt_plan* p, hipStream_t s)
     timer_end(p, s);
 }
 
+// normalize_by_framesum (main.cpp:313-336): fw[f] = mean over frames of the
+// frame sums / frame sum of f (0 for a frame without counts).  The frame sums
+// are the per-(frame, static bin) side sums; the reference's 1/(avg * pixels)
+// factors cancel in the ratio.
+__global__ void __launch_bounds__(256)
+k_frame_sums(const double* __restrict__ fs, int32_t F, int32_t S, double* __restrict__ fw)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    double t = 0.0;
+    for (int b = 0; b < S; b++) t += fs[(int64_t)f * S + b];
+    fw[f] = t;
+}
+
+__global__ void __launch_bounds__(1024)
+k_frame_weights(double* __restrict__ fw, int32_t F)
+{
+    __shared__ double s_red[16];
+    double t = 0.0;
+    for (int f = threadIdx.x; f < F; f += blockDim.x) t += fw[f];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    double total = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) total += s_red[i];
+    const double mean = total / (double)F;
+    __syncthreads();
+    for (int f = threadIdx.x; f < F; f += blockDim.x) fw[f] = fw[f] > 0.0 ? mean / fw[f] : 0.0;
+}
+
+void launch_frame_weights(xpcs_mt_plan* p, hipStream_t s)
+{
+    const HostTables& h = p->h;
+    k_frame_sums<<<(h.F + 255) / 256, 256, 0, s>>>(p->d_fs, h.F, h.ns, p->d_fw);
+    k_frame_weights<<<1, 1024, 0, s>>>(p->d_fw, h.F);
+}
+
 // ---- launchers ------------------------------------------------------------------
 EvFormat ev_format(const xpcs_mt_plan* p)
 {

@@ -33,7 +33,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256)
 k_tt_values(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
             const uint32_t* __restrict__ dense_rows, int32_t c0, int32_t n, int32_t F,
-            const double* __restrict__ weight, double scale, double* __restrict__ X)
+            const double* __restrict__ weight, double scale, const double* __restrict__ fw, double* __restrict__ X)
 {
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int64_t c = c0 + i;
@@ -41,7 +41,7 @@ k_tt_values(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, co
         double* xr = X + i * (int64_t)F;
         if (dense_rows) {
             const uint32_t* r = dense_rows + c * (int64_t)F;
-            for (int f = threadIdx.x; f < F; f += blockDim.x) xr[f] = (double)r[f] * w;
+            for (int f = threadIdx.x; f < F; f += blockDim.x) xr[f] = (double)r[f] * w * (fw ? fw[f] : 1.0);
         } else {
             const int64_t row = rowmap ? (int64_t)rowmap[c] : c;
             const int64_t b = off[row], e = off[row + 1];
@@ -49,7 +49,7 @@ k_tt_values(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, co
                 const uint64_t v = ev[k];
                 const uint32_t f = (uint32_t)(v >> 32);
                 // averaged raw frames share an output frame: several events may add here
-                if (f < (uint32_t)F) atomicAdd(&xr[f], (double)(uint32_t)v * w);
+                if (f < (uint32_t)F) atomicAdd(&xr[f], (double)(uint32_t)v * w * (fw ? fw[f] : 1.0));
             }
         }
     }
@@ -124,11 +124,11 @@ __device__ __forceinline__ double read_lane_f64(double x, int l)
 }
 
 // Event k of the row [b, b + n): output frame and value (count x flat field x
-// scale, divided by the smoothing image when `dv` is set).  Lanes past the row
-// get value 0.
+// scale x frame weight when `fw` is set, divided by the smoothing image when
+// `dv` is set).  Lanes past the row get value 0.
 __device__ __forceinline__ void tt_event(const uint64_t* __restrict__ ev, int64_t b, int64_t n, int64_t k,
-                                         int32_t F, double w, const double* __restrict__ dv, int32_t per_frame,
-                                         uint32_t& f, double& v)
+                                         int32_t F, double w, const double* __restrict__ fw,
+                                         const double* __restrict__ dv, int32_t per_frame, uint32_t& f, double& v)
 {
     f = 0u;
     v = 0.0;
@@ -138,6 +138,7 @@ __device__ __forceinline__ void tt_event(const uint64_t* __restrict__ ev, int64_
         if (fe < (uint32_t)F) {
             f = fe;
             v = (double)(uint32_t)e * w;
+            if (fw) v *= fw[fe];
             if (dv) v = v / (per_frame ? dv[fe] : dv[0]);
         }
     }
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(256)
 k_tt_pairs(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
            int32_t c0, int32_t Pq, int32_t F, const double* __restrict__ weight, double scale,
            const int32_t* __restrict__ bin_of_row, const double* __restrict__ div, int32_t per_frame,
-           double* __restrict__ C)
+           const double* __restrict__ fw, double* __restrict__ C)
 {
     const int lane = lane_id();
     const int wpb = (int)(blockDim.x >> 6);
@@ -169,12 +170,12 @@ k_tt_pairs(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, con
         for (int64_t I = 0; I < n; I += 64) {
             uint32_t fa;
             double va;
-            tt_event(ev, b, n, I + lane, F, w, dv, per_frame, fa, va);
+            tt_event(ev, b, n, I + lane, F, w, fw, dv, per_frame, fa, va);
             const int na = (int)min<int64_t>(64, n - I);
             for (int64_t J = I; J < n; J += 64) {
                 uint32_t fb;
                 double vb;
-                tt_event(ev, b, n, J + lane, F, w, dv, per_frame, fb, vb);
+                tt_event(ev, b, n, J + lane, F, w, fw, dv, per_frame, fb, vb);
                 const bool live = J + lane < n;
                 for (int t = 0; t < na; t++) {
                     const uint32_t f1 = read_lane_u32(fa, t);
@@ -225,7 +226,7 @@ k_tt_pair_count(const int64_t* __restrict__ off, const int32_t* __restrict__ row
 __global__ void __launch_bounds__(256)
 k_tt_sg_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap,
                int32_t c0, int32_t Pq, int32_t F, const double* __restrict__ weight, double scale,
-               const int32_t* __restrict__ bin_of_row, double* __restrict__ sgsum)
+               const int32_t* __restrict__ bin_of_row, const double* __restrict__ fw, double* __restrict__ sgsum)
 {
     const int lane = lane_id();
     const int wpb = (int)(blockDim.x >> 6);
@@ -239,7 +240,7 @@ k_tt_sg_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         for (int64_t k = lane; k < n; k += 64) {
             uint32_t f;
             double v;
-            tt_event(ev, b, n, k, F, w, nullptr, 0, f, v);
+            tt_event(ev, b, n, k, F, w, fw, nullptr, 0, f, v);
             if (v != 0.0) unsafeAtomicAdd(&dst[f], v);
         }
     }
@@ -403,7 +404,8 @@ void launch_tt_values(xpcs_mt_plan* p, int32_t c0, int32_t n, double scale, doub
     launch_fill_u64(reinterpret_cast<uint64_t*>(X), (size_t)n * F, s);
     const unsigned g = (unsigned)std::min<int64_t>(std::max<int32_t>(n, 1), 65535);
     k_tt_values<<<g, 256, 0, s>>>(p->mode == 2 ? nullptr : p->d_ev, p->d_off, p->d_rowmap,
-                                  p->mode == 2 ? p->d_rows : nullptr, c0, n, F, p->d_weight, scale, X);
+                                  p->mode == 2 ? p->d_rows : nullptr, c0, n, F, p->d_weight, scale,
+                                  p->fsnorm ? p->d_fw : nullptr, X);
     timer_end(p, s);
 }
 
@@ -417,7 +419,8 @@ void launch_tt_pairs(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, cons
 {
     timer_begin(p, s, "k_tt_pairs");
     k_tt_pairs<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight, scale,
-                                             d_bin_of_row, d_div, per_frame ? 1 : 0, C);
+                                             d_bin_of_row, d_div, per_frame ? 1 : 0, p->fsnorm ? p->d_fw : nullptr,
+                                             C);
     timer_end(p, s);
 }
 
@@ -433,7 +436,7 @@ void launch_tt_sg_sparse(xpcs_mt_plan* p, int32_t c0, int32_t Pq, double scale, 
 {
     timer_begin(p, s, "k_tt_sg_sparse");
     k_tt_sg_sparse<<<wave_grid(Pq), 256, 0, s>>>(p->d_ev, p->d_off, p->d_rowmap, c0, Pq, p->h.F, p->d_weight,
-                                                 scale, d_bin_of_row, d_sgsum);
+                                                 scale, d_bin_of_row, p->fsnorm ? p->d_fw : nullptr, d_sgsum);
     timer_end(p, s);
 }
 

@@ -176,6 +176,14 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     } else {
         launch_fill_u32(p->d_counters, 4, s);
     }
+    if (p->fsnorm) {
+        // normalize_by_framesum scales the values before two-time as well
+        if (p->mode == 3)
+            return fail(XPCS_ERR_UNSUPPORTED, "twotime: normalize_by_framesum needs frame sums (IMM or packed ingest)");
+        launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
+        launch_side(p, s);
+        launch_frame_weights(p, s);
+    }
     // the reference's values: counts (summed over averaged frames) / average,
     // dense values per dark-frame scale; the flat field per pixel
     const double scale = 1.0 / ((double)h.avg * (double)p->nscale);

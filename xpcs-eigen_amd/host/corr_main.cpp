@@ -19,10 +19,13 @@
 // precedence) UFXC or Rigaku packed words, or HDF5 frames (/entry/data/data);
 // --transposed is accepted (the reference's HDF5 reader does not use it).
 //
+// normalize_by_framesum = 1 scales every value by mean(frameSum) / frameSum of
+// its frame before the correlation (FP64 sums, main.cpp:313-336).
+//
 // Exit status: 0 ok, 1 usage or configuration error (including an unknown
-// two-time smoothing method), 2 option outside this build (--frameout,
-// normalize_by_framesum), 3 unreadable or malformed input file, 4 GPU error,
-// 5 cannot write results.  (The reference exits 0 on most failures.)
+// two-time smoothing method), 2 option outside this build (--frameout),
+// 3 unreadable or malformed input file, 4 GPU error, 5 cannot write results.
+// (The reference exits 0 on most failures.)
 
 #include <fcntl.h>
 #include <sys/mman.h>
@@ -185,8 +188,6 @@ int main(int argc, char** argv)
     // main.cpp:161-164: two-time needs a known smoothing method
     if (cfg.twotime && cfg.smoothing == 0) return die(1, "Smoothing method is not valid");
     if (cfg.twotime && cfg.twotime_window < 1) return die(1, "twotime2onetime_window_size must be >= 1");
-    if (cfg.normalize_by_framesum)
-        return die(2, "normalize_by_framesum rescales counts by frame sums; the exact integer path does not build it");
     if (!flag_str(fl, "imm").empty()) cfg.imm_path = flag_str(fl, "imm");
     else if (fl.pos.size() > 1) cfg.imm_path = fl.pos[1];
     const std::string inp = flag_str(fl, "inpath"), outp = flag_str(fl, "outpath");
@@ -244,7 +245,7 @@ int main(int argc, char** argv)
     d.dqmap = cfg.dqmap.data();
     d.sqmap = cfg.sqmap.data();
     d.flatfield = cfg.flatfield_enabled ? cfg.flatfield.data() : nullptr;
-    d.flags = XPCS_MT_SIDE_STATS;
+    d.flags = XPCS_MT_SIDE_STATS | (cfg.normalize_by_framesum ? XPCS_MT_NORMALIZE_BY_FRAMESUM : 0u);
     xpcs_mt_plan* plan = nullptr;
     int st = xpcs_mt_plan_create(&d, &plan);
     if (st == XPCS_ERR_UNSUPPORTED) return die(2, xpcs_last_error());

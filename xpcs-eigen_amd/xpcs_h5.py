@@ -92,7 +92,8 @@ def write_config(path, *, width, height, dqmap, sqmap, imm_path, data_begin_todo
                  dpl, entry="/xpcs", output="/exchange", compression=True, stride=1, avg=1,
                  dark_begin_todo=0, dark_end_todo=0, lld=0.0, sigma=0.0, static_window=100,
                  flatfield=None, int_dtype=np.int32, float_dtype=np.float32, analysis_type="Multitau",
-                 smoothing_method="StaticMap", smoothing_filter="None", qphi_bins=(1,), twotime_window=1):
+                 smoothing_method="StaticMap", smoothing_filter="None", qphi_bins=(1,), twotime_window=1,
+                 normalize_by_framesum=0):
     """Writes every key Configuration::init reads (src/xpcs/configuration.cpp:80-242);
     the two-time keys (smoothing_method, smoothing_filter, qphi_bin_to_process)
     only with analysis_type "Twotime"."""
@@ -116,7 +117,7 @@ def write_config(path, *, width, height, dqmap, sqmap, imm_path, data_begin_todo
         h.write(entry + "/twotime2onetime_window_size", I(twotime_window))
         h.write(entry + "/stride_frames", np.array([stride], np.int64))
         h.write(entry + "/avg_frames", np.array([avg], np.int64))
-        h.write(entry + "/normalize_by_framesum", I(0))
+        h.write(entry + "/normalize_by_framesum", I(normalize_by_framesum))
         h.write(entry + "/lld", Fv(lld))
         h.write(entry + "/sigma", Fv(sigma))
         for k, v in (("x_pixel_size", 0.075), ("y_pixel_size", 0.075), ("adu_per_photon", 1.0),

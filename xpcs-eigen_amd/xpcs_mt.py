@@ -23,6 +23,7 @@ LIB_PATH = HERE / "libxpcs_mt.so"
 
 XPCS_MT_TIMING = 0x1
 XPCS_MT_SIDE_STATS = 0x2
+XPCS_MT_NORMALIZE_BY_FRAMESUM = 0x4
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "FORMAT", 4: "RANGE", 5: "UNSUPPORTED",
           6: "STATE", 7: "NOMEM", 8: "NO_DEVICE"}
 
@@ -39,7 +40,7 @@ EXPORTS = [
     "xpcs_mt_stream_push_imm_device", "xpcs_mt_stream_frames", "xpcs_mt_stream_snapshot",
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
     "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
-    "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records",
+    "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records", "xpcs_mt_read_sums_f64",
 ]
 FMT_UFXC = 1
 FMT_RIGAKU = 2
@@ -128,6 +129,7 @@ def lib():
             "xpcs_mt_read_twotime": ([vp, i32, vp], i32),
             "xpcs_mt_read_twotime_1d": ([vp, vp, vp, vp], i32),
             "xpcs_mt_ingest_packed": ([vp, vp, C.c_size_t, i32], i32),
+            "xpcs_mt_read_sums_f64": ([vp, vp, vp, vp], i32),
             "xpcs_mt_ingest_frames": ([vp, vp, C.c_int64, i32], i32),
             "xpcs_packed_records": ([vp, vp, C.c_size_t, i32, vp, vp, vp, C.c_int64, vp], i32),
         }
@@ -236,7 +238,7 @@ class Plan:
 
     def __init__(self, width, height, frames, dpl, dqmap, sqmap, stride=1, average=1,
                  frame_from=0, dark_frames=0, lld=0.0, sigma=0.0, flatfield=None, timing=False,
-                 side_stats=False):
+                 side_stats=False, normalize_by_framesum=False):
         self._dq = np.ascontiguousarray(dqmap, np.int32).reshape(-1)
         self._sq = np.ascontiguousarray(sqmap, np.int32).reshape(-1)
         self._ff = None if flatfield is None else np.ascontiguousarray(flatfield, np.float64).reshape(-1)
@@ -244,12 +246,14 @@ class Plan:
             raise ValueError("q-maps must have height*width entries")
         d = Desc(width, height, frames, dpl, stride, average, frame_from, dark_frames, lld, sigma,
                  _p(self._dq), _p(self._sq), _p(self._ff),
-                 (XPCS_MT_TIMING if timing else 0) | (XPCS_MT_SIDE_STATS if side_stats else 0))
+                 (XPCS_MT_TIMING if timing else 0) | (XPCS_MT_SIDE_STATS if side_stats else 0)
+                 | (XPCS_MT_NORMALIZE_BY_FRAMESUM if normalize_by_framesum else 0))
         self._h = C.c_void_p()
         check(lib().xpcs_mt_plan_create(C.byref(d), C.byref(self._h)))
         info = self.info()
         self.T, self.Q, self.S, self.P = info.taus, info.dynamic_bins, info.static_bins, info.pixels
         self.width, self.height, self.frames = width, height, frames
+        self.fsnorm = bool(normalize_by_framesum)
 
     def close(self):
         if self._h:
@@ -372,6 +376,14 @@ class Plan:
         t = Tuning()
         check(lib().xpcs_mt_autotune(self._h, C.byref(t)))
         return t.as_dict()
+
+    def sums_f64(self):
+        """G2/IP/IF as FP64 [P][T] (the normalize_by_framesum sums)."""
+        G2 = np.zeros((self.P, self.T), np.float64)
+        IP = np.zeros((self.P, self.T), np.float64)
+        IF = np.zeros((self.P, self.T), np.float64)
+        check(lib().xpcs_mt_read_sums_f64(self._h, _p(G2), _p(IP), _p(IF)))
+        return G2, IP, IF
 
     def sums(self):
         G2 = np.zeros((self.P, self.T), np.int64)

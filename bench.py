@@ -359,6 +359,28 @@ def run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, g2_ref, reps=
     return out
 
 
+def run_fsnorm(dev, imm, payload_off, dlen, W, H, F, dq, sq, dpl, reps=2):
+    """normalize_by_framesum on the main sparse job: frame weights and the FP64
+    per-pixel series kernel over every pixel (device job time)."""
+    import xpcs_mt
+    plan = xpcs_mt.Plan(W, H, F, dpl, dq, sq, normalize_by_framesum=True, timing=True)
+    plan.ingest_imm_device(imm.data_ptr(), imm.numel(), payload_off, dlen, compressed=True)
+    plan.execute()
+    plan.synchronize()
+    plan.kernel_times_reset()
+    t = time.perf_counter()
+    for _ in range(reps):
+        plan.execute()
+    plan.synchronize()
+    el = (time.perf_counter() - t) / reps
+    kt = {k: v[0] / max(v[1], 1) for k, v in plan.kernel_times().items()}
+    g2, _ = plan.g2()
+    plan.close()
+    return {"workload": f"{W}x{H} sparse, {F} frames, dpl={dpl}, normalize_by_framesum",
+            "ms_per_job": 1e3 * el, "frames_per_s": F / el, "kernel_avg_ms": kt,
+            "g2_finite_fraction": float(np.isfinite(g2).mean())}
+
+
 def synth_corner(dq, sq, W, Wc, Hc):
     """q-maps of the detector's top-left Wc x Hc corner."""
     dq2 = np.asarray(dq).reshape(-1, W)[:Hc, :Wc]
@@ -466,6 +488,7 @@ def main():
     ap.add_argument("--stream-seconds", type=float, default=20.0,
                     help="wall time of the streaming leg (0 = skip)")
     ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
+    ap.add_argument("--no-fsnorm", action="store_true", help="skip the normalize_by_framesum leg")
     ap.add_argument("--twotime-frames", type=int, default=8192,
                     help="frames of the two-time leg on the sparse data (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -594,6 +617,9 @@ def main():
 
     plan.close()
     plan2 = None
+    fsnorm = None
+    if not args.no_fsnorm and not dense:
+        fsnorm = run_fsnorm(dev, imm, payload_off, dlen, W, H, F, dq, sq, dpl)
     formats = None
     if not args.no_formats and not dense:
         formats = run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, g2)
@@ -628,6 +654,7 @@ def main():
         "streaming": streaming,
         "twotime": twotime,
         "formats": formats,
+        "normalize_by_framesum": fsnorm,
         "kernels": kern,
         "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
         "end_to_end_pcie": e2e,

@@ -198,6 +198,9 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
         size_t per = (size_t)h.F * sizeof(uint64_t);
         size_t budget = (size_t)1 << 30;
         p->long_blocks = (int)std::min<size_t>(512, std::max<size_t>(1, budget / per));
+        // normalize_by_framesum runs every pixel through the series kernel,
+        // one wavefront per pixel: more, smaller workgroups (4 GiB of rows)
+        if (p->fsnorm) p->long_blocks = (int)std::min<size_t>(8192, std::max<size_t>(1, 4 * budget / per));
     }
     if (hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {

@@ -710,12 +710,13 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
     timer_end(p, s);
 }
 
-// Every pixel in FP64 with the frame weights (normalize_by_framesum).
+// Every pixel in FP64 with the frame weights (normalize_by_framesum), one
+// wavefront per workgroup: the per-lag reductions stay inside the wave.
 void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
     timer_begin(p, s, "k_mt_long_f64");
-    k_mt_long<double><<<p->long_blocks, kLongBlock, 0, s>>>(
+    k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
         p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
         p->d_counters, dense_rows ? 1 : 0, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
         p->d_rowmap);

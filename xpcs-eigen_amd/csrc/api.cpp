@@ -167,7 +167,7 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     }
     TRY(dalloc(&p->d_long_list, (size_t)h.P));
     TRY(dalloc(&p->d_mid_list, (size_t)h.P));
-    TRY(dalloc(&p->d_counters, 4));
+    TRY(dalloc(&p->d_counters, 8));
     {
         // level-0 prefix boundaries of the dense streaming path
         std::vector<int4> bnd;
@@ -202,7 +202,7 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
         // one wavefront per pixel: more, smaller workgroups (4 GiB of rows)
         if (p->fsnorm) p->long_blocks = (int)std::min<size_t>(8192, std::max<size_t>(1, 4 * budget / per));
     }
-    if (hipMemsetAsync(p->d_counters, 0, 16, s) != hipSuccess ||
+    if (hipMemsetAsync(p->d_counters, 0, 32, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         xpcs_mt_plan_destroy(p);
         return fail(XPCS_ERR_HIP, "plan_create: initialisation failed");
@@ -840,7 +840,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
 void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
 {
     HostTables& h = p->h;
-    launch_fill_u32(p->d_counters, 4, s);
+    launch_fill_u32(p->d_counters, 8, s);
     if (p->d_fs) launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
     if (p->fsnorm) {
         // normalize_by_framesum: frame sums first, then every pixel in FP64

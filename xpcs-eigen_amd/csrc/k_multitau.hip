@@ -64,6 +64,7 @@ __device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uin
 
 // ---- register path: rows of at most 64 events, one event per lane ------------------
 constexpr int kRegMaxTaus = 128;
+constexpr int kMidSmall = 128;   // LDS rows up to this length run with 128-entry wave slices
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 // Lag accumulators are kept in kAccCopies interleaved copies per tau; lane l
 // adds into copy l % kAccCopies, so the lanes of a wave that hit the same lag
@@ -239,9 +240,14 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         const int64_t n64 = off[row + 1] - base;
         if (n64 > kWave || (!reg_ok && n64 > 0)) {
             if (lane == 0) {
-                if (n64 <= mid_cap) {
+                // rows of up to kMidSmall events fill mid_list from the front,
+                // longer ones up to mid_cap from the back
+                if (n64 <= kMidSmall) {
                     const uint32_t q = atomicAdd(&counters[2], 1u);
                     mid_list[q] = (int32_t)p;
+                } else if (n64 <= mid_cap) {
+                    const uint32_t q = atomicAdd(&counters[4], 1u);
+                    mid_list[P - 1 - (int64_t)q] = (int32_t)p;
                 } else {
                     const uint32_t q = atomicAdd(&counters[0], 1u);
                     long_list[q] = (int32_t)p;
@@ -356,17 +362,21 @@ __device__ bool lds_sort_rounds(uint32_t* key, uint32_t* val, int m)
     return false;
 }
 
+// Rows r < *count of the list rows[r * step] (step -1 walks a list filled from
+// the back).  AUX slots hold the level-0 prefix sums (n <= CAP), then the lag
+// accumulators (T taus x copies).
 template <int CAP, int WAVES>
 __global__ void __launch_bounds__(kWave * WAVES)
 k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
-            const int32_t* __restrict__ rows, const uint32_t* __restrict__ counters,
+            const int32_t* __restrict__ rows, int32_t step, const uint32_t* __restrict__ count,
             const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
             int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
             int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap)
 {
+    constexpr int AUX = CAP > 512 ? CAP : 512;
     __shared__ uint32_t s_key[WAVES][CAP];
     __shared__ uint32_t s_val[WAVES][CAP];
-    __shared__ unsigned long long s_aux[WAVES][CAP];   // prefix sums, then lag accumulators
+    __shared__ unsigned long long s_aux[WAVES][AUX];   // prefix sums, then lag accumulators
     const int lane = lane_id();
     const int w = wave_id();
     uint32_t* key = s_key[w];
@@ -374,9 +384,9 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
     unsigned long long* aux = s_aux[w];
     const int64_t nw = (int64_t)gridDim.x * WAVES;
 
-    const int64_t nrows = (int64_t)counters[2];
+    const int64_t nrows = (int64_t)*count;
     for (int64_t r = (int64_t)blockIdx.x * WAVES + w; r < nrows; r += nw) {
-        const int64_t p = rows[r];
+        const int64_t p = rows[r * step];
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
         const int64_t base = off[row];
         const int64_t n64 = off[row + 1] - base;
@@ -445,24 +455,31 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             carry = __shfl(incl, kWave - 1, 64);
         }
         wave_sync();
-        for (int i = lane; i < T; i += kWave) {
-            const TauParam tp = taus[i];
-            const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
-            const uint32_t b_if = (uint32_t)tp.tp << tp.level;
-            const uint32_t b_end = (uint32_t)tp.lf << tp.level;
-            const int j_ip = lds_lower_bound(key, n0, b_ip);
-            const int j_if = lds_lower_bound(key, n0, b_if);
-            const int j_end = lds_lower_bound(key, n0, b_end);
-            const uint64_t s_ip = j_ip > 0 ? aux[j_ip - 1] : 0ull;
-            const uint64_t s_if = j_if > 0 ? aux[j_if - 1] : 0ull;
-            const uint64_t s_end = j_end > 0 ? aux[j_end - 1] : 0ull;
-            iprow[i] = (int64_t)s_ip;
-            ifrow[i] = (int64_t)(s_end - s_if);
+        // S0(lf_L 2^L) depends on the level only: lane L finds it once
+        uint64_t s_lvl = 0ull;
+        if (lane < nlev) {
+            const int j_end = lds_lower_bound(key, n0, (uint32_t)levels[lane].lf << lane);
+            s_lvl = j_end > 0 ? aux[j_end - 1] : 0ull;
+        }
+        for (int i0 = 0; i0 < T; i0 += kWave) {
+            const int i = i0 + lane;
+            const TauParam tp = taus[i < T ? i : T - 1];
+            const uint64_t s_end = __shfl(s_lvl, tp.level, 64);
+            if (i < T) {
+                const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
+                const uint32_t b_if = (uint32_t)tp.tp << tp.level;
+                const int j_ip = lds_lower_bound(key, n0, b_ip);
+                const int j_if = lds_lower_bound(key, n0, b_if);
+                const uint64_t s_ip = j_ip > 0 ? aux[j_ip - 1] : 0ull;
+                const uint64_t s_if = j_if > 0 ? aux[j_if - 1] : 0ull;
+                iprow[i] = (int64_t)s_ip;
+                ifrow[i] = (int64_t)(s_end - s_if);
+            }
         }
         if (lane == 0) pix_total[p] = carry;
         wave_sync();
         // lag accumulators: nc copies per tau while T * nc fits the CAP slots
-        const int nc = T * kAccCopies <= CAP ? kAccCopies : 1;
+        const int nc = T * kAccCopies <= AUX ? kAccCopies : 1;
         for (int i = lane; i < T * nc; i += kWave) aux[i] = 0ull;
         wave_sync();
         // ---- levels: bin in place, then lag products ----
@@ -693,8 +710,11 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
         p->d_rowmap);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
-    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
-        p->d_ev, p->d_off, p->d_mid_list, p->d_counters, p->d_taus, h.T, p->d_levels,
+    k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
+        p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
+        p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
     timer_end(p, s);
 }

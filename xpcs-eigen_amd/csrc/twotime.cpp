@@ -159,7 +159,7 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
     if (p->mode == 1) {
         xpcs_status st = prepare_execute(p);
         if (st) return st;
-        launch_fill_u32(p->d_counters, 4, s);
+        launch_fill_u32(p->d_counters, 8, s);
         launch_transpose(p, s);
     } else if (p->mode == 2) {
         const size_t need = (size_t)h.P * (size_t)F;
@@ -171,10 +171,10 @@ extern "C" xpcs_status xpcs_mt_twotime(xpcs_mt_plan* p, const xpcs_tt_desc* d, v
             if (st) return st;
             p->rows_cap = need;
         }
-        launch_fill_u32(p->d_counters, 4, s);
+        launch_fill_u32(p->d_counters, 8, s);
         launch_dense_transpose(p, s);
     } else {
-        launch_fill_u32(p->d_counters, 4, s);
+        launch_fill_u32(p->d_counters, 8, s);
     }
     if (p->fsnorm) {
         // normalize_by_framesum scales the values before two-time as well

@@ -205,3 +205,39 @@ def test_repeated_execute_is_stable(xmt, orc):
     b = plan.sums()
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_shuffled_rows_two_per_lane(xmt, orc, seed):
+    """Rows of 65..128 events in random frame order (the two-events-per-lane
+    kernel's full sort), some frames repeated (merged at level 0)."""
+    rng = np.random.default_rng(900 + seed)
+    W = H = 12
+    F = 700
+    dq, sq = synth.annular_qmaps(W, H, 4, 2)
+    npd = W * H
+    lens = rng.integers(60, 129, npd)
+    frames, counts = [], []
+    for n in lens:
+        fr = np.sort(rng.choice(F, n, replace=False))
+        if n > 70:
+            fr[5] = fr[4]                       # a repeated frame in the row
+        rng.shuffle(fr)
+        frames.append(fr.astype(np.int32))
+        counts.append(rng.integers(1, 6, n).astype(np.uint32))
+    roff = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    fr_all = np.concatenate(frames)
+    ct_all = np.concatenate(counts)
+    print(f"shuffled rows seed={seed}", flush=True)
+    plan = xmt.Plan(W, H, F, 4, dq, sq)
+    plan.ingest_rows(roff, fr_all, ct_all)
+    plan.execute()
+    qm = orc.build_qmap(dq, sq)
+    order = qm["order"]
+    off = np.concatenate([[0], np.cumsum(lens[order])]).astype(np.int64)
+    csr = dict(off=off, frame=np.concatenate([frames[j] for j in order]),
+               cnt=np.concatenate([counts[j] for j in order]).astype(np.int64))
+    lev, tau = orc.schedule(F, 4)
+    G2, IP, IF = orc.multitau_exact(csr, F, lev, tau)
+    g, a, b = plan.sums()
+    assert np.array_equal(a, IP) and np.array_equal(b, IF) and np.array_equal(g, G2)

@@ -137,7 +137,7 @@ extern "C" xpcs_status xpcs_mt_autotune(xpcs_mt_plan* p, xpcs_mt_tuning* best_ou
             axes.push_back({&xpcs_mt_tuning::tr_chunk_frames, {std::max(1, cf / 2), cf * 2, cf * 4}});
         }
         axes.push_back({&xpcs_mt_tuning::reg_blocks, {1024, 4096}});
-        axes.push_back({&xpcs_mt_tuning::lds_blocks, {256, 4096}});
+        axes.push_back({&xpcs_mt_tuning::lds_blocks, {256, 2048, 4096, 8192}});
     } else if (dense_stream) {
         axes.push_back({&xpcs_mt_tuning::dense_levels,
                         p->h.dpl == 4 ? std::vector<int32_t>{3, 4, 5} : std::vector<int32_t>{2, 3}});

@@ -218,6 +218,339 @@ __device__ __forceinline__ uint32_t reg_prefix_below(uint32_t k, uint32_t pre, i
     return lo > 0 ? s : 0u;
 }
 
+// Lag products of one level, one event per lane (nl events): the partner of
+// lane i is lane i+m, m = 1..hi; products go to the LDS accumulators.  Lanes
+// past nl hold kNoKey, so a valid lane's partner beyond the row has d > hi;
+// only the lanes past nl need masking.
+__device__ __forceinline__ void reg_lags(uint32_t k, uint32_t v, int nl, const LevelParam& lp,
+                                         unsigned long long* acc)
+{
+    const int lane = lane_id();
+    const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
+    const bool valid = lane < nl;
+    const int cp = lane & (kAccCopies - 1);
+    uint32_t kj = k, vj = v;
+    for (uint32_t m = 1; m <= hi; m++) {
+        kj = next_lane_u32(kj, kNoKey);
+        vj = next_lane_u32(vj, 0u);
+        const uint32_t d = kj - k;
+        if (__builtin_amdgcn_ballot_w64(valid && d <= hi) == 0ull) break;
+        if (valid && d - lo <= span)
+            atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp],
+                      (unsigned long long)v * (unsigned long long)vj);
+    }
+}
+
+// ---- two events per lane: rows of 65..128 events --------------------------------
+// Element i of the row is lane i % 64 of register i / 64 (k0/v0, k1/v1); the
+// row is sorted across both.  Levels run in this form until binning leaves at
+// most 64 events, then in the one-register form.
+
+// Number of the first n lanes (sorted keys, kNoKey beyond) with key < b.
+__device__ __forceinline__ int reg_count_below(uint32_t k, int n, uint32_t b)
+{
+    int lo = 0, len = n;
+#pragma unroll
+    for (int step = 0; step < 7; step++) {
+        const int half = len >> 1;
+        const int idx = min(lo + half, kWave - 1);
+        const uint32_t kmid = (uint32_t)__shfl((int)k, idx, 64);
+        const bool live = len > 0;
+        const bool right = live && kmid < b;
+        lo = right ? lo + half + 1 : lo;
+        len = live ? (right ? len - half - 1 : half) : 0;
+    }
+    return lo;
+}
+
+// Compare-exchange of elements i and i^J inside one register (J < 64) for a
+// bitonic network; `up` gives the direction of the element's sequence.
+template <int J>
+__device__ __forceinline__ void bitonic_step_dir(uint32_t& k, uint32_t& v, bool up)
+{
+    const uint32_t pk = xor_lane_u32<J>(k);
+    const uint32_t pv = xor_lane_u32<J>(v);
+    const bool lower = (lane_id() & J) == 0;
+    const bool take = (lower == up) ? (pk < k) : (pk > k);
+    if (take) {
+        k = pk;
+        v = pv;
+    }
+}
+
+template <int KK>
+__device__ __forceinline__ void bitonic_merge_in_reg(uint32_t& k, uint32_t& v, bool up)
+{
+    if constexpr (KK >= 64) bitonic_step_dir<32>(k, v, up);
+    if constexpr (KK >= 32) bitonic_step_dir<16>(k, v, up);
+    if constexpr (KK >= 16) bitonic_step_dir<8>(k, v, up);
+    if constexpr (KK >= 8) bitonic_step_dir<4>(k, v, up);
+    if constexpr (KK >= 4) bitonic_step_dir<2>(k, v, up);
+    if constexpr (KK >= 2) bitonic_step_dir<1>(k, v, up);
+}
+
+// Ascending bitonic sort of the 128 elements of (k0, v0), (k1, v1).
+__device__ void bitonic_sort128(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1)
+{
+    const int lane = lane_id();
+    // stages kk = 2..64 inside each register; element i = lane + 64 r sorts
+    // ascending when (i & kk) == 0 (for kk = 64 that is register 0)
+    bitonic_merge_in_reg<2>(k0, v0, (lane & 2) == 0);
+    bitonic_merge_in_reg<2>(k1, v1, (lane & 2) == 0);
+    bitonic_merge_in_reg<4>(k0, v0, (lane & 4) == 0);
+    bitonic_merge_in_reg<4>(k1, v1, (lane & 4) == 0);
+    bitonic_merge_in_reg<8>(k0, v0, (lane & 8) == 0);
+    bitonic_merge_in_reg<8>(k1, v1, (lane & 8) == 0);
+    bitonic_merge_in_reg<16>(k0, v0, (lane & 16) == 0);
+    bitonic_merge_in_reg<16>(k1, v1, (lane & 16) == 0);
+    bitonic_merge_in_reg<32>(k0, v0, (lane & 32) == 0);
+    bitonic_merge_in_reg<32>(k1, v1, (lane & 32) == 0);
+    bitonic_merge_in_reg<64>(k0, v0, true);
+    bitonic_merge_in_reg<64>(k1, v1, false);
+    // kk = 128: elements i and i + 64 (same lane, other register), then inside
+    if (k1 < k0) {
+        const uint32_t tk = k0, tv = v0;
+        k0 = k1;
+        v0 = v1;
+        k1 = tk;
+        v1 = tv;
+    }
+    bitonic_merge_in_reg<64>(k0, v0, true);
+    bitonic_merge_in_reg<64>(k1, v1, true);
+}
+
+__device__ __forceinline__ bool reg2_unsorted(uint32_t k0, uint32_t k1)
+{
+    const uint32_t first1 = read_lane_u32(k1, 0);
+    return __ballot(next_lane_u32(k0, first1) < k0 || next_lane_u32(k1, kNoKey) < k1) != 0ull;
+}
+
+// Nearly sorted rows: up to 3 odd-even transposition rounds over the 128
+// elements, the bitonic network for whatever is still out of order.
+__device__ void reg2_sort(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1)
+{
+    if (!reg2_unsorted(k0, k1)) return;
+    const int lane = lane_id();
+    const bool odd = (lane & 1) != 0;
+    for (int r = 0; r < 3; r++) {
+        // even phase: (2j, 2j+1), inside each register
+        {
+            const uint32_t pk0 = xor_lane_u32<1>(k0), pv0 = xor_lane_u32<1>(v0);
+            const uint32_t pk1 = xor_lane_u32<1>(k1), pv1 = xor_lane_u32<1>(v1);
+            if (!odd ? (pk0 < k0) : (pk0 > k0)) {
+                k0 = pk0;
+                v0 = pv0;
+            }
+            if (!odd ? (pk1 < k1) : (pk1 > k1)) {
+                k1 = pk1;
+                v1 = pv1;
+            }
+        }
+        // odd phase: (2j+1, 2j+2); element 63 pairs with element 64 across the registers
+        {
+            const uint32_t f1k = read_lane_u32(k1, 0), f1v = read_lane_u32(v1, 0);
+            const uint32_t l0k = read_lane_u32(k0, 63), l0v = read_lane_u32(v0, 63);
+            const uint32_t nk0 = next_lane_u32(k0, f1k), nv0 = next_lane_u32(v0, f1v);
+            const uint32_t bk0 = prev_lane_u32(k0, 0u), bv0 = prev_lane_u32(v0, 0u);
+            const uint32_t nk1 = next_lane_u32(k1, kNoKey), nv1 = next_lane_u32(v1, 0u);
+            const uint32_t bk1 = prev_lane_u32(k1, l0k), bv1 = prev_lane_u32(v1, l0v);
+            if (odd && nk0 < k0) {
+                k0 = nk0;
+                v0 = nv0;
+            } else if (!odd && lane > 0 && bk0 > k0) {
+                k0 = bk0;
+                v0 = bv0;
+            }
+            if (odd && nk1 < k1) {
+                k1 = nk1;
+                v1 = nv1;
+            } else if (!odd && bk1 > k1) {
+                k1 = bk1;
+                v1 = bv1;
+            }
+        }
+        if (!reg2_unsorted(k0, k1)) return;
+    }
+    bitonic_sort128(k0, v0, k1, v1);
+}
+
+// One binning step over both registers (keys >> shift, keys >= lf dropped,
+// equal keys summed), compacted through the wave's staging slots when
+// anything merges or drops.  Returns the new count.
+__device__ int reg2_level(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1, int nl, int shift, uint32_t lf,
+                          uint32_t* __restrict__ stage)
+{
+    const int lane = lane_id();
+    const bool valid0 = lane < nl, valid1 = lane + kWave < nl;
+    const uint32_t kk0 = valid0 ? (k0 >> shift) : kNoKey;
+    const uint32_t kk1 = valid1 ? (k1 >> shift) : kNoKey;
+    const uint32_t l0 = read_lane_u32(kk0, 63);
+    const uint32_t kp0 = prev_lane_u32(kk0, kNoKey), kp1 = prev_lane_u32(kk1, l0);
+    const bool dup0 = valid0 && kk0 == kp0, dup1 = valid1 && kk1 == kp1;
+    const bool drop0 = valid0 && kk0 >= lf, drop1 = valid1 && kk1 >= lf;
+    if (__ballot(dup0 || dup1 || drop0 || drop1) == 0ull) {
+        k0 = kk0;
+        k1 = kk1;
+        return nl;
+    }
+    // stage the shifted row, then every run head sums its run
+    uint32_t* sk = stage;
+    uint32_t* sv = stage + 2 * kWave;
+    uint32_t* ok = stage + 4 * kWave;
+    uint32_t* ov = stage + 6 * kWave;
+    sk[lane] = kk0;
+    sv[lane] = v0;
+    sk[lane + kWave] = kk1;
+    sv[lane + kWave] = v1;
+    wave_sync();
+    const bool head0 = valid0 && !dup0 && !drop0, head1 = valid1 && !dup1 && !drop1;
+    uint32_t sum0 = 0u, sum1 = 0u;
+    if (head0) {
+        sum0 = v0;
+        for (int j = lane + 1; j < nl && sk[j] == kk0; j++) sum0 += sv[j];
+    }
+    if (head1) {
+        sum1 = v1;
+        for (int j = lane + kWave + 1; j < nl && sk[j] == kk1; j++) sum1 += sv[j];
+    }
+    const uint64_t hm0 = __ballot(head0), hm1 = __ballot(head1);
+    const int n0 = __popcll(hm0);
+    const int newn = n0 + __popcll(hm1);
+    if (head0) {
+        const int d = mask_rank(hm0);
+        ok[d] = kk0;
+        ov[d] = sum0;
+    }
+    if (head1) {
+        const int d = n0 + mask_rank(hm1);
+        ok[d] = kk1;
+        ov[d] = sum1;
+    }
+    wave_sync();
+    k0 = lane < newn ? ok[lane] : kNoKey;
+    v0 = lane < newn ? ov[lane] : 0u;
+    k1 = lane + kWave < newn ? ok[lane + kWave] : kNoKey;
+    v1 = lane + kWave < newn ? ov[lane + kWave] : 0u;
+    wave_sync();
+    return newn;
+}
+
+// Lag products of one level over both registers (partner of element i is i+m).
+__device__ __forceinline__ void reg2_lags(uint32_t k0, uint32_t v0, uint32_t k1, uint32_t v1, int nl,
+                                          const LevelParam& lp, unsigned long long* acc)
+{
+    const int lane = lane_id();
+    const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
+    const bool valid0 = lane < nl, valid1 = lane + kWave < nl;
+    const int cp = lane & (kAccCopies - 1);
+    uint32_t kj0 = k0, vj0 = v0, kj1 = k1, vj1 = v1;
+    for (uint32_t m = 1; m <= hi; m++) {
+        const uint32_t f1k = read_lane_u32(kj1, 0), f1v = read_lane_u32(vj1, 0);
+        kj0 = next_lane_u32(kj0, f1k);
+        vj0 = next_lane_u32(vj0, f1v);
+        kj1 = next_lane_u32(kj1, kNoKey);
+        vj1 = next_lane_u32(vj1, 0u);
+        const uint32_t d0 = kj0 - k0, d1 = kj1 - k1;
+        if (__builtin_amdgcn_ballot_w64((valid0 && d0 <= hi) || (valid1 && d1 <= hi)) == 0ull) break;
+        if (valid0 && d0 - lo <= span)
+            atomicAdd(&acc[(lp.first + (int)(d0 - lo)) * kAccCopies + cp],
+                      (unsigned long long)v0 * (unsigned long long)vj0);
+        if (valid1 && d1 - lo <= span)
+            atomicAdd(&acc[(lp.first + (int)(d1 - lo)) * kAccCopies + cp],
+                      (unsigned long long)v1 * (unsigned long long)vj1);
+    }
+}
+
+// Rows r < *count of list[r] with 65..128 events (T <= kRegMaxTaus).
+template <int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES)
+k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
+          const uint32_t* __restrict__ count, int32_t T, const TauParam* __restrict__ taus,
+          const LevelParam* __restrict__ levels, int32_t nlev, int32_t F, int64_t* __restrict__ G2,
+          int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
+          const int32_t* __restrict__ rowmap)
+{
+    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus * kAccCopies];
+    __shared__ uint32_t s_stage[WAVES][8 * kWave];
+    const int lane = lane_id();
+    const int w = wave_id();
+    unsigned long long* acc = s_acc[w];
+    uint32_t* stage = s_stage[w];
+    const int64_t nrows = (int64_t)*count;
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+    for (int64_t r = (int64_t)blockIdx.x * WAVES + w; r < nrows; r += nw) {
+        const int64_t p = list[r];
+        const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+        const int64_t base = off[row];
+        const int n = (int)min<int64_t>(off[row + 1] - base, 2 * kWave);   // 65..128 by construction
+        int64_t* g2row = G2 + p * T;
+        int64_t* iprow = IP + p * T;
+        int64_t* ifrow = IF + p * T;
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = 0ull;
+        uint32_t k0 = kNoKey, v0 = 0u, k1 = kNoKey, v1 = 0u;
+        if (lane < n) {
+            const uint64_t e = ev[base + lane];
+            k0 = (uint32_t)(e >> 32);
+            v0 = (uint32_t)e;
+        }
+        if (lane + kWave < n) {
+            const uint64_t e = ev[base + kWave + lane];
+            k1 = (uint32_t)(e >> 32);
+            v1 = (uint32_t)e;
+        }
+        reg2_sort(k0, v0, k1, v1);
+        int nl = reg2_level(k0, v0, k1, v1, n, 0, (uint32_t)F, stage);
+        // IP and IF from the level-0 prefix sums (as the one-register kernel)
+        {
+            const uint32_t pre0 = wave_incl_scan_u32(v0);
+            const uint32_t t0 = read_lane_u32(pre0, 63);
+            const uint32_t pre1 = wave_incl_scan_u32(v1) + t0;
+            if (lane == 0) pix_total[p] = read_lane_u32(pre1, 63);
+            const int n0 = min(nl, kWave), n1 = max(nl - kWave, 0);
+            auto below = [&](uint32_t b) {
+                const int c0 = reg_count_below(k0, n0, b);
+                const int c1 = reg_count_below(k1, n1, b);
+                const uint32_t s0 = (uint32_t)__shfl((int)pre0, c0 > 0 ? c0 - 1 : 0, 64);
+                const uint32_t s1 = (uint32_t)__shfl((int)pre1, c1 > 0 ? c1 - 1 : 0, 64);
+                if (c0 < n0 || n1 == 0) return c0 > 0 ? s0 : 0u;
+                return c1 > 0 ? s1 : t0;
+            };
+            const uint32_t s_lvl = below(lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
+            for (int i0 = 0; i0 < T; i0 += kWave) {
+                const int i = i0 + lane;
+                const TauParam tp = taus[i < T ? i : T - 1];
+                const uint32_t s_ip = below((uint32_t)(tp.lf - tp.tp) << tp.level);
+                const uint32_t s_if = below((uint32_t)tp.tp << tp.level);
+                const uint32_t s_end = (uint32_t)__shfl((int)s_lvl, tp.level, 64);
+                if (i < T) {
+                    iprow[i] = (int64_t)s_ip;
+                    ifrow[i] = (int64_t)(s_end - s_if);
+                }
+            }
+        }
+        bool two = nl > kWave;
+        for (int L = 0; L < nlev; L++) {
+            const LevelParam lp = levels[L];
+            if (L > 0) {
+                if (two) nl = reg2_level(k0, v0, k1, v1, nl, 1, (uint32_t)lp.lf, stage);
+                else nl = reg_level(k0, v0, nl, 1, (uint32_t)lp.lf, true);
+                two = nl > kWave;
+            }
+            if (nl < 2) break;
+            if (two) reg2_lags(k0, v0, k1, v1, nl, lp, acc);
+            else reg_lags(k0, v0, nl, lp, acc);
+        }
+        wave_sync();
+        for (int i = lane; i < T; i += kWave) {
+            unsigned long long g = 0ull;
+#pragma unroll
+            for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
+            g2row[i] = (int64_t)g;
+        }
+        wave_sync();
+    }
+}
+
 template <int WAVES>
 __global__ void __launch_bounds__(kWave * WAVES)
 k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P, int32_t T,
@@ -303,22 +636,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             const LevelParam lp = levels[L];
             if (L > 0) nl = reg_level(k, v, nl, 1, (uint32_t)lp.lf, true);
             if (nl < 2) break;
-            // lag products: partner of lane i is lane i+m, m = 1..hi
-            const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
-            // lanes past nl hold kNoKey, so a valid lane's partner beyond the
-            // row has d > hi; only the lanes past nl need masking
-            const bool valid = lane < nl;
-            const int cp = lane & (kAccCopies - 1);
-            uint32_t kj = k, vj = v;
-            for (uint32_t m = 1; m <= hi; m++) {
-                kj = next_lane_u32(kj, kNoKey);
-                vj = next_lane_u32(vj, 0u);
-                const uint32_t d = kj - k;
-                if (__builtin_amdgcn_ballot_w64(valid && d <= hi) == 0ull) break;
-                if (valid && d - lo <= span)
-                    atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp],
-                              (unsigned long long)v * (unsigned long long)vj);
-            }
+            reg_lags(k, v, nl, lp, acc);
         }
         wave_sync();
         for (int i = lane; i < T; i += kWave) {
@@ -699,6 +1017,7 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
 constexpr int kSparseWaves = 8;
 constexpr int kSparseCapUsed = 512;
 constexpr int kRegWaves = 8;
+constexpr int kReg2Waves = 4;
 
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
 {
@@ -710,9 +1029,14 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
         p->d_rowmap);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
-    k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
-        p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+    if (h.T <= kRegMaxTaus)
+        k_mt_reg2<kReg2Waves><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
+            p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_taus, p->d_levels,
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+    else
+        k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
+            p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
     k_mt_sparse<kSparseCapUsed, kSparseWaves><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
@@ -756,7 +1080,7 @@ int reg_blocks(const xpcs_mt_plan* p)
 
 int lds_blocks(const xpcs_mt_plan* p)
 {
-    return p->tune.lds_blocks > 0 ? p->tune.lds_blocks : 1024;
+    return p->tune.lds_blocks > 0 ? p->tune.lds_blocks : 2048;
 }
 
 }  // namespace xpcs

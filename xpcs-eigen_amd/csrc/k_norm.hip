@@ -96,88 +96,117 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
     }
 }
 
-__global__ void __launch_bounds__(256)
+// One wavefront per (group, tau): lane l sums the group's chunk partials
+// l, l + 64, ...; the lanes then reduce in a fixed tree (integer sums are exact
+// in any order; weighted FP64 sums come out the same on every run).
+__global__ void __launch_bounds__(64)
 k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ group_chunk_first,
              const int32_t* __restrict__ group_size, int32_t ngroups, const TauParam* __restrict__ taus,
              int32_t T, int weighted, double* __restrict__ g2s)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = blockIdx.x;
     if (k >= (int64_t)ngroups * T) return;
     const int g = (int)(k / T);
     const int i = (int)(k % T);
+    const int lane = (int)threadIdx.x;
+    const int c0 = group_chunk_first[g], c1 = group_chunk_first[g + 1];
     double sg, sp, sf;
     if (weighted) {
-        sg = sp = sf = 0.0;
-#pragma unroll 8
-        for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
-            const NormPartial& r = part[(int64_t)c * T + i];
-            sg += __longlong_as_double(r.g2);
-            sp += __longlong_as_double(r.ip);
-            sf += __longlong_as_double(r.ifs);
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int ch = c0 + lane; ch < c1; ch += 64) {
+            const NormPartial& r = part[(int64_t)ch * T + i];
+            a += __longlong_as_double(r.g2);
+            b += __longlong_as_double(r.ip);
+            c += __longlong_as_double(r.ifs);
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+            c += __shfl_xor(c, o, 64);
+        }
+        sg = a;
+        sp = b;
+        sf = c;
     } else {
-        int64_t ig = 0, ip = 0, iff = 0;
-#pragma unroll 8
-        for (int c = group_chunk_first[g]; c < group_chunk_first[g + 1]; c++) {
-            const NormPartial& r = part[(int64_t)c * T + i];
-            ig += r.g2;
-            ip += r.ip;
-            iff += r.ifs;
+        int64_t a = 0, b = 0, c = 0;
+        for (int ch = c0 + lane; ch < c1; ch += 64) {
+            const NormPartial& r = part[(int64_t)ch * T + i];
+            a += r.g2;
+            b += r.ip;
+            c += r.ifs;
         }
-        sg = (double)ig;
-        sp = (double)ip;
-        sf = (double)iff;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+            c += __shfl_xor(c, o, 64);
+        }
+        sg = (double)a;
+        sp = (double)b;
+        sf = (double)c;
     }
-    g2s[(int64_t)g * T + i] = (double)group_size[g] * (double)taus[i].N * sg / (sp * sf);
+    if (lane == 0) g2s[(int64_t)g * T + i] = (double)group_size[g] * (double)taus[i].N * sg / (sp * sf);
 }
 
-__global__ void __launch_bounds__(256)
+// Chan's merge of Welford states (n, mean, m2); an empty state is the identity.
+__device__ __forceinline__ void welford_merge(double& n, double& mean, double& m2, double rn, double rmean,
+                                              double rm2)
+{
+    if (rn == 0.0) return;
+    if (n == 0.0) {
+        n = rn;
+        mean = rmean;
+        m2 = rm2;
+        return;
+    }
+    const double nt = n + rn;
+    const double d = rmean - mean;
+    mean += d * (rn / nt);
+    m2 += rm2 + d * d * (n * rn / nt);
+    n = nt;
+}
+
+// One wavefront per (q, tau): g2 = mean of the non-NaN static-bin values;
+// stderr from the chunk Welford states, merged lane-strided (lane l takes
+// chunks l, l + 64, ... in order) and then across lanes in a fixed tree.
+__global__ void __launch_bounds__(64)
 k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
          const int32_t* __restrict__ q_first_group, const int32_t* __restrict__ q_chunk_first,
          int32_t nq, int32_t T, double* __restrict__ g2, double* __restrict__ se)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = blockIdx.x;
     if (k >= (int64_t)nq * T) return;
     const int q = (int)(k / T);    // 0-based
     const int i = (int)(k % T);
-    double acc = 0.0, cnt = 0.0;
-    for (int g = q_first_group[q]; g < q_first_group[q + 1]; g++) {
-        const double v = g2s[(int64_t)g * T + i];
-        if (v == v) {
-            acc += v;
-            cnt += 1.0;
-        }
-    }
-    g2[(int64_t)i * nq + q] = acc / cnt;
-    double n = 0.0, mean = 0.0, m2 = 0.0;
-    const int c1 = q_chunk_first[q + 1];
-    for (int c0 = q_chunk_first[q]; c0 < c1; c0 += kNormBatch) {
-        // the chunk states are loaded kNormBatch at a time, then merged in order
-        double rn[kNormBatch], rmean[kNormBatch], rm2[kNormBatch];
-#pragma unroll
-        for (int u = 0; u < kNormBatch; u++) {
-            const NormPartial& r = part[(int64_t)min(c0 + u, c1 - 1) * T + i];
-            rn[u] = c0 + u < c1 ? r.n : 0.0;
-            rmean[u] = r.mean;
-            rm2[u] = r.m2;
-        }
-#pragma unroll
-        for (int u = 0; u < kNormBatch; u++) {
-            if (rn[u] == 0.0) continue;
-            if (n == 0.0) {
-                n = rn[u];
-                mean = rmean[u];
-                m2 = rm2[u];
-                continue;
+    const int lane = (int)threadIdx.x;
+    if (lane == 0) {
+        double acc = 0.0, cnt = 0.0;
+        for (int g = q_first_group[q]; g < q_first_group[q + 1]; g++) {
+            const double v = g2s[(int64_t)g * T + i];
+            if (v == v) {
+                acc += v;
+                cnt += 1.0;
             }
-            const double nt = n + rn[u];
-            const double d = rmean[u] - mean;
-            mean += d * (rn[u] / nt);
-            m2 += rm2[u] + d * d * (n * rn[u] / nt);
-            n = nt;
+        }
+        g2[(int64_t)i * nq + q] = acc / cnt;
+    }
+    double n = 0.0, mean = 0.0, m2 = 0.0;
+    for (int c = q_chunk_first[q] + lane; c < q_chunk_first[q + 1]; c += 64) {
+        const NormPartial& r = part[(int64_t)c * T + i];
+        welford_merge(n, mean, m2, r.n, r.mean, r.m2);
+    }
+    for (int o = 1; o < 64; o <<= 1) {
+        const double rn = __shfl_xor(n, o, 64), rmean = __shfl_xor(mean, o, 64), rm2 = __shfl_xor(m2, o, 64);
+        // the lower lane of each pair keeps the order (lower chunks first)
+        if ((lane & o) == 0) welford_merge(n, mean, m2, rn, rmean, rm2);
+        else {
+            double n2 = rn, mean2 = rmean, m22 = rm2;
+            welford_merge(n2, mean2, m22, n, mean, m2);
+            n = n2;
+            mean = mean2;
+            m2 = m22;
         }
     }
-    se[(int64_t)i * nq + q] = sqrt(1.0 / n) * sqrt(m2 / n);
+    if (lane == 0) se[(int64_t)i * nq + q] = sqrt(1.0 / n) * sqrt(m2 / n);
 }
 
 // Per-pixel sums in the reference's float units, layout [T][W*H] (FP: the
@@ -223,14 +252,12 @@ void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauP
     }
     const int64_t ng = (int64_t)h.ngroups * T;
     if (ng > 0)
-        k_norm_group<<<(int)((ng + 255) / 256), 256, 0, s>>>(p->d_part, p->d_group_chunk_first,
-                                                             p->d_group_size, h.ngroups, taus, T,
-                                                             (fp || p->d_weight) ? 1 : 0, p->d_g2s);
+        k_norm_group<<<(unsigned)ng, 64, 0, s>>>(p->d_part, p->d_group_chunk_first, p->d_group_size, h.ngroups,
+                                                 taus, T, (fp || p->d_weight) ? 1 : 0, p->d_g2s);
     const int64_t nqt = (int64_t)h.nq * T;
     if (nqt > 0)
-        k_norm_q<<<(int)((nqt + 255) / 256), 256, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group,
-                                                          p->d_q_chunk_first, h.nq, T, p->d_g2,
-                                                          p->d_se);
+        k_norm_q<<<(unsigned)nqt, 64, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group, p->d_q_chunk_first,
+                                              h.nq, T, p->d_g2, p->d_se);
     timer_end(p, s);
 }
 

@@ -463,7 +463,7 @@ __device__ __forceinline__ void reg2_lags(uint32_t k0, uint32_t v0, uint32_t k1,
 
 // Rows r < *count of list[r] with 65..128 events (T <= kRegMaxTaus).
 template <int WAVES>
-__global__ void __launch_bounds__(kWave * WAVES)
+__global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
           const uint32_t* __restrict__ count, int32_t T, const TauParam* __restrict__ taus,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F, int64_t* __restrict__ G2,
@@ -551,8 +551,9 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
     }
 }
 
+// 88 scalar registers leave room for 8 waves per SIMD (106 would allow 7)
 template <int WAVES>
-__global__ void __launch_bounds__(kWave * WAVES)
+__global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P, int32_t T,
          const TauParam* __restrict__ taus, const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,

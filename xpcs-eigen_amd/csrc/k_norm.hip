@@ -16,7 +16,7 @@
 
 namespace xpcs {
 
-constexpr int kNormBatch = 16;
+constexpr int kNormBatch = 8;
 
 // FP: the sums are FP64 (frame-sum normalised values) stored in the same
 // 8-byte buffers; they take the weighted (FP64) route with weight 1 when the

@@ -40,6 +40,42 @@ def test_sparse(xmt, orc):
         plan.sums()                   # the int64 reader refuses FP64 sums
 
 
+def mixed_rows_case(W, H, F, seed):
+    """Pixels in three bands of event counts: <= 64 (one event per lane),
+    65..128 (two per lane) and > 128 (the FP64 series kernel)."""
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    lo = synth.ou_counts(W, H, F, 0.05, 10.0, seed)
+    mid = synth.ou_counts(W, H, F, 0.2, 10.0, seed + 1)
+    hi = synth.ou_counts(W, H, F, 0.8, 10.0, seed + 2)
+    band = np.arange(W * H) % 3
+    counts = np.where(band == 0, lo, np.where(band == 1, mid, hi)).astype(np.uint16)
+    ev = (counts > 0).sum(axis=0)
+    assert (ev <= 64).any() and ((ev > 64) & (ev <= 128)).any() and (ev > 128).any()
+    return dict(W=W, H=H, F=F, dpl=4, dq=dq, sq=sq, counts=counts, imm=synth.imm_bytes_sparse(counts, W, H))
+
+
+def test_sparse_row_classes(xmt, orc):
+    c = mixed_rows_case(24, 24, 400, seed=611)
+    plan = run_gpu(xmt, c)
+    check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"]))
+
+
+def test_sparse_row_classes_averaged(xmt, orc):
+    # avg 3 merges three raw frames into one key: runs of equal keys at level 0
+    c = mixed_rows_case(20, 20, 600, seed=612)
+    c["F"] = 200
+    plan = run_gpu(xmt, c, average=3)
+    check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"], avg=3))
+
+
+def test_sparse_many_taus(xmt, orc):
+    # more than 128 taus: every row runs through the FP64 series kernel
+    c = cases.sparse_case(W=16, H=16, F=1200, dpl=32, mu=0.1, tau_c=20.0, seed=613)
+    plan = run_gpu(xmt, c)
+    assert plan.T > 128
+    check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"]))
+
+
 def test_sparse_flatfield_stride_average(xmt, orc):
     c = cases.sparse_case(W=24, H=24, F=900, mu=0.15, tau_c=8.0, seed=602)
     c["F"] = 200

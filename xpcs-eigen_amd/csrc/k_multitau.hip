@@ -63,6 +63,8 @@ __device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uin
 }
 
 // ---- register path: rows of at most 64 events, one event per lane ------------------
+// The row kernels run on counts (V = uint32_t, exact 64-bit lag sums) or, for
+// normalize_by_framesum, on frame-weighted values (V = double, FP64 sums).
 constexpr int kRegMaxTaus = 128;
 constexpr int kMidSmall = 128;   // LDS rows up to this length run with 128-entry wave slices
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
@@ -72,11 +74,75 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 constexpr int kAccCopies = 4;
 static_assert(kAccCopies >= 1 && (kAccCopies & (kAccCopies - 1)) == 0, "copies: a power of two");
 
+// a sum as stored in the int64 [P][T] buffers: integers as such, FP64 as bits
+__device__ __forceinline__ int64_t sum_bits(uint64_t v) { return (int64_t)v; }
+__device__ __forceinline__ int64_t sum_bits(unsigned long long v) { return (int64_t)v; }
+__device__ __forceinline__ int64_t sum_bits(uint32_t v) { return (int64_t)v; }
+__device__ __forceinline__ int64_t sum_bits(double v) { return (int64_t)__double_as_longlong(v); }
+
+// Cross-lane moves of a value: one 32-bit move for counts, two for FP64.
+__device__ __forceinline__ uint32_t lo_half(double x) { return (uint32_t)(uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ uint32_t hi_half(double x) { return (uint32_t)((uint64_t)__double_as_longlong(x) >> 32); }
+__device__ __forceinline__ double from_halves(uint32_t lo, uint32_t hi)
+{
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint32_t next_lane_v(uint32_t x, uint32_t old) { return next_lane_u32(x, old); }
+__device__ __forceinline__ double next_lane_v(double x, double old)
+{
+    return from_halves(next_lane_u32(lo_half(x), lo_half(old)), next_lane_u32(hi_half(x), hi_half(old)));
+}
+__device__ __forceinline__ uint32_t prev_lane_v(uint32_t x, uint32_t old) { return prev_lane_u32(x, old); }
+__device__ __forceinline__ double prev_lane_v(double x, double old)
+{
+    return from_halves(prev_lane_u32(lo_half(x), lo_half(old)), prev_lane_u32(hi_half(x), hi_half(old)));
+}
 template <int J>
-__device__ __forceinline__ void bitonic_step(uint32_t& k, uint32_t& v, int kk)
+__device__ __forceinline__ uint32_t xor_lane_v(uint32_t x) { return xor_lane_u32<J>(x); }
+template <int J>
+__device__ __forceinline__ double xor_lane_v(double x)
+{
+    return from_halves(xor_lane_u32<J>(lo_half(x)), xor_lane_u32<J>(hi_half(x)));
+}
+__device__ __forceinline__ uint32_t push_lane_v(int dst, uint32_t x) { return push_to_lane(dst, x); }
+__device__ __forceinline__ double push_lane_v(int dst, double x)
+{
+    return from_halves(push_to_lane(dst, lo_half(x)), push_to_lane(dst, hi_half(x)));
+}
+__device__ __forceinline__ uint32_t read_lane_v(uint32_t x, int l) { return read_lane_u32(x, l); }
+__device__ __forceinline__ double read_lane_v(double x, int l)
+{
+    return from_halves(read_lane_u32(lo_half(x), l), read_lane_u32(hi_half(x), l));
+}
+__device__ __forceinline__ uint32_t shfl_v(uint32_t x, int src) { return (uint32_t)__shfl((int)x, src, 64); }
+__device__ __forceinline__ double shfl_v(double x, int src) { return __shfl(x, src, 64); }
+__device__ __forceinline__ uint32_t wave_scan_v(uint32_t x) { return wave_incl_scan_u32(x); }
+__device__ __forceinline__ double wave_scan_v(double x)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double t = __shfl_up(x, d, 64);
+        if (lane >= d) x += t;
+    }
+    return x;
+}
+// lag products: exact 64-bit integers for counts, FP64 for weighted values
+__device__ __forceinline__ unsigned long long lag_prod(uint32_t a, uint32_t b)
+{
+    return (unsigned long long)a * (unsigned long long)b;
+}
+__device__ __forceinline__ double lag_prod(double a, double b) { return a * b; }
+template <typename V>
+struct LagAcc { using type = unsigned long long; };
+template <>
+struct LagAcc<double> { using type = double; };
+
+template <int J, typename V>
+__device__ __forceinline__ void bitonic_step(uint32_t& k, V& v, int kk)
 {
     const uint32_t pk = xor_lane_u32<J>(k);
-    const uint32_t pv = xor_lane_u32<J>(v);
+    const V pv = xor_lane_v<J>(v);
     const int lane = lane_id();
     const bool up = (lane & kk) == 0;
     const bool lower = (lane & J) == 0;
@@ -89,7 +155,8 @@ __device__ __forceinline__ void bitonic_step(uint32_t& k, uint32_t& v, int kk)
 
 // Ascending sort of (k, v) over the first m lanes (m a power of two <= 64;
 // lanes >= m hold kNoKey).
-__device__ __forceinline__ void bitonic_sort64(uint32_t& k, uint32_t& v, int m)
+template <typename V>
+__device__ __forceinline__ void bitonic_sort64(uint32_t& k, V& v, int m)
 {
     if (m >= 2) { bitonic_step<1>(k, v, 2); }
     if (m >= 4) { bitonic_step<2>(k, v, 4); bitonic_step<1>(k, v, 4); }
@@ -117,7 +184,8 @@ __device__ __forceinline__ bool reg_unsorted(uint32_t k)
     return __ballot(next_lane_u32(k, kNoKey) < k) != 0ull;
 }
 
-__device__ __forceinline__ void reg_sort(uint32_t& k, uint32_t& v, int n)
+template <typename V>
+__device__ __forceinline__ void reg_sort(uint32_t& k, V& v, int n)
 {
     if (!reg_unsorted(k)) return;
     const int lane = lane_id();
@@ -125,7 +193,7 @@ __device__ __forceinline__ void reg_sort(uint32_t& k, uint32_t& v, int n)
         // even phase: (2j, 2j+1)
         {
             const uint32_t pk = xor_lane_u32<1>(k);
-            const uint32_t pv = xor_lane_u32<1>(v);
+            const V pv = xor_lane_v<1>(v);
             const bool lower = (lane & 1) == 0;
             if (lower ? (pk < k) : (pk > k)) {
                 k = pk;
@@ -134,8 +202,10 @@ __device__ __forceinline__ void reg_sort(uint32_t& k, uint32_t& v, int n)
         }
         // odd phase: (2j+1, 2j+2)
         {
-            const uint32_t nk = next_lane_u32(k, kNoKey), nv = next_lane_u32(v, 0u);
-            const uint32_t bk = prev_lane_u32(k, 0u), bv = prev_lane_u32(v, 0u);
+            const uint32_t nk = next_lane_u32(k, kNoKey);
+            const V nv = next_lane_v(v, V(0));
+            const uint32_t bk = prev_lane_u32(k, 0u);
+            const V bv = prev_lane_v(v, V(0));
             const bool odd = (lane & 1) != 0;
             if (odd && nk < k) {
                 k = nk;
@@ -154,8 +224,8 @@ __device__ __forceinline__ void reg_sort(uint32_t& k, uint32_t& v, int n)
 // summed.  `pairs_only` when runs are at most two long (every level after the
 // first, whose keys are distinct).  Returns the new count; lanes past it hold
 // (kNoKey, 0).  Without repeated or dropped keys no data moves.
-__device__ __forceinline__ int reg_level(uint32_t& k, uint32_t& v, int nl, int shift, uint32_t lf,
-                                         bool pairs_only)
+template <typename V>
+__device__ __forceinline__ int reg_level(uint32_t& k, V& v, int nl, int shift, uint32_t lf, bool pairs_only)
 {
     const int lane = lane_id();
     const bool valid = lane < nl;
@@ -170,28 +240,42 @@ __device__ __forceinline__ int reg_level(uint32_t& k, uint32_t& v, int nl, int s
         return nl;
     }
     const bool head = valid && !dup && !drop;
-    uint32_t sum;
+    V sum;
     if (pairs_only) {
         const uint32_t kn = next_lane_u32(kk, kNoKey);
-        const uint32_t vn = next_lane_u32(v, 0u);
-        sum = v + ((lane + 1 < nl && kn == kk) ? vn : 0u);
+        const V vn = next_lane_v(v, V(0));
+        sum = v + ((lane + 1 < nl && kn == kk) ? vn : V(0));
     } else {
-        const uint32_t pre = wave_incl_scan_u32(valid ? v : 0u);
         const uint64_t starts = __ballot(valid && !dup);
         const uint64_t above = lane == 63 ? 0ull : (starts & ~((2ull << lane) - 1ull));
         const int j = above ? (int)__builtin_ctzll(above) : nl;      // end of this lane's run
-        const uint32_t pj = (uint32_t)__shfl((int)pre, j - 1 < 0 ? 0 : j - 1, 64);
-        const uint32_t pi = prev_lane_u32(pre, 0u);
-        sum = pj - pi;
+        if constexpr (sizeof(V) == 4) {
+            // integer run sums as differences of the (modular) prefix sums
+            const uint32_t pre = wave_incl_scan_u32(valid ? v : 0u);
+            const uint32_t pj = shfl_v(pre, j - 1 < 0 ? 0 : j - 1);
+            const uint32_t pi = prev_lane_u32(pre, 0u);
+            sum = pj - pi;
+        } else {
+            // FP64: a segmented scan, so each run is summed on its own
+            const uint64_t upto = lane == 63 ? starts : (starts & ((2ull << lane) - 1ull));
+            const int st = upto ? 63 - (int)__builtin_clzll(upto) : 0;   // start of this lane's run
+            V x = valid ? v : V(0);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const V t = __shfl_up(x, d, 64);
+                if (lane - d >= st) x += t;
+            }
+            sum = shfl_v(x, j - 1 < 0 ? 0 : j - 1);
+        }
     }
     const uint64_t hm = __ballot(head);
     const int newn = __popcll(hm);
     const int dst = head ? mask_rank(hm) : 63;
     uint32_t k2 = push_to_lane(dst, kk);
-    uint32_t v2 = push_to_lane(dst, sum);
+    V v2 = push_lane_v(dst, sum);
     if (lane >= newn) {
         k2 = kNoKey;
-        v2 = 0u;
+        v2 = V(0);
     }
     k = k2;
     v = v2;
@@ -201,7 +285,8 @@ __device__ __forceinline__ int reg_level(uint32_t& k, uint32_t& v, int nl, int s
 // Sum of the level-0 values of keys < b: keys/prefix are sorted across the
 // first n lanes (kNoKey beyond).  Lane-parallel binary search; every lane takes
 // part in every permute.
-__device__ __forceinline__ uint32_t reg_prefix_below(uint32_t k, uint32_t pre, int n, uint32_t b)
+template <typename V>
+__device__ __forceinline__ V reg_prefix_below(uint32_t k, V pre, int n, uint32_t b)
 {
     int lo = 0, len = n;
 #pragma unroll
@@ -214,30 +299,31 @@ __device__ __forceinline__ uint32_t reg_prefix_below(uint32_t k, uint32_t pre, i
         lo = right ? lo + half + 1 : lo;
         len = live ? (right ? len - half - 1 : half) : 0;
     }
-    const uint32_t s = (uint32_t)__shfl((int)pre, lo > 0 ? lo - 1 : 0, 64);
-    return lo > 0 ? s : 0u;
+    const V s = shfl_v(pre, lo > 0 ? lo - 1 : 0);
+    return lo > 0 ? s : V(0);
 }
 
 // Lag products of one level, one event per lane (nl events): the partner of
 // lane i is lane i+m, m = 1..hi; products go to the LDS accumulators.  Lanes
 // past nl hold kNoKey, so a valid lane's partner beyond the row has d > hi;
 // only the lanes past nl need masking.
-__device__ __forceinline__ void reg_lags(uint32_t k, uint32_t v, int nl, const LevelParam& lp,
-                                         unsigned long long* acc)
+template <typename V>
+__device__ __forceinline__ void reg_lags(uint32_t k, V v, int nl, const LevelParam& lp,
+                                         typename LagAcc<V>::type* acc)
 {
     const int lane = lane_id();
     const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
     const bool valid = lane < nl;
     const int cp = lane & (kAccCopies - 1);
-    uint32_t kj = k, vj = v;
+    uint32_t kj = k;
+    V vj = v;
     for (uint32_t m = 1; m <= hi; m++) {
         kj = next_lane_u32(kj, kNoKey);
-        vj = next_lane_u32(vj, 0u);
+        vj = next_lane_v(vj, V(0));
         const uint32_t d = kj - k;
         if (__builtin_amdgcn_ballot_w64(valid && d <= hi) == 0ull) break;
         if (valid && d - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp],
-                      (unsigned long long)v * (unsigned long long)vj);
+            atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp], lag_prod(v, vj));
     }
 }
 
@@ -265,11 +351,11 @@ __device__ __forceinline__ int reg_count_below(uint32_t k, int n, uint32_t b)
 
 // Compare-exchange of elements i and i^J inside one register (J < 64) for a
 // bitonic network; `up` gives the direction of the element's sequence.
-template <int J>
-__device__ __forceinline__ void bitonic_step_dir(uint32_t& k, uint32_t& v, bool up)
+template <int J, typename V>
+__device__ __forceinline__ void bitonic_step_dir(uint32_t& k, V& v, bool up)
 {
     const uint32_t pk = xor_lane_u32<J>(k);
-    const uint32_t pv = xor_lane_u32<J>(v);
+    const V pv = xor_lane_v<J>(v);
     const bool lower = (lane_id() & J) == 0;
     const bool take = (lower == up) ? (pk < k) : (pk > k);
     if (take) {
@@ -278,8 +364,8 @@ __device__ __forceinline__ void bitonic_step_dir(uint32_t& k, uint32_t& v, bool 
     }
 }
 
-template <int KK>
-__device__ __forceinline__ void bitonic_merge_in_reg(uint32_t& k, uint32_t& v, bool up)
+template <int KK, typename V>
+__device__ __forceinline__ void bitonic_merge_in_reg(uint32_t& k, V& v, bool up)
 {
     if constexpr (KK >= 64) bitonic_step_dir<32>(k, v, up);
     if constexpr (KK >= 32) bitonic_step_dir<16>(k, v, up);
@@ -290,7 +376,8 @@ __device__ __forceinline__ void bitonic_merge_in_reg(uint32_t& k, uint32_t& v, b
 }
 
 // Ascending bitonic sort of the 128 elements of (k0, v0), (k1, v1).
-__device__ void bitonic_sort128(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1)
+template <typename V>
+__device__ void bitonic_sort128(uint32_t& k0, V& v0, uint32_t& k1, V& v1)
 {
     const int lane = lane_id();
     // stages kk = 2..64 inside each register; element i = lane + 64 r sorts
@@ -309,7 +396,8 @@ __device__ void bitonic_sort128(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32
     bitonic_merge_in_reg<64>(k1, v1, false);
     // kk = 128: elements i and i + 64 (same lane, other register), then inside
     if (k1 < k0) {
-        const uint32_t tk = k0, tv = v0;
+        const uint32_t tk = k0;
+        const V tv = v0;
         k0 = k1;
         v0 = v1;
         k1 = tk;
@@ -327,7 +415,8 @@ __device__ __forceinline__ bool reg2_unsorted(uint32_t k0, uint32_t k1)
 
 // Nearly sorted rows: up to 3 odd-even transposition rounds over the 128
 // elements, the bitonic network for whatever is still out of order.
-__device__ void reg2_sort(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1)
+template <typename V>
+__device__ void reg2_sort(uint32_t& k0, V& v0, uint32_t& k1, V& v1)
 {
     if (!reg2_unsorted(k0, k1)) return;
     const int lane = lane_id();
@@ -335,8 +424,8 @@ __device__ void reg2_sort(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1
     for (int r = 0; r < 3; r++) {
         // even phase: (2j, 2j+1), inside each register
         {
-            const uint32_t pk0 = xor_lane_u32<1>(k0), pv0 = xor_lane_u32<1>(v0);
-            const uint32_t pk1 = xor_lane_u32<1>(k1), pv1 = xor_lane_u32<1>(v1);
+            const uint32_t pk0 = xor_lane_u32<1>(k0), pk1 = xor_lane_u32<1>(k1);
+            const V pv0 = xor_lane_v<1>(v0), pv1 = xor_lane_v<1>(v1);
             if (!odd ? (pk0 < k0) : (pk0 > k0)) {
                 k0 = pk0;
                 v0 = pv0;
@@ -348,12 +437,12 @@ __device__ void reg2_sort(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1
         }
         // odd phase: (2j+1, 2j+2); element 63 pairs with element 64 across the registers
         {
-            const uint32_t f1k = read_lane_u32(k1, 0), f1v = read_lane_u32(v1, 0);
-            const uint32_t l0k = read_lane_u32(k0, 63), l0v = read_lane_u32(v0, 63);
-            const uint32_t nk0 = next_lane_u32(k0, f1k), nv0 = next_lane_u32(v0, f1v);
-            const uint32_t bk0 = prev_lane_u32(k0, 0u), bv0 = prev_lane_u32(v0, 0u);
-            const uint32_t nk1 = next_lane_u32(k1, kNoKey), nv1 = next_lane_u32(v1, 0u);
-            const uint32_t bk1 = prev_lane_u32(k1, l0k), bv1 = prev_lane_u32(v1, l0v);
+            const uint32_t f1k = read_lane_u32(k1, 0), l0k = read_lane_u32(k0, 63);
+            const V f1v = read_lane_v(v1, 0), l0v = read_lane_v(v0, 63);
+            const uint32_t nk0 = next_lane_u32(k0, f1k), bk0 = prev_lane_u32(k0, 0u);
+            const uint32_t nk1 = next_lane_u32(k1, kNoKey), bk1 = prev_lane_u32(k1, l0k);
+            const V nv0 = next_lane_v(v0, f1v), bv0 = prev_lane_v(v0, V(0));
+            const V nv1 = next_lane_v(v1, V(0)), bv1 = prev_lane_v(v1, l0v);
             if (odd && nk0 < k0) {
                 k0 = nk0;
                 v0 = nv0;
@@ -374,10 +463,16 @@ __device__ void reg2_sort(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1
     bitonic_sort128(k0, v0, k1, v1);
 }
 
+// Staging slots of one wave for reg2_level: keys and values of the shifted
+// row, then of the compacted row (2 x 64 entries each).
+template <typename V>
+constexpr int reg2_stage_words() { return 2 * 2 * kWave + 2 * 2 * kWave * (int)(sizeof(V) / 4); }
+
 // One binning step over both registers (keys >> shift, keys >= lf dropped,
 // equal keys summed), compacted through the wave's staging slots when
 // anything merges or drops.  Returns the new count.
-__device__ int reg2_level(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1, int nl, int shift, uint32_t lf,
+template <typename V>
+__device__ int reg2_level(uint32_t& k0, V& v0, uint32_t& k1, V& v1, int nl, int shift, uint32_t lf,
                           uint32_t* __restrict__ stage)
 {
     const int lane = lane_id();
@@ -394,17 +489,18 @@ __device__ int reg2_level(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1
         return nl;
     }
     // stage the shifted row, then every run head sums its run
+    constexpr int kVW = 2 * kWave * (int)(sizeof(V) / 4);   // words of 128 values
     uint32_t* sk = stage;
-    uint32_t* sv = stage + 2 * kWave;
-    uint32_t* ok = stage + 4 * kWave;
-    uint32_t* ov = stage + 6 * kWave;
+    V* sv = reinterpret_cast<V*>(stage + 2 * kWave);
+    uint32_t* ok = stage + 2 * kWave + kVW;
+    V* ov = reinterpret_cast<V*>(stage + 4 * kWave + kVW);
     sk[lane] = kk0;
     sv[lane] = v0;
     sk[lane + kWave] = kk1;
     sv[lane + kWave] = v1;
     wave_sync();
     const bool head0 = valid0 && !dup0 && !drop0, head1 = valid1 && !dup1 && !drop1;
-    uint32_t sum0 = 0u, sum1 = 0u;
+    V sum0 = V(0), sum1 = V(0);
     if (head0) {
         sum0 = v0;
         for (int j = lane + 1; j < nl && sk[j] == kk0; j++) sum0 += sv[j];
@@ -428,53 +524,68 @@ __device__ int reg2_level(uint32_t& k0, uint32_t& v0, uint32_t& k1, uint32_t& v1
     }
     wave_sync();
     k0 = lane < newn ? ok[lane] : kNoKey;
-    v0 = lane < newn ? ov[lane] : 0u;
+    v0 = lane < newn ? ov[lane] : V(0);
     k1 = lane + kWave < newn ? ok[lane + kWave] : kNoKey;
-    v1 = lane + kWave < newn ? ov[lane + kWave] : 0u;
+    v1 = lane + kWave < newn ? ov[lane + kWave] : V(0);
     wave_sync();
     return newn;
 }
 
 // Lag products of one level over both registers (partner of element i is i+m).
-__device__ __forceinline__ void reg2_lags(uint32_t k0, uint32_t v0, uint32_t k1, uint32_t v1, int nl,
-                                          const LevelParam& lp, unsigned long long* acc)
+template <typename V>
+__device__ __forceinline__ void reg2_lags(uint32_t k0, V v0, uint32_t k1, V v1, int nl, const LevelParam& lp,
+                                          typename LagAcc<V>::type* acc)
 {
     const int lane = lane_id();
     const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
     const bool valid0 = lane < nl, valid1 = lane + kWave < nl;
     const int cp = lane & (kAccCopies - 1);
-    uint32_t kj0 = k0, vj0 = v0, kj1 = k1, vj1 = v1;
+    uint32_t kj0 = k0, kj1 = k1;
+    V vj0 = v0, vj1 = v1;
     for (uint32_t m = 1; m <= hi; m++) {
-        const uint32_t f1k = read_lane_u32(kj1, 0), f1v = read_lane_u32(vj1, 0);
+        const uint32_t f1k = read_lane_u32(kj1, 0);
+        const V f1v = read_lane_v(vj1, 0);
         kj0 = next_lane_u32(kj0, f1k);
-        vj0 = next_lane_u32(vj0, f1v);
+        vj0 = next_lane_v(vj0, f1v);
         kj1 = next_lane_u32(kj1, kNoKey);
-        vj1 = next_lane_u32(vj1, 0u);
+        vj1 = next_lane_v(vj1, V(0));
         const uint32_t d0 = kj0 - k0, d1 = kj1 - k1;
         if (__builtin_amdgcn_ballot_w64((valid0 && d0 <= hi) || (valid1 && d1 <= hi)) == 0ull) break;
         if (valid0 && d0 - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d0 - lo)) * kAccCopies + cp],
-                      (unsigned long long)v0 * (unsigned long long)vj0);
+            atomicAdd(&acc[(lp.first + (int)(d0 - lo)) * kAccCopies + cp], lag_prod(v0, vj0));
         if (valid1 && d1 - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d1 - lo)) * kAccCopies + cp],
-                      (unsigned long long)v1 * (unsigned long long)vj1);
+            atomicAdd(&acc[(lp.first + (int)(d1 - lo)) * kAccCopies + cp], lag_prod(v1, vj1));
     }
 }
 
-// Rows r < *count of list[r] with 65..128 events (T <= kRegMaxTaus).
-template <int WAVES>
+// The level-0 value of an event: its count, or count x fw[frame].
+template <typename V>
+__device__ __forceinline__ V event_value(uint32_t c, uint32_t f, const double* __restrict__ fw, int32_t F)
+{
+    if constexpr (sizeof(V) == 4) {
+        return c;
+    } else {
+        return f < (uint32_t)F ? (double)c * fw[f] : 0.0;
+    }
+}
+
+// Rows r < *count of list[r] with 65..128 events (T <= kRegMaxTaus).  V is
+// uint32_t (counts) or double (counts x fw[frame], normalize_by_framesum,
+// main.cpp:313-336; pix_total stays the raw count total).
+template <int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
           const uint32_t* __restrict__ count, int32_t T, const TauParam* __restrict__ taus,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F, int64_t* __restrict__ G2,
           int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
-          const int32_t* __restrict__ rowmap)
+          const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
 {
-    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus * kAccCopies];
-    __shared__ uint32_t s_stage[WAVES][8 * kWave];
+    using A = typename LagAcc<V>::type;
+    __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
+    __shared__ uint32_t s_stage[WAVES][reg2_stage_words<V>()];
     const int lane = lane_id();
     const int w = wave_id();
-    unsigned long long* acc = s_acc[w];
+    A* acc = s_acc[w];
     uint32_t* stage = s_stage[w];
     const int64_t nrows = (int64_t)*count;
     const int64_t nw = (int64_t)gridDim.x * WAVES;
@@ -486,45 +597,52 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = 0ull;
-        uint32_t k0 = kNoKey, v0 = 0u, k1 = kNoKey, v1 = 0u;
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        uint32_t k0 = kNoKey, k1 = kNoKey, c0 = 0u, c1 = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
             k0 = (uint32_t)(e >> 32);
-            v0 = (uint32_t)e;
+            c0 = (uint32_t)e;
         }
         if (lane + kWave < n) {
             const uint64_t e = ev[base + kWave + lane];
             k1 = (uint32_t)(e >> 32);
-            v1 = (uint32_t)e;
+            c1 = (uint32_t)e;
+        }
+        V v0 = event_value<V>(c0, k0, fw, F), v1 = event_value<V>(c1, k1, fw, F);
+        if constexpr (sizeof(V) != 4) {
+            const uint32_t tot = read_lane_u32(wave_incl_scan_u32(c0 + c1), 63);
+            if (lane == 0) pix_total[p] = tot;
         }
         reg2_sort(k0, v0, k1, v1);
         int nl = reg2_level(k0, v0, k1, v1, n, 0, (uint32_t)F, stage);
         // IP and IF from the level-0 prefix sums (as the one-register kernel)
         {
-            const uint32_t pre0 = wave_incl_scan_u32(v0);
-            const uint32_t t0 = read_lane_u32(pre0, 63);
-            const uint32_t pre1 = wave_incl_scan_u32(v1) + t0;
-            if (lane == 0) pix_total[p] = read_lane_u32(pre1, 63);
+            const V pre0 = wave_scan_v(v0);
+            const V t0 = read_lane_v(pre0, 63);
+            const V pre1 = wave_scan_v(v1) + t0;
+            if constexpr (sizeof(V) == 4) {
+                if (lane == 0) pix_total[p] = read_lane_u32(pre1, 63);
+            }
             const int n0 = min(nl, kWave), n1 = max(nl - kWave, 0);
             auto below = [&](uint32_t b) {
                 const int c0 = reg_count_below(k0, n0, b);
                 const int c1 = reg_count_below(k1, n1, b);
-                const uint32_t s0 = (uint32_t)__shfl((int)pre0, c0 > 0 ? c0 - 1 : 0, 64);
-                const uint32_t s1 = (uint32_t)__shfl((int)pre1, c1 > 0 ? c1 - 1 : 0, 64);
-                if (c0 < n0 || n1 == 0) return c0 > 0 ? s0 : 0u;
+                const V s0 = shfl_v(pre0, c0 > 0 ? c0 - 1 : 0);
+                const V s1 = shfl_v(pre1, c1 > 0 ? c1 - 1 : 0);
+                if (c0 < n0 || n1 == 0) return c0 > 0 ? s0 : V(0);
                 return c1 > 0 ? s1 : t0;
             };
-            const uint32_t s_lvl = below(lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
+            const V s_lvl = below(lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
             for (int i0 = 0; i0 < T; i0 += kWave) {
                 const int i = i0 + lane;
                 const TauParam tp = taus[i < T ? i : T - 1];
-                const uint32_t s_ip = below((uint32_t)(tp.lf - tp.tp) << tp.level);
-                const uint32_t s_if = below((uint32_t)tp.tp << tp.level);
-                const uint32_t s_end = (uint32_t)__shfl((int)s_lvl, tp.level, 64);
+                const V s_ip = below((uint32_t)(tp.lf - tp.tp) << tp.level);
+                const V s_if = below((uint32_t)tp.tp << tp.level);
+                const V s_end = shfl_v(s_lvl, tp.level);
                 if (i < T) {
-                    iprow[i] = (int64_t)s_ip;
-                    ifrow[i] = (int64_t)(s_end - s_if);
+                    iprow[i] = sum_bits(s_ip);
+                    ifrow[i] = sum_bits((V)(s_end - s_if));
                 }
             }
         }
@@ -542,29 +660,34 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
         }
         wave_sync();
         for (int i = lane; i < T; i += kWave) {
-            unsigned long long g = 0ull;
+            A g = A(0);
 #pragma unroll
             for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
-            g2row[i] = (int64_t)g;
+            g2row[i] = sum_bits(g);
         }
         wave_sync();
     }
 }
 
+// Rows of at most 64 events, one wave each; longer rows are listed for the
+// other kernels: up to min(kMidSmall, mid_cap) events at the front of
+// mid_list, up to mid_cap at its back, the rest in long_list.  With T >
+// kRegMaxTaus every non-empty row is listed.  V as for k_mt_reg2.
 // 88 scalar registers leave room for 8 waves per SIMD (106 would allow 7)
-template <int WAVES>
+template <int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P, int32_t T,
          const TauParam* __restrict__ taus, const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
          uint64_t* __restrict__ pix_total, int32_t* __restrict__ mid_list,
          int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap,
-         const int32_t* __restrict__ rowmap)
+         const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
 {
-    __shared__ unsigned long long s_acc[WAVES][kRegMaxTaus * kAccCopies];
+    using A = typename LagAcc<V>::type;
+    __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
     const int lane = lane_id();
     const int w = wave_id();
-    unsigned long long* acc = s_acc[w];
+    A* acc = s_acc[w];
     const int64_t nw = (int64_t)gridDim.x * WAVES;
     const bool reg_ok = T <= kRegMaxTaus;
 
@@ -576,7 +699,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (lane == 0) {
                 // rows of up to kMidSmall events fill mid_list from the front,
                 // longer ones up to mid_cap from the back
-                if (n64 <= kMidSmall) {
+                if (n64 <= kMidSmall && n64 <= mid_cap) {
                     const uint32_t q = atomicAdd(&counters[2], 1u);
                     mid_list[q] = (int32_t)p;
                 } else if (n64 <= mid_cap) {
@@ -602,34 +725,40 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (lane == 0) pix_total[p] = 0;
             continue;
         }
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = 0ull;
-        uint32_t k = kNoKey, v = 0u;
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        uint32_t k = kNoKey, c = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
             k = (uint32_t)(e >> 32);
-            v = (uint32_t)e;
+            c = (uint32_t)e;
+        }
+        V v = event_value<V>(c, k, fw, F);
+        if constexpr (sizeof(V) != 4) {
+            const uint32_t tot = read_lane_u32(wave_incl_scan_u32(c), 63);
+            if (lane == 0) pix_total[p] = tot;
         }
         reg_sort(k, v, n);
         int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
         // IP and IF of every tau from the level-0 prefix sums:
         // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau')
         {
-            const uint32_t pre = wave_incl_scan_u32(v);
-            if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
+            const V pre = wave_scan_v(v);
+            if constexpr (sizeof(V) == 4) {
+                if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
+            }
             // S0(lf_L 2^L) depends on the level only: lane L holds it
-            const uint32_t s_lvl = reg_prefix_below(
-                k, pre, nl, lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
+            const V s_lvl = reg_prefix_below(k, pre, nl, lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
             for (int i0 = 0; i0 < T; i0 += kWave) {
                 const int i = i0 + lane;
                 const TauParam tp = taus[i < T ? i : T - 1];
                 const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
                 const uint32_t b_if = (uint32_t)tp.tp << tp.level;
-                const uint32_t s_ip = reg_prefix_below(k, pre, nl, b_ip);
-                const uint32_t s_if = reg_prefix_below(k, pre, nl, b_if);
-                const uint32_t s_end = (uint32_t)__shfl((int)s_lvl, tp.level, 64);
+                const V s_ip = reg_prefix_below(k, pre, nl, b_ip);
+                const V s_if = reg_prefix_below(k, pre, nl, b_if);
+                const V s_end = shfl_v(s_lvl, tp.level);
                 if (i < T) {
-                    iprow[i] = (int64_t)s_ip;
-                    ifrow[i] = (int64_t)(s_end - s_if);
+                    iprow[i] = sum_bits(s_ip);
+                    ifrow[i] = sum_bits((V)(s_end - s_if));
                 }
             }
         }
@@ -641,10 +770,10 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         }
         wave_sync();
         for (int i = lane; i < T; i += kWave) {
-            unsigned long long g = 0ull;
+            A g = A(0);
 #pragma unroll
             for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
-            g2row[i] = (int64_t)g;
+            g2row[i] = sum_bits(g);
         }
         wave_sync();
     }
@@ -863,9 +992,6 @@ __device__ double block_sum_f64(double x, unsigned long long* s_red)
 
 __device__ __forceinline__ uint64_t block_sum(uint64_t x, unsigned long long* s_red) { return block_sum_u64(x, s_red); }
 __device__ __forceinline__ double block_sum(double x, unsigned long long* s_red) { return block_sum_f64(x, s_red); }
-// a sum as stored in the int64 [P][T] buffers: integers as such, FP64 as bits
-__device__ __forceinline__ int64_t sum_bits(uint64_t v) { return (int64_t)v; }
-__device__ __forceinline__ int64_t sum_bits(double v) { return (int64_t)__double_as_longlong(v); }
 __device__ __forceinline__ unsigned long long slot_bits(uint64_t v) { return v; }
 __device__ __forceinline__ unsigned long long slot_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
 template <typename AccT>
@@ -940,8 +1066,8 @@ __device__ void long_bin(const SrcT* src, AccT* x, int lf_out)
     }
 }
 
-// One workgroup per pixel row: the listed long rows (or, with all_rows, every
-// pixel) of the sparse events, or every dense row.  With frame weights fw the
+// One workgroup per pixel row: the listed long rows of the sparse events, or
+// every dense row.  With frame weights fw the
 // level-0 value of frame f is count x fw[f] and the sums are FP64
 // (AccT = double, normalize_by_framesum, main.cpp:313-336).
 template <typename AccT>
@@ -949,7 +1075,7 @@ __global__ void __launch_bounds__(kLongBlock)
 k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
           const uint32_t* __restrict__ rows, int32_t P, int32_t T,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F,
-          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense, int all_rows,
+          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense,
           const double* __restrict__ fw, uint64_t* __restrict__ scratch, int64_t* __restrict__ G2,
           int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
           const int32_t* __restrict__ rowmap)
@@ -957,9 +1083,9 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
     __shared__ unsigned long long s_red[kLongBlock / 64];
     __shared__ unsigned long long s_lag[kMaxLagBlock];
     AccT* x = reinterpret_cast<AccT*>(scratch + (size_t)blockIdx.x * (size_t)F);
-    const int64_t nrows = (dense || all_rows) ? (int64_t)P : (int64_t)counters[0];
+    const int64_t nrows = dense ? (int64_t)P : (int64_t)counters[0];
     for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-        const int64_t p = (dense || all_rows) ? r : (int64_t)long_list[r];
+        const int64_t p = dense ? r : (int64_t)long_list[r];
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
@@ -1024,16 +1150,16 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
 {
     const HostTables& h = p->h;
     timer_begin(p, s, "k_mt_reg");
-    k_mt_reg<kRegWaves><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
+    k_mt_reg<kRegWaves, uint32_t><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
         p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed,
-        p->d_rowmap);
+        p->d_rowmap, nullptr);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
     if (h.T <= kRegMaxTaus)
-        k_mt_reg2<kReg2Waves><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
+        k_mt_reg2<kReg2Waves, uint32_t><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_taus, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
     else
         k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
@@ -1050,20 +1176,38 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
     timer_begin(p, s, dense_rows ? "k_mt_long_dense" : "k_mt_long");
     k_mt_long<uint64_t><<<p->long_blocks, kLongBlock, 0, s>>>(
         p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
-        p->d_counters, dense_rows ? 1 : 0, 0, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
+        p->d_counters, dense_rows ? 1 : 0, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
         p->d_rowmap);
     timer_end(p, s);
 }
 
-// Every pixel in FP64 with the frame weights (normalize_by_framesum), one
-// wavefront per workgroup: the per-lag reductions stay inside the wave.
+// normalize_by_framesum: FP64 sums of count x fw[frame].  Sparse rows of up
+// to 128 events run through the register row kernels (with T <= kRegMaxTaus),
+// longer rows and every dense row through the series kernel, one wavefront
+// per workgroup so the per-lag reductions stay inside the wave.
 void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
+    if (!dense_rows) {
+        const bool reg_ok = h.T <= kRegMaxTaus;
+        timer_begin(p, s, "k_mt_reg_f64");
+        k_mt_reg<kRegWaves, double><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
+            p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
+            p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters,
+            reg_ok ? kMidSmall : 0, p->d_rowmap, p->d_fw);
+        timer_end(p, s);
+        if (reg_ok) {
+            timer_begin(p, s, "k_mt_reg2_f64");
+            k_mt_reg2<kReg2Waves, double><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
+                p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_taus, p->d_levels,
+                (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
+            timer_end(p, s);
+        }
+    }
     timer_begin(p, s, "k_mt_long_f64");
     k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
         p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
-        p->d_counters, dense_rows ? 1 : 0, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
+        p->d_counters, dense_rows ? 1 : 0, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
         p->d_rowmap);
     timer_end(p, s);
 }

@@ -139,6 +139,33 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     TRY(upload(&p->d_pix2c, h.pix2c, s));
     TRY(upload(&p->d_taus, h.taus, s));
     TRY(upload(&p->d_levels, h.levels, s));
+    {
+        // distinct frame boundaries of the IP/IF prefix sums, (lf - tau') 2^L,
+        // tau' 2^L and lf 2^L, each tau's indices among them, and per frame
+        // the number of boundaries at or below it
+        std::vector<int32_t> b;
+        for (const TauParam& tp : h.taus) {
+            b.push_back((tp.lf - tp.tp) << tp.level);
+            b.push_back(tp.tp << tp.level);
+            b.push_back(tp.lf << tp.level);
+        }
+        std::sort(b.begin(), b.end());
+        b.erase(std::unique(b.begin(), b.end()), b.end());
+        auto index = [&](int32_t x) { return (int32_t)(std::lower_bound(b.begin(), b.end(), x) - b.begin()); };
+        std::vector<int4> tj;
+        for (const TauParam& tp : h.taus)
+            tj.push_back(make_int4(index((tp.lf - tp.tp) << tp.level), index(tp.tp << tp.level),
+                                   index(tp.lf << tp.level), 0));
+        std::vector<uint16_t> rank((size_t)std::max(h.F, 1));
+        size_t j = 0;
+        for (int32_t f = 0; f < h.F; f++) {
+            while (j < b.size() && b[j] <= f) j++;
+            rank[(size_t)f] = (uint16_t)j;
+        }
+        p->nrank = (int32_t)b.size();
+        TRY(upload(&p->d_tau_bnd, tj, s));
+        TRY(upload(&p->d_frame_rank, rank, s));
+    }
     TRY(upload(&p->d_chunks, h.chunks, s));
     TRY(upload(&p->d_group_chunk_first, h.group_chunk_first, s));
     TRY(upload(&p->d_group_q, h.group_q, s));
@@ -216,7 +243,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
 {
     if (!p) return XPCS_OK;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_chunks);
+    dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_tau_bnd); dfree(p->d_frame_rank); dfree(p->d_chunks);
     dfree(p->d_group_chunk_first); dfree(p->d_group_q); dfree(p->d_group_size);
     dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
     dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);

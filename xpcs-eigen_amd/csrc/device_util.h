@@ -122,6 +122,18 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x)
     return x;
 }
 
+// Inclusive running maximum over the 64 lanes (u32).
+__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x)
+{
+    x = max(x, dpp_u32<kDppRowShr + 1, 0xF, true>(0u, x));
+    x = max(x, dpp_u32<kDppRowShr + 2, 0xF, true>(0u, x));
+    x = max(x, dpp_u32<kDppRowShr + 4, 0xF, true>(0u, x));
+    x = max(x, dpp_u32<kDppRowShr + 8, 0xF, true>(0u, x));
+    x = max(x, dpp_u32<kDppBcast15, 0xA>(0u, x));
+    x = max(x, dpp_u32<kDppBcast31, 0xC>(0u, x));
+    return x;
+}
+
 // Value of lane i+1 (lane 63 gets `old`).
 __device__ __forceinline__ uint32_t next_lane_u32(uint32_t x, uint32_t old)
 {

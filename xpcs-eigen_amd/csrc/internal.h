@@ -188,6 +188,10 @@ struct xpcs_mt_plan {
     size_t series_cap = 0;
     int4* d_bnd = nullptr;               // (frame boundary, op, tau slot, 0), sorted
     int32_t nbnd = 0;
+    // distinct prefix-sum boundaries B_j of the sparse row kernels (sorted)
+    uint16_t* d_frame_rank = nullptr;    // [F] number of B_j <= frame
+    int4* d_tau_bnd = nullptr;           // [T] (j of IP, j of the IF head, j of the level end, 0)
+    int32_t nrank = 0;                   // number of distinct B_j
 
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]

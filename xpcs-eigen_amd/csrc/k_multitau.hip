@@ -73,6 +73,9 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 // spread over banks instead of serialising on one address.
 constexpr int kAccCopies = 4;
 static_assert(kAccCopies >= 1 && (kAccCopies & (kAccCopies - 1)) == 0, "copies: a power of two");
+// the accumulator slots of a wave first hold the prefix sums at the (at most
+// 3 per tau) frame boundaries
+static_assert(3 * kRegMaxTaus <= kRegMaxTaus * kAccCopies, "boundary slots");
 
 // a sum as stored in the int64 [P][T] buffers: integers as such, FP64 as bits
 __device__ __forceinline__ int64_t sum_bits(uint64_t v) { return (int64_t)v; }
@@ -108,6 +111,19 @@ __device__ __forceinline__ uint32_t push_lane_v(int dst, uint32_t x) { return pu
 __device__ __forceinline__ double push_lane_v(int dst, double x)
 {
     return from_halves(push_to_lane(dst, lo_half(x)), push_to_lane(dst, hi_half(x)));
+}
+// Value of lane i+1, zero in lane 63 (DPP bound_ctrl: no old value to copy in).
+__device__ __forceinline__ uint32_t next_lane_or_zero(uint32_t x) { return dpp_u32<kDppWaveShl1, 0xF, true>(0u, x); }
+__device__ __forceinline__ double next_lane_or_zero(double x)
+{
+    return from_halves(next_lane_or_zero(lo_half(x)), next_lane_or_zero(hi_half(x)));
+}
+// Lanes (as a mask) whose x <= b, unsigned.
+__device__ __forceinline__ uint64_t lanes_le(uint32_t x, uint32_t b) { return __builtin_amdgcn_uicmp(x, b, 37); }
+// The first n lanes as a mask (n <= 64; 0 for n <= 0).
+__device__ __forceinline__ uint64_t first_lanes(int n)
+{
+    return n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
 }
 __device__ __forceinline__ uint32_t read_lane_v(uint32_t x, int l) { return read_lane_u32(x, l); }
 __device__ __forceinline__ double read_lane_v(double x, int l)
@@ -282,48 +298,80 @@ __device__ __forceinline__ int reg_level(uint32_t& k, V& v, int nl, int shift, u
     return newn;
 }
 
-// Sum of the level-0 values of keys < b: keys/prefix are sorted across the
-// first n lanes (kNoKey beyond).  Lane-parallel binary search; every lane takes
-// part in every permute.
+// Level-0 prefix sums at the plan's distinct frame boundaries B_0 < B_1 < ...
+// (nb of them): the event that is last among those of equal rank r (r = number
+// of B_j <= its frame) leaves its inclusive prefix sum in M[r]; prefix sums
+// never decrease, so a running maximum over M gives, in M[j], the sum of the
+// values of the frames below B_j.  M holds nb slots (cleared here).
 template <typename V>
-__device__ __forceinline__ V reg_prefix_below(uint32_t k, V pre, int n, uint32_t b)
+__device__ __forceinline__ void boundary_slots_clear(V* M, int nb)
 {
-    int lo = 0, len = n;
+    for (int i = lane_id(); i < nb; i += kWave) M[i] = V(0);
+    wave_sync();
+}
+
+__device__ __forceinline__ uint32_t running_max(uint32_t x) { return wave_incl_max_u32(x); }
+__device__ __forceinline__ double running_max(double x)
+{
+    const int lane = lane_id();
 #pragma unroll
-    for (int step = 0; step < 7; step++) {
-        const int half = len >> 1;
-        const int idx = min(lo + half, kWave - 1);
-        const uint32_t kmid = (uint32_t)__shfl((int)k, idx, 64);
-        const bool live = len > 0;
-        const bool right = live && kmid < b;
-        lo = right ? lo + half + 1 : lo;
-        len = live ? (right ? len - half - 1 : half) : 0;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double t = __shfl_up(x, d, 64);
+        if (lane >= d) x = fmax(x, t);
     }
-    const V s = shfl_v(pre, lo > 0 ? lo - 1 : 0);
-    return lo > 0 ? s : V(0);
+    return x;
+}
+
+// Running maximum over M[0, nb), then IP = S(B_ip), IF = S(B_end) - S(B_if)
+// for every tau (indices per tau in tau_bnd).
+template <typename V>
+__device__ __forceinline__ void boundary_sums(V* M, int nb, const int4* __restrict__ tau_bnd, int T,
+                                              int64_t* __restrict__ iprow, int64_t* __restrict__ ifrow)
+{
+    const int lane = lane_id();
+    wave_sync();
+    V carry = V(0);
+    for (int c0 = 0; c0 < nb; c0 += kWave) {
+        const int i = c0 + lane;
+        V x = i < nb ? M[i] : V(0);
+        x = running_max(x);
+        x = x > carry ? x : carry;
+        if (i < nb) M[i] = x;
+        carry = read_lane_v(x, 63);
+    }
+    wave_sync();
+    for (int i = lane; i < T; i += kWave) {
+        const int4 j = tau_bnd[i];
+        iprow[i] = sum_bits(M[j.x]);
+        ifrow[i] = sum_bits((V)(M[j.z] - M[j.y]));
+    }
+    wave_sync();
 }
 
 // Lag products of one level, one event per lane (nl events): the partner of
-// lane i is lane i+m, m = 1..hi; products go to the LDS accumulators.  Lanes
-// past nl hold kNoKey, so a valid lane's partner beyond the row has d > hi;
-// only the lanes past nl need masking.
+// lane i is lane i+m, m = 1..hi; products go to the LDS accumulators.  Keys
+// are complemented so that the zero a shift brings in past lane 63 reads as
+// kNoKey: a valid lane's partner beyond the row then has d > hi, and only the
+// lanes past nl need masking.
 template <typename V>
 __device__ __forceinline__ void reg_lags(uint32_t k, V v, int nl, const LevelParam& lp,
                                          typename LagAcc<V>::type* acc)
 {
     const int lane = lane_id();
     const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
+    const uint64_t rows = first_lanes(nl);
     const bool valid = lane < nl;
-    const int cp = lane & (kAccCopies - 1);
-    uint32_t kj = k;
+    // slot of lag d: (first + d - lo) copies + this lane's copy
+    const int slot0 = (lp.first - (int)lo) * kAccCopies + (lane & (kAccCopies - 1));
+    const uint32_t kc = ~k;
+    uint32_t kj = kc;
     V vj = v;
     for (uint32_t m = 1; m <= hi; m++) {
-        kj = next_lane_u32(kj, kNoKey);
-        vj = next_lane_v(vj, V(0));
-        const uint32_t d = kj - k;
-        if (__builtin_amdgcn_ballot_w64(valid && d <= hi) == 0ull) break;
-        if (valid && d - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d - lo)) * kAccCopies + cp], lag_prod(v, vj));
+        kj = next_lane_or_zero(kj);
+        vj = next_lane_or_zero(vj);
+        const uint32_t d = kc - kj;   // = key(i+m) - key(i)
+        if ((lanes_le(d, hi) & rows) == 0ull) break;
+        if (valid && d - lo <= span) atomicAdd(&acc[slot0 + (int)d * kAccCopies], lag_prod(v, vj));
     }
 }
 
@@ -331,23 +379,6 @@ __device__ __forceinline__ void reg_lags(uint32_t k, V v, int nl, const LevelPar
 // Element i of the row is lane i % 64 of register i / 64 (k0/v0, k1/v1); the
 // row is sorted across both.  Levels run in this form until binning leaves at
 // most 64 events, then in the one-register form.
-
-// Number of the first n lanes (sorted keys, kNoKey beyond) with key < b.
-__device__ __forceinline__ int reg_count_below(uint32_t k, int n, uint32_t b)
-{
-    int lo = 0, len = n;
-#pragma unroll
-    for (int step = 0; step < 7; step++) {
-        const int half = len >> 1;
-        const int idx = min(lo + half, kWave - 1);
-        const uint32_t kmid = (uint32_t)__shfl((int)k, idx, 64);
-        const bool live = len > 0;
-        const bool right = live && kmid < b;
-        lo = right ? lo + half + 1 : lo;
-        len = live ? (right ? len - half - 1 : half) : 0;
-    }
-    return lo;
-}
 
 // Compare-exchange of elements i and i^J inside one register (J < 64) for a
 // bitonic network; `up` gives the direction of the element's sequence.
@@ -531,30 +562,31 @@ __device__ int reg2_level(uint32_t& k0, V& v0, uint32_t& k1, V& v1, int nl, int 
     return newn;
 }
 
-// Lag products of one level over both registers (partner of element i is i+m).
+// Lag products of one level over both registers (partner of element i is
+// i+m), complemented keys as in reg_lags.
 template <typename V>
 __device__ __forceinline__ void reg2_lags(uint32_t k0, V v0, uint32_t k1, V v1, int nl, const LevelParam& lp,
                                           typename LagAcc<V>::type* acc)
 {
     const int lane = lane_id();
     const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi, span = hi - lo;
+    const uint64_t rows0 = first_lanes(nl), rows1 = first_lanes(nl - kWave);
     const bool valid0 = lane < nl, valid1 = lane + kWave < nl;
-    const int cp = lane & (kAccCopies - 1);
-    uint32_t kj0 = k0, kj1 = k1;
+    const int slot0 = (lp.first - (int)lo) * kAccCopies + (lane & (kAccCopies - 1));
+    const uint32_t kc0 = ~k0, kc1 = ~k1;
+    uint32_t kj0 = kc0, kj1 = kc1;
     V vj0 = v0, vj1 = v1;
     for (uint32_t m = 1; m <= hi; m++) {
         const uint32_t f1k = read_lane_u32(kj1, 0);
         const V f1v = read_lane_v(vj1, 0);
         kj0 = next_lane_u32(kj0, f1k);
         vj0 = next_lane_v(vj0, f1v);
-        kj1 = next_lane_u32(kj1, kNoKey);
-        vj1 = next_lane_v(vj1, V(0));
-        const uint32_t d0 = kj0 - k0, d1 = kj1 - k1;
-        if (__builtin_amdgcn_ballot_w64((valid0 && d0 <= hi) || (valid1 && d1 <= hi)) == 0ull) break;
-        if (valid0 && d0 - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d0 - lo)) * kAccCopies + cp], lag_prod(v0, vj0));
-        if (valid1 && d1 - lo <= span)
-            atomicAdd(&acc[(lp.first + (int)(d1 - lo)) * kAccCopies + cp], lag_prod(v1, vj1));
+        kj1 = next_lane_or_zero(kj1);
+        vj1 = next_lane_or_zero(vj1);
+        const uint32_t d0 = kc0 - kj0, d1 = kc1 - kj1;
+        if (((lanes_le(d0, hi) & rows0) | (lanes_le(d1, hi) & rows1)) == 0ull) break;
+        if (valid0 && d0 - lo <= span) atomicAdd(&acc[slot0 + (int)d0 * kAccCopies], lag_prod(v0, vj0));
+        if (valid1 && d1 - lo <= span) atomicAdd(&acc[slot0 + (int)d1 * kAccCopies], lag_prod(v1, vj1));
     }
 }
 
@@ -575,10 +607,10 @@ __device__ __forceinline__ V event_value(uint32_t c, uint32_t f, const double* _
 template <int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, const int32_t* __restrict__ list,
-          const uint32_t* __restrict__ count, int32_t T, const TauParam* __restrict__ taus,
-          const LevelParam* __restrict__ levels, int32_t nlev, int32_t F, int64_t* __restrict__ G2,
-          int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
-          const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
+          const uint32_t* __restrict__ count, int32_t T, const int4* __restrict__ tau_bnd,
+          const uint16_t* __restrict__ frame_rank, int32_t nb, const LevelParam* __restrict__ levels,
+          int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
+          uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
@@ -597,7 +629,6 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
         uint32_t k0 = kNoKey, k1 = kNoKey, c0 = 0u, c1 = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
@@ -616,36 +647,25 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
         }
         reg2_sort(k0, v0, k1, v1);
         int nl = reg2_level(k0, v0, k1, v1, n, 0, (uint32_t)F, stage);
-        // IP and IF from the level-0 prefix sums (as the one-register kernel)
+        // IP and IF from the level-0 prefix sums at the boundaries (the
+        // accumulator slots serve as the boundary slots first)
         {
+            V* M = reinterpret_cast<V*>(acc);
+            boundary_slots_clear(M, nb);
             const V pre0 = wave_scan_v(v0);
-            const V t0 = read_lane_v(pre0, 63);
-            const V pre1 = wave_scan_v(v1) + t0;
+            const V pre1 = wave_scan_v(v1) + read_lane_v(pre0, 63);
             if constexpr (sizeof(V) == 4) {
                 if (lane == 0) pix_total[p] = read_lane_u32(pre1, 63);
             }
-            const int n0 = min(nl, kWave), n1 = max(nl - kWave, 0);
-            auto below = [&](uint32_t b) {
-                const int c0 = reg_count_below(k0, n0, b);
-                const int c1 = reg_count_below(k1, n1, b);
-                const V s0 = shfl_v(pre0, c0 > 0 ? c0 - 1 : 0);
-                const V s1 = shfl_v(pre1, c1 > 0 ? c1 - 1 : 0);
-                if (c0 < n0 || n1 == 0) return c0 > 0 ? s0 : V(0);
-                return c1 > 0 ? s1 : t0;
-            };
-            const V s_lvl = below(lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
-            for (int i0 = 0; i0 < T; i0 += kWave) {
-                const int i = i0 + lane;
-                const TauParam tp = taus[i < T ? i : T - 1];
-                const V s_ip = below((uint32_t)(tp.lf - tp.tp) << tp.level);
-                const V s_if = below((uint32_t)tp.tp << tp.level);
-                const V s_end = shfl_v(s_lvl, tp.level);
-                if (i < T) {
-                    iprow[i] = sum_bits(s_ip);
-                    ifrow[i] = sum_bits((V)(s_end - s_if));
-                }
-            }
+            const uint32_t r0 = lane < nl ? (uint32_t)frame_rank[k0] : kNoKey;
+            const uint32_t r1 = lane + kWave < nl ? (uint32_t)frame_rank[k1] : kNoKey;
+            const uint32_t rn0 = next_lane_u32(r0, read_lane_u32(r1, 0)), rn1 = next_lane_u32(r1, kNoKey);
+            if (lane < nl && r0 != rn0 && r0 < (uint32_t)nb) M[r0] = pre0;
+            if (lane + kWave < nl && r1 != rn1 && r1 < (uint32_t)nb) M[r1] = pre1;
+            boundary_sums(M, nb, tau_bnd, T, iprow, ifrow);
         }
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        wave_sync();
         bool two = nl > kWave;
         for (int L = 0; L < nlev; L++) {
             const LevelParam lp = levels[L];
@@ -677,7 +697,8 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
 template <int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
 k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32_t P, int32_t T,
-         const TauParam* __restrict__ taus, const LevelParam* __restrict__ levels, int32_t nlev,
+         const int4* __restrict__ tau_bnd, const uint16_t* __restrict__ frame_rank, int32_t nb,
+         const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
          uint64_t* __restrict__ pix_total, int32_t* __restrict__ mid_list,
          int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap,
@@ -725,7 +746,6 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (lane == 0) pix_total[p] = 0;
             continue;
         }
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
         uint32_t k = kNoKey, c = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
@@ -740,28 +760,22 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         reg_sort(k, v, n);
         int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
         // IP and IF of every tau from the level-0 prefix sums:
-        // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau')
+        // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau'), read at the
+        // boundaries (the accumulator slots serve as the boundary slots first)
         {
+            V* M = reinterpret_cast<V*>(acc);
+            boundary_slots_clear(M, nb);
             const V pre = wave_scan_v(v);
             if constexpr (sizeof(V) == 4) {
                 if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
             }
-            // S0(lf_L 2^L) depends on the level only: lane L holds it
-            const V s_lvl = reg_prefix_below(k, pre, nl, lane < nlev ? (uint32_t)levels[lane].lf << lane : 0u);
-            for (int i0 = 0; i0 < T; i0 += kWave) {
-                const int i = i0 + lane;
-                const TauParam tp = taus[i < T ? i : T - 1];
-                const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
-                const uint32_t b_if = (uint32_t)tp.tp << tp.level;
-                const V s_ip = reg_prefix_below(k, pre, nl, b_ip);
-                const V s_if = reg_prefix_below(k, pre, nl, b_if);
-                const V s_end = shfl_v(s_lvl, tp.level);
-                if (i < T) {
-                    iprow[i] = sum_bits(s_ip);
-                    ifrow[i] = sum_bits((V)(s_end - s_if));
-                }
-            }
+            const uint32_t r = lane < nl ? (uint32_t)frame_rank[k] : kNoKey;
+            const uint32_t rn = next_lane_u32(r, kNoKey);
+            if (lane < nl && r != rn && r < (uint32_t)nb) M[r] = pre;
+            boundary_sums(M, nb, tau_bnd, T, iprow, ifrow);
         }
+        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        wave_sync();
         for (int L = 0; L < nlev; L++) {
             const LevelParam lp = levels[L];
             if (L > 0) nl = reg_level(k, v, nl, 1, (uint32_t)lp.lf, true);
@@ -1151,15 +1165,16 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
     const HostTables& h = p->h;
     timer_begin(p, s, "k_mt_reg");
     k_mt_reg<kRegWaves, uint32_t><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
-        p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
-        p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters, kSparseCapUsed,
-        p->d_rowmap, nullptr);
+        p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels,
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list,
+        p->d_long_list, p->d_counters, kSparseCapUsed, p->d_rowmap, nullptr);
     timer_end(p, s);
     timer_begin(p, s, "k_mt_lds");
     if (h.T <= kRegMaxTaus)
         k_mt_reg2<kReg2Waves, uint32_t><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
-            p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_taus, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
+            p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank,
+            p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap,
+            nullptr);
     else
         k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
@@ -1192,15 +1207,16 @@ void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
         const bool reg_ok = h.T <= kRegMaxTaus;
         timer_begin(p, s, "k_mt_reg_f64");
         k_mt_reg<kRegWaves, double><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
-            p->d_ev, p->d_off, h.P, h.T, p->d_taus, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2,
-            p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list, p->d_long_list, p->d_counters,
-            reg_ok ? kMidSmall : 0, p->d_rowmap, p->d_fw);
+            p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels,
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list,
+            p->d_long_list, p->d_counters, reg_ok ? kMidSmall : 0, p->d_rowmap, p->d_fw);
         timer_end(p, s);
         if (reg_ok) {
             timer_begin(p, s, "k_mt_reg2_f64");
             k_mt_reg2<kReg2Waves, double><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
-                p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_taus, p->d_levels,
-                (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
+                p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank,
+                p->nrank, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
+                p->d_rowmap, p->d_fw);
             timer_end(p, s);
         }
     }

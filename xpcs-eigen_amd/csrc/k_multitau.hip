@@ -366,12 +366,18 @@ __device__ __forceinline__ void reg_lags(uint32_t k, V v, int nl, const LevelPar
     const uint32_t kc = ~k;
     uint32_t kj = kc;
     V vj = v;
-    for (uint32_t m = 1; m <= hi; m++) {
-        kj = next_lane_or_zero(kj);
-        vj = next_lane_or_zero(vj);
-        const uint32_t d = kc - kj;   // = key(i+m) - key(i)
-        if ((lanes_le(d, hi) & rows) == 0ull) break;
-        if (valid && d - lo <= span) atomicAdd(&acc[slot0 + (int)d * kAccCopies], lag_prod(v, vj));
+    // two partners per pass: one exit test and one loop test for both (keys
+    // are distinct, so d >= m and the loop ends by m = hi + 1)
+    for (int m = 0; m < kWave; m += 2) {
+        const uint32_t kj1 = next_lane_or_zero(kj);
+        const V vj1 = next_lane_or_zero(vj);
+        const uint32_t d1 = kc - kj1;   // = key(i+m+1) - key(i)
+        if ((lanes_le(d1, hi) & rows) == 0ull) break;
+        kj = next_lane_or_zero(kj1);
+        vj = next_lane_or_zero(vj1);
+        const uint32_t d2 = kc - kj;
+        if (valid && d1 - lo <= span) atomicAdd(&acc[slot0 + (int)d1 * kAccCopies], lag_prod(v, vj1));
+        if (valid && d2 - lo <= span) atomicAdd(&acc[slot0 + (int)d2 * kAccCopies], lag_prod(v, vj));
     }
 }
 
@@ -576,17 +582,25 @@ __device__ __forceinline__ void reg2_lags(uint32_t k0, V v0, uint32_t k1, V v1, 
     const uint32_t kc0 = ~k0, kc1 = ~k1;
     uint32_t kj0 = kc0, kj1 = kc1;
     V vj0 = v0, vj1 = v1;
-    for (uint32_t m = 1; m <= hi; m++) {
-        const uint32_t f1k = read_lane_u32(kj1, 0);
-        const V f1v = read_lane_v(vj1, 0);
-        kj0 = next_lane_u32(kj0, f1k);
-        vj0 = next_lane_v(vj0, f1v);
-        kj1 = next_lane_or_zero(kj1);
-        vj1 = next_lane_or_zero(vj1);
-        const uint32_t d0 = kc0 - kj0, d1 = kc1 - kj1;
-        if (((lanes_le(d0, hi) & rows0) | (lanes_le(d1, hi) & rows1)) == 0ull) break;
-        if (valid0 && d0 - lo <= span) atomicAdd(&acc[slot0 + (int)d0 * kAccCopies], lag_prod(v0, vj0));
-        if (valid1 && d1 - lo <= span) atomicAdd(&acc[slot0 + (int)d1 * kAccCopies], lag_prod(v1, vj1));
+    // two partners per pass, as in reg_lags
+    for (int m = 0; m < 2 * kWave; m += 2) {
+        uint32_t f1k = read_lane_u32(kj1, 0);
+        V f1v = read_lane_v(vj1, 0);
+        const uint32_t ka0 = next_lane_u32(kj0, f1k), ka1 = next_lane_or_zero(kj1);
+        const V va0 = next_lane_v(vj0, f1v), va1 = next_lane_or_zero(vj1);
+        const uint32_t da0 = kc0 - ka0, da1 = kc1 - ka1;
+        if (((lanes_le(da0, hi) & rows0) | (lanes_le(da1, hi) & rows1)) == 0ull) break;
+        f1k = read_lane_u32(ka1, 0);
+        f1v = read_lane_v(va1, 0);
+        kj0 = next_lane_u32(ka0, f1k);
+        kj1 = next_lane_or_zero(ka1);
+        vj0 = next_lane_v(va0, f1v);
+        vj1 = next_lane_or_zero(va1);
+        const uint32_t db0 = kc0 - kj0, db1 = kc1 - kj1;
+        if (valid0 && da0 - lo <= span) atomicAdd(&acc[slot0 + (int)da0 * kAccCopies], lag_prod(v0, va0));
+        if (valid1 && da1 - lo <= span) atomicAdd(&acc[slot0 + (int)da1 * kAccCopies], lag_prod(v1, va1));
+        if (valid0 && db0 - lo <= span) atomicAdd(&acc[slot0 + (int)db0 * kAccCopies], lag_prod(v0, vj0));
+        if (valid1 && db1 - lo <= span) atomicAdd(&acc[slot0 + (int)db1 * kAccCopies], lag_prod(v1, vj1));
     }
 }
 

@@ -9,6 +9,13 @@
 //   IF = sum_{tau'<=t<lf_L} x_L[t] = S0(2^L lf_L) - S0(2^L tau'),
 // where S0(b) is the sum of the level-0 counts of frames < b.
 //
+// k_mt_reg / k_mt_reg2: one wavefront per row of at most 64 / 128 events held
+// in registers (one / two per lane).  The row is sorted (odd-even passes, a
+// bitonic network when still unsorted), duplicates merged, IP/IF taken from
+// the level-0 prefix sums at the plan's frame boundaries (per-frame boundary
+// ranks, one running maximum per row), every level binned with ballot/permute
+// compaction, and lag partners found by shifting the row along the lanes.
+//
 // k_mt_sparse: one wavefront per pixel row of at most CAP events.  The row is
 // sorted on chip (register bitonic network up to 64 events, LDS network
 // beyond), duplicates merged, IP/IF read off the level-0 prefix sums by binary

@@ -84,6 +84,24 @@ def test_mid_rows(xmt, orc):
     check_against_oracle(xmt, orc, c)
 
 
+def test_transpose_staged_ranges(xmt, orc):
+    """About 100 events per pixel: buckets of 256 pixels hold ~25 k events, so
+    the row placement assembles each bucket in several LDS ranges."""
+    c = cases.sparse_case(W=32, H=32, F=2000, dpl=4, mu=0.05, tau_c=10.0, seed=21)
+    ev = (c["counts"] > 0).sum(axis=0)
+    assert 8192 < ev[:256].sum() < 4 * 7000
+    check_against_oracle(xmt, orc, c)
+
+
+def test_transpose_hot_pixel_direct(xmt, orc):
+    """A pixel with a count in every one of 5000 frames: its bucket's rows are
+    placed straight into HBM (a row longer than half the staging buffer)."""
+    c = cases.sparse_case(W=16, H=16, F=5000, dpl=4, mu=0.01, tau_c=20.0, seed=22)
+    c["counts"][:, 37] = np.maximum(c["counts"][:, 37], 1)
+    c["imm"] = synth.imm_bytes_sparse(c["counts"], 16, 16)
+    check_against_oracle(xmt, orc, c)
+
+
 def test_many_taus(xmt, orc):
     """More than 128 taus: every non-empty row takes the LDS kernel (dpl=16)."""
     c = cases.sparse_case(W=12, H=12, F=5000, dpl=16, mu=0.01, tau_c=30.0, seed=17)

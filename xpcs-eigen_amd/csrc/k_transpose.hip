@@ -28,7 +28,6 @@ namespace xpcs {
 //                follow from the bucket start, and events land in their rows
 //                through LDS cursors.
 // Only LDS atomics are used; every global write goes to a small, hot region.
-constexpr int kTrBlock = 256;
 constexpr int kTrChunkBlock = 1024;   // workgroup of the per-chunk kernels
 __device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total);
 
@@ -167,58 +166,150 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
     }
 }
 
-// Launched with kTrBlock threads (2^bshift >= kTrBlock row counters: a whole
-// number per thread).
-__global__ void __launch_bounds__(kTrBlock)
+// Row placement, one bucket at a time per workgroup.  The bucket's events are
+// split into G contiguous slices, one per group of waves (G = 16 single waves
+// for buckets of up to 256 pixels); every group counts its slice per pixel,
+// and its cursors start after the earlier slices' events of the same pixel,
+// so each row receives its events in bucket order (frame order, up to the
+// order inside a transpose chunk) without barriers between the groups.  A wave
+// keeps the first kTrRowsRegs * 64 events of its share in registers (read from
+// HBM once, by the counting pass).  Rows are assembled in LDS a pixel range at
+// a time: the range holds the pixels whose row starts fall in a window of
+// kTrRowsCap - (longest row) events, so its rows fit the staging buffer, and
+// it is written out as one contiguous, coalesced span.  A bucket whose longest
+// row exceeds half the buffer, or that would need more than four ranges,
+// places its events straight into HBM through the same cursors.
+constexpr int kTrRowsBlock = 1024;
+constexpr int kTrRowsWaves = kTrRowsBlock / 64;
+constexpr int kTrRowsCap = 15872;     // staged events per range (124 KB of LDS)
+constexpr int kTrRowsTable = 4096;    // per-group count and cursor entries
+constexpr int kTrRowsRegs = 16;       // register-held events per lane
+
+// Index of the first of the n nondecreasing values s[] that is >= x.
+__device__ __forceinline__ int lds_first_ge(const uint32_t* s, int n, uint32_t x)
+{
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (s[lo + half] < x) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// One workgroup per CU: 124 KB of staging, 16 KB of cursors and up to 16 KB
+// of row starts (4096-pixel buckets) stay within the 160 KB of LDS.
+__global__ void __launch_bounds__(kTrRowsBlock)
 k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int32_t nblk, int bshift,
           int32_t nb, int32_t P, int64_t* __restrict__ off, uint64_t* __restrict__ ev)
 {
-    extern __shared__ unsigned long long s_row[];   // [2^bshift] counts, then cursors
+    __shared__ unsigned long long s_stage[kTrRowsCap];
+    __shared__ uint32_t s_tab[kTrRowsTable];   // [G][2^bshift] counts, then cursors
     __shared__ int64_t s_warp[33];
+    __shared__ uint32_t s_max;
+    extern __shared__ uint32_t s_start[];      // [2^bshift] row starts (from the bucket start)
     const int nloc = 1 << bshift;
+    const uint32_t lmask = (uint32_t)nloc - 1u;
+    const int G = nloc >= kTrRowsTable ? 1 : min(kTrRowsWaves, kTrRowsTable / nloc);
+    const int wpg = kTrRowsWaves / G;          // waves per group
+    const int w = wave_id(), lane = lane_id();
+    const int g = w / wpg, wj = w % wpg;
+    uint32_t* cur = s_tab + g * nloc;
+    const uint32_t step = (uint32_t)wpg * 64u;
+    const int per = (nloc + kTrRowsBlock - 1) / kTrRowsBlock;
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         const int64_t s0 = hoff[(int64_t)b * nblk];
         const int64_t s1 = hoff[(int64_t)(b + 1) * nblk];   // hoff[nb*nblk] is the total
-        for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_row[i] = 0ull;
-        __syncthreads();
-        const int64_t bd = blockDim.x;
-        for (int64_t e0 = s0 + threadIdx.x; e0 < s1; e0 += 4 * bd) {
-            uint64_t x[4];
+        const uint32_t n = (uint32_t)(s1 - s0);
+        const uint64_t* seg = b1 + s0;
+        const uint32_t L = (n + (uint32_t)G - 1u) / (uint32_t)G;
+        const uint32_t ga = min(n, (uint32_t)g * L), gb = min(n, ga + L);   // this group's slice
+        const uint32_t e_first = ga + (uint32_t)wj * 64u + (uint32_t)lane;
+        const uint32_t e_rest = ga + (uint32_t)wj * 64u + (uint32_t)kTrRowsRegs * step;   // beyond the registers
+        for (int i = threadIdx.x; i < G * nloc; i += kTrRowsBlock) s_tab[i] = 0u;
+        if (threadIdx.x == 0) s_max = 0u;
+        uint64_t xr[kTrRowsRegs];
 #pragma unroll
-            for (int u = 0; u < 4; u++) x[u] = (e0 + u * bd < s1) ? b1[e0 + u * bd] : ~0ull;
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (x[u] != ~0ull) atomicAdd(&s_row[(uint32_t)(x[u] >> 16) & (uint32_t)(nloc - 1)], 1ull);
+        for (int t = 0; t < kTrRowsRegs; t++) {
+            const uint32_t e = e_first + (uint32_t)t * step;
+            xr[t] = e < gb ? seg[e] : ~0ull;
         }
         __syncthreads();
-        // exclusive scan of the nloc counts (nloc is a multiple of blockDim.x)
-        const int per = nloc / (int)blockDim.x;
-        int64_t acc = 0;
-        for (int u = 0; u < per; u++) acc += (int64_t)s_row[threadIdx.x * per + u];
+#pragma unroll
+        for (int t = 0; t < kTrRowsRegs; t++)
+            if (xr[t] != ~0ull) atomicAdd(&cur[(uint32_t)(xr[t] >> 16) & lmask], 1u);
+        for (uint32_t e0 = e_rest; e0 < gb; e0 += step) {
+            const uint32_t e = e0 + (uint32_t)lane;
+            if (e < gb) atomicAdd(&cur[(uint32_t)(seg[e] >> 16) & lmask], 1u);
+        }
+        __syncthreads();
+        // row lengths, their exclusive scan (row offsets), the longest row
+        const int i0 = threadIdx.x * per;
+        uint32_t sum = 0u, mx = 0u;
+        for (int u = 0; u < per; u++)
+            if (i0 + u < nloc) {
+                uint32_t c = 0u;
+                for (int q = 0; q < G; q++) c += s_tab[q * nloc + i0 + u];
+                sum += c;
+                mx = max(mx, c);
+            }
+        mx = read_lane_u32(wave_incl_max_u32(mx), 63);
+        if (lane == 0 && mx) atomicMax(&s_max, mx);
         int64_t tot;
-        int64_t ex = block_excl_scan(acc, s_warp, tot) + s0;
+        uint32_t ex = (uint32_t)block_excl_scan((int64_t)sum, s_warp, tot);
         for (int u = 0; u < per; u++) {
-            const int i = threadIdx.x * per + u;
-            const int64_t cnt = (int64_t)s_row[i];
-            const int64_t c = ((int64_t)b << bshift) + i;
-            if (c < P) off[c] = ex;
-            s_row[i] = (unsigned long long)ex;
-            ex += cnt;
+            const int i = i0 + u;
+            if (i < nloc) {
+                s_start[i] = ex;
+                const int64_t c_id = ((int64_t)b << bshift) + i;
+                if (c_id < P) off[c_id] = s0 + ex;
+                for (int q = 0; q < G; q++) {   // cursors: after the earlier slices' events
+                    const uint32_t c = s_tab[q * nloc + i];
+                    s_tab[q * nloc + i] = ex;
+                    ex += c;
+                }
+            }
         }
         if (b == nb - 1 && threadIdx.x == 0) off[P] = s1;
         __syncthreads();
-        for (int64_t e0 = s0 + threadIdx.x; e0 < s1; e0 += 4 * bd) {
-            uint64_t x[4], pos[4];
+        const uint32_t maxrow = s_max;
+        const bool direct = 2 * maxrow > (uint32_t)kTrRowsCap || n > 4 * ((uint32_t)kTrRowsCap - maxrow);
+        const uint32_t span = direct ? n + 1u : (uint32_t)kTrRowsCap - maxrow;
+        for (uint32_t lo = 0; lo < n; lo += span) {
+            const uint32_t hi = lo + span;
+            // pixels whose rows start in [lo, hi) and the window their rows fill
+            const int pa = lds_first_ge(s_start, nloc, lo);
+            const int pb = lds_first_ge(s_start, nloc, hi);
+            const uint32_t wlo = pa < nloc ? s_start[pa] : n;
+            const uint32_t whi = pb < nloc ? s_start[pb] : n;
+            if (whi == wlo) continue;
+            auto place = [&](uint64_t x) {
+                const uint32_t p = (uint32_t)(x >> 16) & lmask;
+                const uint64_t y = (x & 0xFFFFFFFF00000000ull) | (x & 0xFFFFull);
+                if (direct) {
+                    ev[s0 + atomicAdd(&cur[p], 1u)] = y;
+                } else {
+                    const uint32_t st = s_start[p];
+                    if (st >= lo && st < hi) s_stage[atomicAdd(&cur[p], 1u) - wlo] = y;
+                }
+            };
 #pragma unroll
-            for (int u = 0; u < 4; u++) x[u] = (e0 + u * bd < s1) ? b1[e0 + u * bd] : ~0ull;
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (x[u] != ~0ull) pos[u] = atomicAdd(&s_row[(uint32_t)(x[u] >> 16) & (uint32_t)(nloc - 1)], 1ull);
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (x[u] != ~0ull) ev[pos[u]] = (x[u] & 0xFFFFFFFF00000000ull) | (x[u] & 0xFFFFull);
+            for (int t = 0; t < kTrRowsRegs; t++)
+                if (xr[t] != ~0ull) place(xr[t]);
+            for (uint32_t e0 = e_rest; e0 < gb; e0 += step) {
+                const uint32_t e = e0 + (uint32_t)lane;
+                if (e < gb) place(seg[e]);
+            }
+            __syncthreads();
+            if (!direct)
+                for (uint32_t i = threadIdx.x; i < whi - wlo; i += kTrRowsBlock) ev[s0 + wlo + i] = s_stage[i];
+            __syncthreads();
         }
-        __syncthreads();
+        __syncthreads();   // s_max and the row tables are rewritten for the next bucket
     }
 }
 
@@ -544,10 +635,7 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
         t.bshift, t.nb, t.nblk, p->d_hoff, p->d_b1);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_rows");
-    // 256 threads: placement tiles of 1024 events hold about one event per
-    // pixel, so rows come out nearly in frame order (cheap to sort downstream)
-    const int rows_threads = kTrBlock;
-    k_tr_rows<<<t.nb < 4096 ? t.nb : 4096, rows_threads, (size_t)sizeof(unsigned long long) << t.bshift, s>>>(
+    k_tr_rows<<<t.nb < 256 ? t.nb : 256, kTrRowsBlock, (size_t)sizeof(uint32_t) << t.bshift, s>>>(
         p->d_b1, p->d_hoff, t.nblk, t.bshift, t.nb, (int32_t)p->nrows, p->d_off, p->d_ev);
     timer_end(p, s);
 }

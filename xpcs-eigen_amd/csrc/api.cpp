@@ -836,7 +836,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
             if (st) return st;
             p->hist_cap = nh + 1;
         }
-        const int64_t nbs = nh / 4096 + 2;
+        const int64_t nbs = bucket_scan_scratch(p->tshape.nb, p->tshape.nblk);
         if (nbs > p->bsum_cap) {
             xpcs_status st = dalloc(&p->d_bsum, (size_t)nbs);
             if (st) return st;

@@ -160,10 +160,10 @@ struct xpcs_mt_plan {
     uint64_t* d_ev = nullptr;            // (frame << 32) | count
     int64_t* d_off = nullptr;            // [P+1]
     uint64_t* d_b1 = nullptr;            // bucketed events (transpose stage)
-    uint32_t* d_hist = nullptr;          // [nb][nblk] events per (bucket, chunk)
-    int64_t* d_hoff = nullptr;           // [nb*nblk+1] segment starts
+    uint32_t* d_hist = nullptr;          // [nblk][nb] events per (chunk, bucket)
+    int64_t* d_hoff = nullptr;           // [nblk][nb] segment starts, then the total
     int64_t hist_cap = 0;
-    int64_t* d_bsum = nullptr;           // scan block sums
+    int64_t* d_bsum = nullptr;           // bucket scan scratch
     int64_t bsum_cap = 0;
     xpcs::TransposeShape tshape;
     int64_t events = 0;                  // known after execute (host copy)
@@ -240,6 +240,7 @@ xpcs_status fail(xpcs_status st, const char* fmt, ...);
 
 // kernel launchers (kernels_*.hip); all asynchronous on `s`
 TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames);
+int64_t bucket_scan_scratch(int32_t nb, int32_t nblk);
 void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
 EvFormat ev_format(const xpcs_mt_plan* p);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);

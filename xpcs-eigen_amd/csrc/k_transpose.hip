@@ -17,8 +17,8 @@ namespace xpcs {
 // ---- bucketed transpose (no per-event global atomics) -------------------------------
 // Pixels are grouped into buckets of 2^bshift consecutive compact ids.
 //   k_tr_count   per chunk of CF raw frames: LDS histogram of events per bucket,
-//                written to hist[bucket][chunk].
-//   (scan)       exclusive scan of hist in bucket-major order: the start of
+//                written to hist[chunk][bucket].
+//   (scan)       column scan of hist into bucket-major order: the start of
 //                every (bucket, chunk) segment of the bucketed array B1.
 //   k_tr_bucket  per chunk: each event takes a slot of its (bucket, chunk)
 //                segment through an LDS cursor and is written to B1 as
@@ -88,6 +88,54 @@ __device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen,
     return v != 0 && ((mapped[idx >> 5] >> (idx & 31u)) & 1u);
 }
 
+// The events of a chunk's raw frames form one flat index space, visited in
+// groups of up to kTrGroupFrames frames: the prefix sums of the group's record
+// lengths and the records' offsets sit in LDS, and each thread keeps 4 events
+// in flight across frame boundaries (frames hold a few thousand events, so a
+// thread's frame index advances by at most a step or two per event).
+constexpr int kTrGroupFrames = 256;
+
+struct ChunkGroup {
+    uint32_t pre[kTrGroupFrames + 1];   // events before each frame of the group
+    int64_t rec[kTrGroupFrames];        // byte offset of each frame's record
+    int32_t frame[kTrGroupFrames];      // output frame of each raw frame
+    int64_t warp[33];
+};
+
+// Loads group [g0, g0 + ng) into LDS; returns its event count.  Every thread
+// of the workgroup calls it (it synchronises).
+__device__ uint32_t chunk_group_load(ChunkGroup& cg, const int64_t* __restrict__ sel_off,
+                                     const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame,
+                                     int64_t g0, int ng)
+{
+    const int t = threadIdx.x;
+    const uint32_t dl = t < ng ? sel_dlen[g0 + t] : 0u;
+    int64_t tot;
+    const int64_t ex = block_excl_scan((int64_t)dl, cg.warp, tot);
+    if (t < ng) {
+        cg.pre[t] = (uint32_t)ex;
+        cg.rec[t] = sel_off[g0 + t];
+        if (sel_frame) cg.frame[t] = sel_frame[g0 + t];
+    }
+    if (t == 0) cg.pre[ng] = (uint32_t)tot;
+    __syncthreads();
+    return (uint32_t)tot;
+}
+
+// Event q of the group (q < its event count); kf is the frame of the thread's
+// previous event at or below q.
+__device__ __forceinline__ bool chunk_event(const ChunkGroup& cg, const uint8_t* __restrict__ imm, uint32_t q,
+                                            int& kf, EvFormat fm, const uint32_t* __restrict__ mapped,
+                                            uint32_t npix_det, uint32_t* __restrict__ counters, uint32_t& r,
+                                            uint32_t& v)
+{
+    while (cg.pre[kf + 1] <= q) kf++;
+    const uint8_t* base = imm + cg.rec[kf];
+    const uint32_t dlen = cg.pre[kf + 1] - cg.pre[kf];
+    const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
+    return tr_event_raw(base, dlen, al4, q - cg.pre[kf], fm, mapped, npix_det, counters, r, v);
+}
+
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf, EvFormat fm,
@@ -95,31 +143,30 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters)
 {
     extern __shared__ uint32_t s_h[];
+    __shared__ ChunkGroup cg;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0u;
-    __syncthreads();
     const int64_t k0 = (int64_t)blockIdx.x * cf;
     const int64_t k1 = min(nsel, k0 + cf);
-    for (int64_t k = k0; k < k1; k++) {
-        const uint8_t* base = imm + sel_off[k];
-        const uint32_t dlen = sel_dlen[k];
-        const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        const uint32_t bd = blockDim.x;
-        for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
-            uint32_t c[4];
-            uint32_t v[4];
+    const uint32_t bd = blockDim.x;
+    for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
+        const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
+        const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, nullptr, g0, ng);
+        int kf[4] = {0, 0, 0, 0};
+        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += 4 * bd) {
+            uint32_t c[4], v[4];
             bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, fm, mapped, npix_det, counters, c[u], v[u]);
+                const uint32_t q = q0 + u * bd;
+                ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, counters, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
                 if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
         }
+        __syncthreads();   // the group tables are reloaded
     }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)b * nblk + blockIdx.x] = s_h[b];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)blockIdx.x * nb + b] = s_h[b];
 }
 
 __global__ void __launch_bounds__(kTrChunkBlock)
@@ -130,29 +177,26 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
 {
     extern __shared__ unsigned long long s_base[];   // [nb] segment starts of this chunk
     uint32_t* s_cur = reinterpret_cast<uint32_t*>(s_base + nb);
+    __shared__ ChunkGroup cg;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         s_cur[b] = 0u;
-        s_base[b] = (unsigned long long)hoff[(int64_t)b * nblk + blockIdx.x];
+        s_base[b] = (unsigned long long)hoff[(int64_t)blockIdx.x * nb + b];
     }
-    __syncthreads();
     const uint32_t lmask = (1u << bshift) - 1u;
     const int64_t k0 = (int64_t)blockIdx.x * cf;
     const int64_t k1 = min(nsel, k0 + cf);
-    for (int64_t k = k0; k < k1; k++) {
-        const uint8_t* base = imm + sel_off[k];
-        const uint32_t dlen = sel_dlen[k];
-        const uint64_t f = (uint64_t)(uint32_t)sel_frame[k] << 32;
-        const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        const uint32_t bd = blockDim.x;
-        for (uint32_t j0 = threadIdx.x; j0 < dlen; j0 += 4 * bd) {
-            uint32_t c[4];
-            uint32_t v[4];
+    const uint32_t bd = blockDim.x;
+    for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
+        const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
+        const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, sel_frame, g0, ng);
+        int kf[4] = {0, 0, 0, 0};
+        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += 4 * bd) {
+            uint32_t c[4], v[4], slot[4];
             bool ok[4];
-            uint32_t slot[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint32_t j = j0 + u * bd;
-                ok[u] = j < dlen && tr_event_raw(base, dlen, al4, j, fm, mapped, npix_det, nullptr, c[u], v[u]);
+                const uint32_t q = q0 + u * bd;
+                ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, nullptr, c[u], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -160,9 +204,10 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
 #pragma unroll
             for (int u = 0; u < 4; u++)
                 if (ok[u])
-                    b1[s_base[c[u] >> bshift] + slot[u]] =
-                        f | ((uint64_t)((uint32_t)c[u] & lmask) << 16) | (uint64_t)v[u];
+                    b1[s_base[c[u] >> bshift] + slot[u]] = ((uint64_t)(uint32_t)cg.frame[kf[u]] << 32) |
+                                                           ((uint64_t)(c[u] & lmask) << 16) | (uint64_t)v[u];
         }
+        __syncthreads();   // the group tables are reloaded
     }
 }
 
@@ -222,8 +267,8 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
     const uint32_t step = (uint32_t)wpg * 64u;
     const int per = (nloc + kTrRowsBlock - 1) / kTrRowsBlock;
     for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int64_t s0 = hoff[(int64_t)b * nblk];
-        const int64_t s1 = hoff[(int64_t)(b + 1) * nblk];   // hoff[nb*nblk] is the total
+        const int64_t s0 = hoff[b];   // chunk 0's segment starts the bucket
+        const int64_t s1 = b + 1 < nb ? hoff[b + 1] : hoff[(int64_t)nblk * nb];
         const uint32_t n = (uint32_t)(s1 - s0);
         const uint64_t* seg = b1 + s0;
         const uint32_t L = (n + (uint32_t)G - 1u) / (uint32_t)G;
@@ -313,10 +358,8 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
     }
 }
 
-// ---- exclusive scan of the row counters ------------------------------------------
+// ---- bucket offsets --------------------------------------------------------------
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kScanBlock * kScanItems;
 
 // Block-wide exclusive scan of one value per thread; returns the block total.
 __device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total)
@@ -340,63 +383,70 @@ __device__ int64_t block_excl_scan(int64_t x, int64_t* s_warp, int64_t& total)
     return r;
 }
 
+// The per-chunk histograms are chunk-major, hist[c][b], so each chunk writes
+// and reads its nb counters contiguously.  Segment (bucket b, chunk c) of the
+// bucketed array starts at hoff[c][b] = (events of buckets < b) + (events of
+// bucket b in chunks < c); hoff[nblk][0] holds the total.  The column scan
+// runs over segments of kColChunks chunks: per-segment sums, their scan down
+// each bucket's column with the bucket totals, one workgroup's scan of the
+// totals (bucket bases), then the offsets.
+constexpr int kColChunks = 16;
+
 __global__ void __launch_bounds__(kScanBlock)
-k_scan_reduce(const uint32_t* __restrict__ cnt, int64_t n, int64_t* __restrict__ bsum)
+k_tr_colsum(const uint32_t* __restrict__ hist, int32_t nb, int32_t nblk, int64_t* __restrict__ part)
 {
-    __shared__ int64_t s[33];
-    int64_t b0 = (int64_t)blockIdx.x * kScanTile;
-    int64_t acc = 0;
-    for (int i = 0; i < kScanItems; i++) {
-        int64_t k = b0 + (int64_t)i * kScanBlock + threadIdx.x;
-        if (k < n) acc += cnt[k];
-    }
-    int64_t tot;
-    block_excl_scan(acc, s, tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+    const int32_t b = blockIdx.x * kScanBlock + threadIdx.x;
+    if (b >= nb) return;
+    const int32_t c0 = (int32_t)blockIdx.y * kColChunks, c1 = min(nblk, c0 + kColChunks);
+    int64_t sum = 0;
+    for (int32_t c = c0; c < c1; c++) sum += hist[(int64_t)c * nb + b];
+    part[(int64_t)blockIdx.y * nb + b] = sum;
 }
 
+__global__ void __launch_bounds__(kScanBlock)
+k_tr_colscan(int64_t* __restrict__ part, int32_t nb, int32_t nseg, int64_t* __restrict__ total)
+{
+    const int32_t b = blockIdx.x * kScanBlock + threadIdx.x;
+    if (b >= nb) return;
+    int64_t run = 0;
+    for (int32_t r = 0; r < nseg; r++) {
+        const int64_t v = part[(int64_t)r * nb + b];
+        part[(int64_t)r * nb + b] = run;
+        run += v;
+    }
+    total[b] = run;
+}
+
+// In place: total[b] becomes the exclusive scan, total[nb] the sum.
 __global__ void __launch_bounds__(1024)
-k_scan_bsum(int64_t* __restrict__ bsum, int64_t nb)
+k_tr_bucketscan(int64_t* __restrict__ total, int32_t nb)
 {
     __shared__ int64_t s[33];
     int64_t carry = 0;
-    for (int64_t b0 = 0; b0 < nb; b0 += blockDim.x) {
-        int64_t k = b0 + threadIdx.x;
-        int64_t v = k < nb ? bsum[k] : 0;
+    for (int32_t b0 = 0; b0 < nb; b0 += blockDim.x) {
+        const int32_t b = b0 + threadIdx.x;
+        const int64_t v = b < nb ? total[b] : 0;
         int64_t tot;
-        int64_t ex = block_excl_scan(v, s, tot);
-        if (k < nb) bsum[k] = carry + ex;
+        const int64_t ex = block_excl_scan(v, s, tot);
+        if (b < nb) total[b] = carry + ex;
         carry += tot;
     }
+    if (threadIdx.x == 0) total[nb] = carry;
 }
 
 __global__ void __launch_bounds__(kScanBlock)
-k_scan_tile(const uint32_t* __restrict__ cnt, int64_t n, const int64_t* __restrict__ bsum,
-            int64_t* __restrict__ off)
+k_tr_coloff(const uint32_t* __restrict__ hist, const int64_t* __restrict__ part, const int64_t* __restrict__ base,
+            int32_t nb, int32_t nblk, int64_t* __restrict__ hoff)
 {
-    __shared__ int64_t s[33];
-    int64_t b0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    int64_t acc = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-        int64_t k = b0 + i;
-        v[i] = k < n ? cnt[k] : 0u;
-        acc += v[i];
+    const int32_t b = blockIdx.x * kScanBlock + threadIdx.x;
+    if (b >= nb) return;
+    const int32_t c0 = (int32_t)blockIdx.y * kColChunks, c1 = min(nblk, c0 + kColChunks);
+    int64_t run = base[b] + part[(int64_t)blockIdx.y * nb + b];
+    for (int32_t c = c0; c < c1; c++) {
+        hoff[(int64_t)c * nb + b] = run;
+        run += hist[(int64_t)c * nb + b];
     }
-    int64_t tot;
-    int64_t ex = block_excl_scan(acc, s, tot) + bsum[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-        int64_t k = b0 + i;
-        if (k < n) off[k] = ex;
-        ex += v[i];
-    }
-}
-
-__global__ void k_scan_total(const uint32_t* __restrict__ cnt, int64_t n, int64_t* __restrict__ off)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) off[n] = (n > 0 ? off[n - 1] + cnt[n - 1] : 0);
+    if (b == 0 && blockIdx.y == 0) hoff[(int64_t)nblk * nb] = base[nb];
 }
 
 // ---- dense frames -> pixel-major rows ---------------------------------------------
@@ -585,14 +635,24 @@ static int grid_for(int64_t n, int64_t cap)
     return (int)(n < cap ? n : cap);
 }
 
-// Exclusive scan of n u32 values into out[0..n] (out[n] = total).
-static void scan_u32(const uint32_t* in, int64_t n, int64_t* out, int64_t* bsum, hipStream_t s)
+// Bucket-major segment offsets from the chunk-major histograms; scratch holds
+// bucket_scan_scratch(nb, nblk) values.
+static void bucket_scan(const uint32_t* hist, int32_t nb, int32_t nblk, int64_t* hoff, int64_t* scratch,
+                        hipStream_t s)
 {
-    const int64_t nb = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<<<(int)nb, kScanBlock, 0, s>>>(in, n, bsum);
-    k_scan_bsum<<<1, 1024, 0, s>>>(bsum, nb);
-    k_scan_tile<<<(int)nb, kScanBlock, 0, s>>>(in, n, bsum, out);
-    k_scan_total<<<1, 64, 0, s>>>(in, n, out);
+    const int32_t nseg = (nblk + kColChunks - 1) / kColChunks;
+    int64_t* part = scratch;
+    int64_t* total = scratch + (int64_t)nseg * nb;
+    const dim3 grid((nb + kScanBlock - 1) / kScanBlock, nseg);
+    k_tr_colsum<<<grid, kScanBlock, 0, s>>>(hist, nb, nblk, part);
+    k_tr_colscan<<<grid.x, kScanBlock, 0, s>>>(part, nb, nseg, total);
+    k_tr_bucketscan<<<1, 1024, 0, s>>>(total, nb);
+    k_tr_coloff<<<grid, kScanBlock, 0, s>>>(hist, part, total, nb, nblk, hoff);
+}
+
+int64_t bucket_scan_scratch(int32_t nb, int32_t nblk)
+{
+    return (int64_t)((nblk + kColChunks - 1) / kColChunks) * nb + nb + 1;
 }
 
 // Buckets of 2^bshift pixel rows (at least 256: k_tr_rows scans a bucket with
@@ -620,14 +680,13 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
 {
     const TransposeShape& t = p->tshape;
     const uint32_t npd = (uint32_t)((int64_t)p->h.W * p->h.H);
-    const int64_t nh = (int64_t)t.nb * t.nblk;
     timer_begin(p, s, "k_tr_count");
     k_tr_count<<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
         p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
         t.nblk, p->d_hist, p->d_counters);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_scan");
-    scan_u32(p->d_hist, nh, p->d_hoff, p->d_bsum, s);
+    bucket_scan(p->d_hist, t.nb, t.nblk, p->d_hoff, p->d_bsum, s);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_bucket");
     k_tr_bucket<<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(

@@ -72,6 +72,32 @@ def test_snapshots_equal_plans_of_the_prefix(xmt, orc, dpl):
     s.close()
 
 
+def test_frequent_snapshots(xmt):
+    """A snapshot after every push of 1..3 frames recomputes only the levels
+    that changed since the previous one; compared with plans of the prefix
+    around power-of-two frame counts (where many levels change at once)."""
+    c = cases.sparse_case(W=16, H=16, F=600, dpl=4, mu=0.3, tau_c=10.0, seed=131)
+    F = c["F"]
+    s = xmt.Stream(c["W"], c["H"], F, 4, c["dq"], c["sq"])
+    rng = np.random.default_rng(7)
+    checks = {8, 31, 32, 33, 63, 64, 127, 128, 129, 255, 256, 300, 511, 512, 600}
+    n = 0
+    while n < F:
+        k = int(min(F - n, rng.integers(1, 4)))
+        nxt = min([x for x in checks if x > n] + [F])
+        k = min(k, nxt - n)
+        s.push_imm(synth.imm_bytes_sparse(c["counts"][n:n + k], c["W"], c["H"]))
+        n += k
+        if s.frames()[1] > 0:
+            s.snapshot()
+        if n in checks:
+            print("snapshot at", n, flush=True)
+            G2, IP, IF = s.sums()
+            g2, se = s.g2()
+            assert_same((G2, IP, IF, g2, se), plan_results(xmt, c, n))
+    s.close()
+
+
 def test_device_push_equals_host_push(xmt):
     import torch
     W = H = 32

@@ -256,7 +256,8 @@ int lds_blocks(const xpcs_mt_plan* p);
 void launch_fill_u32(uint32_t* d, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
 xpcs_status prepare_execute(xpcs_mt_plan* p);
-void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus);
+void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
+                           const TauParam* taus);
 void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_frame_weights(xpcs_mt_plan* p, hipStream_t s);
 // two-time (k_twotime.hip)
@@ -306,8 +307,8 @@ struct StreamLaunch {
 };
 void launch_stream_update(const StreamLaunch& a, hipStream_t s);
 void launch_stream_snapshot(const StreamLaunch& a, int32_t n, const LevelParam* levels, int32_t nlev,
-                            const TauParam* taus_max, int32_t T, TauParam* taus, int64_t* G2, int64_t* IP,
-                            int64_t* IF, hipStream_t s);
+                            const TauParam* taus_max, int32_t t0, int32_t t1, int32_t S, TauParam* taus,
+                            int64_t* G2, int64_t* IP, int64_t* IF, hipStream_t s);
 void enqueue_execute(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);

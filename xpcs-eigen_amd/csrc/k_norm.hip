@@ -24,13 +24,13 @@ constexpr int kNormBatch = 8;
 template <bool FP>
 __global__ void __launch_bounds__(256)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
-               int32_t T, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
+               int32_t t0, int32_t t1, int32_t S, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                const int64_t* __restrict__ IF, const double* __restrict__ weight,
                NormPartial* __restrict__ part)
 {
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const NormChunk ch = chunks[c];
-        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        for (int i = t0 + (int)threadIdx.x; i < t1; i += blockDim.x) {
             const double N = (double)taus[i].N;
             int64_t sg = 0, sp = 0, sf = 0;
             double wg = 0.0, wp = 0.0, wf = 0.0;
@@ -41,9 +41,9 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
 #pragma unroll
               for (int u = 0; u < kNormBatch; u++) {
                   const int pu = min(p0 + u, ch.end - 1);
-                  gg[u] = G2[(int64_t)pu * T + i];
-                  aa[u] = IP[(int64_t)pu * T + i];
-                  bb[u] = IF[(int64_t)pu * T + i];
+                  gg[u] = G2[(int64_t)pu * S + i];
+                  aa[u] = IP[(int64_t)pu * S + i];
+                  bb[u] = IF[(int64_t)pu * S + i];
               }
               for (int u = 0; u < kNormBatch && p0 + u < ch.end; u++) {
                 const int p = p0 + u;
@@ -91,7 +91,7 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
             r.n = n;
             r.mean = mean;
             r.m2 = m2;
-            part[c * T + i] = r;
+            part[c * S + i] = r;
         }
     }
 }
@@ -102,19 +102,20 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
 __global__ void __launch_bounds__(64)
 k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ group_chunk_first,
              const int32_t* __restrict__ group_size, int32_t ngroups, const TauParam* __restrict__ taus,
-             int32_t T, int weighted, double* __restrict__ g2s)
+             int32_t t0, int32_t t1, int32_t S, int weighted, double* __restrict__ g2s)
 {
     const int64_t k = blockIdx.x;
-    if (k >= (int64_t)ngroups * T) return;
-    const int g = (int)(k / T);
-    const int i = (int)(k % T);
+    const int nt = t1 - t0;
+    if (k >= (int64_t)ngroups * nt) return;
+    const int g = (int)(k / nt);
+    const int i = t0 + (int)(k % nt);
     const int lane = (int)threadIdx.x;
     const int c0 = group_chunk_first[g], c1 = group_chunk_first[g + 1];
     double sg, sp, sf;
     if (weighted) {
         double a = 0.0, b = 0.0, c = 0.0;
         for (int ch = c0 + lane; ch < c1; ch += 64) {
-            const NormPartial& r = part[(int64_t)ch * T + i];
+            const NormPartial& r = part[(int64_t)ch * S + i];
             a += __longlong_as_double(r.g2);
             b += __longlong_as_double(r.ip);
             c += __longlong_as_double(r.ifs);
@@ -130,7 +131,7 @@ k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ g
     } else {
         int64_t a = 0, b = 0, c = 0;
         for (int ch = c0 + lane; ch < c1; ch += 64) {
-            const NormPartial& r = part[(int64_t)ch * T + i];
+            const NormPartial& r = part[(int64_t)ch * S + i];
             a += r.g2;
             b += r.ip;
             c += r.ifs;
@@ -144,7 +145,7 @@ k_norm_group(const NormPartial* __restrict__ part, const int32_t* __restrict__ g
         sp = (double)b;
         sf = (double)c;
     }
-    if (lane == 0) g2s[(int64_t)g * T + i] = (double)group_size[g] * (double)taus[i].N * sg / (sp * sf);
+    if (lane == 0) g2s[(int64_t)g * S + i] = (double)group_size[g] * (double)taus[i].N * sg / (sp * sf);
 }
 
 // Chan's merge of Welford states (n, mean, m2); an empty state is the identity.
@@ -171,17 +172,18 @@ __device__ __forceinline__ void welford_merge(double& n, double& mean, double& m
 __global__ void __launch_bounds__(64)
 k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
          const int32_t* __restrict__ q_first_group, const int32_t* __restrict__ q_chunk_first,
-         int32_t nq, int32_t T, double* __restrict__ g2, double* __restrict__ se)
+         int32_t nq, int32_t t0, int32_t t1, int32_t S, double* __restrict__ g2, double* __restrict__ se)
 {
     const int64_t k = blockIdx.x;
-    if (k >= (int64_t)nq * T) return;
-    const int q = (int)(k / T);    // 0-based
-    const int i = (int)(k % T);
+    const int nt = t1 - t0;
+    if (k >= (int64_t)nq * nt) return;
+    const int q = (int)(k / nt);    // 0-based
+    const int i = t0 + (int)(k % nt);
     const int lane = (int)threadIdx.x;
     if (lane == 0) {
         double acc = 0.0, cnt = 0.0;
         for (int g = q_first_group[q]; g < q_first_group[q + 1]; g++) {
-            const double v = g2s[(int64_t)g * T + i];
+            const double v = g2s[(int64_t)g * S + i];
             if (v == v) {
                 acc += v;
                 cnt += 1.0;
@@ -191,7 +193,7 @@ k_norm_q(const NormPartial* __restrict__ part, const double* __restrict__ g2s,
     }
     double n = 0.0, mean = 0.0, m2 = 0.0;
     for (int c = q_chunk_first[q] + lane; c < q_chunk_first[q + 1]; c += 64) {
-        const NormPartial& r = part[(int64_t)c * T + i];
+        const NormPartial& r = part[(int64_t)c * S + i];
         welford_merge(n, mean, m2, r.n, r.mean, r.m2);
     }
     for (int o = 1; o < 64; o <<= 1) {
@@ -234,36 +236,41 @@ k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const 
     oIF[o] = (float)(sum_value(IF[c * T + i], FP) * s / (lv * N));
 }
 
-// Normalisation of the first T taus; sums are [P][T] rows, taus gives N per tau.
-void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t T, const TauParam* taus)
+// Normalisation of taus [t0, t1): sums are rows of stride S ([P][S]), taus
+// gives N per tau; the chunk partials and group values use the same stride,
+// g2 and stderr are [tau][q].  Values of other taus are left as they are.
+void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
+                           const TauParam* taus)
 {
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
+    const int32_t nt = t1 - t0;
+    if (nt <= 0) return;
     timer_begin(p, s, "k_norm");
     const bool fp = p->fsnorm;
     const unsigned nb = nchunks < 65536 ? nchunks : 65536;
-    if (nchunks > 0 && T > 0) {
+    if (nchunks > 0) {
         if (fp)
-            k_norm_partial<true><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF,
-                                                    p->d_weight, p->d_part);
+            k_norm_partial<true><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2, p->d_IP,
+                                                    p->d_IF, p->d_weight, p->d_part);
         else
-            k_norm_partial<false><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, T, p->d_G2, p->d_IP, p->d_IF,
-                                                     p->d_weight, p->d_part);
+            k_norm_partial<false><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2, p->d_IP,
+                                                     p->d_IF, p->d_weight, p->d_part);
     }
-    const int64_t ng = (int64_t)h.ngroups * T;
+    const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)
         k_norm_group<<<(unsigned)ng, 64, 0, s>>>(p->d_part, p->d_group_chunk_first, p->d_group_size, h.ngroups,
-                                                 taus, T, (fp || p->d_weight) ? 1 : 0, p->d_g2s);
-    const int64_t nqt = (int64_t)h.nq * T;
+                                                 taus, t0, t1, S, (fp || p->d_weight) ? 1 : 0, p->d_g2s);
+    const int64_t nqt = (int64_t)h.nq * nt;
     if (nqt > 0)
         k_norm_q<<<(unsigned)nqt, 64, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group, p->d_q_chunk_first,
-                                              h.nq, T, p->d_g2, p->d_se);
+                                              h.nq, t0, t1, S, p->d_g2, p->d_se);
     timer_end(p, s);
 }
 
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s)
 {
-    launch_normalize_with(p, s, p->h.T, p->d_taus);
+    launch_normalize_with(p, s, 0, p->h.T, p->h.T, p->d_taus);
 }
 
 void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF)

@@ -106,18 +106,20 @@ k_stream_update(const uint32_t* __restrict__ tile, int32_t nb, int32_t P, int32_
     }
 }
 
-// Sums for the schedule of n frames (its first T taus), [P][T], and the tau
-// table (lf, N for n frames) the normalisation reads.
+// Sums for the schedule of n frames, taus [t0, t1) of it, into rows of stride
+// S ([P][S]), and those taus' table entries (lf, N for n frames) that the
+// normalisation reads.  Only the levels holding those taus are read.
 template <int D2>
 __global__ void __launch_bounds__(256)
 k_stream_snapshot(const uint32_t* __restrict__ win, const uint32_t* __restrict__ head,
                   const uint64_t* __restrict__ acc, const uint32_t* __restrict__ pend,
                   const uint64_t* __restrict__ total, int32_t P, int32_t n, const LevelParam* __restrict__ levels,
-                  int32_t nlev, const TauParam* __restrict__ taus_max, int32_t T, TauParam* __restrict__ taus,
-                  int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF)
+                  int32_t nlev, const TauParam* __restrict__ taus_max, int32_t t0, int32_t t1, int32_t S,
+                  TauParam* __restrict__ taus, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
+                  int64_t* __restrict__ IF)
 {
     if (blockIdx.x == 0) {
-        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        for (int i = t0 + (int)threadIdx.x; i < t1; i += blockDim.x) {
             TauParam t = taus_max[i];
             t.lf = n >> t.level;
             t.N = t.lf - t.tp;
@@ -130,17 +132,18 @@ k_stream_snapshot(const uint32_t* __restrict__ win, const uint32_t* __restrict__
     for (int L = 0; L < nlev; L++) {
         if (L > 0 && ((n >> (L - 1)) & 1)) TL -= pend[(int64_t)(L - 1) * P + c];
         const LevelParam lp = levels[L];
-        const int cnt = min(lp.count, T - lp.first);   // taus of this level within the first T
-        if (cnt <= 0) break;
+        if (lp.first >= t1) break;
+        const int a = max(lp.first, t0), b = min(lp.first + lp.count, t1);   // this level's taus in range
+        if (a >= b) continue;
         const int64_t lb = (int64_t)L * D2 * P;
         uint64_t sw = 0, sh = 0;   // sums of the newest / oldest d values
 #pragma unroll
         for (int d = 1; d <= D2; d++) {
             sw += win[lb + (int64_t)(d - 1) * P + c];
             sh += head[lb + (int64_t)(d - 1) * P + c];
-            const int r = d - lp.lo;
-            if (r >= 0 && r < cnt) {
-                const int64_t o = c * T + lp.first + r;
+            const int i = lp.first + d - lp.lo;
+            if (i >= a && i < b) {
+                const int64_t o = c * S + i;
                 G2[o] = (int64_t)acc[lb + (int64_t)(d - 1) * P + c];
                 IP[o] = (int64_t)(TL - sw);
                 IF[o] = (int64_t)(TL - sh);
@@ -163,16 +166,17 @@ void launch_stream_update(const StreamLaunch& a, hipStream_t s)
 }
 
 void launch_stream_snapshot(const StreamLaunch& a, int32_t n, const LevelParam* levels, int32_t nlev,
-                            const TauParam* taus_max, int32_t T, TauParam* taus, int64_t* G2, int64_t* IP,
-                            int64_t* IF, hipStream_t s)
+                            const TauParam* taus_max, int32_t t0, int32_t t1, int32_t S, TauParam* taus,
+                            int64_t* G2, int64_t* IP, int64_t* IF, hipStream_t s)
 {
+    if (t1 <= t0) return;
     const unsigned gp = (unsigned)((a.P + 255) / 256) + 1;   // block 0 also writes the tau table
     if (a.d2 == 8)
         k_stream_snapshot<8><<<gp, 256, 0, s>>>(a.win, a.head, a.acc, a.pend, a.total, a.P, n, levels, nlev,
-                                                taus_max, T, taus, G2, IP, IF);
+                                                taus_max, t0, t1, S, taus, G2, IP, IF);
     else
         k_stream_snapshot<16><<<gp, 256, 0, s>>>(a.win, a.head, a.acc, a.pend, a.total, a.P, n, levels, nlev,
-                                                 taus_max, T, taus, G2, IP, IF);
+                                                 taus_max, t0, t1, S, taus, G2, IP, IF);
 }
 
 }  // namespace xpcs

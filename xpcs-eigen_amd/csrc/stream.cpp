@@ -262,9 +262,46 @@ extern "C" xpcs_status xpcs_mt_stream_snapshot(xpcs_mt_stream* s)
             return fail(XPCS_ERR_STATE, "stream_snapshot: schedule of %d frames is not a prefix", s->n);
     hipStream_t q = s->plan->stream;
     const StreamLaunch a = launch_args(s);
-    launch_stream_snapshot(a, s->n, s->plan->d_levels, (int32_t)h.levels.size(), s->plan->d_taus, T,
-                           s->d_taus_snap, s->plan->d_G2, s->plan->d_IP, s->plan->d_IF, q);
-    launch_normalize_with(s->plan, q, T, s->d_taus_snap);
+    // Only what changed since the previous snapshot is recomputed.  Level L of
+    // n frames differs from level L of the previous count n' only when
+    // n >> L != n' >> L (the levels up to the highest bit of n ^ n'), and the
+    // taus beyond the previous schedule are new; the sums, chunk partials and
+    // g2 rows of every other tau are the previous snapshot's.  Sums and
+    // partials use the maximum schedule's row stride, so they stay in place.
+    const int32_t S = h.T;
+    int32_t rng[2][2];
+    int nr = 0;
+    if (s->snap_T <= 0) {
+        rng[nr][0] = 0;
+        rng[nr++][1] = T;
+    } else {
+        const uint32_t x = (uint32_t)(s->n ^ s->snap_n);
+        int32_t e = 0;   // taus [0, e) belong to changed levels
+        if (x) {
+            const int lc = 31 - __builtin_clz(x);
+            e = lc + 1 < (int)h.levels.size() ? h.levels[(size_t)lc + 1].first : T;
+            e = std::min(e, T);
+        }
+        const int32_t tp = s->snap_T;
+        if (e >= tp) {
+            rng[nr][0] = 0;
+            rng[nr++][1] = T;
+        } else {
+            if (e > 0) {
+                rng[nr][0] = 0;
+                rng[nr++][1] = e;
+            }
+            if (T > tp) {
+                rng[nr][0] = tp;
+                rng[nr++][1] = T;
+            }
+        }
+    }
+    for (int r = 0; r < nr; r++) {
+        launch_stream_snapshot(a, s->n, s->plan->d_levels, (int32_t)h.levels.size(), s->plan->d_taus, rng[r][0],
+                               rng[r][1], S, s->d_taus_snap, s->plan->d_G2, s->plan->d_IP, s->plan->d_IF, q);
+        launch_normalize_with(s->plan, q, rng[r][0], rng[r][1], S, s->d_taus_snap);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(s->done, q));
     s->snap_n = s->n;
@@ -314,10 +351,12 @@ extern "C" xpcs_status xpcs_mt_stream_read_sums_i64(xpcs_mt_stream* s, int64_t* 
     if (!s) return fail(XPCS_ERR_INVALID_ARG, "stream_read_sums: NULL stream");
     xpcs_status st = stream_wait(s);
     if (st) return st;
-    const size_t n = (size_t)s->plan->h.P * s->snap_T * sizeof(int64_t);
-    if (G2) HIP_TRY(hipMemcpy(G2, s->plan->d_G2, n, hipMemcpyDeviceToHost));
-    if (IP) HIP_TRY(hipMemcpy(IP, s->plan->d_IP, n, hipMemcpyDeviceToHost));
-    if (IF) HIP_TRY(hipMemcpy(IF, s->plan->d_IF, n, hipMemcpyDeviceToHost));
+    // device rows have the maximum schedule's stride
+    const size_t w = (size_t)s->snap_T * sizeof(int64_t), pitch = (size_t)s->plan->h.T * sizeof(int64_t);
+    const size_t rows = (size_t)s->plan->h.P;
+    if (G2) HIP_TRY(hipMemcpy2D(G2, w, s->plan->d_G2, pitch, w, rows, hipMemcpyDeviceToHost));
+    if (IP) HIP_TRY(hipMemcpy2D(IP, w, s->plan->d_IP, pitch, w, rows, hipMemcpyDeviceToHost));
+    if (IF) HIP_TRY(hipMemcpy2D(IF, w, s->plan->d_IF, pitch, w, rows, hipMemcpyDeviceToHost));
     return XPCS_OK;
 }
 

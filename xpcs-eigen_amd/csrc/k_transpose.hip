@@ -90,10 +90,12 @@ __device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen,
 
 // The events of a chunk's raw frames form one flat index space, visited in
 // groups of up to kTrGroupFrames frames: the prefix sums of the group's record
-// lengths and the records' offsets sit in LDS, and each thread keeps 4 events
-// in flight across frame boundaries (frames hold a few thousand events, so a
-// thread's frame index advances by at most a step or two per event).
+// lengths and the records' offsets sit in LDS, and each thread keeps
+// kTrInFlight events in flight across frame boundaries (frames hold a few
+// thousand events, so a thread's frame index advances by at most a step or two
+// per event; 2 measured faster than 1, 4 or 8).
 constexpr int kTrGroupFrames = 256;
+constexpr int kTrInFlight = 2;
 
 struct ChunkGroup {
     uint32_t pre[kTrGroupFrames + 1];   // events before each frame of the group
@@ -151,17 +153,17 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
     for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, nullptr, g0, ng);
-        int kf[4] = {0, 0, 0, 0};
-        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += 4 * bd) {
-            uint32_t c[4], v[4];
-            bool ok[4];
+        int kf[kTrInFlight] = {};
+        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kTrInFlight * bd) {
+            uint32_t c[kTrInFlight], v[kTrInFlight];
+            bool ok[kTrInFlight];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kTrInFlight; u++) {
                 const uint32_t q = q0 + u * bd;
                 ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, counters, c[u], v[u]);
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kTrInFlight; u++)
                 if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
         }
         __syncthreads();   // the group tables are reloaded
@@ -189,20 +191,20 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
     for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, sel_frame, g0, ng);
-        int kf[4] = {0, 0, 0, 0};
-        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += 4 * bd) {
-            uint32_t c[4], v[4], slot[4];
-            bool ok[4];
+        int kf[kTrInFlight] = {};
+        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kTrInFlight * bd) {
+            uint32_t c[kTrInFlight], v[kTrInFlight], slot[kTrInFlight];
+            bool ok[kTrInFlight];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < kTrInFlight; u++) {
                 const uint32_t q = q0 + u * bd;
                 ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, nullptr, c[u], v[u]);
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kTrInFlight; u++)
                 if (ok[u]) slot[u] = atomicAdd(&s_cur[c[u] >> bshift], 1u);
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < kTrInFlight; u++)
                 if (ok[u])
                     b1[s_base[c[u] >> bshift] + slot[u]] = ((uint64_t)(uint32_t)cg.frame[kf[u]] << 32) |
                                                            ((uint64_t)(c[u] & lmask) << 16) | (uint64_t)v[u];

@@ -102,6 +102,23 @@ def test_transpose_hot_pixel_direct(xmt, orc):
     check_against_oracle(xmt, orc, c)
 
 
+@pytest.mark.parametrize("bshift,cf", [(8, 20), (11, 7), (12, 50)])
+def test_transpose_bucket_sizes(xmt, orc, bshift, cf):
+    """Row placement with 256-pixel buckets (one wave per slice) and with
+    2048 / 4096-pixel buckets (wave groups sharing a slice, events beyond the
+    register-held ones re-read per range), exact against the oracle."""
+    c = cases.sparse_case(W=64, H=64, F=400, dpl=4, mu=0.02, tau_c=10.0, seed=23)
+    print(f"bucket shift {bshift}, chunk frames {cf}", flush=True)
+    plan = xmt.Plan(c["W"], c["H"], c["F"], c["dpl"], c["dq"], c["sq"])
+    plan.set_tuning(tr_bucket_shift=bshift, tr_chunk_frames=cf)
+    plan.ingest_imm(c["imm"])
+    plan.execute()
+    assert plan.get_tuning()["tr_bucket_shift"] == bshift
+    r = orc.run_exact(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"])
+    G2, IP, IF = plan.sums()
+    assert np.array_equal(IP, r["IP"]) and np.array_equal(IF, r["IF"]) and np.array_equal(G2, r["G2"])
+
+
 def test_many_taus(xmt, orc):
     """More than 128 taus: every non-empty row takes the LDS kernel (dpl=16)."""
     c = cases.sparse_case(W=12, H=12, F=5000, dpl=16, mu=0.01, tau_c=30.0, seed=17)

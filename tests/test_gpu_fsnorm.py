@@ -68,6 +68,21 @@ def test_sparse_row_classes_averaged(xmt, orc):
     check(plan, orc.run_fsnorm(c["imm"], c["dq"], c["sq"], c["F"], c["dpl"], avg=3))
 
 
+def test_sparse_rows_beyond_lds(xmt, orc):
+    """Rows of ~800 events (more than the 512-entry LDS kernel holds) beside
+    sparse ones: the FP64 series kernel takes the long rows."""
+    W = H = 16
+    F = 900
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    lo = synth.ou_counts(W, H, F, 0.05, 10.0, 614)
+    hi = synth.ou_counts(W, H, F, 3.0, 10.0, 615)
+    counts = np.where(np.arange(W * H) % 4 == 0, hi, lo).astype(np.uint16)
+    assert (counts > 0).sum(axis=0).max() > 512
+    c = dict(W=W, H=H, F=F, dpl=4, dq=dq, sq=sq, imm=synth.imm_bytes_sparse(counts, W, H))
+    plan = run_gpu(xmt, c)
+    check(plan, orc.run_fsnorm(c["imm"], dq, sq, F, 4))
+
+
 def test_sparse_many_taus(xmt, orc):
     # more than 128 taus: every row runs through the FP64 series kernel
     c = cases.sparse_case(W=16, H=16, F=1200, dpl=32, mu=0.1, tau_c=20.0, seed=613)

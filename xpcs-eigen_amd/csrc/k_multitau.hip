@@ -36,7 +36,8 @@ namespace xpcs {
 // key >= lf are dropped, runs of equal keys are summed.  Returns the new count.
 // Reads of a chunk complete before its writes, and writes only go to slots at
 // or below the slot being read, so the pass is safe in place.
-__device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uint32_t lf)
+template <typename V>
+__device__ int level_compact(uint32_t* key, V* val, int n, int shift, uint32_t lf)
 {
     const int lane = lane_id();
     int carry = 0;
@@ -48,7 +49,7 @@ __device__ int level_compact(uint32_t* key, uint32_t* val, int n, int shift, uin
         uint32_t kp = __shfl_up(k, 1, 64);
         if (lane == 0) kp = prevk;
         const bool head = valid && k < lf && k != kp;
-        uint32_t sum = 0;
+        V sum = V(0);
         if (head) {
             sum = val[i];
             for (int j = i + 1; j < n && (key[j] >> shift) == k; j++) sum += val[j];
@@ -150,6 +151,9 @@ __device__ __forceinline__ double wave_scan_v(double x)
     }
     return x;
 }
+// inclusive scans of the 64-bit accumulator type (prefix sums of a row)
+__device__ __forceinline__ unsigned long long wave_scan_acc(unsigned long long x) { return wave_incl_scan_u64(x); }
+__device__ __forceinline__ double wave_scan_acc(double x) { return wave_scan_v(x); }
 // lag products: exact 64-bit integers for counts, FP64 for weighted values
 __device__ __forceinline__ unsigned long long lag_prod(uint32_t a, uint32_t b)
 {
@@ -816,7 +820,8 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
 
 // LDS rows of m (a power of two > 64) entries, kNoKey-padded: up to 3
 // odd-even transposition rounds; true when the row ends up sorted.
-__device__ bool lds_sort_rounds(uint32_t* key, uint32_t* val, int m)
+template <typename V>
+__device__ bool lds_sort_rounds(uint32_t* key, V* val, int m)
 {
     const int lane = lane_id();
     auto sorted = [&]() {
@@ -833,7 +838,7 @@ __device__ bool lds_sort_rounds(uint32_t* key, uint32_t* val, int m)
                 if (b < a) {
                     key[i] = b;
                     key[i + 1] = a;
-                    const uint32_t va = val[i];
+                    const V va = val[i];
                     val[i] = val[i + 1];
                     val[i + 1] = va;
                 }
@@ -848,23 +853,25 @@ __device__ bool lds_sort_rounds(uint32_t* key, uint32_t* val, int m)
 // Rows r < *count of the list rows[r * step] (step -1 walks a list filled from
 // the back).  AUX slots hold the level-0 prefix sums (n <= CAP), then the lag
 // accumulators (T taus x copies).
-template <int CAP, int WAVES>
+template <int CAP, int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES)
 k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const int32_t* __restrict__ rows, int32_t step, const uint32_t* __restrict__ count,
             const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
             int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
-            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap)
+            int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap,
+            const double* __restrict__ fw)
 {
+    using A = typename LagAcc<V>::type;
     constexpr int AUX = CAP > 512 ? CAP : 512;
     __shared__ uint32_t s_key[WAVES][CAP];
-    __shared__ uint32_t s_val[WAVES][CAP];
-    __shared__ unsigned long long s_aux[WAVES][AUX];   // prefix sums, then lag accumulators
+    __shared__ V s_val[WAVES][CAP];
+    __shared__ A s_aux[WAVES][AUX];   // prefix sums, then lag accumulators
     const int lane = lane_id();
     const int w = wave_id();
     uint32_t* key = s_key[w];
-    uint32_t* val = s_val[w];
-    unsigned long long* aux = s_aux[w];
+    V* val = s_val[w];
+    A* aux = s_aux[w];
     const int64_t nw = (int64_t)gridDim.x * WAVES;
 
     const int64_t nrows = (int64_t)*count;
@@ -880,26 +887,30 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         const int m = next_pow2(n);
         bool m_sorted = false;
         // ---- load and sort by frame ----
+        uint32_t counts = 0u;   // the row's raw count total (FP64 values: pixelSum precedes the normalisation)
         if (m <= kWave) {
-            uint32_t k = 0xFFFFFFFFu, v = 0;
+            uint32_t k = 0xFFFFFFFFu, c = 0u;
             if (lane < n) {
                 uint64_t e = ev[base + lane];
                 k = (uint32_t)(e >> 32);
-                v = (uint32_t)e;
+                c = (uint32_t)e;
             }
+            counts = c;
+            V v = event_value<V>(c, k, fw, F);
             reg_sort(k, v, n);
             key[lane] = k;
             val[lane] = v;
         } else {
             for (int i = lane; i < m; i += kWave) {
-                uint32_t k = 0xFFFFFFFFu, v = 0;
+                uint32_t k = 0xFFFFFFFFu, c = 0u;
                 if (i < n) {
                     uint64_t e = ev[base + i];
                     k = (uint32_t)(e >> 32);
-                    v = (uint32_t)e;
+                    c = (uint32_t)e;
                 }
+                counts += c;
                 key[i] = k;
-                val[i] = v;
+                val[i] = event_value<V>(c, k, fw, F);
             }
             wave_sync();
             // nearly sorted rows: odd-even transposition rounds, the bitonic
@@ -916,7 +927,7 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                             if (ki != kl && ((ki > kl) == up)) {
                                 key[i] = kl;
                                 key[l] = ki;
-                                const uint32_t vi = val[i];
+                                const V vi = val[i];
                                 val[i] = val[l];
                                 val[l] = vi;
                             }
@@ -929,41 +940,46 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         wave_sync();
         // ---- level 0: merge repeated frames, prefix sums, IP/IF ----
         const int n0 = level_compact(key, val, n, 0, (uint32_t)F);
-        uint64_t carry = 0;
+        A carry = A(0);
         for (int b0 = 0; b0 < n0; b0 += kWave) {
             const int i = b0 + lane;
-            const uint64_t v = i < n0 ? (uint64_t)val[i] : 0ull;
-            const uint64_t incl = wave_incl_scan_u64(v) + carry;
+            const A v = i < n0 ? (A)val[i] : A(0);
+            const A incl = wave_scan_acc(v) + carry;
             if (i < n0) aux[i] = incl;
             carry = __shfl(incl, kWave - 1, 64);
         }
         wave_sync();
         // S0(lf_L 2^L) depends on the level only: lane L finds it once
-        uint64_t s_lvl = 0ull;
+        A s_lvl = A(0);
         if (lane < nlev) {
             const int j_end = lds_lower_bound(key, n0, (uint32_t)levels[lane].lf << lane);
-            s_lvl = j_end > 0 ? aux[j_end - 1] : 0ull;
+            s_lvl = j_end > 0 ? aux[j_end - 1] : A(0);
         }
         for (int i0 = 0; i0 < T; i0 += kWave) {
             const int i = i0 + lane;
             const TauParam tp = taus[i < T ? i : T - 1];
-            const uint64_t s_end = __shfl(s_lvl, tp.level, 64);
+            const A s_end = __shfl(s_lvl, tp.level, 64);
             if (i < T) {
                 const uint32_t b_ip = (uint32_t)(tp.lf - tp.tp) << tp.level;
                 const uint32_t b_if = (uint32_t)tp.tp << tp.level;
                 const int j_ip = lds_lower_bound(key, n0, b_ip);
                 const int j_if = lds_lower_bound(key, n0, b_if);
-                const uint64_t s_ip = j_ip > 0 ? aux[j_ip - 1] : 0ull;
-                const uint64_t s_if = j_if > 0 ? aux[j_if - 1] : 0ull;
-                iprow[i] = (int64_t)s_ip;
-                ifrow[i] = (int64_t)(s_end - s_if);
+                const A s_ip = j_ip > 0 ? aux[j_ip - 1] : A(0);
+                const A s_if = j_if > 0 ? aux[j_if - 1] : A(0);
+                iprow[i] = sum_bits(s_ip);
+                ifrow[i] = sum_bits((A)(s_end - s_if));
             }
         }
-        if (lane == 0) pix_total[p] = carry;
+        if constexpr (sizeof(V) == 4) {
+            if (lane == 0) pix_total[p] = carry;
+        } else {
+            const uint64_t tot = wave_sum_u64((uint64_t)counts);
+            if (lane == 0) pix_total[p] = tot;
+        }
         wave_sync();
         // lag accumulators: nc copies per tau while T * nc fits the CAP slots
         const int nc = T * kAccCopies <= AUX ? kAccCopies : 1;
-        for (int i = lane; i < T * nc; i += kWave) aux[i] = 0ull;
+        for (int i = lane; i < T * nc; i += kWave) aux[i] = A(0);
         wave_sync();
         // ---- levels: bin in place, then lag products ----
         int nl = n0;
@@ -974,21 +990,20 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
             for (int i = lane; i < nl; i += kWave) {
                 const uint32_t ki = key[i];
-                const uint64_t vi = val[i];
+                const V vi = val[i];
                 for (int j = i + 1; j < nl; j++) {
                     const uint32_t d = key[j] - ki;
                     if (d > hi) break;
                     if (d >= lo)
-                        atomicAdd(&aux[(lp.first + (int)(d - lo)) * nc + (lane & (nc - 1))],
-                                  (unsigned long long)(vi * val[j]));
+                        atomicAdd(&aux[(lp.first + (int)(d - lo)) * nc + (lane & (nc - 1))], lag_prod(vi, val[j]));
                 }
             }
             wave_sync();
         }
         for (int i = lane; i < T; i += kWave) {
-            unsigned long long g = 0ull;
+            A g = A(0);
             for (int c = 0; c < nc; c++) g += aux[i * nc + c];
-            g2row[i] = (int64_t)g;
+            g2row[i] = sum_bits(g);
         }
         wave_sync();
     }
@@ -1197,12 +1212,12 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
             p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap,
             nullptr);
     else
-        k_mt_sparse<kMidSmall, kSparseWaves><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
+        k_mt_sparse<kMidSmall, kSparseWaves, uint32_t><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
-    k_mt_sparse<kSparseCapUsed, kSparseWaves><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
+    k_mt_sparse<kSparseCapUsed, kSparseWaves, uint32_t><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
     timer_end(p, s);
 }
 
@@ -1217,29 +1232,36 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
     timer_end(p, s);
 }
 
-// normalize_by_framesum: FP64 sums of count x fw[frame].  Sparse rows of up
-// to 128 events run through the register row kernels (with T <= kRegMaxTaus),
-// longer rows and every dense row through the series kernel, one wavefront
-// per workgroup so the per-lag reductions stay inside the wave.
+// normalize_by_framesum: FP64 sums of count x fw[frame], through the same row
+// kernels as the exact path (register rows, two per lane, the LDS kernels)
+// with V = double; rows over the LDS capacity and every dense row run through
+// the series kernel, one wavefront per workgroup so the per-lag reductions
+// stay inside the wave.
 void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
     if (!dense_rows) {
-        const bool reg_ok = h.T <= kRegMaxTaus;
         timer_begin(p, s, "k_mt_reg_f64");
         k_mt_reg<kRegWaves, double><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
             p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels,
             (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list,
-            p->d_long_list, p->d_counters, reg_ok ? kMidSmall : 0, p->d_rowmap, p->d_fw);
+            p->d_long_list, p->d_counters, kSparseCapUsed, p->d_rowmap, p->d_fw);
         timer_end(p, s);
-        if (reg_ok) {
-            timer_begin(p, s, "k_mt_reg2_f64");
+        timer_begin(p, s, "k_mt_lds_f64");
+        if (h.T <= kRegMaxTaus)
             k_mt_reg2<kReg2Waves, double><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
                 p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank,
                 p->nrank, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
                 p->d_rowmap, p->d_fw);
-            timer_end(p, s);
-        }
+        else
+            k_mt_sparse<kMidSmall, kSparseWaves, double><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
+                p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
+                (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
+        k_mt_sparse<kSparseCapUsed, kSparseWaves, double><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0,
+                                                            s>>>(
+            p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
+        timer_end(p, s);
     }
     timer_begin(p, s, "k_mt_long_f64");
     k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(

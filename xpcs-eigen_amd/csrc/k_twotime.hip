@@ -259,7 +259,9 @@ constexpr int kGramLoads = kGramK * kGramTile / 256;   // doubles per thread and
 // registers while the current one is multiplied from LDS.
 // MFMA operand layout (16x16x4 FP64): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; its result register r holds D[l/16 + 4r][l%16], r = 0..3.
-__global__ void __launch_bounds__(256)
+// Two waves per SIMD: the accumulators move from AGPRs to VGPRs (210 in all,
+// no spills), which hides the operand loads better than one wave with 332.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_tt_gram(const double* __restrict__ X, int32_t n, int32_t F, int32_t ntile, int32_t ksplit, double inv_pq,
           double* __restrict__ C)
 {

@@ -112,6 +112,41 @@ def test_dense_with_darks(xmt, orc):
     check(plan, orc.run_fsnorm(c["imm"], dq, sq, F, 4, darks=2))
 
 
+def test_dense_tiles_dpl8(xmt, orc):
+    # 64x64 uncompressed frames at a fixed step: the staged-tile raw pass, then
+    # FP64 series passes for the higher levels
+    W = H = 64
+    F = 1100
+    dq, sq = synth.annular_qmaps(W, H, 8, 4)
+    c = dict(W=W, H=H, F=F, dpl=8, dq=dq, sq=sq,
+             imm=synth.imm_bytes_dense(synth.ou_counts(W, H, F, 3.0, 25.0, seed=614), W, H))
+    plan = run_gpu(xmt, c)
+    check(plan, orc.run_fsnorm(c["imm"], dq, sq, F, 8))
+
+
+def test_dense_stride_average_flatfield(xmt, orc):
+    # 20x20 pixels (not a multiple of 64): the per-lane raw pass with averaging
+    W = H = 20
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    raw = synth.ou_counts(W, H, 820, 2.0, 12.0, seed=615)
+    ff = np.random.default_rng(616).uniform(0.5, 1.5, (H, W))
+    c = dict(W=W, H=H, F=200, dpl=4, dq=dq, sq=sq, imm=synth.imm_bytes_dense(raw, W, H))
+    plan = run_gpu(xmt, c, stride=2, average=2, frame_from=3, flatfield=ff)
+    check(plan, orc.run_fsnorm(c["imm"], dq, sq, 200, 4, stride=2, avg=2, frame_from=3, flatfield=ff))
+
+
+def test_dense_dpl16_series_rows(xmt, orc):
+    # delays per level other than 4 and 8: dense frames go through the
+    # transposed per-pixel series instead of the streaming passes
+    W = H = 16
+    F = 600
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    c = dict(W=W, H=H, F=F, dpl=16, dq=dq, sq=sq,
+             imm=synth.imm_bytes_dense(synth.ou_counts(W, H, F, 3.0, 20.0, seed=617), W, H))
+    plan = run_gpu(xmt, c)
+    check(plan, orc.run_fsnorm(c["imm"], dq, sq, F, 16))
+
+
 def test_side_stats_are_the_raw_counts(xmt):
     c = cases.sparse_case(W=24, H=24, F=300, mu=0.2, tau_c=8.0, seed=606)
     a = xmt.Plan(c["W"], c["H"], c["F"], 4, c["dq"], c["sq"], side_stats=True, normalize_by_framesum=True)

@@ -816,7 +816,8 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
     if (p->fsnorm && p->mode == 3)
         return fail(XPCS_ERR_UNSUPPORTED, "execute: normalize_by_framesum needs frame sums (IMM or packed ingest)");
     HostTables& h = p->h;
-    const bool long_rows = p->fsnorm || p->mode != 2 || !dense_stream_ok(p);
+    const bool dense_stream = p->mode == 2 && dense_stream_ok(p);
+    const bool long_rows = !dense_stream;
     if (long_rows && !p->d_scratch) {
         xpcs_status st = dalloc(&p->d_scratch, (size_t)p->long_blocks * h.F);
         if (st) return st;
@@ -842,7 +843,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
             if (st) return st;
             p->bsum_cap = nbs;
         }
-    } else if (p->mode == 2 && dense_stream_ok(p) && !p->fsnorm) {
+    } else if (dense_stream) {
         const size_t need = dense_series_elems(p);
         if (need > p->series_cap) {
             xpcs_status st = dalloc(&p->d_series_a, need);
@@ -869,7 +870,13 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
     HostTables& h = p->h;
     launch_fill_u32(p->d_counters, 8, s);
     if (p->d_fs) launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
-    if (p->fsnorm) {
+    if (p->fsnorm && p->mode == 2 && dense_stream_ok(p)) {
+        // normalize_by_framesum on dense frames: frame sums from the raw
+        // frames, then the streaming kernels over frame-weighted values
+        launch_side_dense_raw(p, s);
+        launch_frame_weights(p, s);
+        launch_mt_dense(p, s, true);
+    } else if (p->fsnorm) {
         // normalize_by_framesum: frame sums first, then every pixel in FP64
         if (p->mode == 1) launch_transpose(p, s);
         else launch_dense_transpose(p, s);
@@ -884,7 +891,7 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
     } else if (p->mode == 2 && dense_stream_ok(p)) {
         // thread-per-pixel streaming multi-tau over the raw frames
         if (p->d_fs) launch_side_dense_raw(p, s);
-        launch_mt_dense(p, s);
+        launch_mt_dense(p, s, false);
     } else if (p->mode == 2) {
         launch_dense_transpose(p, s);
         if (p->d_fs) launch_side(p, s);

@@ -310,7 +310,7 @@ void launch_stream_snapshot(const StreamLaunch& a, int32_t n, const LevelParam* 
                             const TauParam* taus_max, int32_t t0, int32_t t1, int32_t S, TauParam* taus,
                             int64_t* G2, int64_t* IP, int64_t* IF, hipStream_t s);
 void enqueue_execute(xpcs_mt_plan* p, hipStream_t s);
-void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s);
+void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s, bool fsnorm);
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s);
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s);
 void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float* IF);

@@ -28,6 +28,7 @@
 // column of a pixel is its raster index); each further pass reads the
 // previous series with the same coalesced pattern.
 #include <climits>
+#include <type_traits>
 
 #include "device_util.h"
 #include "internal.h"
@@ -61,16 +62,18 @@ struct FState {
     double pend[K];           // first value of an incomplete pair (levels above the group)
 };
 
+// ST: uint32_t for counts, double for frame-weighted values (normalize_by_framesum)
+template <typename ST>
 struct SeriesOut {
-    uint32_t* col;            // element k of the output series at col[k * stride]
+    ST* col;                  // element k of the output series at col[k * stride]
     int64_t stride;
     uint32_t lf;              // values of the output level
 };
 
 // One value of level l (index k, wave-uniform): lag products, window shift,
 // and pairing into level l+1 by the parity of k.
-template <int DPL, int K, bool FIRST, int l>
-__device__ __forceinline__ void push_one(FState<DPL, K>& s, double v, uint32_t k, const SeriesOut& o)
+template <int DPL, int K, bool FIRST, int l, typename ST>
+__device__ __forceinline__ void push_one(FState<DPL, K>& s, double v, uint32_t k, const SeriesOut<ST>& o)
 {
     if constexpr (l < K) {
         constexpr int dlo = lag_lo<DPL, FIRST, l>();
@@ -82,7 +85,7 @@ __device__ __forceinline__ void push_one(FState<DPL, K>& s, double v, uint32_t k
         if (k & 1u) push_one<DPL, K, FIRST, l + 1>(s, s.pend[l] + v, k >> 1, o);
         else s.pend[l] = v;
     } else {
-        if (o.col && k < o.lf) o.col[(int64_t)k * o.stride] = (uint32_t)v;
+        if (o.col && k < o.lf) o.col[(int64_t)k * o.stride] = (ST)v;
     }
 }
 
@@ -90,9 +93,9 @@ __device__ __forceinline__ void push_one(FState<DPL, K>& s, double v, uint32_t k
 // TAIL: the last, partial group; values with index >= lf of their level are
 // zeroed (they do not exist in the series) and nothing reaches the levels
 // above the group.
-template <int DPL, int K, bool FIRST, bool TAIL, int l, int N>
+template <int DPL, int K, bool FIRST, bool TAIL, int l, int N, typename ST>
 __device__ __forceinline__ void push_block(FState<DPL, K>& s, double (&v)[N], uint32_t kb, uint32_t n_in,
-                                           const SeriesOut& o)
+                                           const SeriesOut<ST>& o)
 {
     if constexpr (TAIL) {
         const uint32_t lim = n_in >> l;
@@ -127,7 +130,7 @@ __device__ __forceinline__ void push_block(FState<DPL, K>& s, double (&v)[N], ui
     } else {
 #pragma unroll
         for (int j = 0; j < N; j++)
-            if (o.col && kb + j < o.lf) o.col[(int64_t)(kb + j) * o.stride] = (uint32_t)v[j];
+            if (o.col && kb + j < o.lf) o.col[(int64_t)(kb + j) * o.stride] = (ST)v[j];
     }
 }
 
@@ -147,18 +150,26 @@ enum DenseInput { kSeries = 0, kRawLanes = 1, kRawTiles = 2 };
 constexpr int kTileDepth = 3;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-template <int DPL, int K, int MODE>
+// FP (normalize_by_framesum, main.cpp:313-336): level-0 values are the counts
+// times fw[frame], the series between passes are FP64, and G2/IP/IF are FP64
+// sums stored as bit patterns (no exactness bound, no integer redo);
+// pix_total stays the raw count total.
+template <int DPL, int K, int MODE, bool FP>
 __global__ void __launch_bounds__(256)
 k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int64_t tile_off0,
                int64_t tile_step, int avg, const int32_t* __restrict__ pix2c, uint32_t npix,
                const int64_t* __restrict__ dark_sum, const int64_t* __restrict__ thr, int32_t nscale,
-               const uint32_t* __restrict__ series_in, uint32_t n_in, const LevelParam* __restrict__ levels,
-               int32_t L0, int32_t nlev_pass, const int4* __restrict__ bnd, int32_t nbnd, int32_t T,
-               int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
-               uint64_t* __restrict__ pix_total, uint32_t* __restrict__ series_out, uint32_t lf_out,
-               uint32_t* __restrict__ redo_count, int32_t* __restrict__ redo_list)
+               const std::conditional_t<FP, double, uint32_t>* __restrict__ series_in, uint32_t n_in,
+               const LevelParam* __restrict__ levels, int32_t L0, int32_t nlev_pass, const int4* __restrict__ bnd,
+               int32_t nbnd, int32_t T, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
+               int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
+               std::conditional_t<FP, double, uint32_t>* __restrict__ series_out, uint32_t lf_out,
+               uint32_t* __restrict__ redo_count, int32_t* __restrict__ redo_list, const double* __restrict__ fw)
 {
+    using ST = std::conditional_t<FP, double, uint32_t>;   // series element
     constexpr bool RAW = MODE != kSeries;
+    // values of a group: integers from the raw frames, or the input series
+    using XT = std::conditional_t<FP && !RAW, double, uint32_t>;
     constexpr bool TILES = MODE == kRawTiles;
     __shared__ u32x4 s_tile[TILES ? 4 : 1][2][64];
 
@@ -182,8 +193,8 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
             s.acc[l][j] = 0.0;
         }
     }
-    const SeriesOut o{(active && series_out && nlev_pass == K) ? series_out + gid : nullptr, (int64_t)npix,
-                      lf_out};
+    const SeriesOut<ST> o{(active && series_out && nlev_pass == K) ? series_out + gid : nullptr, (int64_t)npix,
+                          lf_out};
     int32_t nds = 0, th = 0;
     if constexpr (RAW) {
         if (active) {
@@ -194,48 +205,82 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
     }
     const int64_t rowT = (int64_t)(active ? c : 0) * T;
     uint32_t S = 0;
+    double Sd = 0.0;   // FP: running sum of the weighted values
     int bi = 0;
     if constexpr (RAW) {
         while (bi < nbnd && bnd[bi].x == 0) bi++;
     }
     int nextb = (RAW && bi < nbnd) ? bnd[bi].x : INT_MAX;
 
-    // boundary captures inside group t0 (wave-uniform test), then the running sum
-    auto captures = [&](const uint32_t (&x)[kGroup], uint32_t t0) {
+    // boundary captures inside group t0 (wave-uniform test), then the running sums
+    auto captures = [&](const XT (&x)[kGroup], const double (&v)[kGroup], uint32_t t0) {
         if constexpr (RAW) {
             while (nextb <= (int)(t0 + kGroup)) {
                 const int4 b = bnd[bi];
                 const int n = b.x - (int)t0;
-                uint32_t part = S;
+                if constexpr (FP) {
+                    double part = Sd;
 #pragma unroll
-                for (int u = 0; u < kGroup; u++) part += (u < n) ? x[u] : 0u;
-                if (active) {
-                    if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
-                    else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
-                    else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                    for (int u = 0; u < kGroup; u++) part += (u < n) ? v[u] : 0.0;
+                    if (active) {
+                        if (b.y == 0) IP[rowT + b.z] = (int64_t)__double_as_longlong(part);
+                        else if (b.y == 1) IF[rowT + b.z] = (int64_t)__double_as_longlong(part);
+                        else
+                            IF[rowT + b.z] =
+                                (int64_t)__double_as_longlong(part - __longlong_as_double((long long)IF[rowT + b.z]));
+                    }
+                } else {
+                    uint32_t part = S;
+#pragma unroll
+                    for (int u = 0; u < kGroup; u++) part += (u < n) ? x[u] : 0u;
+                    if (active) {
+                        if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
+                        else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
+                        else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                    }
                 }
                 bi++;
                 nextb = bi < nbnd ? bnd[bi].x : INT_MAX;
             }
 #pragma unroll
-            for (int u = 0; u < kGroup; u++) S += x[u];
+            for (int u = 0; u < kGroup; u++) S += (uint32_t)x[u];
+            if constexpr (FP) {
+#pragma unroll
+                for (int u = 0; u < kGroup; u++) Sd += v[u];
+            }
         }
     };
-    auto group = [&](uint32_t (&x)[kGroup], uint32_t t0) {
-        captures(x, t0);
-        double v[kGroup];
+    // the group's values (counts x fw[frame] for FP raw input)
+    auto weigh = [&](const XT (&x)[kGroup], double (&v)[kGroup], uint32_t t0) {
 #pragma unroll
-        for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
+        for (int u = 0; u < kGroup; u++) {
+            v[u] = (double)x[u];
+            if constexpr (FP && RAW) v[u] *= fw[min(t0 + (uint32_t)u, n_in - 1)];
+        }
+    };
+    auto group = [&](XT (&x)[kGroup], uint32_t t0) {
+        double v[kGroup];
+        if constexpr (FP) {
+            weigh(x, v, t0);
+            captures(x, v, t0);
+        } else {   // counts: captures read x only
+            captures(x, v, t0);
+            weigh(x, v, t0);
+        }
         push_block<DPL, K, RAW, false, 0, kGroup>(s, v, t0, n_in, o);
     };
-    auto tail_group = [&](uint32_t (&x)[kGroup], uint32_t t0) {
+    auto tail_group = [&](XT (&x)[kGroup], uint32_t t0) {
 #pragma unroll
         for (int u = 0; u < kGroup; u++)
-            if (t0 + (uint32_t)u >= n_in) x[u] = 0u;
-        captures(x, t0);
+            if (t0 + (uint32_t)u >= n_in) x[u] = XT(0);
         double v[kGroup];
-#pragma unroll
-        for (int u = 0; u < kGroup; u++) v[u] = (double)x[u];
+        if constexpr (FP) {
+            weigh(x, v, t0);
+            captures(x, v, t0);
+        } else {
+            captures(x, v, t0);
+            weigh(x, v, t0);
+        }
         push_block<DPL, K, RAW, true, 0, kGroup>(s, v, t0, n_in, o);
     };
     const uint32_t nfull = n_in / kGroup * kGroup;
@@ -286,9 +331,9 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
         }
     } else {
         const uint16_t* col16 = reinterpret_cast<const uint16_t*>(imm) + gid;
-        const uint32_t* col32 = RAW ? nullptr : series_in + gid;
+        const ST* col32 = RAW ? nullptr : series_in + gid;
         // loads of group t0 (frame indices clamped into range; the tail zeroes them)
-        auto load = [&](uint32_t (&r)[kGroup], uint32_t t0) {
+        auto load = [&](XT (&r)[kGroup], uint32_t t0) {
 #pragma unroll
             for (int u = 0; u < kGroup; u++) {
                 const uint32_t t = min(t0 + (uint32_t)u, n_in - 1);
@@ -297,7 +342,7 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
                 else r[u] = col32[(int64_t)t * npix];
             }
         };
-        auto values = [&](uint32_t (&x)[kGroup], const uint32_t (&r)[kGroup], uint32_t t0) {
+        auto values = [&](XT (&x)[kGroup], const XT (&r)[kGroup], uint32_t t0) {
             if constexpr (RAW) {
                 if (avg == 1) {
 #pragma unroll
@@ -320,30 +365,30 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
                 for (int u = 0; u < kGroup; u++) x[u] = r[u];
             }
         };
-        uint32_t rcur[kGroup];
+        XT rcur[kGroup];
         if (RAW && avg != 1) {
 #pragma unroll
-            for (int u = 0; u < kGroup; u++) rcur[u] = 0u;
+            for (int u = 0; u < kGroup; u++) rcur[u] = XT(0);
         } else {
             load(rcur, 0);
         }
         for (uint32_t t0 = 0; t0 < nfull; t0 += kGroup) {
             // next group's loads are issued before this group is processed
-            uint32_t rnext[kGroup];
+            XT rnext[kGroup];
             if (RAW && avg != 1) {
 #pragma unroll
-                for (int u = 0; u < kGroup; u++) rnext[u] = 0u;
+                for (int u = 0; u < kGroup; u++) rnext[u] = XT(0);
             } else {
                 load(rnext, t0 + kGroup);
             }
-            uint32_t x[kGroup];
+            XT x[kGroup];
             values(x, rcur, t0);
             group(x, t0);
 #pragma unroll
             for (int u = 0; u < kGroup; u++) rcur[u] = rnext[u];
         }
         if (nfull < n_in) {
-            uint32_t x[kGroup];
+            XT x[kGroup];
             values(x, rcur, nfull);
             tail_group(x, nfull);
         }
@@ -358,7 +403,7 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
             if (b.y == 0) IP[rowT + b.z] = 0;
             else if (b.y == 1) IF[rowT + b.z] = 0;
         }
-        if (S > kExactF64Total) redo_list[atomicAdd(redo_count, 1u)] = (int32_t)gid;
+        if (!FP && S > kExactF64Total) redo_list[atomicAdd(redo_count, 1u)] = (int32_t)gid;
     }
 #pragma unroll
     for (int l = 0; l < K; l++) {
@@ -368,7 +413,9 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
 #pragma unroll
             for (int d = 1; d <= 2 * DPL; d++) {
                 const int rel = d - lp.lo;
-                if (d >= dlo && rel >= 0 && rel < lp.count) G2[rowT + lp.first + rel] = (int64_t)s.acc[l][d - 1];
+                if (d >= dlo && rel >= 0 && rel < lp.count)
+                    G2[rowT + lp.first + rel] =
+                        FP ? (int64_t)__double_as_longlong(s.acc[l][d - 1]) : (int64_t)s.acc[l][d - 1];
             }
         }
     }
@@ -516,17 +563,23 @@ size_t dense_series_elems(const xpcs_mt_plan* p)
 {
     const HostTables& h = p->h;
     const size_t K = (size_t)dense_levels(p);
-    return h.levels.size() > K ? (size_t)h.levels[K].lf * (size_t)h.W * (size_t)h.H : 1;
+    // 32-bit words: u32 level values, or FP64 frame-weighted values (2 words)
+    const size_t words = p->fsnorm ? 2 : 1;
+    return h.levels.size() > K ? words * (size_t)h.levels[K].lf * (size_t)h.W * (size_t)h.H : 1;
 }
 
-template <int DPL, int K>
+template <int DPL, int K, bool FP>
 static void run_passes(xpcs_mt_plan* p, hipStream_t s)
 {
+    using ST = std::conditional_t<FP, double, uint32_t>;
     const HostTables& h = p->h;
     const int nlev = (int)h.levels.size();
     const uint32_t npd = (uint32_t)(h.W * h.H);
     const unsigned grid = (npd + 255) / 256;
     uint32_t* redo_count = p->d_counters + 3;
+    ST* const sa = reinterpret_cast<ST*>(p->d_series_a);
+    ST* const sb = reinterpret_cast<ST*>(p->d_series_b);
+    const double* fw = FP ? p->d_fw : nullptr;
     int L0 = 0;
     uint32_t n_in = (uint32_t)h.F;
     int pass = 0;
@@ -534,37 +587,41 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
         const int k = std::min(K, nlev - L0);
         const bool more = L0 + k < nlev;
         const uint32_t lf_out = more ? (uint32_t)h.levels[L0 + k].lf : 0u;
-        uint32_t* out = more ? (pass % 2 == 0 ? p->d_series_a : p->d_series_b) : nullptr;
-        const uint32_t* in = pass == 0 ? nullptr : (pass % 2 == 1 ? p->d_series_a : p->d_series_b);
+        ST* out = more ? (pass % 2 == 0 ? sa : sb) : nullptr;
+        const ST* in = pass == 0 ? nullptr : (pass % 2 == 1 ? sa : sb);
         if (pass == 0) {
-            timer_begin(p, s, "k_mt_dense_raw");
+            timer_begin(p, s, FP ? "k_mt_dense_raw_f64" : "k_mt_dense_raw");
             if (p->tile_off0 >= 0)
-                k_mt_dense_f64<DPL, K, kRawTiles><<<grid, 256, 0, s>>>(
+                k_mt_dense_f64<DPL, K, kRawTiles, FP><<<grid, 256, 0, s>>>(
                     p->d_imm, p->d_sel_off, p->tile_off0, p->tile_step, h.avg, p->d_pix2c, npd, p->d_dark_sum,
                     p->d_thr, (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T,
-                    p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
+                    p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw);
             else
-                k_mt_dense_f64<DPL, K, kRawLanes><<<grid, 256, 0, s>>>(
+                k_mt_dense_f64<DPL, K, kRawLanes, FP><<<grid, 256, 0, s>>>(
                     p->d_imm, p->d_sel_off, 0, 0, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr,
                     (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2,
-                    p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list);
+                    p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw);
             timer_end(p, s);
-            timer_begin(p, s, "k_mt_dense_exact");
-            k_mt_dense_exact<DPL, K, true><<<256, 256, 0, s>>>(
-                p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
-                nullptr, n_in, p->d_levels, L0, k, h.T, p->d_G2, redo_count, p->d_long_list);
-            timer_end(p, s);
+            if constexpr (!FP) {
+                timer_begin(p, s, "k_mt_dense_exact");
+                k_mt_dense_exact<DPL, K, true><<<256, 256, 0, s>>>(
+                    p->d_imm, p->d_sel_off, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
+                    nullptr, n_in, p->d_levels, L0, k, h.T, p->d_G2, redo_count, p->d_long_list);
+                timer_end(p, s);
+            }
         } else {
-            timer_begin(p, s, "k_mt_dense_series");
-            k_mt_dense_f64<DPL, K, kSeries><<<grid, 256, 0, s>>>(
+            timer_begin(p, s, FP ? "k_mt_dense_series_f64" : "k_mt_dense_series");
+            k_mt_dense_f64<DPL, K, kSeries, FP><<<grid, 256, 0, s>>>(
                 nullptr, nullptr, 0, 0, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k,
-                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr);
+                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr, fw);
             timer_end(p, s);
-            timer_begin(p, s, "k_mt_dense_exact");
-            k_mt_dense_exact<DPL, K, false><<<256, 256, 0, s>>>(
-                nullptr, nullptr, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k, h.T,
-                p->d_G2, redo_count, p->d_long_list);
-            timer_end(p, s);
+            if constexpr (!FP) {
+                timer_begin(p, s, "k_mt_dense_exact");
+                k_mt_dense_exact<DPL, K, false><<<256, 256, 0, s>>>(
+                    nullptr, nullptr, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k, h.T,
+                    p->d_G2, redo_count, p->d_long_list);
+                timer_end(p, s);
+            }
         }
         n_in = lf_out;
         L0 += k;
@@ -572,17 +629,24 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
     }
 }
 
-void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s)
+template <bool FP>
+static void launch_dense_passes(xpcs_mt_plan* p, hipStream_t s)
 {
     const int k = dense_levels(p);
     if (p->h.dpl == 4) {
-        if (k == 3) run_passes<4, 3>(p, s);
-        else if (k == 5) run_passes<4, 5>(p, s);
-        else run_passes<4, 4>(p, s);
+        if (k == 3) run_passes<4, 3, FP>(p, s);
+        else if (k == 5) run_passes<4, 5, FP>(p, s);
+        else run_passes<4, 4, FP>(p, s);
     } else {
-        if (k == 2) run_passes<8, 2>(p, s);
-        else run_passes<8, 3>(p, s);
+        if (k == 2) run_passes<8, 2, FP>(p, s);
+        else run_passes<8, 3, FP>(p, s);
     }
+}
+
+void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s, bool fsnorm)
+{
+    if (fsnorm) launch_dense_passes<true>(p, s);
+    else launch_dense_passes<false>(p, s);
 }
 
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s)

@@ -495,6 +495,11 @@ void dense_tiling(xpcs_mt_plan* p, const std::vector<int64_t>& so, const uint8_t
     p->tile_step = 0;
     const HostTables& h = p->h;
     const int64_t npd = (int64_t)h.W * h.H;
+    // 16-byte loads of 8 pixels in the frame-sum kernel: every selected frame
+    // 16-byte aligned and the pixel count a multiple of 8
+    p->side_vec = npd % 8 == 0;
+    for (int64_t o : so)
+        if (((uintptr_t)base + (uintptr_t)o) % 16 != 0) p->side_vec = false;
     if (h.avg != 1 || so.empty() || npd % 8 != 0) return;
     const int64_t step = so.size() > 1 ? so[1] - so[0] : 16;
     if (step <= 0 || step % 16 != 0 || ((uintptr_t)base + (uintptr_t)so[0]) % 16 != 0) return;

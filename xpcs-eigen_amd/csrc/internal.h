@@ -142,6 +142,7 @@ struct xpcs_mt_plan {
     uint32_t* d_mapped = nullptr;        // bit per detector pixel: dqmap, sqmap >= 1
     int64_t tile_off0 = -1;
     int64_t tile_step = 0;
+    bool side_vec = false;   // every selected dense frame 16-byte aligned, pixels % 8 == 0
     int64_t nsel = 0;                    // selected raw frames
     int64_t* d_sel_off = nullptr;        // payload byte offset per selected raw frame
     uint32_t* d_sel_dlen = nullptr;

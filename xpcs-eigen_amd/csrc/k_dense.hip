@@ -511,31 +511,165 @@ k_mt_dense_exact(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
 }
 
 // Side statistics of the dense path straight from the raw frames: per
-// (output frame, static bin) sums of the thresholded values.
+// (output frame, static bin) sums of the thresholded, flat-field weighted
+// values (the per-frame partition sums of dense_filter.cpp / sparse_filter.cpp
+// :134-191). A lane owns 8 consecutive detector pixels (one 16-byte load per
+// raw frame when VEC) and keeps their bin, dark, threshold and weight in
+// registers for a chunk of FC frames. Pixels of equal bin are summed inside the
+// lane (the runs before the lane's last bin change go to LDS directly), the
+// last runs across lanes by a segmented scan whose segments (equal bins of
+// neighbouring lanes) are fixed for the whole kernel; segment tails add into
+// the block's LDS table [FC][S], which goes to fs with one global atomic per
+// nonzero entry. Values in 32-bit arithmetic: the caller guarantees
+// 65535 * nscale < 2^31.
+constexpr int kSidePix = 8;
+constexpr int kSideLdsDoubles = 4096;   // LDS table of one block (32 KB)
+constexpr int kSideRing = 3;            // raw frames loaded ahead (VEC, avg 1)
+
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)u);
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <bool VEC>
 __global__ void __launch_bounds__(256)
 k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg, int32_t F,
                  const int32_t* __restrict__ pix2c, uint32_t npix_det, const int64_t* __restrict__ dark_sum,
-                 const int64_t* __restrict__ thr, int64_t nscale, const int32_t* __restrict__ sbin_of_c,
-                 const double* __restrict__ weight, int32_t S, double* __restrict__ fs)
+                 const int64_t* __restrict__ thr, int32_t nscale, const int32_t* __restrict__ sbin_of_c,
+                 const double* __restrict__ weight, int32_t S, int32_t FC, double* __restrict__ fs)
 {
-    __shared__ double s_acc[4096];
-    for (int f = blockIdx.x; f < F; f += gridDim.x) {
-        for (int b = threadIdx.x; b < S; b += blockDim.x) s_acc[b] = 0.0;
+    extern __shared__ double s_acc[];   // [FC][S]
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t j0 = g * kSidePix;
+    const bool live = j0 < npix_det;    // VEC: npix_det % 8 == 0
+    int32_t bin[kSidePix], nds[kSidePix], th[kSidePix];
+    double w[kSidePix];
+#pragma unroll
+    for (int k = 0; k < kSidePix; k++) {
+        const uint32_t j = j0 + (uint32_t)k;
+        const int32_t c = j < npix_det ? pix2c[j] : -1;
+        if (c >= 0) {
+            bin[k] = sbin_of_c[c] - 1;
+            nds[k] = dark_sum ? -(int32_t)dark_sum[c] : 0;
+            const int64_t t = thr ? thr[c] : 0;
+            th[k] = t < 0 ? 0 : (t > INT32_MAX ? INT32_MAX : (int32_t)t);
+            w[k] = weight ? weight[c] : 1.0;
+        } else {
+            bin[k] = -1;
+            nds[k] = 0;
+            th[k] = INT32_MAX;   // no value passes
+            w[k] = 0.0;
+        }
+    }
+    uint32_t brk = 0;   // bit k: pixel k starts a new run inside the lane
+#pragma unroll
+    for (int k = 1; k < kSidePix; k++)
+        if (bin[k] != bin[k - 1]) brk |= 1u << k;
+    // segments of equal last-run bins across the lanes
+    const int32_t lb = bin[kSidePix - 1];
+    const int32_t lb_prev = (int32_t)prev_lane_u32((uint32_t)lb, 0xFFFFFFFEu);
+    const int32_t lb_next = (int32_t)next_lane_u32((uint32_t)lb, 0xFFFFFFFEu);
+    const uint32_t start = wave_incl_max_u32(lane == 0 || lb != lb_prev ? lane : 0u);
+    const bool tail = lane == 63 || lb != lb_next;
+    const bool p1 = lane >= start + 1, p2 = lane >= start + 2, p4 = lane >= start + 4, p8 = lane >= start + 8;
+    const bool pb15 = start < (lane & 0x30u);   // rows 1, 3 add lane 15 / 47 of the row before
+    const bool pb31 = start < 32u;              // rows 2, 3 add lane 31
+
+    const int32_t nchunks = (F + FC - 1) / FC;
+    for (int32_t ch = blockIdx.y; ch < nchunks; ch += gridDim.y) {
+        const int32_t f0 = ch * FC;
+        const int32_t nf = min(FC, F - f0);
+        for (int i = threadIdx.x; i < FC * S; i += 256) s_acc[i] = 0.0;
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < npix_det; j += blockDim.x) {
-            const int32_t c = pix2c[j];
-            if (c < 0) continue;
-            uint32_t acc = 0;
-            for (int a = 0; a < avg; a++) {
-                const uint16_t* fr = reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)f * avg + a]);
-                const int64_t y = nscale * (int64_t)fr[j] - (dark_sum ? dark_sum[c] : 0);
-                if (y > (thr ? thr[c] : 0) && y > 0) acc += (uint32_t)y;
+        auto frame = [&](const uint32_t (&x)[kSidePix], int32_t fl) {
+            double* row = s_acc + (int64_t)fl * S;
+            double run = 0.0;
+#pragma unroll
+            for (int k = 0; k < kSidePix; k++) {
+                if (k > 0 && ((brk >> k) & 1u)) {
+                    if (bin[k - 1] >= 0 && run != 0.0) atomicAdd(&row[bin[k - 1]], run);
+                    run = 0.0;
+                }
+                run += (double)x[k] * w[k];
             }
-            if (acc) atomicAdd(&s_acc[sbin_of_c[c] - 1], (double)acc * (weight ? weight[c] : 1.0));
+            double u;
+            u = dpp_f64<kDppRowShr + 1, 0xF, true>(run);
+            run += p1 ? u : 0.0;
+            u = dpp_f64<kDppRowShr + 2, 0xF, true>(run);
+            run += p2 ? u : 0.0;
+            u = dpp_f64<kDppRowShr + 4, 0xF, true>(run);
+            run += p4 ? u : 0.0;
+            u = dpp_f64<kDppRowShr + 8, 0xF, true>(run);
+            run += p8 ? u : 0.0;
+            u = dpp_f64<kDppBcast15, 0xA, false>(run);
+            run += pb15 ? u : 0.0;
+            u = dpp_f64<kDppBcast31, 0xC, false>(run);
+            run += pb31 ? u : 0.0;
+            if (tail && lb >= 0 && run != 0.0) atomicAdd(&row[lb], run);
+        };
+        auto decode = [&](const u32x4& r, uint32_t (&x)[kSidePix]) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                x[2 * q] = dense_value(r[q] & 0xFFFFu, nscale, nds[2 * q], th[2 * q]);
+                x[2 * q + 1] = dense_value(r[q] >> 16, nscale, nds[2 * q + 1], th[2 * q + 1]);
+            }
+        };
+        if constexpr (VEC) {
+            if (avg == 1) {
+                auto issue = [&](u32x4& r, int32_t fl) {
+                    const int32_t f = f0 + min(fl, nf - 1);
+                    r = live ? *reinterpret_cast<const u32x4*>(imm + sel_off[f] + 2 * (int64_t)j0) : u32x4{0, 0, 0, 0};
+                };
+                u32x4 ring[kSideRing];
+#pragma unroll
+                for (int d = 0; d < kSideRing; d++) issue(ring[d], d);
+                for (int32_t fb = 0; fb < nf; fb += kSideRing) {
+#pragma unroll
+                    for (int d = 0; d < kSideRing; d++) {
+                        if (fb + d < nf) {
+                            uint32_t x[kSidePix];
+                            decode(ring[d], x);
+                            issue(ring[d], fb + d + kSideRing);
+                            frame(x, fb + d);
+                        }
+                    }
+                }
+            } else {
+                for (int32_t fl = 0; fl < nf; fl++) {
+                    uint32_t x[kSidePix] = {};
+                    for (int a = 0; a < avg; a++) {
+                        const u32x4 r = live ? *reinterpret_cast<const u32x4*>(
+                                                   imm + sel_off[(int64_t)(f0 + fl) * avg + a] + 2 * (int64_t)j0)
+                                             : u32x4{0, 0, 0, 0};
+                        uint32_t y[kSidePix];
+                        decode(r, y);
+#pragma unroll
+                        for (int k = 0; k < kSidePix; k++) x[k] += y[k];
+                    }
+                    frame(x, fl);
+                }
+            }
+        } else {
+            for (int32_t fl = 0; fl < nf; fl++) {
+                uint32_t x[kSidePix] = {};
+                for (int a = 0; a < avg; a++) {
+                    const uint16_t* fr =
+                        reinterpret_cast<const uint16_t*>(imm + sel_off[(int64_t)(f0 + fl) * avg + a]);
+#pragma unroll
+                    for (int k = 0; k < kSidePix; k++)
+                        if (j0 + (uint32_t)k < npix_det) x[k] += dense_value(fr[j0 + k], nscale, nds[k], th[k]);
+                }
+                frame(x, fl);
+            }
         }
         __syncthreads();
-        for (int b = threadIdx.x; b < S; b += blockDim.x)
-            if (s_acc[b] != 0.0) atomicAdd(&fs[(int64_t)f * S + b], s_acc[b]);
+        for (int i = threadIdx.x; i < nf * S; i += 256)
+            if (s_acc[i] != 0.0) atomicAdd(&fs[(int64_t)f0 * S + i], s_acc[i]);
         __syncthreads();
     }
 }
@@ -652,10 +786,21 @@ void launch_mt_dense(xpcs_mt_plan* p, hipStream_t s, bool fsnorm)
 void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s)
 {
     const HostTables& h = p->h;
+    const uint32_t npd = (uint32_t)(h.W * h.H);
+    const uint32_t groups = (npd + kSidePix - 1) / kSidePix;
+    const int32_t fc = std::max(1, std::min(64, kSideLdsDoubles / std::max(h.ns, 1)));
+    const int32_t nchunks = (h.F + fc - 1) / fc;
+    const dim3 grid((groups + 255) / 256, (unsigned)std::min(nchunks, 65535));
+    const size_t lds = (size_t)fc * (size_t)h.ns * sizeof(double);
     timer_begin(p, s, "k_side");
-    k_side_dense_raw<<<h.F < 65536 ? h.F : 65536, 256, 0, s>>>(
-        p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, (uint32_t)(h.W * h.H), p->d_dark_sum, p->d_thr,
-        p->nscale, p->d_sbin_of_c, p->d_weight, h.ns, p->d_fs);
+    if (p->side_vec)
+        k_side_dense_raw<true><<<grid, 256, lds, s>>>(p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, npd,
+                                                      p->d_dark_sum, p->d_thr, (int32_t)p->nscale, p->d_sbin_of_c,
+                                                      p->d_weight, h.ns, fc, p->d_fs);
+    else
+        k_side_dense_raw<false><<<grid, 256, lds, s>>>(p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, npd,
+                                                       p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
+                                                       p->d_sbin_of_c, p->d_weight, h.ns, fc, p->d_fs);
     timer_end(p, s);
 }
 

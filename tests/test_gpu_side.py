@@ -1,12 +1,13 @@
-"""Side statistics of dense frames on the device: the per-(frame, static bin)
-sums of the thresholded, flat-field weighted values (the partition sums behind
-frameSum and partition-mean-partial, sparse_filter.cpp:134-191 and the dense
-filter's frame loop) against sums over the oracle's dense transpose of the same
-frames. The values are integers (times a flat-field weight), so the sums agree
+"""Side statistics on the device: the per-(frame, static bin) sums of the
+thresholded, flat-field weighted values (the partition sums behind frameSum and
+partition-mean-partial, sparse_filter.cpp:134-191 and the dense filter's frame
+loop) against sums over the oracle's transpose of the same frames, dense and
+sparse. The values are integers (times a flat-field weight), so the sums agree
 to float32 rounding of the reported means."""
 import numpy as np
 import pytest
 
+import cases
 import synth
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +29,7 @@ def oracle_sums(orc, imm, dq, sq, F, **kw):
 
 
 def check_side(xmt, orc, W, H, F, dq, sq, imm, darks=0, avg=1, stride=1, frame_from=0, flatfield=None):
-    print(f"dense side W={W} H={H} F={F} darks={darks} avg={avg} S={int(np.max(sq))}", flush=True)
+    print(f"side W={W} H={H} F={F} darks={darks} avg={avg} S={int(np.max(sq))}", flush=True)
     kw = dict(darks=darks, avg=avg, stride=stride, frame_from=frame_from, flatfield=flatfield)
     fs, qm = oracle_sums(orc, imm, dq, sq, F, **kw)
     plan = xmt.Plan(W, H, F, 4, dq, sq, stride=stride, average=avg, frame_from=frame_from, dark_frames=darks,
@@ -96,4 +97,29 @@ def test_dense_uniform_bins_long_segments(xmt, orc):
     sq[:, W // 2:] = 2
     dq = np.ones((H, W), np.int32)
     imm = synth.imm_bytes_dense(synth.ou_counts(W, H, F, 5.0, 10.0, seed=708), W, H)
+    check_side(xmt, orc, W, H, F, dq, sq, imm)
+
+
+def test_sparse_events(xmt, orc):
+    # compressed frames: events in raster order, one workgroup per frame
+    c = cases.sparse_case(W=48, H=40, F=400, mu=0.3, tau_c=8.0, seed=709, S=12, Q=4)
+    check_side(xmt, orc, 48, 40, 400, c["dq"], c["sq"], c["imm"])
+
+
+def test_sparse_events_flatfield_average(xmt, orc):
+    c = cases.sparse_case(W=32, H=32, F=800, mu=0.4, tau_c=8.0, seed=710, S=9, Q=3)
+    ff = np.random.default_rng(711).uniform(0.5, 1.5, (32, 32))
+    check_side(xmt, orc, 32, 32, 190, c["dq"], c["sq"], c["imm"], avg=2, stride=2, frame_from=5, flatfield=ff)
+
+
+def test_sparse_events_many_bins(xmt, orc):
+    # 300 static bins (a 2-byte per-pixel bin table), scattered over the
+    # detector so that neighbouring events rarely share a bin
+    W, H = 48, 32
+    F = 300
+    rng = np.random.default_rng(712)
+    bins = np.concatenate([np.repeat(np.arange(1, 301), 4), np.zeros(W * H - 1200, np.int64)])
+    sq = rng.permutation(bins).reshape(H, W).astype(np.int32)
+    dq = np.where(sq > 0, 1 + (sq - 1) % 6, 0).astype(np.int32)
+    imm = synth.imm_bytes_sparse(synth.ou_counts(W, H, F, 0.3, 8.0, seed=713), W, H)
     check_side(xmt, orc, W, H, F, dq, sq, imm)

@@ -217,6 +217,24 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
             return fail(XPCS_ERR_UNSUPPORTED, "plan: side statistics need S <= 4096 and F*S <= 2^26");
         }
         TRY(dalloc(&p->d_fs, (size_t)h.F * h.ns));
+        // static bin (1..S, 0 unmapped) per raster pixel, one byte when S < 256:
+        // one small gather per sparse event instead of the pixel's row id and
+        // then its bin
+        if (h.ns < 256) {
+            std::vector<uint8_t> sbp((size_t)h.W * h.H, 0);
+            for (int c = 0; c < h.P; c++) sbp[(size_t)h.order[(size_t)c]] = (uint8_t)h.sbin_of_c[(size_t)c];
+            uint8_t* d = nullptr;
+            TRY(upload(&d, sbp, s));
+            p->d_sbin_pix = d;
+            p->sbin_pix_bytes = 1;
+        } else {
+            std::vector<uint16_t> sbp((size_t)h.W * h.H, 0);
+            for (int c = 0; c < h.P; c++) sbp[(size_t)h.order[(size_t)c]] = (uint16_t)h.sbin_of_c[(size_t)c];
+            uint16_t* d = nullptr;
+            TRY(upload(&d, sbp, s));
+            p->d_sbin_pix = d;
+            p->sbin_pix_bytes = 2;
+        }
     }
     if (p->fsnorm) TRY(dalloc(&p->d_fw, (size_t)h.F));
     // dense-row scratch (allocated by the first execute that needs it): one row
@@ -246,7 +264,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_tau_bnd); dfree(p->d_frame_rank); dfree(p->d_chunks);
     dfree(p->d_group_chunk_first); dfree(p->d_group_q); dfree(p->d_group_size);
     dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
-    dfree(p->d_sbin_of_c); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
+    dfree(p->d_sbin_of_c); dfree(p->d_sbin_pix); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
     dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
     dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total); dfree(p->d_fw);

@@ -175,4 +175,50 @@ __device__ __forceinline__ uint32_t read_lane_u32(uint32_t x, int l)
     return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
 }
 
+// DPP move of an FP64 value (two 32-bit halves); lanes the pattern does not
+// write (row mask) or that read outside their row (BOUND_ZERO) get 0.0.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)u);
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Segmented inclusive FP64 sum over the 64 lanes: lane i gets the sum of x over
+// lanes start..i of its segment, `start` being the segment's first lane (as
+// from a running maximum of segment heads). Row-shift steps 1, 2, 4, 8 within
+// each 16-lane row, then the row-15 and row-31 broadcasts; a lane takes a
+// partner's partial sum only while that partner lies in its segment.
+__device__ __forceinline__ double wave_seg_scan_f64(double x, uint32_t start)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    double u;
+    u = dpp_f64<kDppRowShr + 1, 0xF, true>(x);
+    x += lane >= start + 1 ? u : 0.0;
+    u = dpp_f64<kDppRowShr + 2, 0xF, true>(x);
+    x += lane >= start + 2 ? u : 0.0;
+    u = dpp_f64<kDppRowShr + 4, 0xF, true>(x);
+    x += lane >= start + 4 ? u : 0.0;
+    u = dpp_f64<kDppRowShr + 8, 0xF, true>(x);
+    x += lane >= start + 8 ? u : 0.0;
+    u = dpp_f64<kDppBcast15, 0xA, false>(x);   // rows 1, 3 from lanes 15, 47
+    x += start < (lane & 0x30u) ? u : 0.0;
+    u = dpp_f64<kDppBcast31, 0xC, false>(x);   // rows 2, 3 from lane 31
+    x += start < 32u ? u : 0.0;
+    return x;
+}
+
+// Sum of x over each run of equal `key` in neighbouring lanes; `tail` marks the
+// run's last lane, which holds the run's total. All 64 lanes must be active.
+__device__ __forceinline__ double wave_run_sum_f64(double x, uint32_t key, bool& tail)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t kp = prev_lane_u32(key, ~key);
+    const uint32_t kn = next_lane_u32(key, ~key);
+    tail = kn != key;
+    return wave_seg_scan_f64(x, wave_incl_max_u32(kp != key ? lane : 0u));
+}
+
 }  // namespace xpcs

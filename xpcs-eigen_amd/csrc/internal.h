@@ -117,6 +117,8 @@ struct xpcs_mt_plan {
     double* d_weight = nullptr;
     int32_t* d_c2pix = nullptr;
     int32_t* d_sbin_of_c = nullptr;
+    void* d_sbin_pix = nullptr;          // static bin per raster pixel (0: unmapped), side statistics only
+    int sbin_pix_bytes = 0;              // its element size: 1 (S < 256) or 2
 
     // ingest state: 0 none, 1 sparse IMM, 2 dense IMM, 3 rows
     int mode = 0;

@@ -526,15 +526,6 @@ constexpr int kSidePix = 8;
 constexpr int kSideLdsDoubles = 4096;   // LDS table of one block (32 KB)
 constexpr int kSideRing = 3;            // raw frames loaded ahead (VEC, avg 1)
 
-template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
-__device__ __forceinline__ double dpp_f64(double x)
-{
-    const uint64_t u = (uint64_t)__double_as_longlong(x);
-    const uint32_t lo = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)u);
-    const uint32_t hi = dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(0u, (uint32_t)(u >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 template <bool VEC>
 __global__ void __launch_bounds__(256)
 k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg, int32_t F,
@@ -576,9 +567,6 @@ k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
     const int32_t lb_next = (int32_t)next_lane_u32((uint32_t)lb, 0xFFFFFFFEu);
     const uint32_t start = wave_incl_max_u32(lane == 0 || lb != lb_prev ? lane : 0u);
     const bool tail = lane == 63 || lb != lb_next;
-    const bool p1 = lane >= start + 1, p2 = lane >= start + 2, p4 = lane >= start + 4, p8 = lane >= start + 8;
-    const bool pb15 = start < (lane & 0x30u);   // rows 1, 3 add lane 15 / 47 of the row before
-    const bool pb31 = start < 32u;              // rows 2, 3 add lane 31
 
     const int32_t nchunks = (F + FC - 1) / FC;
     for (int32_t ch = blockIdx.y; ch < nchunks; ch += gridDim.y) {
@@ -597,19 +585,7 @@ k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
                 }
                 run += (double)x[k] * w[k];
             }
-            double u;
-            u = dpp_f64<kDppRowShr + 1, 0xF, true>(run);
-            run += p1 ? u : 0.0;
-            u = dpp_f64<kDppRowShr + 2, 0xF, true>(run);
-            run += p2 ? u : 0.0;
-            u = dpp_f64<kDppRowShr + 4, 0xF, true>(run);
-            run += p4 ? u : 0.0;
-            u = dpp_f64<kDppRowShr + 8, 0xF, true>(run);
-            run += p8 ? u : 0.0;
-            u = dpp_f64<kDppBcast15, 0xA, false>(run);
-            run += pb15 ? u : 0.0;
-            u = dpp_f64<kDppBcast31, 0xC, false>(run);
-            run += pb31 ? u : 0.0;
+            run = wave_seg_scan_f64(run, start);
             if (tail && lb >= 0 && run != 0.0) atomicAdd(&row[lb], run);
         };
         auto decode = [&](const u32x4& r, uint32_t (&x)[kSidePix]) {

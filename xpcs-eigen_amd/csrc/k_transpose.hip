@@ -511,27 +511,55 @@ k_dense_transpose(const uint8_t* __restrict__ imm, const int64_t* __restrict__ s
 // fs[f][s-1] = sum over masked pixels with sqmap == s of count * flatfield
 // (SparseFilter/DenseFilter accumulate partitions_mean_ / partial means and the
 // frame sums from the same values, src/xpcs/filter/sparse_filter.cpp:168-191).
-constexpr int kSideMaxBins = 4096;
+constexpr int kSideEv = 4;   // events per thread in flight (decode, pixel id, bin)
 
+// One workgroup per selected raw frame; an LDS table of the S bins (dynamic,
+// S doubles). Each thread keeps kSideEv events in flight: their decodes, then
+// their bins (a 1- or 2-byte per-pixel table; with a flat field also the row
+// id for the weight) are loaded before the sums. The events of a
+// frame come in raster order, so neighbouring lanes mostly share a bin: runs of
+// equal bins are summed across the wave first and only each run's last lane
+// adds into LDS (one lane per address instead of up to 64).
+template <bool WEIGHT, typename BT>
 __global__ void __launch_bounds__(256)
 k_side_sparse(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
               const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
               EvFormat fm, const int32_t* __restrict__ pix2c, uint32_t npix_det,
-              const int32_t* __restrict__ sbin_of_c,
+              const BT* __restrict__ sbin_pix,
               const double* __restrict__ weight, int32_t S, double* __restrict__ fs)
 {
-    __shared__ double s_acc[kSideMaxBins];
+    extern __shared__ double s_acc[];   // [S]
     for (int64_t k = blockIdx.x; k < nsel; k += gridDim.x) {
         for (int b = threadIdx.x; b < S; b += blockDim.x) s_acc[b] = 0.0;
         __syncthreads();
         const uint8_t* base = imm + sel_off[k];
         const uint32_t dlen = sel_dlen[k];
         const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-        for (uint32_t j = threadIdx.x; j < dlen; j += blockDim.x) {
-            int32_t c;
-            uint32_t v;
-            if (!tr_event(base, dlen, al4, j, fm, pix2c, npix_det, nullptr, c, v)) continue;
-            atomicAdd(&s_acc[sbin_of_c[c] - 1], (double)v * (weight ? weight[c] : 1.0));
+        // the loop bound is uniform: every lane takes part in the wave sums
+        for (uint32_t jb = 0; jb < dlen; jb += blockDim.x * kSideEv) {
+            uint32_t idx[kSideEv], v[kSideEv];
+            int32_t c[kSideEv], b[kSideEv];
+#pragma unroll
+            for (int e = 0; e < kSideEv; e++) {
+                const uint32_t j = jb + (uint32_t)e * blockDim.x + threadIdx.x;
+                idx[e] = npix_det;
+                v[e] = 0;
+                if (j < dlen) tr_decode(base, dlen, al4, j, fm, idx[e], v[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < kSideEv; e++) {
+                const bool ok = v[e] != 0 && idx[e] < npix_det;
+                b[e] = ok ? (int32_t)sbin_pix[idx[e]] - 1 : -1;
+                if constexpr (WEIGHT) c[e] = ok ? pix2c[idx[e]] : 0;
+            }
+#pragma unroll
+            for (int e = 0; e < kSideEv; e++) {
+                double x = b[e] >= 0 ? (double)v[e] : 0.0;
+                if constexpr (WEIGHT) x *= b[e] >= 0 ? weight[c[e]] : 0.0;
+                bool tail;
+                const double run = wave_run_sum_f64(x, (uint32_t)b[e], tail);
+                if (tail && b[e] >= 0 && run != 0.0) atomicAdd(&s_acc[b[e]], run);
+            }
         }
         __syncthreads();
         double* row = fs + (int64_t)sel_frame[k] * S;
@@ -572,9 +600,24 @@ void launch_side(xpcs_mt_plan* p, hipStream_t s)
     const HostTables& h = p->h;
     timer_begin(p, s, "k_side");
     if (p->mode == 1) {
-        k_side_sparse<<<p->nsel < 65536 ? (int)p->nsel : 65536, 256, 0, s>>>(
-            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, ev_format(p), p->d_pix2c,
-            (uint32_t)((int64_t)h.W * h.H), p->d_sbin_of_c, p->d_weight, h.ns, p->d_fs);
+        const unsigned grid = p->nsel < 65536 ? (unsigned)p->nsel : 65536u;
+        const size_t lds = (size_t)h.ns * sizeof(double);
+        const uint32_t npd = (uint32_t)((int64_t)h.W * h.H);
+        auto go = [&](auto weighted, const auto* bins) {
+            k_side_sparse<decltype(weighted)::value><<<grid, 256, lds, s>>>(
+                p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, ev_format(p), p->d_pix2c, npd,
+                bins, p->d_weight, h.ns, p->d_fs);
+        };
+        const bool one = p->sbin_pix_bytes == 1;
+        const auto* b8 = static_cast<const uint8_t*>(p->d_sbin_pix);
+        const auto* b16 = static_cast<const uint16_t*>(p->d_sbin_pix);
+        if (p->d_weight) {
+            if (one) go(std::true_type{}, b8);
+            else go(std::true_type{}, b16);
+        } else {
+            if (one) go(std::false_type{}, b8);
+            else go(std::false_type{}, b16);
+        }
     } else if (p->mode == 2) {
         const int chunk = 256;
         dim3 grid((unsigned)((h.F + 255) / 256), (unsigned)((h.P + chunk - 1) / chunk));

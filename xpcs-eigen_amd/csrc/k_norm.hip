@@ -3,8 +3,9 @@
 //
 //   k_norm_partial  one workgroup per chunk (<= 256 pixels of one (q, s) group):
 //                   integer sums of G2/IP/IF per tau (flat-field weighted sums in
-//                   FP64 when a flat field is set) and a sequential Welford of
-//                   x_p = N G2_p / (IP_p IF_p) with 0/0 -> 0 over the chunk.
+//                   FP64 when a flat field is set) and a Welford of
+//                   x_p = N G2_p / (IP_p IF_p) with 0/0 -> 0 over each of the
+//                   chunk's two pixel slices, the two merged in order.
 //   k_norm_group    per (group, tau): sums of the group's chunks in order;
 //                   g2s = c_s N sum G2 / (sum IP sum IF)            (FP64)
 //   k_norm_q        per (q, tau): mean of the non-NaN g2s of q; the chunk
@@ -17,81 +18,136 @@
 namespace xpcs {
 
 constexpr int kNormBatch = 8;
+constexpr int kNormBlock = 256;
+constexpr int kNormSlices = 2;                        // pixel slices of a chunk (2: measured best of 1, 2, 4, 8)
+constexpr int kNormLanes = kNormBlock / kNormSlices;  // taus per pass
 
-// FP: the sums are FP64 (frame-sum normalised values) stored in the same
-// 8-byte buffers; they take the weighted (FP64) route with weight 1 when the
-// plan has no flat field.
+// Running state of one (chunk slice, tau): the sums and a Welford of x_p.
+struct NormAcc {
+    int64_t sg = 0, sp = 0, sf = 0;
+    double wg = 0.0, wp = 0.0, wf = 0.0;
+    double n = 0.0, mean = 0.0, m2 = 0.0;
+};
+
+// Pixels [pb, pe) of tau column i in order. FP: the sums are FP64
+// (frame-sum normalised values) stored in the same 8-byte buffers; they take
+// the weighted (FP64) route with weight 1 when the plan has no flat field.
 template <bool FP>
-__global__ void __launch_bounds__(256)
+__device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
+                                                const int64_t* __restrict__ IF, const double* __restrict__ weight,
+                                                int32_t S, int i, double N, int pb, int pe, NormAcc& a)
+{
+    // kNormBatch pixels' loads in flight per thread
+    for (int p0 = pb; p0 < pe; p0 += kNormBatch) {
+        int64_t gg[kNormBatch], aa[kNormBatch], bb[kNormBatch];
+#pragma unroll
+        for (int u = 0; u < kNormBatch; u++) {
+            const int pu = min(p0 + u, pe - 1);
+            gg[u] = G2[(int64_t)pu * S + i];
+            aa[u] = IP[(int64_t)pu * S + i];
+            bb[u] = IF[(int64_t)pu * S + i];
+        }
+        for (int u = 0; u < kNormBatch && p0 + u < pe; u++) {
+            const int p = p0 + u;
+            const int64_t g = gg[u];
+            const int64_t ip = aa[u];
+            const int64_t f = bb[u];
+            double x;
+            if (FP) {
+                const double w = weight ? weight[p] : 1.0;
+                const double gd = __longlong_as_double(g), ad = __longlong_as_double(ip),
+                             bd = __longlong_as_double(f);
+                a.wg += w * w * gd;
+                a.wp += w * ad;
+                a.wf += w * bd;
+                x = (w == 0.0) ? 0.0 : N * gd / (ad * bd);
+            } else if (weight) {
+                const double w = weight[p];
+                a.wg += w * w * (double)g;
+                a.wp += w * (double)ip;
+                a.wf += w * (double)f;
+                x = (w == 0.0) ? 0.0 : N * (double)g / ((double)ip * (double)f);
+            } else {
+                a.sg += g;
+                a.sp += ip;
+                a.sf += f;
+                x = N * (double)g / ((double)ip * (double)f);
+            }
+            if (x != x) x = 0.0;
+            a.n += 1.0;
+            const double d = x - a.mean;
+            a.mean += d / a.n;
+            a.m2 += d * (x - a.mean);
+        }
+    }
+}
+
+// a followed by b: sums added, Welford states merged (Chan et al.).
+__device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b)
+{
+    a.sg += b.sg;
+    a.sp += b.sp;
+    a.sf += b.sf;
+    a.wg += b.wg;
+    a.wp += b.wp;
+    a.wf += b.wf;
+    if (b.n == 0.0) return;
+    if (a.n == 0.0) {
+        a.n = b.n;
+        a.mean = b.mean;
+        a.m2 = b.m2;
+        return;
+    }
+    const double n = a.n + b.n;
+    const double d = b.mean - a.mean;
+    a.mean += d * (b.n / n);
+    a.m2 += b.m2 + d * d * (a.n * b.n / n);
+    a.n = n;
+}
+
+// One workgroup per chunk: its pixels split into kNormSlices consecutive
+// slices, thread (slice, lane) taking tau t0 + tb + lane of slice `slice` for
+// each block tb of kNormLanes taus; the slice states are merged in slice
+// order through LDS. The split does not depend on the tau range, so a range
+// of taus gets the same partials as the full schedule.
+template <bool FP>
+__global__ void __launch_bounds__(kNormBlock)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
                int32_t t0, int32_t t1, int32_t S, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                const int64_t* __restrict__ IF, const double* __restrict__ weight,
                NormPartial* __restrict__ part)
 {
+    __shared__ __align__(16) unsigned char s_raw[kNormBlock * sizeof(NormAcc)];
+    NormAcc* s_acc = reinterpret_cast<NormAcc*>(s_raw);
+    const int lt = (int)threadIdx.x % kNormLanes;
+    const int sl = (int)threadIdx.x / kNormLanes;
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const NormChunk ch = chunks[c];
-        for (int i = t0 + (int)threadIdx.x; i < t1; i += blockDim.x) {
-            const double N = (double)taus[i].N;
-            int64_t sg = 0, sp = 0, sf = 0;
-            double wg = 0.0, wp = 0.0, wf = 0.0;
-            double n = 0.0, mean = 0.0, m2 = 0.0;
-            // kNormBatch pixels' loads in flight per thread
-            for (int p0 = ch.begin; p0 < ch.end; p0 += kNormBatch) {
-              int64_t gg[kNormBatch], aa[kNormBatch], bb[kNormBatch];
+        const int len = ch.end - ch.begin;
+        const int pb = ch.begin + (int)((int64_t)len * sl / kNormSlices);
+        const int pe = ch.begin + (int)((int64_t)len * (sl + 1) / kNormSlices);
+        for (int tb = t0; tb < t1; tb += kNormLanes) {   // uniform trip count
+            const int i = tb + lt;
+            if (i < t1) {
+                NormAcc a;
+                norm_accumulate<FP>(G2, IP, IF, weight, S, i, (double)taus[i].N, pb, pe, a);
+                s_acc[threadIdx.x] = a;
+            }
+            __syncthreads();
+            if (sl == 0 && i < t1) {
+                NormAcc a = s_acc[lt];
 #pragma unroll
-              for (int u = 0; u < kNormBatch; u++) {
-                  const int pu = min(p0 + u, ch.end - 1);
-                  gg[u] = G2[(int64_t)pu * S + i];
-                  aa[u] = IP[(int64_t)pu * S + i];
-                  bb[u] = IF[(int64_t)pu * S + i];
-              }
-              for (int u = 0; u < kNormBatch && p0 + u < ch.end; u++) {
-                const int p = p0 + u;
-                const int64_t g = gg[u];
-                const int64_t a = aa[u];
-                const int64_t b = bb[u];
-                double x;
-                if (FP) {
-                    const double w = weight ? weight[p] : 1.0;
-                    const double gd = __longlong_as_double(g), ad = __longlong_as_double(a),
-                                 bd = __longlong_as_double(b);
-                    wg += w * w * gd;
-                    wp += w * ad;
-                    wf += w * bd;
-                    x = (w == 0.0) ? 0.0 : N * gd / (ad * bd);
-                } else if (weight) {
-                    const double w = weight[p];
-                    wg += w * w * (double)g;
-                    wp += w * (double)a;
-                    wf += w * (double)b;
-                    x = (w == 0.0) ? 0.0 : N * (double)g / ((double)a * (double)b);
-                } else {
-                    sg += g;
-                    sp += a;
-                    sf += b;
-                    x = N * (double)g / ((double)a * (double)b);
-                }
-                if (x != x) x = 0.0;
-                n += 1.0;
-                const double d = x - mean;
-                mean += d / n;
-                m2 += d * (x - mean);
-              }
+                for (int k = 1; k < kNormSlices; k++) norm_merge(a, s_acc[k * kNormLanes + lt]);
+                NormPartial r;
+                r.g2 = (FP || weight) ? __double_as_longlong(a.wg) : a.sg;
+                r.ip = (FP || weight) ? __double_as_longlong(a.wp) : a.sp;
+                r.ifs = (FP || weight) ? __double_as_longlong(a.wf) : a.sf;
+                r.n = a.n;
+                r.mean = a.mean;
+                r.m2 = a.m2;
+                part[c * S + i] = r;
             }
-            NormPartial r;
-            if (FP || weight) {
-                r.g2 = __double_as_longlong(wg);
-                r.ip = __double_as_longlong(wp);
-                r.ifs = __double_as_longlong(wf);
-            } else {
-                r.g2 = sg;
-                r.ip = sp;
-                r.ifs = sf;
-            }
-            r.n = n;
-            r.mean = mean;
-            r.m2 = m2;
-            part[c * S + i] = r;
+            __syncthreads();
         }
     }
 }
@@ -251,11 +307,11 @@ void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     const unsigned nb = nchunks < 65536 ? nchunks : 65536;
     if (nchunks > 0) {
         if (fp)
-            k_norm_partial<true><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2, p->d_IP,
-                                                    p->d_IF, p->d_weight, p->d_part);
+            k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2,
+                                                           p->d_IP, p->d_IF, p->d_weight, p->d_part);
         else
-            k_norm_partial<false><<<nb, 128, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2, p->d_IP,
-                                                     p->d_IF, p->d_weight, p->d_part);
+            k_norm_partial<false><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2,
+                                                            p->d_IP, p->d_IF, p->d_weight, p->d_part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)

@@ -210,6 +210,17 @@ xpcs_status xpcs_imm_index(const void* imm, size_t nbytes, int64_t* payload_off,
                            int32_t* compressed);
 
 /*
+ * Ingest frame-major events from a caller's own decoder (what io::Imm hands a
+ * SparseFilter for a compressed file, src/xpcs/io/imm.cpp:87-99): raw frame k
+ * holds the events frame_off[k] .. frame_off[k+1]-1 of pixel[] (raster index
+ * y*W+x) and count[].  Frame selection as for a compressed IMM file
+ * (frame_from, stride, average; no dark frames); pixels outside the detector
+ * are reported by execute, as for IMM input.
+ */
+xpcs_status xpcs_mt_ingest_events(xpcs_mt_plan* plan, const int32_t* pixel, const uint16_t* count,
+                                  const int64_t* frame_off, int64_t nframes);
+
+/*
  * Ingest pixel-major rows (the SparseData layout of
  * src/xpcs/data_structure/sparse_data.h:57-79 as filled by a Filter):
  * row_off[W*H+1] into frame[] (ascending per row) and count[] (integer

@@ -142,3 +142,47 @@ def test_packed_needs_no_darks(xmt):
     plan = xmt.Plan(W, H, 10, 4, dq, sq, dark_frames=2)
     with pytest.raises(xmt.XpcsError):
         plan.ingest_packed(np.zeros(4, np.uint32), xmt.FMT_UFXC)
+
+
+@pytest.mark.parametrize("stride,avg,frame_from", [(1, 1, 0), (2, 3, 5)])
+def test_events(xmt, orc, stride, avg, frame_from):
+    """Frame-major event arrays: the same results as the compressed IMM file
+    holding those events (with empty frames and a repeated ingest)."""
+    W, H, F = 40, 28, 150
+    dq, sq = maps(W, H)
+    raw = frame_from + F * stride * avg
+    counts = synth.ou_counts(W, H, raw, 0.3, 10.0, seed=410)
+    counts[7] = 0
+    counts[frame_from + 1] = 0
+    imm = synth.imm_bytes_sparse(counts, W, H)
+    nz = [np.flatnonzero(c) for c in counts]
+    pixel = np.concatenate(nz).astype(np.int32)
+    count = np.concatenate([c[i] for c, i in zip(counts, nz)]).astype(np.uint16)
+    frame_off = np.concatenate([[0], np.cumsum([len(i) for i in nz])]).astype(np.int64)
+    print(f"events W={W} H={H} F={F} stride={stride} avg={avg} from={frame_from}", flush=True)
+    plan = xmt.Plan(W, H, F, 4, dq, sq, stride=stride, average=avg, frame_from=frame_from)
+    for _ in range(2):
+        plan.ingest_events(pixel, count, frame_off)
+        plan.execute()
+    check_same(plan, orc.run_exact(imm, dq, sq, F, 4, stride=stride, avg=avg, frame_from=frame_from))
+
+
+def test_events_pixel_out_of_range_is_an_error(xmt):
+    W, H = 8, 8
+    dq, sq = maps(W, H, 2, 1)
+    plan = xmt.Plan(W, H, 4, 2, dq, sq)
+    plan.ingest_events(np.array([3, 64, 5], np.int32), np.array([1, 1, 1], np.uint16),
+                       np.array([0, 1, 2, 3, 3], np.int64))
+    plan.execute()
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.g2()
+    assert e.value.status == 4
+
+
+def test_events_bad_offsets_are_refused(xmt):
+    W, H = 8, 8
+    dq, sq = maps(W, H, 2, 1)
+    plan = xmt.Plan(W, H, 4, 2, dq, sq)
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.ingest_events(np.array([3, 4], np.int32), np.array([1, 1], np.uint16), np.array([0, 2, 1, 2], np.int64))
+    assert e.value.status == 1

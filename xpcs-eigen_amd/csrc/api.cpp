@@ -609,6 +609,73 @@ extern "C" xpcs_status xpcs_mt_ingest_imm(xpcs_mt_plan* p, const void* imm, size
     return XPCS_OK;
 }
 
+// Frame-major events from a caller's own decoder: the selected raw frames are
+// laid out as compressed-IMM payload records (int32 pixel[n], u16 count[n],
+// padded to 4 bytes) and take the sparse path exactly as a compressed IMM
+// file does (src/xpcs/io/imm.cpp:87-99, main.cpp:222-268).
+extern "C" xpcs_status xpcs_mt_ingest_events(xpcs_mt_plan* p, const int32_t* pixel, const uint16_t* count,
+                                             const int64_t* frame_off, int64_t nframes)
+{
+    if (!p || !frame_off) return fail(XPCS_ERR_INVALID_ARG, "ingest_events: NULL argument");
+    if (nframes < 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_events: nframes < 0");
+    if (frame_off[0] != 0) return fail(XPCS_ERR_INVALID_ARG, "ingest_events: frame_off[0] != 0");
+    for (int64_t k = 0; k < nframes; k++) {
+        if (frame_off[k + 1] < frame_off[k])
+            return fail(XPCS_ERR_INVALID_ARG, "ingest_events: frame_off decreases at frame %lld", (long long)k);
+        if (frame_off[k + 1] - frame_off[k] > (int64_t)UINT32_MAX)
+            return fail(XPCS_ERR_RANGE, "ingest_events: frame %lld has more than 2^32-1 events", (long long)k);
+    }
+    if (frame_off[nframes] > 0 && (!pixel || !count))
+        return fail(XPCS_ERR_INVALID_ARG, "ingest_events: NULL pixel/count arrays");
+    HostTables& h = p->h;
+    std::vector<int64_t> sel;
+    std::vector<int32_t> ofr;
+    int64_t first = 0;
+    xpcs_status st = select_frames(h, nframes, true, &first, sel, ofr);
+    if (st) return st;
+    std::vector<int64_t> so(sel.size());
+    std::vector<uint32_t> sd(sel.size());
+    size_t bytes = 0;
+    for (size_t k = 0; k < sel.size(); k++) {
+        const int64_t n = frame_off[sel[k] + 1] - frame_off[sel[k]];
+        so[k] = (int64_t)bytes;
+        sd[k] = (uint32_t)n;
+        bytes += ((size_t)n * 6 + 3) & ~(size_t)3;
+    }
+    std::vector<uint8_t> buf(bytes, 0);
+    for (size_t k = 0; k < sel.size(); k++) {
+        const int64_t e0 = frame_off[sel[k]];
+        const size_t n = sd[k];
+        uint8_t* rec = buf.data() + so[k];
+        if (n) {
+            memcpy(rec, pixel + e0, n * 4);
+            memcpy(rec + n * 4, count + e0, n * 2);
+        }
+    }
+    const size_t pad = 256;
+    if (bytes + pad > p->imm_cap) {
+        st = dalloc(&p->d_imm_owned, bytes + pad);
+        if (st) return st;
+        p->imm_cap = bytes + pad;
+    }
+    st = h2d_staged(p->d_imm_owned, buf.data(), bytes, p->stream);
+    if (st) return st;
+    p->d_imm = p->d_imm_owned;
+    st = finish_selection(p, so, sd, ofr);
+    if (st) return st;
+    p->mode = 1;
+    p->fmt = kFmtImm;
+    p->nscale = 1;
+    p->clock.clear();
+    p->ticks.clear();
+    p->nrows = (int64_t)h.W * h.H;
+    p->d_rowmap = p->d_c2pix;
+    p->executed = false;
+    p->generation++;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return XPCS_OK;
+}
+
 extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* dev_imm, size_t nbytes,
                                                  const int64_t* payload_off, const uint32_t* dlen,
                                                  int64_t raw_frames, int32_t compressed)

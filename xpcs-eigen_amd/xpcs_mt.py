@@ -31,7 +31,7 @@ EXPORTS = [
     "xpcs_version", "xpcs_last_error", "xpcs_level_max", "xpcs_delay_count",
     "xpcs_delays_per_level", "xpcs_mt_plan_create", "xpcs_mt_plan_destroy", "xpcs_mt_plan_info",
     "xpcs_mt_schedule", "xpcs_mt_pixel_order", "xpcs_mt_ingest_imm", "xpcs_mt_ingest_imm_device",
-    "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_execute", "xpcs_mt_synchronize",
+    "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_ingest_events", "xpcs_mt_execute", "xpcs_mt_synchronize",
     "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
     "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
     "xpcs_mt_read_dark", "xpcs_mt_set_tuning", "xpcs_mt_get_tuning", "xpcs_mt_autotune",
@@ -95,6 +95,7 @@ def lib():
             "xpcs_mt_ingest_imm_device": ([vp, vp, C.c_size_t, vp, vp, i64, i32], i32),
             "xpcs_imm_index": ([vp, C.c_size_t, vp, vp, vp, vp, i64, vp, vp], i32),
             "xpcs_mt_ingest_rows": ([vp, vp, vp, vp], i32),
+            "xpcs_mt_ingest_events": ([vp, vp, vp, vp, i64], i32),
             "xpcs_mt_execute": ([vp, vp], i32),
             "xpcs_mt_synchronize": ([vp], i32),
             "xpcs_mt_read_sums_i64": ([vp, vp, vp, vp], i32),
@@ -322,6 +323,18 @@ class Plan:
         if fr.size < ro[-1] or ct.size < ro[-1]:
             raise ValueError(f"frame/count hold fewer than row_off[-1] = {ro[-1]} events")
         check(lib().xpcs_mt_ingest_rows(self._h, _p(ro), _p(fr), _p(ct)))
+
+    def ingest_events(self, pixel, count, frame_off):
+        """Frame-major events: raw frame k holds pixel[frame_off[k]:frame_off[k+1]]
+        (raster indices) with count[...] (u16)."""
+        fo = np.ascontiguousarray(frame_off, np.int64).reshape(-1)
+        px = np.ascontiguousarray(pixel, np.int32).reshape(-1)
+        ct = np.ascontiguousarray(count, np.uint16).reshape(-1)
+        if fo.size < 1:
+            raise ValueError("frame_off needs nframes+1 entries")
+        if px.size < fo[-1] or ct.size < fo[-1]:
+            raise ValueError(f"pixel/count hold fewer than frame_off[-1] = {fo[-1]} events")
+        check(lib().xpcs_mt_ingest_events(self._h, _p(px), _p(ct), _p(fo), fo.size - 1))
 
     def execute(self, stream=None):
         check(lib().xpcs_mt_execute(self._h, None if stream is None else C.c_void_p(stream)))

@@ -230,7 +230,7 @@ constexpr int kTrRowsBlock = 1024;
 constexpr int kTrRowsWaves = kTrRowsBlock / 64;
 constexpr int kTrRowsCap = 15872;     // staged events per range (124 KB of LDS)
 constexpr int kTrRowsTable = 4096;    // per-group count and cursor entries
-constexpr int kTrRowsRegs = 16;       // register-held events per lane
+constexpr int kTrRowsRegs = 48;       // register-held events per lane (2 VGPRs each; 128 in all)
 
 // Index of the first of the n nondecreasing values s[] that is >= x.
 __device__ __forceinline__ int lds_first_ge(const uint32_t* s, int n, uint32_t x)
@@ -344,6 +344,11 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
                     if (st >= lo && st < hi) s_stage[atomicAdd(&cur[p], 1u) - wlo] = y;
                 }
             };
+            // an empty register barrier per held event: without it the
+            // compiler hoists each event's pixel and output word out of this
+            // loop, 2-3 more VGPRs per event, and the held events spill
+#pragma unroll
+            for (int t = 0; t < kTrRowsRegs; t++) asm volatile("" : "+v"(xr[t]));
 #pragma unroll
             for (int t = 0; t < kTrRowsRegs; t++)
                 if (xr[t] != ~0ull) place(xr[t]);

@@ -105,11 +105,13 @@ __device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b)
     a.n = n;
 }
 
-// One workgroup per chunk: its pixels split into kNormSlices consecutive
-// slices, thread (slice, lane) taking tau t0 + tb + lane of slice `slice` for
-// each block tb of kNormLanes taus; the slice states are merged in slice
-// order through LDS. The split does not depend on the tau range, so a range
-// of taus gets the same partials as the full schedule.
+// Each chunk's pixels are split into kNormSlices consecutive slices, one
+// thread per (slice, tau) for each pass of up to kNormLanes taus, and the
+// slice states are merged in slice order through LDS. A workgroup takes
+// kNormBlock / (kNormSlices * lanes) chunks at once (several when the tau
+// range is narrow, as in stream snapshots). Neither the tau range nor the
+// number of chunks per workgroup changes a chunk's computation, so a range of
+// taus gets the same partials as the full schedule.
 template <bool FP>
 __global__ void __launch_bounds__(kNormBlock)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
@@ -119,33 +121,42 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
 {
     __shared__ __align__(16) unsigned char s_raw[kNormBlock * sizeof(NormAcc)];
     NormAcc* s_acc = reinterpret_cast<NormAcc*>(s_raw);
-    const int lt = (int)threadIdx.x % kNormLanes;
-    const int sl = (int)threadIdx.x / kNormLanes;
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const NormChunk ch = chunks[c];
+    const int nt = t1 - t0;
+    const int lanes = min(nt, kNormLanes);              // taus per pass
+    const int per_chunk = kNormSlices * lanes;
+    const int cpb = max(1, kNormBlock / per_chunk);     // chunks per workgroup
+    const int cc = (int)threadIdx.x / per_chunk;
+    const int r = (int)threadIdx.x % per_chunk;
+    const int sl = r / lanes, lt = r % lanes;
+    for (int64_t c0 = (int64_t)blockIdx.x * cpb; c0 < nchunks; c0 += (int64_t)gridDim.x * cpb) {
+        const int64_t c = c0 + cc;
+        const bool live = cc < cpb && c < nchunks;
+        NormChunk ch{0, 0};
+        if (live) ch = chunks[c];
         const int len = ch.end - ch.begin;
         const int pb = ch.begin + (int)((int64_t)len * sl / kNormSlices);
         const int pe = ch.begin + (int)((int64_t)len * (sl + 1) / kNormSlices);
-        for (int tb = t0; tb < t1; tb += kNormLanes) {   // uniform trip count
+        for (int tb = t0; tb < t1; tb += lanes) {   // uniform trip count
             const int i = tb + lt;
-            if (i < t1) {
+            const bool act = live && i < t1;
+            if (act) {
                 NormAcc a;
                 norm_accumulate<FP>(G2, IP, IF, weight, S, i, (double)taus[i].N, pb, pe, a);
                 s_acc[threadIdx.x] = a;
             }
             __syncthreads();
-            if (sl == 0 && i < t1) {
-                NormAcc a = s_acc[lt];
+            if (act && sl == 0) {
+                NormAcc a = s_acc[threadIdx.x];
 #pragma unroll
-                for (int k = 1; k < kNormSlices; k++) norm_merge(a, s_acc[k * kNormLanes + lt]);
-                NormPartial r;
-                r.g2 = (FP || weight) ? __double_as_longlong(a.wg) : a.sg;
-                r.ip = (FP || weight) ? __double_as_longlong(a.wp) : a.sp;
-                r.ifs = (FP || weight) ? __double_as_longlong(a.wf) : a.sf;
-                r.n = a.n;
-                r.mean = a.mean;
-                r.m2 = a.m2;
-                part[c * S + i] = r;
+                for (int k = 1; k < kNormSlices; k++) norm_merge(a, s_acc[threadIdx.x + k * lanes]);
+                NormPartial o;
+                o.g2 = (FP || weight) ? __double_as_longlong(a.wg) : a.sg;
+                o.ip = (FP || weight) ? __double_as_longlong(a.wp) : a.sp;
+                o.ifs = (FP || weight) ? __double_as_longlong(a.wf) : a.sf;
+                o.n = a.n;
+                o.mean = a.mean;
+                o.m2 = a.m2;
+                part[c * S + i] = o;
             }
             __syncthreads();
         }
@@ -304,7 +315,10 @@ void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     if (nt <= 0) return;
     timer_begin(p, s, "k_norm");
     const bool fp = p->fsnorm;
-    const unsigned nb = nchunks < 65536 ? nchunks : 65536;
+    // chunks per workgroup as the kernel derives them from the tau range
+    const int cpb = std::max(1, kNormBlock / (kNormSlices * std::min(nt, kNormLanes)));
+    const int64_t nwg = (nchunks + cpb - 1) / cpb;
+    const unsigned nb = nwg < 65536 ? (unsigned)nwg : 65536u;
     if (nchunks > 0) {
         if (fp)
             k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2,

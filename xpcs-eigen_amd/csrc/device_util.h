@@ -210,15 +210,28 @@ __device__ __forceinline__ double wave_seg_scan_f64(double x, uint32_t start)
     return x;
 }
 
-// Sum of x over each run of equal `key` in neighbouring lanes; `tail` marks the
-// run's last lane, which holds the run's total. All 64 lanes must be active.
-__device__ __forceinline__ double wave_run_sum_f64(double x, uint32_t key, bool& tail)
+// Runs of equal `key` in neighbouring lanes: returns the first lane of this
+// lane's run; `tail` marks the run's last lane. All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_run_start(uint32_t key, bool& tail)
 {
     const uint32_t lane = (uint32_t)lane_id();
     const uint32_t kp = prev_lane_u32(key, ~key);
     const uint32_t kn = next_lane_u32(key, ~key);
     tail = kn != key;
-    return wave_seg_scan_f64(x, wave_incl_max_u32(kp != key ? lane : 0u));
+    return wave_incl_max_u32(kp != key ? lane : 0u);
+}
+
+// Sum of x over each run of equal `key` in neighbouring lanes; `tail` marks the
+// run's last lane, which holds the run's total. All 64 lanes must be active.
+__device__ __forceinline__ double wave_run_sum_f64(double x, uint32_t key, bool& tail)
+{
+    return wave_seg_scan_f64(x, wave_run_start(key, tail));
+}
+
+// Lane l receives x of lane src (ds_bpermute).
+__device__ __forceinline__ uint32_t pull_from_lane(int src, uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)x);
 }
 
 }  // namespace xpcs

@@ -154,6 +154,8 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, nullptr, g0, ng);
         int kf[kTrInFlight] = {};
+        // (one atomic per event: aggregating runs of equal buckets across the
+        // wave, as k_tr_bucket does, measured no faster here)
         for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kTrInFlight * bd) {
             uint32_t c[kTrInFlight], v[kTrInFlight];
             bool ok[kTrInFlight];
@@ -192,17 +194,32 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, sel_frame, g0, ng);
         int kf[kTrInFlight] = {};
-        for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kTrInFlight * bd) {
+        // uniform trip count: every lane takes part in the run detection
+        for (uint32_t qb = 0; qb < total; qb += kTrInFlight * bd) {
             uint32_t c[kTrInFlight], v[kTrInFlight], slot[kTrInFlight];
             bool ok[kTrInFlight];
 #pragma unroll
             for (int u = 0; u < kTrInFlight; u++) {
-                const uint32_t q = q0 + u * bd;
+                const uint32_t q = qb + u * bd + threadIdx.x;
                 ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, nullptr, c[u], v[u]);
             }
+            // one LDS cursor atomic per run of equal buckets: the run's last
+            // lane takes the run's slots and hands the base to the run's first
+            // lane, from which every lane of the run reads it (slots in lane order)
 #pragma unroll
-            for (int u = 0; u < kTrInFlight; u++)
-                if (ok[u]) slot[u] = atomicAdd(&s_cur[c[u] >> bshift], 1u);
+            for (int u = 0; u < kTrInFlight; u++) {
+                const uint32_t key = ok[u] ? c[u] >> bshift : 0xFFFFFFFFu;
+                bool tail;
+                const uint32_t lane = (uint32_t)lane_id();
+                const uint32_t start = wave_run_start(key, tail);
+                uint32_t base = 0u;
+                if (tail && ok[u]) base = atomicAdd(&s_cur[key], lane - start + 1u);
+                // the lanes of a run push in a rotation (lane -> lane + 1, last ->
+                // first) so that no two lanes target one lane; lane 63 always
+                // ends a run
+                const uint32_t at_head = push_to_lane(tail ? (int)start : (int)lane + 1, base);
+                slot[u] = pull_from_lane((int)start, at_head) + (lane - start);
+            }
 #pragma unroll
             for (int u = 0; u < kTrInFlight; u++)
                 if (ok[u])

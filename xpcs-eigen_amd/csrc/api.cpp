@@ -968,14 +968,16 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
         launch_mt_dense(p, s, true);
     } else if (p->fsnorm) {
         // normalize_by_framesum: frame sums first, then every pixel in FP64
-        if (p->mode == 1) launch_transpose(p, s);
+        const bool fused = side_fused(p);   // frame sums inside the transpose's count pass
+        if (p->mode == 1) launch_transpose(p, s, fused);
         else launch_dense_transpose(p, s);
-        launch_side(p, s);
+        if (!fused) launch_side(p, s);
         launch_frame_weights(p, s);
         launch_mt_fsnorm(p, s, p->mode == 2);
     } else if (p->mode == 1) {
-        launch_transpose(p, s);
-        if (p->d_fs) launch_side(p, s);
+        const bool fused = side_fused(p);
+        launch_transpose(p, s, fused);
+        if (p->d_fs && !fused) launch_side(p, s);
         launch_mt_sparse(p, s);
         launch_mt_long(p, s, false);
     } else if (p->mode == 2 && dense_stream_ok(p)) {

@@ -244,7 +244,11 @@ xpcs_status fail(xpcs_status st, const char* fmt, ...);
 // kernel launchers (kernels_*.hip); all asynchronous on `s`
 TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, int32_t chunk_frames);
 int64_t bucket_scan_scratch(int32_t nb, int32_t nblk);
-void launch_transpose(xpcs_mt_plan* p, hipStream_t s);
+// with_side: also accumulate the side statistics' frame sums into d_fs (which
+// the caller has zeroed); only when side_fused(p)
+void launch_transpose(xpcs_mt_plan* p, hipStream_t s, bool with_side = false);
+constexpr int64_t kSideFuseDoubles = 4096;   // LDS frame-sum table of the fused count pass (32 KB)
+bool side_fused(const xpcs_mt_plan* p);
 EvFormat ev_format(const xpcs_mt_plan* p);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);

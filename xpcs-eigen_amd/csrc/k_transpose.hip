@@ -138,21 +138,62 @@ __device__ __forceinline__ bool chunk_event(const ChunkGroup& cg, const uint8_t*
     return tr_event_raw(base, dlen, al4, q - cg.pre[kf], fm, mapped, npix_det, counters, r, v);
 }
 
+// SIDE: the side statistics' per-(output frame, static bin) sums of the
+// chunk's events in the same pass (sparse_filter.cpp:168-191; no flat field,
+// static bins < 256 from the 1-byte per-pixel table): an LDS table
+// [output frames of the chunk][S], runs of equal (frame, bin) summed across
+// the wave first, flushed into fs with one global atomic per nonzero entry.
+template <bool SIDE>
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf, EvFormat fm,
            const uint32_t* __restrict__ mapped, uint32_t npix_det, int bshift, int32_t nb,
-           int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters)
+           int32_t nblk, uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
+           const int32_t* __restrict__ sel_frame, const uint8_t* __restrict__ sbin_pix, int32_t S,
+           double* __restrict__ fs)
 {
-    extern __shared__ uint32_t s_h[];
+    extern __shared__ uint32_t s_h[];   // [nb] bucket counts; SIDE: then [frames][S] doubles
     __shared__ ChunkGroup cg;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) s_h[b] = 0u;
     const int64_t k0 = (int64_t)blockIdx.x * cf;
     const int64_t k1 = min(nsel, k0 + cf);
     const uint32_t bd = blockDim.x;
+    double* s_fs = reinterpret_cast<double*>(s_h + ((nb + 1) & ~1));
+    int32_t f_lo = 0, f_span = 0;
+    if constexpr (SIDE) {
+        f_lo = sel_frame[k0];
+        f_span = sel_frame[k1 - 1] - f_lo + 1;
+        for (int i = threadIdx.x; i < f_span * S; i += blockDim.x) s_fs[i] = 0.0;
+    }
     for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
-        const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, nullptr, g0, ng);
+        const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, SIDE ? sel_frame : nullptr, g0, ng);
+        if constexpr (SIDE) {
+            int kf[kTrInFlight] = {};
+            // uniform trip count: every lane takes part in the wave sums
+            for (uint32_t qb = 0; qb < total; qb += kTrInFlight * bd) {
+                uint32_t c[kTrInFlight], v[kTrInFlight];
+                bool ok[kTrInFlight];
+#pragma unroll
+                for (int u = 0; u < kTrInFlight; u++) {
+                    const uint32_t q = qb + u * bd + threadIdx.x;
+                    ok[u] = q < total &&
+                            chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, counters, c[u], v[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kTrInFlight; u++) {
+                    if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
+                    const uint32_t bin = ok[u] ? (uint32_t)sbin_pix[c[u]] : 0u;   // 1..S, 0 unmapped
+                    const uint32_t key =
+                        bin ? (uint32_t)(cg.frame[kf[u]] - f_lo) * 256u + (bin - 1u) : 0xFFFFFFFFu;
+                    bool tail;
+                    const double run = wave_run_sum_f64(bin ? (double)v[u] : 0.0, key, tail);
+                    if (tail && bin) atomicAdd(&s_fs[(key >> 8) * (uint32_t)S + (key & 255u)], run);
+                }
+            }
+            __syncthreads();   // the group tables are reloaded
+            continue;
+        }
         int kf[kTrInFlight] = {};
         // (one atomic per event: aggregating runs of equal buckets across the
         // wave, as k_tr_bucket does, measured no faster here)
@@ -171,6 +212,10 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         __syncthreads();   // the group tables are reloaded
     }
     for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)blockIdx.x * nb + b] = s_h[b];
+    if constexpr (SIDE) {
+        for (int i = threadIdx.x; i < f_span * S; i += blockDim.x)
+            if (s_fs[i] != 0.0) atomicAdd(&fs[(int64_t)f_lo * S + i], s_fs[i]);
+    }
 }
 
 __global__ void __launch_bounds__(kTrChunkBlock)
@@ -743,14 +788,29 @@ TransposeShape transpose_shape(int64_t P, int64_t nsel, int32_t bucket_shift, in
     return t;
 }
 
-void launch_transpose(xpcs_mt_plan* p, hipStream_t s)
+bool side_fused(const xpcs_mt_plan* p)
+{
+    return p->mode == 1 && p->d_fs && !p->d_weight && p->sbin_pix_bytes == 1 &&
+           p->tshape.cf * (int64_t)p->h.ns <= kSideFuseDoubles;
+}
+
+void launch_transpose(xpcs_mt_plan* p, hipStream_t s, bool with_side)
 {
     const TransposeShape& t = p->tshape;
     const uint32_t npd = (uint32_t)((int64_t)p->h.W * p->h.H);
     timer_begin(p, s, "k_tr_count");
-    k_tr_count<<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
-        t.nblk, p->d_hist, p->d_counters);
+    if (with_side) {
+        // the chunk's output frames never exceed its cf raw frames
+        const size_t lds = (size_t)((t.nb + 1) & ~1) * sizeof(uint32_t) + (size_t)t.cf * p->h.ns * sizeof(double);
+        k_tr_count<true><<<t.nblk, kTrChunkBlock, lds, s>>>(
+            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
+            t.nblk, p->d_hist, p->d_counters, p->d_sel_frame, static_cast<const uint8_t*>(p->d_sbin_pix), p->h.ns,
+            p->d_fs);
+    } else {
+        k_tr_count<false><<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
+            t.nblk, p->d_hist, p->d_counters, nullptr, nullptr, 0, nullptr);
+    }
     timer_end(p, s);
     timer_begin(p, s, "k_tr_scan");
     bucket_scan(p->d_hist, t.nb, t.nblk, p->d_hoff, p->d_bsum, s);

@@ -48,7 +48,7 @@ k_stream_scatter(const uint8_t* __restrict__ imm, const StreamFrames fr, int32_t
 // time in registers.
 template <int D2>
 __global__ void __launch_bounds__(256)
-k_stream_update(const uint32_t* __restrict__ tile, int32_t nb, int32_t P, int32_t n0, int32_t nlev,
+k_stream_update(uint32_t* __restrict__ tile, int32_t nb, int32_t P, int32_t n0, int32_t nlev,
                 uint32_t* __restrict__ win, uint32_t* __restrict__ head, uint64_t* __restrict__ acc,
                 uint32_t* __restrict__ pend, uint64_t* __restrict__ total)
 {
@@ -59,6 +59,8 @@ k_stream_update(const uint32_t* __restrict__ tile, int32_t nb, int32_t P, int32_
     uint64_t add = 0;
     for (int i = 0; i < nb; i++) {
         const uint32_t v = active ? tile[(int64_t)i * P + c] : 0u;
+        // read entries go back to zero, ready for the next push's scatter
+        if (active && v) tile[(int64_t)i * P + c] = 0u;
         s_list[i][tid] = v;
         add += v;
     }
@@ -155,7 +157,8 @@ k_stream_snapshot(const uint32_t* __restrict__ win, const uint32_t* __restrict__
 void launch_stream_update(const StreamLaunch& a, hipStream_t s)
 {
     const unsigned gp = (unsigned)((a.P + 255) / 256);
-    launch_fill_u32(a.tile, (size_t)a.nb * a.P, s);
+    // the tile is zero here: cleared when the stream opens, and every entry a
+    // scatter sets is cleared again by the update that reads it
     k_stream_scatter<<<a.nb, 256, 0, s>>>(a.imm, a.fr, a.nb, a.pix2c, a.npix_det, a.P, a.tile, a.counters);
     if (a.d2 == 8)
         k_stream_update<8><<<gp, 256, 0, s>>>(a.tile, a.nb, a.P, a.n0, a.nlev, a.win, a.head, a.acc, a.pend,

@@ -156,6 +156,7 @@ extern "C" xpcs_status xpcs_mt_stream_open(const xpcs_mt_desc* desc, int32_t max
         launch_fill_u64(s->acc, L * D2 * P, q);
         launch_fill_u32(s->pend, L * P, q);
         launch_fill_u64(s->total, P, q);
+        launch_fill_u32(s->tile, (size_t)kStreamChunk * P, q);
         launch_fill_u32(s->plan->d_counters, 8, q);
         if (hipEventCreateWithFlags(&s->done, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&s->copied, hipEventDisableTiming) != hipSuccess ||

@@ -56,6 +56,18 @@ def test_dense_darks_vector_loads(xmt, orc):
     check_side(xmt, orc, W, H, F, dq, sq, imm, darks=3)
 
 
+@pytest.mark.parametrize("W,H,darks", [(32, 32, 2), (15, 15, 0)])
+def test_dense_integer_sums(xmt, orc, W, H, darks):
+    # F >= 512 without a flat field: the kernel keeps its sums in u32 until
+    # the LDS table (16-byte loads for 32x32, per-pixel loads for 15x15)
+    F = 600
+    dq, sq = synth.annular_qmaps(W, H, 8, 4)
+    sig = synth.ou_counts(W, H, F, 6.0, 20.0, seed=714 + W)
+    frames = sig if not darks else np.concatenate([synth.dark_frames(W, H, darks, seed=715), sig + 2], axis=0)
+    imm = synth.imm_bytes_dense(frames, W, H)
+    check_side(xmt, orc, W, H, F, dq, sq, imm, darks=darks)
+
+
 def test_dense_odd_pixels_flatfield(xmt, orc):
     # 15x15 = 225 pixels: per-pixel 2-byte loads, last lane partly outside
     W = H = 15

@@ -210,6 +210,27 @@ __device__ __forceinline__ double wave_seg_scan_f64(double x, uint32_t start)
     return x;
 }
 
+// Segmented inclusive u32 sum over the 64 lanes, as wave_seg_scan_f64 (the
+// caller guarantees that a segment's sum fits 32 bits).
+__device__ __forceinline__ uint32_t wave_seg_scan_u32(uint32_t x, uint32_t start)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    uint32_t u;
+    u = dpp_u32<kDppRowShr + 1, 0xF, true>(0u, x);
+    x += lane >= start + 1 ? u : 0u;
+    u = dpp_u32<kDppRowShr + 2, 0xF, true>(0u, x);
+    x += lane >= start + 2 ? u : 0u;
+    u = dpp_u32<kDppRowShr + 4, 0xF, true>(0u, x);
+    x += lane >= start + 4 ? u : 0u;
+    u = dpp_u32<kDppRowShr + 8, 0xF, true>(0u, x);
+    x += lane >= start + 8 ? u : 0u;
+    u = dpp_u32<kDppBcast15, 0xA>(0u, x);
+    x += start < (lane & 0x30u) ? u : 0u;
+    u = dpp_u32<kDppBcast31, 0xC>(0u, x);
+    x += start < 32u ? u : 0u;
+    return x;
+}
+
 // Runs of equal `key` in neighbouring lanes: returns the first lane of this
 // lane's run; `tail` marks the run's last lane. All 64 lanes must be active.
 __device__ __forceinline__ uint32_t wave_run_start(uint32_t key, bool& tail)

@@ -521,12 +521,14 @@ k_mt_dense_exact(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
 // neighbouring lanes) are fixed for the whole kernel; segment tails add into
 // the block's LDS table [FC][S], which goes to fs with one global atomic per
 // nonzero entry. Values in 32-bit arithmetic: the caller guarantees
-// 65535 * nscale < 2^31.
+// 65535 * nscale < 2^31. INT (no flat field, F >= 512): the sums stay u32 up to
+// the LDS atomic -- one value is below 2^32 / F (F * 65535 * nscale * avg <
+// 2^32), so a wave segment of at most 512 pixels sums below 2^32.
 constexpr int kSidePix = 8;
 constexpr int kSideLdsDoubles = 4096;   // LDS table of one block (32 KB)
 constexpr int kSideRing = 3;            // raw frames loaded ahead (VEC, avg 1)
 
-template <bool VEC>
+template <bool VEC, bool INT>
 __global__ void __launch_bounds__(256)
 k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off, int avg, int32_t F,
                  const int32_t* __restrict__ pix2c, uint32_t npix_det, const int64_t* __restrict__ dark_sum,
@@ -576,6 +578,20 @@ k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
         __syncthreads();
         auto frame = [&](const uint32_t (&x)[kSidePix], int32_t fl) {
             double* row = s_acc + (int64_t)fl * S;
+            if constexpr (INT) {
+                uint32_t run = 0u;
+#pragma unroll
+                for (int k = 0; k < kSidePix; k++) {
+                    if (k > 0 && ((brk >> k) & 1u)) {
+                        if (bin[k - 1] >= 0 && run) atomicAdd(&row[bin[k - 1]], (double)run);
+                        run = 0u;
+                    }
+                    run += x[k];
+                }
+                run = wave_seg_scan_u32(run, start);
+                if (tail && lb >= 0 && run) atomicAdd(&row[lb], (double)run);
+                return;
+            }
             double run = 0.0;
 #pragma unroll
             for (int k = 0; k < kSidePix; k++) {
@@ -769,14 +785,19 @@ void launch_side_dense_raw(xpcs_mt_plan* p, hipStream_t s)
     const dim3 grid((groups + 255) / 256, (unsigned)std::min(nchunks, 65535));
     const size_t lds = (size_t)fc * (size_t)h.ns * sizeof(double);
     timer_begin(p, s, "k_side");
-    if (p->side_vec)
-        k_side_dense_raw<true><<<grid, 256, lds, s>>>(p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, npd,
-                                                      p->d_dark_sum, p->d_thr, (int32_t)p->nscale, p->d_sbin_of_c,
-                                                      p->d_weight, h.ns, fc, p->d_fs);
-    else
-        k_side_dense_raw<false><<<grid, 256, lds, s>>>(p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, npd,
-                                                       p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
-                                                       p->d_sbin_of_c, p->d_weight, h.ns, fc, p->d_fs);
+    auto go = [&](auto vec, auto integer) {
+        k_side_dense_raw<decltype(vec)::value, decltype(integer)::value><<<grid, 256, lds, s>>>(
+            p->d_imm, p->d_sel_off, h.avg, h.F, p->d_pix2c, npd, p->d_dark_sum, p->d_thr, (int32_t)p->nscale,
+            p->d_sbin_of_c, p->d_weight, h.ns, fc, p->d_fs);
+    };
+    const bool integer = !p->d_weight && h.F >= 512;
+    if (p->side_vec) {
+        if (integer) go(std::true_type{}, std::true_type{});
+        else go(std::true_type{}, std::false_type{});
+    } else {
+        if (integer) go(std::false_type{}, std::true_type{});
+        else go(std::false_type{}, std::false_type{});
+    }
     timer_end(p, s);
 }
 

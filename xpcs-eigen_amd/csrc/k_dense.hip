@@ -615,7 +615,10 @@ k_side_dense_raw(const uint8_t* __restrict__ imm, const int64_t* __restrict__ se
             if (avg == 1) {
                 auto issue = [&](u32x4& r, int32_t fl) {
                     const int32_t f = f0 + min(fl, nf - 1);
-                    r = live ? *reinterpret_cast<const u32x4*>(imm + sel_off[f] + 2 * (int64_t)j0) : u32x4{0, 0, 0, 0};
+                    // each frame is streamed once: nontemporal (about 1 % faster)
+                    r = live ? __builtin_nontemporal_load(
+                                   reinterpret_cast<const u32x4*>(imm + sel_off[f] + 2 * (int64_t)j0))
+                             : u32x4{0, 0, 0, 0};
                 };
                 u32x4 ring[kSideRing];
 #pragma unroll

@@ -496,8 +496,9 @@ def main():
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the pixel-sharded multi-GPU design is "
-                                   "described in DESIGN.md but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding the pixels over several GPUs "
+                                   "(one all-reduce of the per-bin sums) is designed but not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch

@@ -345,7 +345,8 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
 #pragma unroll
         for (int t = 0; t < kTrRowsRegs; t++) {
             const uint32_t e = e_first + (uint32_t)t * step;
-            xr[t] = e < gb ? seg[e] : ~0ull;
+            // read once into registers: nontemporal (0.232 -> 0.222 ms on 1 Mpx)
+            xr[t] = e < gb ? __builtin_nontemporal_load(seg + e) : ~0ull;
         }
         __syncthreads();
 #pragma unroll

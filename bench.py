@@ -99,23 +99,6 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def batched_jobs(n_jobs=64):
-    """The batched workload: job j draws (seed 4000 + j) frames U[500, 20000], a
-    square crop U[128, 1024] px, occupancy log-U[0.1 %, 5 %] (mu = occ / (1 - occ)),
-    OU correlation time log-U[1, 1000] frames, dpl 4 or 8, Q ~ U[2, 12], S = 3Q."""
-    jobs = []
-    for j in range(n_jobs):
-        r = np.random.default_rng(4000 + j)
-        F = int(r.integers(500, 20001))
-        W = int(r.integers(128, 1025))
-        occ = float(np.exp(r.uniform(np.log(0.001), np.log(0.05))))
-        tau = float(np.exp(r.uniform(0.0, np.log(1000.0))))
-        dpl = int(r.choice([4, 8]))
-        Q = int(r.integers(2, 13))
-        jobs.append(dict(F=F, W=W, mu=occ / (1.0 - occ), tau=tau, dpl=dpl, Q=Q, S=3 * Q, seed=4000 + j))
-    return jobs
-
-
 def run_batched(dev, n_jobs, reps):
     """64 uneven sparse jobs: the plain path (one job after another, each
     executed and synchronised) against one autotuned, graph-captured batch
@@ -123,7 +106,7 @@ def run_batched(dev, n_jobs, reps):
     import torch
     import synth
     import xpcs_mt
-    jobs = batched_jobs(n_jobs)
+    jobs = synth.batched_jobs(n_jobs)
     t0 = time.time()
     plans, keep = [], []
     frames = 0
@@ -180,28 +163,6 @@ def run_batched(dev, n_jobs, reps):
             "frames_per_s": frames / tb, "g2_same_as_plain": bool(same), "branches": 8,
             "launch": "one hipGraph launch per batch (all jobs captured on 8 branches)",
             "autotune_and_capture_s": tune_s, "distinct_tuned_shapes": len(tunes), "setup_s": gen_s}
-
-
-class DeviceFrames:
-    """Frames of one synthetic detector stream, already in device memory,
-    arriving at `fps` from `start` (wall clock of the serving loop)."""
-
-    def __init__(self, imm, off, dlen, total_frames, start, fps):
-        self.imm, self.off, self.dlen = imm, off, dlen
-        self.frames = total_frames          # the stream's full length (admission size)
-        self.generated = len(off)           # frames that exist for this run
-        self.start, self.fps = start, fps
-
-    def arrival(self, t):
-        return self.start + t / self.fps
-
-    def available(self, now):
-        if now < self.start:
-            return 0
-        return int(min(self.generated, np.floor((now - self.start) * self.fps) + 1))
-
-    def push(self, stream, a, b):
-        stream.push_imm_device(self.imm.data_ptr(), self.imm.numel(), self.off[a:b], self.dlen[a:b])
 
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
@@ -435,7 +396,7 @@ def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
             k = pending.pop(0)
             sp = specs[k]
             svc.submit(k, sp["W"], sp["W"], sp["dpl"], sp["dq"], sp["sq"],
-                       DeviceFrames(sp["imm"], sp["off"], sp["dl"], sp["length"], sp["start"], fps), now=now)
+                       serve.DeviceFrames(sp["imm"], sp["off"], sp["dl"], sp["length"], sp["start"], fps), now=now)
         svc.step(now)
         steps += 1
         if now >= seconds:

@@ -109,7 +109,12 @@ public:
 
     // Stands in for the Configuration singleton: width, height, frames,
     // delays per level, stride, average, q-maps and flat field of the job.
-    static void configure(const xpcs_mt_desc& desc) { state().desc = desc; state().configured = true; }
+    static void configure(const xpcs_mt_desc& desc)
+    {
+        state().reset();
+        state().desc = desc;
+        state().configured = true;
+    }
 
     // Corr::multiTau2 (corr.cpp:315-431): G2, IP and IF of every pixel of
     // `data` for every tau, ADDED into the caller's float arrays laid out
@@ -162,21 +167,30 @@ public:
         }
     }
 
-    // Corr::normalizeG2s (corr.cpp:927-1091) of the last multiTau2: g2 and
-    // stderr [tau][q] (what the reference writes as norm-0-g2/norm-0-stderr),
-    // from that run's exact per-pixel sums still held on the device.
-    static void normalizeG2s(double* g2, double* stderr_)
+    // Corr::normalizeG2s(G2, IP, IF) (corr.h:106-108, corr.cpp:927-1091) of
+    // the caller's arrays, as main.cpp:441-445 calls it on the accumulated
+    // G2s/IPs/IFs: each a pixels x taus matrix stored column-major, i.e.
+    // [tau][W*H] floats (rows = W*H, cols = taus).  The reference writes the
+    // result to HDF5 as norm-0-g2/norm-0-stderr; here g2 and stderr_ receive
+    // it, [tau][q] doubles (taus x lastShape's qbins).  Uses the configured
+    // q-maps and schedule; the arrays are whatever the caller holds (one
+    // multiTau2, several accumulated, or edited), as in the reference.
+    static void normalizeG2s(const float* G2, const float* IP, const float* IF, int rows, int cols, double* g2,
+                             double* stderr_)
     {
         State& st = state();
-        if (!st.plan) throw std::runtime_error("normalizeG2s: no multiTau2 result");
-        detail::check(xpcs_mt_read_g2(st.plan, g2, stderr_), "normalizeG2s");
+        if (!st.configured) throw std::runtime_error("normalizeG2s: Corr::configure() not called");
+        if (!st.plan) detail::check(xpcs_mt_plan_create(&st.desc, &st.plan), "normalizeG2s");
+        detail::check(xpcs_mt_normalize_ref_f32(st.plan, G2, IP, IF, rows, cols, g2, stderr_), "normalizeG2s");
     }
 
-    // Taus and dynamic q-bins of the last multiTau2 (the shapes of its outputs).
+    // Taus and dynamic q-bins of the configured job (the shapes of
+    // normalizeG2s's outputs).
     static void lastShape(int* taus, int* qbins)
     {
         State& st = state();
-        if (!st.plan) throw std::runtime_error("lastShape: no multiTau2 result");
+        if (!st.configured) throw std::runtime_error("lastShape: Corr::configure() not called");
+        if (!st.plan) detail::check(xpcs_mt_plan_create(&st.desc, &st.plan), "lastShape");
         xpcs_mt_info info;
         detail::check(xpcs_mt_plan_info(st.plan, &info), "lastShape");
         *taus = info.taus;

@@ -256,6 +256,24 @@ xpcs_status xpcs_mt_read_ref_f32(xpcs_mt_plan* plan, float* G2, float* IP, float
 xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* plan, double* g2, double* stderr_);
 
 /*
+ * Normalise a caller's per-pixel arrays: Corr::normalizeG2s(G2, IP, IF)
+ * (src/xpcs/corr.h:106-108, src/xpcs/corr.cpp:927-1091, called at
+ * src/xpcs/main.cpp:441-445 on the accumulated G2s/IPs/IFs).  G2, IP and IF
+ * are float arrays in the reference's units and layout, [taus][pixels] with
+ * pixels = width*height (the column-major pixels x taus matrices the
+ * reference maps over them; what multiTau2 accumulated, or
+ * xpcs_mt_read_ref_f32 returned).  Uses only the plan's q-map and schedule
+ * (no ingest needed): static-bin sums of each array, g2s = (sum G2 / c) /
+ * ((sum IP / c) (sum IF / c)), the NaN-skipping mean per dynamic bin, and the
+ * stderr of G2/(IP IF) over the pixels of each dynamic bin with NaN counted
+ * as 0.  The values already carry any flat field, so it is not applied again.
+ * FP64 on the device; g2 and stderr are [taus][Q] and also become the plan's
+ * xpcs_mt_read_g2 results.  Synchronous; the caller's arrays are not kept.
+ */
+xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* plan, const float* G2, const float* IP, const float* IF,
+                                      int64_t pixels, int32_t taus, double* g2, double* stderr_);
+
+/*
  * Dark image of a dense ingest (DarkImage, src/xpcs/data_structure/dark_image.cpp:81-106):
  * per-pixel Welford mean and standard deviation, [W*H] float64 each (zeros
  * without dark frames); the --darkout DarkAvg/DarkStd datasets.

@@ -81,7 +81,11 @@ done:
     return rc;
 }
 
-/* src/xpcs/data_structure/dark_image.cpp:81-106 (Welford in double). */
+/* src/xpcs/data_structure/dark_image.cpp:81-106 (Welford in double).  Dark
+ * values are uint16, as every count here (the reference reads them as int16,
+ * src/xpcs/io/imm.cpp:87; the two agree below 32768), so the mean and standard
+ * deviation that set the threshold describe the same values as the integer
+ * dark sums that are subtracted. */
 void xo_dark_image(const uint8_t* bytes, const xo_imm_frame* frames, int darks,
                    int npix_det, const double* flatfield, double* dark_avg, double* dark_std,
                    int64_t* dark_sum)
@@ -91,9 +95,8 @@ void xo_dark_image(const uint8_t* bytes, const xo_imm_frame* frames, int darks,
         const uint8_t* vp = bytes + frames[i].payload_off;
         for (int j = 0; j < npix_det; j++) {
             uint16_t raw = rd_u16(vp + 2 * (int64_t)j);
-            float data = (float)(int16_t)raw;
             double tmp = dark_avg[j];
-            double pix = (double)data * (flatfield ? flatfield[j] : 1.0);
+            double pix = (double)raw * (flatfield ? flatfield[j] : 1.0);
             dark_avg[j] += ((pix - dark_avg[j]) / (double)(i + 1));
             dark_std[j] += ((pix - tmp) * (pix - dark_avg[j]));
             dark_sum[j] += raw;
@@ -146,7 +149,8 @@ int64_t xo_dense_transpose(const uint8_t* bytes, const xo_imm_frame* frames,
                     /* exact, n-scaled integer value */
                     int64_t y = n * (int64_t)raw - (darks > 0 ? dark_sum[j] : 0);
                     if (y > thr[j]) { hit[c] = 1; acc_c[c] += y > 0 ? y : 0; }
-                    /* reference float value */
+                    /* reference float value (its int16 read; the dark mean is
+                     * the uint16 one, so this matches the reference below 32768) */
                     float v = (float)(int16_t)raw;
                     float thresh = 0.0f;
                     if (darks > 0) {

@@ -4,7 +4,7 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
-for p in (ROOT / "xpcs-eigen_amd", ROOT / "oracle", ROOT / "tests"):
+for p in (ROOT / "xpcs-eigen_amd", ROOT / "oracle", ROOT / "tests", ROOT / "tests" / "golden"):
     if str(p) not in sys.path:
         sys.path.insert(0, str(p))
 
@@ -34,3 +34,10 @@ def orc():
     import oracle
     oracle.lib()
     return oracle
+
+
+def pytest_terminal_summary(terminalreporter):
+    # name the benchmark-workload tests in the run's last lines
+    done = [r.nodeid.split("::")[-1] for r in terminalreporter.stats.get("passed", []) if "::test_config" in r.nodeid]
+    if done:
+        terminalreporter.write_line("benchmark-workload tests passed: " + ", ".join(done))

@@ -97,10 +97,22 @@ def check_against_oracle(tmp_path, orc, c, F, dpl, avg=1, ff=None, repeats=1, **
                                         stale_tail=False)
     for got, want in ((out["G2"], G2c), (out["IP"], IPc), (out["IF"], IFc)):
         np.testing.assert_allclose(got, repeats * want, rtol=1e-4, atol=1e-6)
+    # normalizeG2s of the accumulated arrays (corr.cpp:927-1091): repeats
+    # times every sum divides g2 and stderr by repeats
     ok = ~np.isnan(r["g2"])
     assert np.array_equal(np.isnan(out["g2"]), ~ok)
-    np.testing.assert_allclose(out["g2"][ok], r["g2"][ok], rtol=1e-6)
-    np.testing.assert_allclose(out["se"], r["stderr"], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(out["g2"][ok], r["g2"][ok] / repeats, rtol=1e-4)
+    np.testing.assert_allclose(out["se"], r["stderr"] / repeats, rtol=1e-4, atol=1e-12)
+    # the same arrays through the FP64 normalisation of FP64 sums (pinned to the
+    # exact one) and through the float replica of the reference's normalizeG2s
+    order = r["qm"]["order"]
+    rows = [np.ascontiguousarray(out[k][:, order].T, np.float64) for k in ("G2", "IP", "IF")]
+    g2w, sew = orc.normalize_f64_sums(*rows, np.ones(out["level"].size), r["qm"])
+    np.testing.assert_allclose(out["g2"][ok], g2w[ok], rtol=1e-9)
+    np.testing.assert_allclose(out["se"], sew, rtol=1e-9, atol=1e-15)
+    g2f, sef = orc.normalize_compat(out["G2"], out["IP"], out["IF"], r["qm"])
+    np.testing.assert_allclose(out["g2"][ok], g2f[ok], rtol=1e-4)
+    np.testing.assert_allclose(out["se"], sef, rtol=1e-4, atol=1e-9)
 
 
 @pytest.mark.gpu

@@ -159,6 +159,24 @@ def dense_case(W, H, F_raw, dpl, mu, tau_c, seed, darks=2, S=9, Q=3):
     return dict(W=W, H=H, F=F_raw, dpl=dpl, dq=dq, sq=sq, imm=synth.imm_bytes_dense(frames, W, H))
 
 
+@pytest.mark.parametrize("dpl", [4, 2])
+def test_dense_darks_above_int16(xmt, orc, dpl):
+    """Dark frames straddling 32768 at some pixels (32767 and 32769: uint16
+    standard deviation 1) with a sigma threshold: the streaming kernel (dpl 4)
+    and the transpose path (dpl 2) keep and subtract exactly as the oracle."""
+    W = H = 16
+    F = 600
+    dq, sq = synth.annular_qmaps(W, H, 9, 3)
+    sig = synth.ou_counts(W, H, F, 3.0, 15.0, seed=57).astype(np.int64)
+    dark = np.full((2, W * H), 3, np.int64)
+    hot = np.arange(5, W * H, 23)
+    dark[0, hot], dark[1, hot] = 32767, 32769
+    base = np.where(np.isin(np.arange(W * H), hot), 32768, 3)
+    frames = np.concatenate([dark, sig + base[None, :]]).astype(np.uint16)
+    c = dict(W=W, H=H, F=F, dpl=dpl, dq=dq, sq=sq, imm=synth.imm_bytes_dense(frames, W, H))
+    check_against_oracle(xmt, orc, c, dark_frames=2, sigma=1.0, lld=0.5)
+
+
 def test_dense_dpl8_two_passes(xmt, orc):
     """Streaming dense kernel, dpl=8, 9 levels (two passes of the level cascade)."""
     c = dense_case(16, 16, 3000, 8, 3.0, 40.0, seed=41)
@@ -217,6 +235,84 @@ def test_ref_units_vs_float_replica(xmt, orc):
     for got, want in ((G2r, G2c), (IPr, IPc), (IFr, IFc)):
         rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
         assert rel.max() <= REF_RTOL
+
+
+@pytest.mark.parametrize("ff", [False, True])
+def test_normalize_ref_f32_arrays(xmt, orc, ff):
+    """Corr::normalizeG2s(G2, IP, IF) on reference-unit float arrays, by a
+    plan that ingested nothing: the arrays read back from an executed plan
+    normalise to that plan's g2 (within float rounding of the arrays), to the
+    FP64 normalisation of the same arrays, and to the reference's float
+    normalisation of them."""
+    c = cases.sparse_case(W=32, H=32, F=600, dpl=4, mu=0.1, tau_c=10.0, seed=9)
+    flat = np.random.default_rng(0).uniform(0.5, 1.5, (32, 32)) if ff else None
+    plan = run_gpu(xmt, c, flatfield=flat)
+    G2r, IPr, IFr = plan.ref_f32()
+    g2, se = plan.g2()
+    fresh = xmt.Plan(c["W"], c["H"], c["F"], c["dpl"], c["dq"], c["sq"], flatfield=flat)
+    g2n, sen = fresh.normalize_ref_f32(G2r, IPr, IFr)
+    assert np.array_equal(fresh.g2()[0], g2n, equal_nan=True)
+    assert_g2_close(g2n, g2, 1e-5)
+    assert_g2_close(sen, se, REF_RTOL)
+    qm = orc.build_qmap(c["dq"], c["sq"])
+    rows = [np.ascontiguousarray(a[:, qm["order"]].T, np.float64) for a in (G2r, IPr, IFr)]
+    g2w, sew = orc.normalize_f64_sums(*rows, np.ones(plan.T), qm)
+    assert_g2_close(g2n, g2w, 1e-9)
+    assert_g2_close(sen, sew, 1e-9)
+    g2f, sef = orc.normalize_compat(G2r, IPr, IFr, qm)
+    assert_g2_close(g2n, g2f.astype(np.float64), REF_RTOL)
+    a = np.ascontiguousarray(G2r)
+    out = np.zeros(plan.T * plan.Q)
+    st = xmt.lib().xpcs_mt_normalize_ref_f32(fresh._h, xmt._p(a), xmt._p(a), xmt._p(a), a.shape[1], plan.T - 1,
+                                             xmt._p(out), xmt._p(out))
+    assert st == 1 and b"taus" in xmt.lib().xpcs_last_error()
+    fresh.close()
+    plan.close()
+
+
+def test_sparse_detector_beyond_transpose_limit_is_refused(xmt):
+    """4100 x 4100 raster pixels need transpose buckets of more than 4096 rows:
+    execute refuses the job (nothing is launched)."""
+    W = H = 4100
+    dq = np.zeros((H, W), np.int32)
+    sq = np.zeros((H, W), np.int32)
+    dq[:2, :2] = 1
+    sq[:2, :2] = 1
+    ev = [(np.array([0, W * H - 1], np.int32), np.array([1, 1], np.uint16))] * 8
+    plan = xmt.Plan(W, H, 8, 2, dq, sq)
+    plan.ingest_imm(synth.imm_bytes_events(ev, W, H))
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.execute()
+    assert e.value.status == 5 and "4096 x 4096" in str(e.value)
+    plan.close()
+
+
+def test_device_ingest_checks_dark_frames(xmt):
+    """Dark frames of a device-resident dense image are bounds- and size-checked
+    before anything is read."""
+    import torch
+    W = H = 8
+    frames = np.full((6, W * H), 4, np.uint16)
+    imm = synth.imm_bytes_dense(frames, W, H)
+    dev = torch.from_numpy(np.frombuffer(imm, np.uint8).copy()).cuda()
+    idx = xmt.imm_index(imm)
+    off, dl = idx["payload_off"].copy(), idx["dlen"].copy()
+    dq, sq = synth.annular_qmaps(W, H, 2, 1)
+    plan = xmt.Plan(W, H, 3, 2, dq, sq, dark_frames=2)
+    bad = off.copy()
+    bad[1] = len(imm) - 10
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.ingest_imm_device(dev.data_ptr(), dev.numel(), bad, dl, compressed=False)
+    assert e.value.status == 3 and "dark frame 1" in str(e.value)
+    short = dl.copy()
+    short[0] = 5
+    with pytest.raises(xmt.XpcsError) as e:
+        plan.ingest_imm_device(dev.data_ptr(), dev.numel(), off, short, compressed=False)
+    assert e.value.status == 3 and "dark frame 0" in str(e.value)
+    plan.ingest_imm_device(dev.data_ptr(), dev.numel(), off, dl, compressed=False)
+    plan.execute()
+    plan.g2()
+    plan.close()
 
 
 def test_pixel_out_of_range_is_an_error(xmt):

@@ -104,3 +104,36 @@ def test_dense_transpose_dark_threshold(orc):
     assert p0.tolist() == [8]                  # frame 0 raw 0 dropped, frame 1 raw 7 -> 2*4
     p2 = csr["cnt"][csr["off"][2]:csr["off"][3]]
     assert p2.tolist() == [4]                  # raw 5 kept (2*2), raw 2 dropped
+
+
+def dark_straddle_case():
+    """Pixel 5 has dark frames 32767 and 32769 (mean 32768, standard deviation
+    1 as uint16 values; as int16 they would be 32767 and -32767, standard
+    deviation 32767) and data 32768 + s for s = 0..3 in frames 0..3; every other
+    pixel has dark 3 and data 3 + s."""
+    W = H = 4
+    dark = np.full((2, 16), 3, np.uint16)
+    dark[0, 5], dark[1, 5] = 32767, 32769
+    s = np.arange(4, dtype=np.uint16)[:, None]
+    sig = np.repeat(3 + s, 16, axis=1).astype(np.uint16)
+    sig[:, 5] = 32768 + s[:, 0]
+    imm = synth.imm_bytes_dense(np.concatenate([dark, sig]), W, H)
+    return dict(W=W, H=H, F=4, dq=np.ones((4, 4), np.int32), sq=np.ones((4, 4), np.int32), imm=imm)
+
+
+def test_dense_dark_above_int16_known_answer(orc):
+    """Dark mean, deviation, threshold and subtraction all use uint16 values:
+    with lld 0 and sigma 1 pixel 5's threshold is 2 x 1 (n = 2 dark frames) and
+    its values are 2 (raw - 32768), so s = 2, 3 are kept as 4, 6."""
+    c = dark_straddle_case()
+    qm = orc.build_qmap(c["dq"], c["sq"])
+    fr = orc.imm_index(c["imm"])
+    sel, of = orc.select_frames(fr.size, 4, darks=2)
+    csr = orc.dense_transpose(c["imm"], fr, sel, of, qm, darks=2, lld=0.0, sigma=1.0)
+    assert csr["dark_avg"][5] == 32768.0 and csr["dark_std"][5] == 1.0
+    c5 = int(np.flatnonzero(qm["order"] == 5)[0])
+    a, b = csr["off"][c5], csr["off"][c5 + 1]
+    assert csr["frame"][a:b].tolist() == [2, 3] and csr["cnt"][a:b].tolist() == [4, 6]
+    c0 = int(np.flatnonzero(qm["order"] == 0)[0])
+    a, b = csr["off"][c0], csr["off"][c0 + 1]
+    assert csr["frame"][a:b].tolist() == [1, 2, 3] and csr["cnt"][a:b].tolist() == [2, 4, 6]

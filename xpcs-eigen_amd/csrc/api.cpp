@@ -384,7 +384,10 @@ xpcs_status select_frames(const HostTables& h, int64_t n_raw, bool compressed, i
 
 // Dark image of the first `darks` frames (DarkImage::Compute,
 // src/xpcs/data_structure/dark_image.cpp:81-106, Welford in double) plus the
-// exact integer sums used for the n-scaled threshold of the dense path.
+// exact integer sums used for the n-scaled threshold of the dense path.  Dark
+// values are uint16 like every count (the reference reads int16,
+// src/xpcs/io/imm.cpp:87; equal below 32768), so the threshold's standard
+// deviation and the subtracted sums describe the same values.
 xpcs_status dense_prepare(xpcs_mt_plan* p, const uint8_t* const* dark_frames)
 {
     HostTables& h = p->h;
@@ -401,7 +404,7 @@ xpcs_status dense_prepare(xpcs_mt_plan* p, const uint8_t* const* dark_frames)
         for (int i = 0; i < h.darks; i++) {
             const uint16_t* v = reinterpret_cast<const uint16_t*>(dark_frames[i]);
             for (int64_t j = 0; j < npd; j++) {
-                const double x = (double)(float)(int16_t)v[j];
+                const double x = (double)v[j];
                 const double prev = p->dark_avg[(size_t)j];
                 p->dark_avg[(size_t)j] += (x - p->dark_avg[(size_t)j]) / (double)(i + 1);
                 p->dark_std[(size_t)j] += (x - prev) * (x - p->dark_avg[(size_t)j]);
@@ -696,6 +699,15 @@ extern "C" xpcs_status xpcs_mt_ingest_imm_device(xpcs_mt_plan* p, const void* de
             return fail(XPCS_ERR_FORMAT, "ingest_imm_device: uncompressed frame %lld has the wrong size", (long long)r);
     }
     if (!compressed) {
+        for (int64_t i = 0; i < h.darks; i++) {
+            if (i >= raw_frames)
+                return fail(XPCS_ERR_FORMAT, "ingest_imm_device: fewer frames than dark frames");
+            if ((int64_t)dlen[i] != (int64_t)h.W * h.H)
+                return fail(XPCS_ERR_FORMAT, "ingest_imm_device: dark frame %lld has the wrong size", (long long)i);
+            if (payload_off[i] < 0 || (size_t)payload_off[i] + (size_t)dlen[i] * 2 > nbytes)
+                return fail(XPCS_ERR_FORMAT, "ingest_imm_device: dark frame %lld lies outside the buffer",
+                            (long long)i);
+        }
         std::vector<std::vector<uint8_t>> dark_host((size_t)h.darks);
         std::vector<const uint8_t*> darks;
         for (int i = 0; i < h.darks; i++) {
@@ -920,6 +932,11 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
             p->ev_cap = p->raw_events;
         }
         p->tshape = transpose_shape(p->nrows, p->nsel, p->tune.tr_bucket_shift, p->tune.tr_chunk_frames);
+        // the row placement holds one bucket's cursors in LDS: 4096 rows at most
+        if (p->tshape.bshift > 12)
+            return fail(XPCS_ERR_UNSUPPORTED,
+                        "execute: the sparse transpose takes at most 4096 x 4096 detector pixels, not %lld",
+                        (long long)p->nrows);
         const int64_t nh = (int64_t)p->tshape.nb * p->tshape.nblk;
         if (nh + 1 > p->hist_cap) {
             xpcs_status st = dalloc(&p->d_hist, (size_t)nh);
@@ -1100,6 +1117,48 @@ extern "C" xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* p, double* g2, double* se)
     const size_t n = (size_t)p->h.nq * p->h.T * sizeof(double);
     if (g2) HIP_TRY(hipMemcpy(g2, p->d_g2, n, hipMemcpyDeviceToHost));
     if (se) HIP_TRY(hipMemcpy(se, p->d_se, n, hipMemcpyDeviceToHost));
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* p, const float* G2, const float* IP,
+                                                 const float* IF, int64_t pixels, int32_t taus, double* g2,
+                                                 double* se)
+{
+    if (!p || !G2 || !IP || !IF) return fail(XPCS_ERR_INVALID_ARG, "normalize_ref_f32: NULL argument");
+    const HostTables& h = p->h;
+    const int64_t npd = (int64_t)h.W * h.H;
+    if (pixels != npd || taus != h.T)
+        return fail(XPCS_ERR_INVALID_ARG, "normalize_ref_f32: arrays are %d x %lld, the plan needs %d taus x %lld pixels",
+                    taus, (long long)pixels, h.T, (long long)npd);
+    // an execute still in flight writes the same partial and result buffers
+    if (p->executed) HIP_TRY(hipStreamSynchronize(p->stream_last ? p->stream_last : p->stream));
+    const size_t n_in = (size_t)npd * (size_t)h.T;
+    const size_t n_out = std::max<size_t>(1, (size_t)h.P * (size_t)h.T);
+    float* d_in = nullptr;
+    int64_t* d_sum = nullptr;
+    hipError_t e = hipMalloc(&d_in, 3 * n_in * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_sum, 3 * n_out * sizeof(int64_t));
+    hipStream_t s = p->stream;
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in, G2, n_in * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in + n_in, IP, n_in * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in + 2 * n_in, IF, n_in * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        launch_from_ref_f32(p, s, d_in, d_in + n_in, d_in + 2 * n_in, d_sum, d_sum + n_out, d_sum + 2 * n_out);
+        // FP64 sums, no flat-field weights (the values carry them)
+        launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, d_sum, d_sum + n_out, d_sum + 2 * n_out, true,
+                              nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const size_t nr = (size_t)h.nq * h.T * sizeof(double);
+    if (e == hipSuccess && g2) e = hipMemcpy(g2, p->d_g2, nr, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && se) e = hipMemcpy(se, p->d_se, nr, hipMemcpyDeviceToHost);
+    if (d_in) (void)hipFree(d_in);
+    if (d_sum) (void)hipFree(d_sum);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? XPCS_ERR_NOMEM : XPCS_ERR_HIP, "normalize_ref_f32: %s",
+                    hipGetErrorString(e));
+    collect_timers(p);
     return XPCS_OK;
 }
 

@@ -303,39 +303,78 @@ k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const 
     oIF[o] = (float)(sum_value(IF[c * T + i], FP) * s / (lv * N));
 }
 
-// Normalisation of taus [t0, t1): sums are rows of stride S ([P][S]), taus
-// gives N per tau; the chunk partials and group values use the same stride,
-// g2 and stderr are [tau][q].  Values of other taus are left as they are.
-void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
-                           const TauParam* taus)
+// Reference-unit float arrays [T][W*H] (the G2s/IPs/IFs of a caller, as
+// Corr::normalizeG2s receives them) as FP64 sums in summed-bin units, [P][T]
+// in compact pixel order: G2 = v 4^L N, IP = v 2^L N, IF = v 2^L N.
+__global__ void __launch_bounds__(256)
+k_from_ref_f32(const float* __restrict__ G2, const float* __restrict__ IP, const float* __restrict__ IF,
+               const int32_t* __restrict__ c2pix, int32_t P, const TauParam* __restrict__ taus, int32_t T,
+               int64_t npix_det, int64_t* __restrict__ oG2, int64_t* __restrict__ oIP, int64_t* __restrict__ oIF)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)P * T) return;
+    const int64_t c = k / T;
+    const int i = (int)(k % T);
+    const TauParam tp = taus[i];
+    const double lv = (double)(1ll << tp.level);
+    const double N = (double)tp.N;
+    const int64_t src = (int64_t)i * npix_det + c2pix[c];
+    oG2[k] = __double_as_longlong((double)G2[src] * (lv * lv * N));
+    oIP[k] = __double_as_longlong((double)IP[src] * (lv * N));
+    oIF[k] = __double_as_longlong((double)IF[src] * (lv * N));
+}
+
+// Normalisation of taus [t0, t1) of the sums G2/IP/IF: rows of stride S
+// ([P][S]), int64 integers or (fp) FP64 bit patterns, each pixel weighted by
+// weight[c] (NULL: 1); taus gives N per tau; the chunk partials and group
+// values use the same stride, g2 and stderr are [tau][q].  Values of other
+// taus are left as they are.
+void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
+                           const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
+                           const double* weight)
 {
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
     const int32_t nt = t1 - t0;
     if (nt <= 0) return;
     timer_begin(p, s, "k_norm");
-    const bool fp = p->fsnorm;
     // chunks per workgroup as the kernel derives them from the tau range
     const int cpb = std::max(1, kNormBlock / (kNormSlices * std::min(nt, kNormLanes)));
     const int64_t nwg = (nchunks + cpb - 1) / cpb;
     const unsigned nb = nwg < 65536 ? (unsigned)nwg : 65536u;
     if (nchunks > 0) {
         if (fp)
-            k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2,
-                                                           p->d_IP, p->d_IF, p->d_weight, p->d_part);
+            k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
+                                                           weight, p->d_part);
         else
-            k_norm_partial<false><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, p->d_G2,
-                                                            p->d_IP, p->d_IF, p->d_weight, p->d_part);
+            k_norm_partial<false><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
+                                                            weight, p->d_part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)
         k_norm_group<<<(unsigned)ng, 64, 0, s>>>(p->d_part, p->d_group_chunk_first, p->d_group_size, h.ngroups,
-                                                 taus, t0, t1, S, (fp || p->d_weight) ? 1 : 0, p->d_g2s);
+                                                 taus, t0, t1, S, (fp || weight) ? 1 : 0, p->d_g2s);
     const int64_t nqt = (int64_t)h.nq * nt;
     if (nqt > 0)
         k_norm_q<<<(unsigned)nqt, 64, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group, p->d_q_chunk_first,
                                               h.nq, t0, t1, S, p->d_g2, p->d_se);
     timer_end(p, s);
+}
+
+void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
+                           const TauParam* taus)
+{
+    launch_normalize_sums(p, s, t0, t1, S, taus, p->d_G2, p->d_IP, p->d_IF, p->fsnorm, p->d_weight);
+}
+
+void launch_from_ref_f32(xpcs_mt_plan* p, hipStream_t s, const float* G2, const float* IP, const float* IF,
+                         int64_t* oG2, int64_t* oIP, int64_t* oIF)
+{
+    const HostTables& h = p->h;
+    const int64_t n = (int64_t)h.P * h.T;
+    if (n == 0) return;
+    k_from_ref_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(G2, IP, IF, p->d_c2pix, h.P, p->d_taus, h.T,
+                                                               (int64_t)h.W * h.H, oG2, oIP, oIF);
 }
 
 void launch_normalize(xpcs_mt_plan* p, hipStream_t s)

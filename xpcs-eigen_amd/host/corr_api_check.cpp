@@ -1,7 +1,7 @@
 // Drives the xpcs::Corr facade (include/xpcs_corr.hpp) the way the
 // reference's main.cpp:169-201 and :428-455 drive xpcs::Corr: rows of a
 // SparseData filled by the caller, multiTau2 into zeroed [tau][W*H] float
-// arrays, then normalizeG2s.
+// arrays, then normalizeG2s on those arrays.
 //
 //   corr_api_check <in.bin> <out.bin> [repeats]
 //
@@ -95,7 +95,8 @@ int run(int argc, char** argv)
     xpcs::Corr::lastShape(&taus, &qbins);
     if ((size_t)taus != T) throw std::runtime_error("delaysPerLevel and the plan disagree on T");
     std::vector<double> g2(T * (size_t)qbins), se(T * (size_t)qbins);
-    xpcs::Corr::normalizeG2s(g2.data(), se.data());
+    // main.cpp:441-445: the accumulated arrays as pixels x taus matrices
+    xpcs::Corr::normalizeG2s(G2.data(), IP.data(), IF.data(), (int)npix, (int)T, g2.data(), se.data());
     xpcs::Corr::release();
 
     std::vector<int32_t> level(T), tau(T);

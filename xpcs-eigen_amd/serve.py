@@ -122,3 +122,27 @@ class StreamService:
         a = np.array(self.latencies)
         return {"p50_ms": 1e3 * float(np.percentile(a, 50)), "p99_ms": 1e3 * float(np.percentile(a, 99)),
                 "max_ms": 1e3 * float(a.max()), "frames": int(a.size)}
+
+
+class DeviceFrames:
+    """A source whose frames are already in device memory (a compressed IMM
+    image, its payload offsets and event counts), arriving at `fps` from
+    `start` on the serving loop's clock.  `total_frames` is the stream's full
+    length (its admission size); only the frames of `off` exist."""
+
+    def __init__(self, imm, off, dlen, total_frames, start, fps):
+        self.imm, self.off, self.dlen = imm, off, dlen
+        self.frames = total_frames
+        self.generated = len(off)
+        self.start, self.fps = start, fps
+
+    def arrival(self, t):
+        return self.start + t / self.fps
+
+    def available(self, now):
+        if now < self.start:
+            return 0
+        return int(min(self.generated, np.floor((now - self.start) * self.fps + 1e-9) + 1))
+
+    def push(self, stream, a, b):
+        stream.push_imm_device(self.imm.data_ptr(), self.imm.numel(), self.off[a:b], self.dlen[a:b])

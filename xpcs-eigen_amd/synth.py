@@ -336,3 +336,20 @@ def imm_device_dense(width, height, frames, mu, tau_c, seed, darks=2, dark_mean=
         view[t:t + n, HEADER_BYTES:] = (c + d16[None, :]).clamp_(max=65535).to(torch.int16).view(torch.uint8)
     offs = np.arange(total, dtype=np.int64) * rec + HEADER_BYTES
     return imm, offs, np.full(total, npix, np.uint32)
+
+
+def batched_jobs(n_jobs=64):
+    """The batched workload: job j draws (seed 4000 + j) frames U[500, 20000], a
+    square crop U[128, 1024] px, occupancy log-U[0.1 %, 5 %] (mu = occ / (1 - occ)),
+    OU correlation time log-U[1, 1000] frames, dpl 4 or 8, Q ~ U[2, 12], S = 3Q."""
+    jobs = []
+    for j in range(n_jobs):
+        r = np.random.default_rng(4000 + j)
+        F = int(r.integers(500, 20001))
+        W = int(r.integers(128, 1025))
+        occ = float(np.exp(r.uniform(np.log(0.001), np.log(0.05))))
+        tau = float(np.exp(r.uniform(0.0, np.log(1000.0))))
+        dpl = int(r.choice([4, 8]))
+        Q = int(r.integers(2, 13))
+        jobs.append(dict(F=F, W=W, mu=occ / (1.0 - occ), tau=tau, dpl=dpl, Q=Q, S=3 * Q, seed=4000 + j))
+    return jobs

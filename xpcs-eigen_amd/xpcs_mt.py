@@ -41,6 +41,7 @@ EXPORTS = [
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
     "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
     "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records", "xpcs_mt_read_sums_f64",
+    "xpcs_mt_normalize_ref_f32",
 ]
 FMT_UFXC = 1
 FMT_RIGAKU = 2
@@ -133,6 +134,7 @@ def lib():
             "xpcs_mt_read_sums_f64": ([vp, vp, vp, vp], i32),
             "xpcs_mt_ingest_frames": ([vp, vp, C.c_int64, i32], i32),
             "xpcs_packed_records": ([vp, vp, C.c_size_t, i32, vp, vp, vp, C.c_int64, vp], i32),
+            "xpcs_mt_normalize_ref_f32": ([vp, vp, vp, vp, i64, i32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -417,6 +419,17 @@ class Plan:
         g2 = np.zeros((self.T, self.Q), np.float64)
         se = np.zeros((self.T, self.Q), np.float64)
         check(lib().xpcs_mt_read_g2(self._h, _p(g2), _p(se)))
+        return g2, se
+
+    def normalize_ref_f32(self, G2, IP, IF):
+        """Corr::normalizeG2s of caller arrays in the reference's units,
+        [T][W*H] float32 each (src/xpcs/corr.cpp:927-1091): g2, stderr [T][Q]."""
+        n = self.width * self.height
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(self.T, n) for a in (G2, IP, IF)]
+        g2 = np.zeros((self.T, self.Q), np.float64)
+        se = np.zeros((self.T, self.Q), np.float64)
+        check(lib().xpcs_mt_normalize_ref_f32(self._h, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), n, self.T,
+                                              _p(g2), _p(se)))
         return g2, se
 
     def device_results(self):

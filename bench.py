@@ -38,30 +38,38 @@ WORKLOADS = {
 }
 
 
-def transpose_hist_entries(P, nsel):
-    """Entries of the (bucket, chunk) histogram the sparse transpose uses."""
-    bshift = 8
-    while ((P + (1 << bshift) - 1) >> bshift) > 4096:
-        bshift += 1
-    nb = (P + (1 << bshift) - 1) >> bshift
-    cf = (nsel + 1023) // 1024 if nsel > 1024 else 1
-    nblk = max(1, (nsel + cf - 1) // cf)
-    return nb * nblk
+def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev):
+    """Algorithmic bytes per launch of each sparse-path timer: inputs read once,
+    outputs written once, each row kernel charged only for its own rows.
 
+    px / ev: pixels and level-0 events per row class (plan.row_stats(): empty,
+    1..64, 65..128, 129..512, longer).  Row classes and their kernels: empty
+    and 1..64 -> k_mt_reg (which also lists the others: 4 B each); 65..512 ->
+    the k_mt_lds timer (k_mt_reg2 and the LDS kernel); longer -> k_mt_long.
+    Per row kernel: its rows' events (8 B), compact-to-row map (4 B) and row
+    offsets (2 x 8 B) in, the int64 G2/IP/IF of every tau (24 B per tau) and
+    the pixel total (8 B) out."""
+    P = int(px.sum())
+    E0 = int(ev.sum())
+    bshift = int(tuning["tr_bucket_shift"])
+    nb = (nrows + (1 << bshift) - 1) >> bshift
+    cf = int(tuning["tr_chunk_frames"])
+    nh = nb * max(1, (nsel + cf - 1) // cf)
 
-def algorithmic_bytes(name, P, T, E_raw, E0, n_mapped, nsel, F=0, dense_raw=0):
-    """Bytes each kernel must move at minimum (inputs read once, outputs written once)."""
-    nh = transpose_hist_entries(P, nsel)
+    def rows(k):
+        n = int(px[list(k)].sum())
+        return 28 * n + 8 * int(ev[list(k)].sum()) + 24 * n * T + 8 * n
+
     return {
-        "k_tr_count": 6 * E_raw + 4 * nh,                 # raw events in, histogram out
-        "k_tr_scan": 4 * nh + 8 * (nh + 1),               # histogram in, segment offsets out
-        "k_tr_bucket": 6 * E_raw + 8 * E0,                # raw events in, bucketed events out
-        "k_tr_rows": 8 * E0 + 8 * E0 + 8 * (P + 1),       # bucketed events in, rows + offsets out
-        "k_mt_reg": 8 * E0 + 8 * (P + 1) + 3 * 8 * P * T,  # rows + offsets in, int64 sums out
-        "k_norm": 3 * 8 * n_mapped * T,                   # int64 sums in
-        "k_dense_transpose": 2 * dense_raw + 4 * P * F,   # u16 frames in, u32 pixel rows out
-        "k_mt_long_dense": 4 * P * F + 3 * 8 * P * T,     # u32 rows in, int64 sums out
-    }.get(name, 0)
+        "k_tr_count": 6 * E_raw + 4 * nh,                       # raw events in, histogram out
+        "k_tr_scan": 4 * nh + 8 * (nh + 1),                     # histogram in, segment offsets out
+        "k_tr_bucket": 6 * E_raw + 8 * E0,                      # raw events in, bucketed events out
+        "k_tr_rows": 8 * E0 + 8 * E0 + 8 * (nrows + 1),         # bucketed events in, rows + offsets out
+        "k_mt_reg": rows((0, 1)) + 4 * int(px[2:].sum()),       # + the list entries of the longer rows
+        "k_mt_lds": rows((2, 3)) + 4 * int(px[2:4].sum()),      # + their list entries in
+        "k_mt_long": rows((4,)) + 4 * int(px[4]),
+        "k_norm": 3 * 8 * n_mapped * T,                         # int64 sums in (partials out are ~1e-3 of it)
+    }
 
 
 def dense_pass_bytes(P, F, avg, dpl, levels):
@@ -132,6 +140,15 @@ def run_batched(dev, n_jobs, reps):
             p.execute()
             p.synchronize()
         plain.append(time.perf_counter() - t1)
+    # each job on its own (execute + synchronise), to see where the plain time goes
+    per_job = []
+    for p in plans:
+        t1 = time.perf_counter()
+        p.execute()
+        p.synchronize()
+        per_job.append(1e3 * (time.perf_counter() - t1))
+    per_job = np.array(per_job)
+    events = np.array([int(p.info().events) for p in plans], np.float64)
     ref = [p.g2()[0] for p in plans[:4]]
     # batched: autotune + capture once, then graph launches
     b = xpcs_mt.Batch(branches=8)
@@ -162,7 +179,13 @@ def run_batched(dev, n_jobs, reps):
             "speedup_vs_plain": tp / tb, "batch_ms": 1e3 * tb, "plain_ms": 1e3 * tp,
             "frames_per_s": frames / tb, "g2_same_as_plain": bool(same), "branches": 8,
             "launch": "one hipGraph launch per batch (all jobs captured on 8 branches)",
-            "autotune_and_capture_s": tune_s, "distinct_tuned_shapes": len(tunes), "setup_s": gen_s}
+            "autotune_and_capture_s": tune_s, "distinct_tuned_shapes": len(tunes), "setup_s": gen_s,
+            "plain_job_ms": {"min": float(per_job.min()), "median": float(np.median(per_job)),
+                             "max": float(per_job.max()), "largest8_share": float(np.sort(per_job)[-8:].sum()
+                                                                                  / per_job.sum()),
+                             "under_2ms_jobs": int((per_job < 2.0).sum()),
+                             "under_2ms_share": float(per_job[per_job < 2.0].sum() / per_job.sum())},
+            "events_total": float(events.sum())}
 
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
@@ -435,49 +458,77 @@ def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sparse-1mpx", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-frames", type=int, default=2000,
-                    help="frames of the CPU baseline sample (full detector)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes for the main workload")
-    ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
-    ap.add_argument("--stream-seconds", type=float, default=20.0,
-                    help="wall time of the streaming leg (0 = skip)")
-    ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
-    ap.add_argument("--no-fsnorm", action="store_true", help="skip the normalize_by_framesum leg")
-    ap.add_argument("--twotime-frames", type=int, default=8192,
-                    help="frames of the two-time leg on the sparse data (0 = skip)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="run only a plain 1 GiB elementwise copy (read 1 GiB, write 1 GiB) three times, "
-                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; sharding the pixels over several GPUs "
-                                   "(one all-reduce of the per-bin sums) is designed but not built",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def measured_issue(workload, kernel):
+    """VALU issue of `kernel` on `workload` from the committed rocprofv3 SQ
+    counter summary (profiles/valu_issue.txt): (VALU wave instructions per
+    dispatch, fraction of the chip's VALU issue slots used), or None."""
+    path = ROOT / "profiles" / "valu_issue.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) >= 4 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return float(f[2]), float(f[3])
+    return None
 
+
+def cpu_model():
+    """CPU model name and logical CPUs of this host (lscpu), for the baseline."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return None
+
+
+def cpu_baseline_quarter(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl):
+    """The oracle's reference-structured CPU path (serial full-detector
+    transpose, OpenMP multi-tau, serial normalisation) on a quarter of the
+    detector -- rows 0 .. H/4-1 -- over all F frames with the job's schedule,
+    on every host core; frames/s of the whole detector = F / (4 x seconds)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    import synth
+    Hq = H // 4
+    lim = Hq * W
+    frames = []
+    for f in range(F):
+        n = int(dlen[f])
+        o = int(payload_off[f])
+        idx = np.frombuffer(host_imm, np.int32, n, o)
+        val = np.frombuffer(host_imm, np.uint16, n, o + 4 * n)
+        m = idx < lim
+        frames.append((idx[m], val[m]))
+    sample = np.frombuffer(synth.imm_bytes_events(frames, W, Hq), np.uint8)
+    dqq = np.ascontiguousarray(np.asarray(dq).reshape(H, W)[:Hq])
+    sqq = np.ascontiguousarray(np.asarray(sq).reshape(H, W)[:Hq])
+    nth = len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    _, _, stages = oracle.cpu_pipeline(sample, W, Hq, dqq, sqq, F, dpl, nth)
+    cpu_s = time.perf_counter() - t
+    return {"value": F / (4.0 * cpu_s), "unit": "frames/s", "cores": nth, "kind": "port",
+            "cpu_model": cpu_model(), "seconds": cpu_s,
+            "sample": f"rows 0-{Hq - 1} of the same {W}x{H} workload (a quarter of the detector) over all {F} "
+                      f"frames with the job's schedule (dpl={dpl}); reference-structured CPU path on {nth} threads: "
+                      f"serial transpose {stages[1]:.2f}s, OpenMP multi-tau {stages[2]:.2f}s, serial normalise "
+                      f"{stages[3]:.2f}s; value = frames / (4 x seconds)"}
+
+
+def run_job(name, steps, warmup, autotune=True, e2e=True):
+    """One workload of WORKLOADS, IMM generated on the device and resident in
+    HBM: `warmup` untimed and `steps` timed whole jobs (transpose or dense
+    passes, multi-tau, normalisation; g2 / stderr in HBM), per-kernel HIP-event
+    times and algorithmic bytes, the roofline entry of the dominant kernel and,
+    with e2e, the PCIe-inclusive rate (host IMM bytes -> g2 on the host).
+    Returns (entry, context for the other legs)."""
     import torch
     import synth
     import xpcs_mt
-
-    if args.copy_calibration:
-        n = 1 << 28
-        src = torch.ones(n, dtype=torch.int32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.add(src, 1, out=dst)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_kernel": "elementwise add int32", "bytes_read": 4 * n, "bytes_written": 4 * n,
-                          "launches": 3}))
-        return
-
-    wl = WORKLOADS[args.workload]
+    wl = WORKLOADS[name]
     W, H, F, dpl = wl["W"], wl["H"], wl["F"], wl["dpl"]
     dq, sq = synth.annular_qmaps(W, H, wl["S"], wl["Q"])
     lo, hi = wl["tau"]
@@ -499,14 +550,14 @@ def main():
     plan.ingest_imm_device(imm.data_ptr(), nbytes, payload_off, dlen, compressed=not dense)
     # launch shapes from the occupancy-keyed autotuner (untimed, as a serving
     # process would after its first job of this shape)
-    tuned = plan.autotune() if not args.no_autotune else None
-    for _ in range(args.warmup):
+    tuned = plan.autotune() if autotune else None
+    for _ in range(warmup):
         plan.execute()
         plan.synchronize()
     plan.kernel_times_reset()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         plan.execute()
     plan.synchronize()
     torch.cuda.synchronize()
@@ -514,114 +565,176 @@ def main():
     ktimes = plan.kernel_times()
     info = plan.info()
     P, T = plan.P, plan.T
-    E0 = info.events
-    n_mapped = info.mapped_pixels
-    ms_per_step = 1000.0 * elapsed / args.steps
-    frames_per_s = F * args.steps / elapsed
-
+    shapes = plan.get_tuning()
     levels, _ = plan.schedule()
-    dbytes = dense_pass_bytes(P, F, wl.get("avg", 1), dpl, np.asarray(levels)) if dense else {}
+    if dense:
+        abytes = dense_pass_bytes(P, F, wl.get("avg", 1), dpl, np.asarray(levels))
+        abytes["k_norm"] = 3 * 8 * info.mapped_pixels * T
+        px = ev = None
+    else:
+        px, ev = plan.row_stats()
+        abytes = sparse_kernel_bytes(T, int(dlen.sum()), len(dlen), W * H, info.mapped_pixels, shapes, px, ev)
     kern = {}
     for k, (ms, n) in ktimes.items():
         avg_ms = ms / max(n, 1)
-        b = dbytes.get(k) or algorithmic_bytes(k, P, T, int(dlen.sum()), E0, n_mapped, len(dlen), F,
-                                               int(dlen[darks:].sum()) * wl.get('avg', 1) if dense else 0)
-        kern[k] = dict(avg_ms=avg_ms, launches=n, alg_bytes=b,
+        b = abytes.get(k, 0)
+        kern[k] = dict(avg_ms=avg_ms, launches=n // steps, alg_bytes=b,
                        gbs=(b / (avg_ms * 1e-3) / 1e9) if b and avg_ms > 0 else None)
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     d = kern[dom]
+    traffic = measured_traffic(name, dom)
+    issue = measured_issue(name, dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None,
-                "traffic": measured_traffic(args.workload, dom), "traffic_source": "profiles/hbm_traffic.txt",
-                "alg_bytes_per_launch": d["alg_bytes"], "avg_ms": d["avg_ms"]}
-    job_bytes = sum(v["alg_bytes"] * v["launches"] for v in kern.values()) / args.steps
-    job_kernel_ms = sum(v["avg_ms"] * v["launches"] for v in kern.values()) / args.steps
-
-    # correctness spot check of this run (cheap invariants)
+                "traffic": traffic, "traffic_source": "profiles/hbm_traffic.txt",
+                "alg_bytes_per_launch": d["alg_bytes"], "avg_ms": d["avg_ms"],
+                "valu_issue_frac": issue[1] if issue else None,
+                "valu_source": "profiles/valu_issue.txt"}
+    job_bytes = sum(v["alg_bytes"] * v["launches"] for v in kern.values())
+    job_kernel_ms = sum(v["avg_ms"] * v["launches"] for v in kern.values())
     g2, se = plan.g2()
-    finite = float(np.isfinite(g2).mean())
-    plan_shapes = plan.get_tuning()
+    entry = {"workload": f"{name}: {W}x{H}, {F} frames, dpl={dpl}, mu={wl['mu']}, "
+                         f"{wl['S']} static / {wl['Q']} dynamic rings" + (f", {darks} dark frames" if darks else ""),
+             "frames": F, "taus": T, "pixels": P, "events_level0": info.events,
+             "raw_events": int(dlen.sum()) if not dense else None,
+             "row_classes": {"pixels": px.tolist(), "events": ev.tolist()} if px is not None else None,
+             "launch_shapes": shapes, "autotuned": tuned is not None,
+             "ms_per_job": 1000.0 * elapsed / steps, "frames_per_s": F * steps / elapsed,
+             "roofline": roofline, "kernels": kern,
+             "job_alg_bytes": job_bytes, "job_kernel_ms": job_kernel_ms,
+             "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
+             "exact_pixels": info.exact_pixels if dense else None,
+             "g2_finite_fraction": float(np.isfinite(g2).mean()), "setup_seconds": gen_s}
+    plan.close()
+    host_imm = None
+    if e2e:
+        # PCIe-inclusive end-to-end (host IMM bytes -> g2 on host): a cold pass
+        # (first ingest of the process allocates the pinned staging pool) and a
+        # warm pass with a new plan
+        host_imm = imm.cpu().numpy()
+        res = {}
+        for leg in ("cold", "warm"):
+            t2 = time.perf_counter()
+            plan2 = xpcs_mt.Plan(W, H, F, dpl, dq, sq, dark_frames=darks)
+            t3 = time.perf_counter()
+            plan2.ingest_imm(host_imm)
+            t3b = time.perf_counter()
+            plan2.execute()
+            plan2.g2()
+            t4 = time.perf_counter()
+            plan2.close()
+            res[leg] = dict(frames_per_s=F / (t4 - t3), seconds=t4 - t3, ingest_seconds=t3b - t3,
+                            ingest_gbs=host_imm.nbytes / (t3b - t3) / 1e9, plan_seconds=t3 - t2)
+        entry["end_to_end_pcie"] = res
+    ctx = dict(W=W, H=H, F=F, dpl=dpl, dq=dq, sq=sq, imm=imm, payload_off=payload_off, dlen=dlen,
+               host_imm=host_imm, g2=g2, dev=dev)
+    return entry, ctx
 
-    # PCIe-inclusive end-to-end (host IMM bytes -> g2 on host): a cold pass
-    # (first ingest of the process allocates the pinned staging pool) and a
-    # warm pass with a new plan
-    host_imm = imm.cpu().numpy()
-    e2e = {}
-    for leg in ("cold", "warm"):
-        t2 = time.perf_counter()
-        plan2 = xpcs_mt.Plan(W, H, F, dpl, dq, sq, dark_frames=darks)
-        t3 = time.perf_counter()
-        plan2.ingest_imm(host_imm)
-        t3b = time.perf_counter()
-        plan2.execute()
-        plan2.g2()
-        t4 = time.perf_counter()
-        plan2.close()
-        e2e[leg] = dict(frames_per_s=F / (t4 - t3), seconds=t4 - t3, ingest_seconds=t3b - t3,
-                        ingest_gbs=host_imm.nbytes / (t3b - t3) / 1e9, plan_seconds=t3 - t2)
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sparse-1mpx", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes for the main workload")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive passes of the main workload")
+    ap.add_argument("--dense-steps", type=int, default=10,
+                    help="timed jobs of the dense 512x512 x 20000-frame leg (0 = skip)")
+    ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
+    ap.add_argument("--stream-seconds", type=float, default=20.0,
+                    help="wall time of the streaming leg (0 = skip)")
+    ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
+    ap.add_argument("--no-fsnorm", action="store_true", help="skip the normalize_by_framesum leg")
+    ap.add_argument("--twotime-frames", type=int, default=8192,
+                    help="frames of the two-time leg on the sparse data (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB elementwise copy (read 1 GiB, write 1 GiB) three times, "
+                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; sharding the pixels over several GPUs "
+                                   "(one all-reduce of the per-bin sums) is designed but not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_calibration:
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 1, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel": "elementwise add int32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "launches": 3}))
+        return
+
+    main_job, ctx = run_job(args.workload, args.steps, args.warmup, autotune=not args.no_autotune,
+                            e2e=not args.no_e2e)
+    if ctx["host_imm"] is None and (not args.no_cpu or not args.no_formats):
+        ctx["host_imm"] = ctx["imm"].cpu().numpy()
+    dense = WORKLOADS[args.workload].get("dense", False)
+    W, H, F, dpl, dq, sq = (ctx[k] for k in ("W", "H", "F", "dpl", "dq", "sq"))
+    imm, payload_off, dlen, host_imm, dev = (ctx[k] for k in ("imm", "payload_off", "dlen", "host_imm", "dev"))
 
     cpu = None
     if not args.no_cpu and not dense:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle
-        fs = min(args.cpu_frames, F)
-        end = int(payload_off[fs - 1]) + 6 * int(dlen[fs - 1])
-        sample = host_imm[:end]
-        nth = len(os.sched_getaffinity(0))
-        nth = min(nth, 16)
-        t5 = time.perf_counter()
-        _, _, stages = oracle.cpu_pipeline(sample, W, H, dq, sq, fs, dpl, nth)
-        cpu_s = time.perf_counter() - t5
-        cpu = {"value": fs / cpu_s, "unit": "frames/s", "cores": nth, "kind": "port",
-               "sample": f"first {fs} frames of the same {W}x{H} workload (dpl={dpl}, schedule for {fs} frames), "
-                         f"reference-structured CPU path: serial transpose {stages[1]:.2f}s, OpenMP multi-tau "
-                         f"{stages[2]:.2f}s, serial normalise {stages[3]:.2f}s"}
-
-    plan.close()
-    plan2 = None
+        cpu = cpu_baseline_quarter(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl)
     fsnorm = None
     if not args.no_fsnorm and not dense:
         fsnorm = run_fsnorm(dev, imm, payload_off, dlen, W, H, F, dq, sq, dpl)
     formats = None
     if not args.no_formats and not dense:
-        formats = run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, g2)
+        formats = run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, ctx["g2"])
     twotime = None
     if args.twotime_frames > 0 and not dense:
         twotime = run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, min(args.twotime_frames, F),
                               cpu=not args.no_cpu)
+    del ctx, imm, host_imm
+    torch.cuda.empty_cache()
+    dense_leg = None
+    if args.dense_steps > 0 and not dense:
+        dense_leg, dctx = run_job("dense-512", args.dense_steps, 2, autotune=not args.no_autotune, e2e=False)
+        del dctx
+        torch.cuda.empty_cache()
     batched = run_batched(dev, args.batched_jobs, 3) if args.batched_jobs > 0 else None
     streaming = run_streaming(dev, args.stream_seconds) if args.stream_seconds > 0 else None
 
     out = {
         "metric": "sparse 1Mpx frames/s + HBM GB/s; batched jobs/s vs plain; p99 stream latency",
-        "value": frames_per_s,
+        "value": main_job["frames_per_s"],
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_job["ms_per_job"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
-        "data": "synthetic (OU speckle, Poisson counts, compressed IMM generated on device)",
-        "config": {"workload": f"{args.workload}: {W}x{H}, {F} frames, dpl={dpl}, mu={wl['mu']}, "
-                               f"{wl['S']} static / {wl['Q']} dynamic rings",
-                   "frames": F, "taus": T, "pixels": P, "events_level0": E0,
-                   "raw_events": int(dlen.sum()), "parallelism": "single GPU",
-                   "launch_shapes": plan_shapes, "autotuned": tuned is not None},
-        "roofline": roofline,
+        "data": "synthetic (OU speckle, Poisson counts, IMM generated on device)",
+        "config": {"workload": main_job["workload"], "frames": F, "taus": main_job["taus"],
+                   "pixels": main_job["pixels"], "events_level0": main_job["events_level0"],
+                   "raw_events": main_job["raw_events"], "parallelism": "single GPU",
+                   "launch_shapes": main_job["launch_shapes"], "autotuned": main_job["autotuned"]},
+        "roofline": main_job["roofline"],
         "cpu_baseline": cpu,
+        "dense": dense_leg,
         "batched": batched,
         "streaming": streaming,
         "twotime": twotime,
         "formats": formats,
         "normalize_by_framesum": fsnorm,
-        "kernels": kern,
-        "job_alg_gbs": job_bytes / (job_kernel_ms * 1e-3) / 1e9 if job_kernel_ms > 0 else None,
-        "end_to_end_pcie": e2e,
-        "g2_finite_fraction": finite,
-        "setup_seconds": gen_s,
+        "kernels": main_job["kernels"],
+        "row_classes": main_job["row_classes"],
+        "job_alg_gbs": main_job["job_alg_gbs"],
+        "job_hbm_frac": (main_job["job_alg_gbs"] / HBM_PEAK_GBS) if main_job["job_alg_gbs"] else None,
+        "end_to_end_pcie": main_job.get("end_to_end_pcie"),
+        "g2_finite_fraction": main_job["g2_finite_fraction"],
+        "setup_seconds": main_job["setup_seconds"],
     }
     print(json.dumps(out))
 

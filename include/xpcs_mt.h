@@ -295,6 +295,16 @@ xpcs_status xpcs_mt_read_side(xpcs_mt_plan* plan, int32_t window, float* pixel_s
                               float* frame_sum, float* partition_mean, float* partition_partial);
 
 /*
+ * Row classes of the last sparse execute (no reference counterpart: the
+ * reference treats every pixel row alike, src/xpcs/corr.cpp:329-430).  For
+ * class k the compact pixels and level-0 events whose rows hold: 0 no event,
+ * 1 1..64 events (one event per lane), 2 65..128 (two per lane), 3 129..512
+ * (LDS rows), 4 more (dense-series rows).  Five entries each; either array
+ * may be NULL.  XPCS_ERR_STATE for dense ingests (no rows are formed).
+ */
+xpcs_status xpcs_mt_row_stats(xpcs_mt_plan* plan, int64_t* pixels, int64_t* events);
+
+/*
  * Kernel timing (desc.flags & XPCS_MT_TIMING): per kernel name, the summed
  * event-measured milliseconds and launch count since the last reset.
  * Names are written as a '\n'-separated list into names (cap bytes).

@@ -102,17 +102,19 @@ def check_against_oracle(tmp_path, orc, c, F, dpl, avg=1, ff=None, repeats=1, **
     ok = ~np.isnan(r["g2"])
     assert np.array_equal(np.isnan(out["g2"]), ~ok)
     np.testing.assert_allclose(out["g2"][ok], r["g2"][ok] / repeats, rtol=1e-4)
-    np.testing.assert_allclose(out["se"], r["stderr"] / repeats, rtol=1e-4, atol=1e-12)
+    # (stderr of cells whose exact ratios are all equal is 0; from float arrays
+    # with a flat field the ratios differ by float rounding, ~1e-8)
+    np.testing.assert_allclose(out["se"], r["stderr"] / repeats, rtol=1e-4, atol=1e-7)
     # the same arrays through the FP64 normalisation of FP64 sums (pinned to the
     # exact one) and through the float replica of the reference's normalizeG2s
     order = r["qm"]["order"]
     rows = [np.ascontiguousarray(out[k][:, order].T, np.float64) for k in ("G2", "IP", "IF")]
     g2w, sew = orc.normalize_f64_sums(*rows, np.ones(out["level"].size), r["qm"])
     np.testing.assert_allclose(out["g2"][ok], g2w[ok], rtol=1e-9)
-    np.testing.assert_allclose(out["se"], sew, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(out["se"], sew, rtol=1e-9, atol=1e-13)
     g2f, sef = orc.normalize_compat(out["G2"], out["IP"], out["IF"], r["qm"])
     np.testing.assert_allclose(out["g2"][ok], g2f[ok], rtol=1e-4)
-    np.testing.assert_allclose(out["se"], sef, rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(out["se"], sef, rtol=1e-4, atol=1e-6)   # float Welford noise near 0
 
 
 @pytest.mark.gpu

@@ -253,12 +253,14 @@ def test_normalize_ref_f32_arrays(xmt, orc, ff):
     g2n, sen = fresh.normalize_ref_f32(G2r, IPr, IFr)
     assert np.array_equal(fresh.g2()[0], g2n, equal_nan=True)
     assert_g2_close(g2n, g2, 1e-5)
-    assert_g2_close(sen, se, REF_RTOL)
+    # stderr of cells whose exact ratios are all equal is 0; from float arrays
+    # with a flat field the ratios differ by float rounding
+    np.testing.assert_allclose(sen, se, rtol=REF_RTOL, atol=1e-7)
     qm = orc.build_qmap(c["dq"], c["sq"])
     rows = [np.ascontiguousarray(a[:, qm["order"]].T, np.float64) for a in (G2r, IPr, IFr)]
     g2w, sew = orc.normalize_f64_sums(*rows, np.ones(plan.T), qm)
     assert_g2_close(g2n, g2w, 1e-9)
-    assert_g2_close(sen, sew, 1e-9)
+    np.testing.assert_allclose(sen, sew, rtol=1e-9, atol=1e-13)
     g2f, sef = orc.normalize_compat(G2r, IPr, IFr, qm)
     assert_g2_close(g2n, g2f.astype(np.float64), REF_RTOL)
     a = np.ascontiguousarray(G2r)

@@ -1111,9 +1111,11 @@ extern "C" xpcs_status xpcs_mt_read_ref_f32(xpcs_mt_plan* p, float* G2, float* I
 extern "C" xpcs_status xpcs_mt_read_g2(xpcs_mt_plan* p, double* g2, double* se)
 {
     if (!p) return fail(XPCS_ERR_INVALID_ARG, "read_g2: NULL plan");
-    if (!p->executed) return fail(XPCS_ERR_STATE, "read_g2: execute first");
-    xpcs_status st = sync_and_check(p);
-    if (st) return st;
+    if (!p->executed && !p->normalized) return fail(XPCS_ERR_STATE, "read_g2: execute first");
+    if (p->executed) {
+        xpcs_status st = sync_and_check(p);
+        if (st) return st;
+    }
     const size_t n = (size_t)p->h.nq * p->h.T * sizeof(double);
     if (g2) HIP_TRY(hipMemcpy(g2, p->d_g2, n, hipMemcpyDeviceToHost));
     if (se) HIP_TRY(hipMemcpy(se, p->d_se, n, hipMemcpyDeviceToHost));
@@ -1159,6 +1161,7 @@ extern "C" xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* p, const float* G
         return fail(e == hipErrorOutOfMemory ? XPCS_ERR_NOMEM : XPCS_ERR_HIP, "normalize_ref_f32: %s",
                     hipGetErrorString(e));
     collect_timers(p);
+    p->normalized = true;
     return XPCS_OK;
 }
 
@@ -1226,6 +1229,31 @@ extern "C" xpcs_status xpcs_mt_read_side(xpcs_mt_plan* p, int32_t window, float*
         }
         for (int s = 0; s < h.ns; s++)
             partition_mean[s] = (float)(acc[(size_t)s] / ((double)h.pixels_per_bin[(size_t)s] * (double)h.F));
+    }
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_row_stats(xpcs_mt_plan* p, int64_t* pixels, int64_t* events)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "row_stats: NULL plan");
+    if (!p->executed) return fail(XPCS_ERR_STATE, "row_stats: execute first");
+    if (p->mode == 2) return fail(XPCS_ERR_STATE, "row_stats: dense ingests form no rows");
+    xpcs_status st = sync_and_check(p);
+    if (st) return st;
+    const HostTables& h = p->h;
+    std::vector<int64_t> off((size_t)p->nrows + 1);
+    HIP_TRY(hipMemcpy(off.data(), p->d_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t np[5] = {0, 0, 0, 0, 0}, ne[5] = {0, 0, 0, 0, 0};
+    for (int c = 0; c < h.P; c++) {
+        const int64_t row = p->d_rowmap ? (int64_t)h.order[(size_t)c] : c;
+        const int64_t n = off[(size_t)row + 1] - off[(size_t)row];
+        const int k = n == 0 ? 0 : n <= 64 ? 1 : n <= 128 ? 2 : n <= sparse_cap() ? 3 : 4;
+        np[k]++;
+        ne[k] += n;
+    }
+    for (int k = 0; k < 5; k++) {
+        if (pixels) pixels[k] = np[k];
+        if (events) events[k] = ne[k];
     }
     return XPCS_OK;
 }

@@ -216,6 +216,7 @@ struct xpcs_mt_plan {
     // side statistics
     double* d_fs = nullptr;              // [F][S] side statistics (XPCS_MT_SIDE_STATS)
     bool executed = false;
+    bool normalized = false;             // g2/stderr hold an xpcs_mt_normalize_ref_f32 result
 
     // timing
     std::vector<xpcs::KernelTimer> pending;

@@ -41,7 +41,7 @@ EXPORTS = [
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
     "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
     "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records", "xpcs_mt_read_sums_f64",
-    "xpcs_mt_normalize_ref_f32",
+    "xpcs_mt_normalize_ref_f32", "xpcs_mt_row_stats",
 ]
 FMT_UFXC = 1
 FMT_RIGAKU = 2
@@ -135,6 +135,7 @@ def lib():
             "xpcs_mt_ingest_frames": ([vp, vp, C.c_int64, i32], i32),
             "xpcs_packed_records": ([vp, vp, C.c_size_t, i32, vp, vp, vp, C.c_int64, vp], i32),
             "xpcs_mt_normalize_ref_f32": ([vp, vp, vp, vp, i64, i32, vp, vp], i32),
+            "xpcs_mt_row_stats": ([vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -420,6 +421,14 @@ class Plan:
         se = np.zeros((self.T, self.Q), np.float64)
         check(lib().xpcs_mt_read_g2(self._h, _p(g2), _p(se)))
         return g2, se
+
+    def row_stats(self):
+        """Pixels and level-0 events per row class (empty, <=64, 65..128,
+        129..512, longer) of the last sparse execute."""
+        px = np.zeros(5, np.int64)
+        ev = np.zeros(5, np.int64)
+        check(lib().xpcs_mt_row_stats(self._h, _p(px), _p(ev)))
+        return px, ev
 
     def normalize_ref_f32(self, G2, IP, IF):
         """Corr::normalizeG2s of caller arrays in the reference's units,

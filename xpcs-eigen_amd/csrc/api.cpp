@@ -1148,7 +1148,7 @@ extern "C" xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* p, const float* G
         launch_from_ref_f32(p, s, d_in, d_in + n_in, d_in + 2 * n_in, d_sum, d_sum + n_out, d_sum + 2 * n_out);
         // FP64 sums, no flat-field weights (the values carry them)
         launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, d_sum, d_sum + n_out, d_sum + 2 * n_out, true,
-                              nullptr);
+                              nullptr, plan_norm_out(p));
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);

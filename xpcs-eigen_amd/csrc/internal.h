@@ -266,9 +266,18 @@ void launch_fill_u64(uint64_t* d, size_t n, hipStream_t s);
 xpcs_status prepare_execute(xpcs_mt_plan* p);
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus);
+// where a normalisation writes: chunk partials [chunks][S], group values
+// [groups][S], g2 and stderr [tau][Q]
+struct NormOut {
+    NormPartial* part;
+    double* g2s;
+    double* g2;
+    double* se;
+};
+NormOut plan_norm_out(xpcs_mt_plan* p);
 void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
-                           const double* weight);
+                           const double* weight, const NormOut& out);
 // reference-unit [T][W*H] floats -> FP64 sums [P][T] (summed-bin units)
 void launch_from_ref_f32(xpcs_mt_plan* p, hipStream_t s, const float* G2, const float* IP, const float* IF,
                          int64_t* oG2, int64_t* oIP, int64_t* oIF);

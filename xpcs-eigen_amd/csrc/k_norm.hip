@@ -328,10 +328,11 @@ k_from_ref_f32(const float* __restrict__ G2, const float* __restrict__ IP, const
 // ([P][S]), int64 integers or (fp) FP64 bit patterns, each pixel weighted by
 // weight[c] (NULL: 1); taus gives N per tau; the chunk partials and group
 // values use the same stride, g2 and stderr are [tau][q].  Values of other
-// taus are left as they are.
+// taus are left as they are.  The partials, group values and results go to
+// `out` (the plan's buffers, or caller / scratch memory).
 void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
-                           const double* weight)
+                           const double* weight, const NormOut& out)
 {
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
@@ -345,26 +346,29 @@ void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     if (nchunks > 0) {
         if (fp)
             k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
-                                                           weight, p->d_part);
+                                                           weight, out.part);
         else
             k_norm_partial<false><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
-                                                            weight, p->d_part);
+                                                            weight, out.part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)
-        k_norm_group<<<(unsigned)ng, 64, 0, s>>>(p->d_part, p->d_group_chunk_first, p->d_group_size, h.ngroups,
-                                                 taus, t0, t1, S, (fp || weight) ? 1 : 0, p->d_g2s);
+        k_norm_group<<<(unsigned)ng, 64, 0, s>>>(out.part, p->d_group_chunk_first, p->d_group_size, h.ngroups,
+                                                 taus, t0, t1, S, (fp || weight) ? 1 : 0, out.g2s);
     const int64_t nqt = (int64_t)h.nq * nt;
     if (nqt > 0)
-        k_norm_q<<<(unsigned)nqt, 64, 0, s>>>(p->d_part, p->d_g2s, p->d_q_first_group, p->d_q_chunk_first,
-                                              h.nq, t0, t1, S, p->d_g2, p->d_se);
+        k_norm_q<<<(unsigned)nqt, 64, 0, s>>>(out.part, out.g2s, p->d_q_first_group, p->d_q_chunk_first,
+                                              h.nq, t0, t1, S, out.g2, out.se);
     timer_end(p, s);
 }
+
+NormOut plan_norm_out(xpcs_mt_plan* p) { return NormOut{p->d_part, p->d_g2s, p->d_g2, p->d_se}; }
 
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus)
 {
-    launch_normalize_sums(p, s, t0, t1, S, taus, p->d_G2, p->d_IP, p->d_IF, p->fsnorm, p->d_weight);
+    launch_normalize_sums(p, s, t0, t1, S, taus, p->d_G2, p->d_IP, p->d_IF, p->fsnorm, p->d_weight,
+                          plan_norm_out(p));
 }
 
 void launch_from_ref_f32(xpcs_mt_plan* p, hipStream_t s, const float* G2, const float* IP, const float* IF,

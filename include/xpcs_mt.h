@@ -274,6 +274,24 @@ xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* plan, const float* G2, const
                                       int64_t pixels, int32_t taus, double* g2, double* stderr_);
 
 /*
+ * xpcs_mt_normalize_ref_f32 for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  No counterpart in the
+ * reference.  G2, IP, IF are DEVICE arrays [taus][pixels] and g2, stderr_
+ * DEVICE arrays [taus][Q]; the work is enqueued on `stream` (a hipStream_t;
+ * NULL = the plan's stream) and the call returns without waiting, allocating
+ * or freeing anything.  Intermediates go to a fixed 64 MiB device scratch
+ * area per GPU, held from library load until the process ends: jobs needing
+ * more (about 24 x pixels x taus bytes of sums plus the partials) are refused
+ * with XPCS_ERR_UNSUPPORTED.  Every call on one GPU shares that area, so calls
+ * that overlap on one GPU (other threads, other streams) race; serialise them.
+ * Results equal xpcs_mt_normalize_ref_f32's bit for bit; the plan's own
+ * xpcs_mt_read_g2 results are not touched.
+ */
+xpcs_status xpcs_mt_normalize_ref_f32_noalloc(xpcs_mt_plan* plan, const float* G2, const float* IP,
+                                              const float* IF, int64_t pixels, int32_t taus, double* g2,
+                                              double* stderr_, void* stream);
+
+/*
  * Dark image of a dense ingest (DarkImage, src/xpcs/data_structure/dark_image.cpp:81-106):
  * per-pixel Welford mean and standard deviation, [W*H] float64 each (zeros
  * without dark frames); the --darkout DarkAvg/DarkStd datasets.

@@ -119,6 +119,11 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
     }
     HostTables& h = p->h;
     (void)hipGetDevice(&p->device);
+    st = noalloc_setup(p->device);
+    if (st) {
+        delete p;
+        return st;
+    }
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         delete p;
         return fail(XPCS_ERR_HIP, "plan_create: stream creation failed");

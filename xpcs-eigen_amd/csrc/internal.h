@@ -278,6 +278,9 @@ NormOut plan_norm_out(xpcs_mt_plan* p);
 void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
                            const double* weight, const NormOut& out);
+// the statically placed scratch of the allocation-free entry point
+// (noalloc.cpp): its address is resolved once per device, at plan creation
+xpcs_status noalloc_setup(int device);
 // reference-unit [T][W*H] floats -> FP64 sums [P][T] (summed-bin units)
 void launch_from_ref_f32(xpcs_mt_plan* p, hipStream_t s, const float* G2, const float* IP, const float* IF,
                          int64_t* oG2, int64_t* oIP, int64_t* oIF);

@@ -41,7 +41,7 @@ EXPORTS = [
     "xpcs_mt_stream_poll", "xpcs_mt_stream_read_g2", "xpcs_mt_stream_read_sums_i64", "xpcs_mt_stream_close",
     "xpcs_mt_twotime", "xpcs_mt_twotime_info", "xpcs_mt_read_twotime", "xpcs_mt_read_twotime_1d",
     "xpcs_mt_ingest_packed", "xpcs_mt_ingest_frames", "xpcs_packed_records", "xpcs_mt_read_sums_f64",
-    "xpcs_mt_normalize_ref_f32", "xpcs_mt_row_stats",
+    "xpcs_mt_normalize_ref_f32", "xpcs_mt_row_stats", "xpcs_mt_normalize_ref_f32_noalloc",
 ]
 FMT_UFXC = 1
 FMT_RIGAKU = 2
@@ -136,6 +136,7 @@ def lib():
             "xpcs_packed_records": ([vp, vp, C.c_size_t, i32, vp, vp, vp, C.c_int64, vp], i32),
             "xpcs_mt_normalize_ref_f32": ([vp, vp, vp, vp, i64, i32, vp, vp], i32),
             "xpcs_mt_row_stats": ([vp, vp, vp], i32),
+            "xpcs_mt_normalize_ref_f32_noalloc": ([vp, vp, vp, vp, i64, i32, vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -38,38 +38,53 @@ WORKLOADS = {
 }
 
 
-def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev):
+def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev, tdeep=None, dstride=0):
     """Algorithmic bytes per launch of each sparse-path timer: inputs read once,
     outputs written once, each row kernel charged only for its own rows.
 
     px / ev: pixels and level-0 events per row class (plan.row_stats(): empty,
-    1..64, 65..128, 129..512, longer).  Row classes and their kernels: empty
-    and 1..64 -> k_mt_reg (which also lists the others: 4 B each); 65..512 ->
-    the k_mt_lds timer (k_mt_reg2 and the LDS kernel); longer -> k_mt_long.
-    Per row kernel: its rows' events (8 B), compact-to-row map (4 B) and row
-    offsets (2 x 8 B) in, the int64 G2/IP/IF of every tau (24 B per tau) and
-    the pixel total (8 B) out."""
+    1..64, 65..128, 129..512, 513..2048, longer).  Row classes and their
+    kernels: empty and 1..64 -> the k_mt_reg timer (the classification pass,
+    which lists the others at 4 B each, and the register kernel); 65..2048 ->
+    the k_mt_lds timer (k_mt_reg2 and the two LDS kernels); longer ->
+    k_mt_long.  Per row kernel: its rows' events (8 B), compact-to-row map
+    (4 B) and row offsets (2 x 8 B) in, the int64 G2/IP/IF of every tau
+    (24 B per tau) and the pixel total (8 B) out.  With the deep-level
+    hand-over (tdeep = first tau of the hand-over level, dstride = u32 per
+    series row) the non-empty rows up to 128 events write G2 of the taus
+    before tdeep and their series row (4 B x dstride) instead, and k_mt_deep
+    reads every pixel's series row, row offsets and map and writes the G2 of
+    the deep taus of those rows."""
     P = int(px.sum())
     E0 = int(ev.sum())
     bshift = int(tuning["tr_bucket_shift"])
     nb = (nrows + (1 << bshift) - 1) >> bshift
     cf = int(tuning["tr_chunk_frames"])
     nh = nb * max(1, (nsel + cf - 1) // cf)
+    deep = tdeep is not None and dstride > 0
 
     def rows(k):
         n = int(px[list(k)].sum())
-        return 28 * n + 8 * int(ev[list(k)].sum()) + 24 * n * T + 8 * n
+        b = 28 * n + 8 * int(ev[list(k)].sum()) + 16 * n * T + 8 * n
+        for c in k:
+            handed = deep and c in (1, 2)
+            b += int(px[c]) * (8 * (tdeep if handed else T) + (4 * dstride if handed else 0))
+        return b
 
-    return {
+    out = {
         "k_tr_count": 6 * E_raw + 4 * nh,                       # raw events in, histogram out
         "k_tr_scan": 4 * nh + 8 * (nh + 1),                     # histogram in, segment offsets out
         "k_tr_bucket": 6 * E_raw + 8 * E0,                      # raw events in, bucketed events out
         "k_tr_rows": 8 * E0 + 8 * E0 + 8 * (nrows + 1),         # bucketed events in, rows + offsets out
-        "k_mt_reg": rows((0, 1)) + 4 * int(px[2:].sum()),       # + the list entries of the longer rows
-        "k_mt_lds": rows((2, 3)) + 4 * int(px[2:4].sum()),      # + their list entries in
-        "k_mt_long": rows((4,)) + 4 * int(px[4]),
+        "k_mt_classify": 20 * P + 4 * int(px[2:].sum()),        # map + offsets in, list entries out
+        "k_mt_reg": rows((0, 1)),
+        "k_mt_lds": rows((2, 3, 4)) + 4 * int(px[2:5].sum()),   # + their list entries in
+        "k_mt_long": rows((5,)) + 4 * int(px[5]),
         "k_norm": 3 * 8 * n_mapped * T,                         # int64 sums in (partials out are ~1e-3 of it)
     }
+    if deep:
+        out["k_mt_deep"] = P * (4 * dstride + 4 + 16) + 8 * (T - tdeep) * int(px[1] + px[2])
+    return out
 
 
 def dense_pass_bytes(P, F, avg, dpl, levels):
@@ -458,6 +473,51 @@ def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
     return out
 
 
+def run_corr_cli(host_imm, W, H, F, dpl, dq, sq, g2_ref, reps=2):
+    """The drop-in `corr <config.hdf5> --imm <file>` on the same job: the IMM
+    written to a file and a configuration written with every key the reference
+    reads, then the tool run `reps` times (the first with the file just
+    written, so mostly in the page cache).  Wall time of the whole process
+    (start-up, configuration, mmap of the file, ingest, execute, every output
+    dataset written into the HDF5 file, exit) and the stage times the tool
+    prints; norm-0-g2 is read back and compared with the plan's g2."""
+    import shutil
+    import subprocess
+    import tempfile
+    import xpcs_h5
+    corr = ROOT / "xpcs-eigen_amd" / "corr"
+    tmp = Path(tempfile.mkdtemp(prefix="xpcs_bench_"))
+    try:
+        imm = tmp / "data.imm"
+        host_imm.tofile(imm)
+        conf = tmp / "cfg.h5"
+        runs = []
+        for _ in range(reps):
+            xpcs_h5.write_config(conf, width=W, height=H, dqmap=dq, sqmap=sq, imm_path=imm, data_end_todo=F,
+                                 dpl=dpl)
+            t = time.perf_counter()
+            r = subprocess.run([str(corr), str(conf)], capture_output=True, text=True, timeout=300)
+            wall = time.perf_counter() - t
+            if r.returncode != 0:
+                return {"error": r.stderr[-500:]}
+            stages = {}
+            for line in r.stdout.splitlines():
+                if line.startswith("[info] ") and " took " in line and line.endswith(" ms"):
+                    name, ms = line[7:-3].rsplit(" took ", 1)
+                    stages[name] = float(ms)
+            runs.append(dict(seconds=wall, frames_per_s=F / wall, stage_ms=stages))
+        with xpcs_h5.H5(conf) as h:
+            g2 = h.read("/exchange/norm-0-g2", np.float32)
+        ok = ~np.isnan(g2_ref)
+        rel = float(np.max(np.abs(g2[ok] - g2_ref[ok]) / np.abs(g2_ref[ok]))) if ok.any() else 0.0
+        return {"command": "corr cfg.h5 (IMM path from /xpcs/input_file_local)",
+                "imm_bytes": int(host_imm.nbytes), "runs": runs,
+                "best_frames_per_s": max(x["frames_per_s"] for x in runs),
+                "g2_max_rel_diff_vs_plan": rel}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def measured_issue(workload, kernel):
     """VALU issue of `kernel` on `workload` from the committed rocprofv3 SQ
     counter summary (profiles/valu_issue.txt): (VALU wave instructions per
@@ -573,7 +633,10 @@ def run_job(name, steps, warmup, autotune=True, e2e=True):
         px = ev = None
     else:
         px, ev = plan.row_stats()
-        abytes = sparse_kernel_bytes(T, int(dlen.sum()), len(dlen), W * H, info.mapped_pixels, shapes, px, ev)
+        lev = np.asarray(levels)
+        tdeep = int((lev < info.deep_level).sum())
+        abytes = sparse_kernel_bytes(T, int(dlen.sum()), len(dlen), W * H, info.mapped_pixels, shapes, px, ev,
+                                     tdeep if info.deep_stride > 0 else None, info.deep_stride)
     kern = {}
     for k, (ms, n) in ktimes.items():
         avg_ms = ms / max(n, 1)
@@ -599,6 +662,7 @@ def run_job(name, steps, warmup, autotune=True, e2e=True):
              "raw_events": int(dlen.sum()) if not dense else None,
              "row_classes": {"pixels": px.tolist(), "events": ev.tolist()} if px is not None else None,
              "launch_shapes": shapes, "autotuned": tuned is not None,
+             "deep_level": (info.deep_level if info.deep_stride > 0 else None) if not dense else None,
              "ms_per_job": 1000.0 * elapsed / steps, "frames_per_s": F * steps / elapsed,
              "roofline": roofline, "kernels": kern,
              "job_alg_bytes": job_bytes, "job_kernel_ms": job_kernel_ms,
@@ -647,6 +711,7 @@ def main():
                     help="wall time of the streaming leg (0 = skip)")
     ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
     ap.add_argument("--no-fsnorm", action="store_true", help="skip the normalize_by_framesum leg")
+    ap.add_argument("--no-corr", action="store_true", help="skip the timed `corr` command-line leg")
     ap.add_argument("--twotime-frames", type=int, default=8192,
                     help="frames of the two-time leg on the sparse data (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -683,6 +748,12 @@ def main():
     cpu = None
     if not args.no_cpu and not dense:
         cpu = cpu_baseline_quarter(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl)
+    corr_cli = None
+    if not args.no_corr and not dense:
+        if ctx["host_imm"] is None:
+            ctx["host_imm"] = ctx["imm"].cpu().numpy()
+            host_imm = ctx["host_imm"]
+        corr_cli = run_corr_cli(ctx["host_imm"], W, H, F, dpl, dq, sq, ctx["g2"])
     fsnorm = None
     if not args.no_fsnorm and not dense:
         fsnorm = run_fsnorm(dev, imm, payload_off, dlen, W, H, F, dq, sq, dpl)
@@ -733,6 +804,7 @@ def main():
         "job_alg_gbs": main_job["job_alg_gbs"],
         "job_hbm_frac": (main_job["job_alg_gbs"] / HBM_PEAK_GBS) if main_job["job_alg_gbs"] else None,
         "end_to_end_pcie": main_job.get("end_to_end_pcie"),
+        "corr_cli": corr_cli,
         "g2_finite_fraction": main_job["g2_finite_fraction"],
         "setup_seconds": main_job["setup_seconds"],
     }

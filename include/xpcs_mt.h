@@ -95,7 +95,12 @@ typedef struct xpcs_mt_info {
                                    were recomputed in 64-bit integers (total
                                    count above 94906265, beyond exact FP64) */
     int64_t events;             /* level-0 events after the last ingest        */
-    int64_t long_pixels;        /* pixels handled by the dense-row kernel      */
+    int64_t long_pixels;        /* dense ingests: every pixel (dense-row or
+                                   streaming kernels); sparse: pixels whose rows
+                                   hold more than 1024 events                 */
+    int32_t deep_level;         /* sparse path: first level of the one-lane-per-pixel
+                                   kernel (levels above it; = level count: none) */
+    int32_t deep_stride;        /* its series row, u32 values per pixel (0: none) */
 } xpcs_mt_info;
 
 /* Library version string. */
@@ -317,8 +322,9 @@ xpcs_status xpcs_mt_read_side(xpcs_mt_plan* plan, int32_t window, float* pixel_s
  * reference treats every pixel row alike, src/xpcs/corr.cpp:329-430).  For
  * class k the compact pixels and level-0 events whose rows hold: 0 no event,
  * 1 1..64 events (one event per lane), 2 65..128 (two per lane), 3 129..512
- * (LDS rows), 4 more (dense-series rows).  Five entries each; either array
- * may be NULL.  XPCS_ERR_STATE for dense ingests (no rows are formed).
+ * (LDS rows, 8 waves a workgroup), 4 513..2048 (LDS rows, 2 waves a
+ * workgroup), 5 more (dense-series rows).  Six entries each; either array may
+ * be NULL.  XPCS_ERR_STATE for dense ingests (no rows are formed).
  */
 xpcs_status xpcs_mt_row_stats(xpcs_mt_plan* plan, int64_t* pixels, int64_t* events);
 
@@ -341,6 +347,11 @@ xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* plan);
  *   dense_levels     levels per pass of the dense streaming kernel, i.e. the
  *                    depth of its register-resident lag window
  *                    (dpl 4: 3, 4, 5; dpl 8: 2, 3)
+ *   deep_frames      sparse rows of at most 128 events hand their levels over
+ *                    to a one-lane-per-pixel kernel from the first level with
+ *                    at most deep_frames binned frames (16..96; negative: no
+ *                    hand-over; 0: 96).  Only for dpl 4 or 8, T <= 128 and at
+ *                    most 48 taus from that level on (otherwise no hand-over).
  */
 typedef struct xpcs_mt_tuning {
     int32_t tr_bucket_shift;
@@ -348,7 +359,8 @@ typedef struct xpcs_mt_tuning {
     int32_t reg_blocks;
     int32_t lds_blocks;
     int32_t dense_levels;
-    int32_t reserved[3];
+    int32_t deep_frames;
+    int32_t reserved[2];
 } xpcs_mt_tuning;
 
 xpcs_status xpcs_mt_set_tuning(xpcs_mt_plan* plan, const xpcs_mt_tuning* tuning);

@@ -78,6 +78,54 @@ def test_long_rows(xmt, orc):
     assert plan.info().long_pixels > 0
 
 
+@pytest.mark.parametrize("F,mu,cls", [(4000, 0.2, 4), (3000, 1.0, 4), (7000, 3.0, 5)])
+def test_long_row_classes(xmt, orc, F, mu, cls):
+    """Rows of 513..1024 and 1025..2048 events (the two larger LDS kernels,
+    one list shared with the series kernel) and longer rows (series kernel),
+    exact against the oracle; the row statistics see the class."""
+    c = cases.sparse_case(W=12, H=12, F=F, dpl=4, mu=mu, tau_c=20.0, seed=31 + F)
+    plan, r = check_against_oracle(xmt, orc, c)
+    px, ev = plan.row_stats()
+    n = np.diff(r["csr"]["off"])
+    want = np.array([(n == 0).sum(), ((n >= 1) & (n <= 64)).sum(), ((n > 64) & (n <= 128)).sum(),
+                     ((n > 128) & (n <= 512)).sum(), ((n > 512) & (n <= 2048)).sum(), (n > 2048).sum()])
+    assert np.array_equal(px, want) and ev.sum() == n.sum()
+    assert px[cls] > 0
+
+
+@pytest.mark.parametrize("dpl,F,mu", [(4, 1000, 0.05), (8, 3000, 0.03), (8, 2500, 0.03)])
+def test_deep_level_handover(xmt, orc, dpl, F, mu):
+    """The deep levels of rows up to 128 events run one lane per pixel from the
+    first level with at most deep_frames binned frames: every hand-over level
+    (none, 16, 48, 96 frames) gives the oracle's sums bit for bit, rows of the
+    one- and two-register kernels alike."""
+    c = cases.sparse_case(W=24, H=24, F=F, dpl=dpl, mu=mu, tau_c=15.0, seed=61 + F + dpl)
+    r = orc.run_exact(c["imm"], c["dq"], c["sq"], F, dpl)
+    n = np.diff(r["csr"]["off"])
+    assert (n > 64).any() and ((n >= 1) & (n <= 64)).any()
+    levels = []
+    nlev = len(set(orc.schedule(F, dpl)[0].tolist()))
+    for frames in (-1, 16, 48, 96):
+        plan = xmt.Plan(c["W"], c["H"], F, dpl, c["dq"], c["sq"])
+        plan.set_tuning(deep_frames=frames)
+        plan.ingest_imm(c["imm"])
+        plan.execute()
+        G2, IP, IF = plan.sums()
+        assert np.array_equal(IP, r["IP"]) and np.array_equal(IF, r["IF"])
+        assert np.array_equal(G2, r["G2"]), f"G2 differs with deep_frames={frames}"
+        g2, _ = plan.g2()
+        assert_g2_close(g2, r["g2"])
+        info = plan.info()
+        levels.append(info.deep_level)
+        if info.deep_level < nlev:      # a hand-over: a level with at most `frames` binned frames
+            assert frames > 0 and (F >> info.deep_level) <= frames and info.deep_stride >= (F >> info.deep_level)
+        else:
+            assert info.deep_stride == 0
+        plan.close()
+    assert levels[0] == nlev                       # none: the level count
+    assert nlev >= levels[1] >= levels[2] >= levels[3] >= 1 and levels[3] < nlev
+
+
 def test_mid_rows(xmt, orc):
     """Rows of 65..512 events take the LDS kernel."""
     c = cases.sparse_case(W=16, H=16, F=2000, dpl=4, mu=0.08, tau_c=12.0, seed=13)

@@ -22,7 +22,7 @@ using xpcs::fail;
     } while (0)
 
 namespace xpcs {
-int sparse_cap();
+int sparse_cap_long();
 
 template <typename T>
 static xpcs_status dalloc(T** p, size_t n)
@@ -273,6 +273,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
     dfree(p->d_sel_frame); dfree(p->d_dark_sum); dfree(p->d_thr); dfree(p->d_ev); dfree(p->d_off);
     dfree(p->d_b1); dfree(p->d_hist); dfree(p->d_hoff); dfree(p->d_bsum); dfree(p->d_rows); dfree(p->d_long_list); dfree(p->d_mid_list); dfree(p->d_counters);
     dfree(p->d_scratch); dfree(p->d_G2); dfree(p->d_IP); dfree(p->d_IF); dfree(p->d_pix_total); dfree(p->d_fw);
+    dfree(p->d_deep);
     dfree(p->d_part); dfree(p->d_g2s); dfree(p->d_g2); dfree(p->d_se); dfree(p->d_fs); dfree(p->d_series_a); dfree(p->d_series_b); dfree(p->d_bnd);
     for (auto& t : p->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : p->event_pool) (void)hipEventDestroy(e);
@@ -299,6 +300,9 @@ extern "C" xpcs_status xpcs_mt_plan_info(const xpcs_mt_plan* p, xpcs_mt_info* in
         (void)hipMemcpy(cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost);
     info->exact_pixels = p->mode == 2 ? (int32_t)cnt[3] : 0;
     info->long_pixels = p->mode == 2 ? h.P : cnt[0];
+    int stride = 0;
+    info->deep_level = (p->mode == 1 || p->mode == 3) ? deep_level(p, &stride) : (int32_t)h.levels.size();
+    info->deep_stride = stride;
     return XPCS_OK;
 }
 
@@ -929,6 +933,17 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
         xpcs_status st = dalloc(&p->d_scratch, (size_t)p->long_blocks * h.F);
         if (st) return st;
     }
+    if (p->mode == 1 || p->mode == 3) {
+        int stride = 0;
+        if (deep_level(p, &stride) < (int)h.levels.size()) {
+            const size_t need = (size_t)h.P * (size_t)stride;
+            if (need > p->deep_cap) {
+                xpcs_status st = dalloc(&p->d_deep, need);
+                if (st) return st;
+                p->deep_cap = need;
+            }
+        }
+    }
     if (p->mode == 1) {
         if (p->raw_events > p->ev_cap) {
             xpcs_status st = dalloc(&p->d_ev, (size_t)p->raw_events);
@@ -1001,7 +1016,6 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
         launch_transpose(p, s, fused);
         if (p->d_fs && !fused) launch_side(p, s);
         launch_mt_sparse(p, s);
-        launch_mt_long(p, s, false);
     } else if (p->mode == 2 && dense_stream_ok(p)) {
         // thread-per-pixel streaming multi-tau over the raw frames
         if (p->d_fs) launch_side_dense_raw(p, s);
@@ -1012,7 +1026,6 @@ void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
         launch_mt_long(p, s, true);
     } else {
         launch_mt_sparse(p, s);
-        launch_mt_long(p, s, false);
     }
     launch_normalize(p, s);
 }
@@ -1248,15 +1261,15 @@ extern "C" xpcs_status xpcs_mt_row_stats(xpcs_mt_plan* p, int64_t* pixels, int64
     const HostTables& h = p->h;
     std::vector<int64_t> off((size_t)p->nrows + 1);
     HIP_TRY(hipMemcpy(off.data(), p->d_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-    int64_t np[5] = {0, 0, 0, 0, 0}, ne[5] = {0, 0, 0, 0, 0};
+    int64_t np[6] = {0, 0, 0, 0, 0, 0}, ne[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < h.P; c++) {
         const int64_t row = p->d_rowmap ? (int64_t)h.order[(size_t)c] : c;
         const int64_t n = off[(size_t)row + 1] - off[(size_t)row];
-        const int k = n == 0 ? 0 : n <= 64 ? 1 : n <= 128 ? 2 : n <= sparse_cap() ? 3 : 4;
+        const int k = n == 0 ? 0 : n <= 64 ? 1 : n <= 128 ? 2 : n <= 512 ? 3 : n <= sparse_cap_long() ? 4 : 5;
         np[k]++;
         ne[k] += n;
     }
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < 6; k++) {
         if (pixels) pixels[k] = np[k];
         if (events) events[k] = ne[k];
     }

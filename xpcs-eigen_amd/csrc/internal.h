@@ -199,7 +199,10 @@ struct xpcs_mt_plan {
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]
     int32_t* d_mid_list = nullptr;       // [P] rows of 65..CAP events
-    uint32_t* d_counters = nullptr;      // [0] long rows, [1] error flags, [2] mid rows, [3] dense exact-redo pixels
+    uint32_t* d_counters = nullptr;      // [0] rows over 1024 events, [1] error flags, [2] rows of 65..128,
+                                         // [3] dense exact-redo pixels, [4] rows of 129..512, [5] 513..1024
+    uint32_t* d_deep = nullptr;          // [P][deep stride] level series handed to the one-lane-per-pixel kernel
+    size_t deep_cap = 0;
     uint64_t* d_scratch = nullptr;
     size_t scratch_cap = 0;
     int long_blocks = 0;
@@ -252,6 +255,9 @@ constexpr int64_t kSideFuseDoubles = 4096;   // LDS frame-sum table of the fused
 bool side_fused(const xpcs_mt_plan* p);
 EvFormat ev_format(const xpcs_mt_plan* p);
 void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
+// first level of the one-lane-per-pixel kernel of the sparse path (the level
+// count: none) and its series row length in u32 (0: none)
+int deep_level(const xpcs_mt_plan* p, int* stride);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_side(xpcs_mt_plan* p, hipStream_t s);

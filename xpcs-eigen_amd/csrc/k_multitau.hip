@@ -626,6 +626,28 @@ __device__ __forceinline__ V event_value(uint32_t c, uint32_t f, const double* _
     }
 }
 
+// Hand-over of the deep levels to k_mt_deep: the series of the first level
+// with at most kDeepMaxFrames binned frames, written by the register kernels
+// as one dense row of `stride` u32 per pixel ([P][stride]).
+constexpr int kDeepMaxFrames = 96;
+constexpr int kDeepMaxTaus = 48;     // deep G2 staged through the series area: 64 x 48 x 8 B fits
+constexpr int kDeepMaxLevels = 6;
+
+// One row's level series (keys < stride, distinct) as a dense row at dst,
+// staged through the wave's LDS words ser[0, stride).
+__device__ __forceinline__ void write_deep_row(uint32_t* ser, int stride, uint32_t k0, uint32_t v0, bool has0,
+                                               uint32_t k1, uint32_t v1, bool has1, uint32_t* __restrict__ dst)
+{
+    const int lane = lane_id();
+    for (int t = lane; t < stride; t += kWave) ser[t] = 0u;
+    wave_sync();
+    if (has0) ser[k0] = v0;
+    if (has1) ser[k1] = v1;
+    wave_sync();
+    for (int t = lane; t < stride; t += kWave) dst[t] = ser[t];
+    wave_sync();
+}
+
 // Rows r < *count of list[r] with 65..128 events (T <= kRegMaxTaus).  V is
 // uint32_t (counts) or double (counts x fw[frame], normalize_by_framesum,
 // main.cpp:313-336; pix_total stays the raw count total).
@@ -635,11 +657,13 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
           const uint32_t* __restrict__ count, int32_t T, const int4* __restrict__ tau_bnd,
           const uint16_t* __restrict__ frame_rank, int32_t nb, const LevelParam* __restrict__ levels,
           int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
-          uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
+          uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
+          int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
     __shared__ uint32_t s_stage[WAVES][reg2_stage_words<V>()];
+    const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
     const int lane = lane_id();
     const int w = wave_id();
     A* acc = s_acc[w];
@@ -699,12 +723,22 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
                 else nl = reg_level(k0, v0, nl, 1, (uint32_t)lp.lf, true);
                 two = nl > kWave;
             }
-            if (nl < 2) break;
+            if constexpr (sizeof(V) == 4) {
+                if (L == ldeep) {   // the staging slots are free between levels
+                    write_deep_row(stage, dstride, k0, v0, lane < nl, k1, v1, two && lane + kWave < nl,
+                                   deep + p * dstride);
+                    break;
+                }
+            }
+            if (nl < 2) {
+                if (ldeep >= nlev) break;
+                continue;   // no more pairs; bin what is left down to the hand-over level
+            }
             if (two) reg2_lags(k0, v0, k1, v1, nl, lp, acc);
             else reg_lags(k0, v0, nl, lp, acc);
         }
         wave_sync();
-        for (int i = lane; i < T; i += kWave) {
+        for (int i = lane; i < tdeep; i += kWave) {
             A g = A(0);
 #pragma unroll
             for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
@@ -714,10 +748,82 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
     }
 }
 
-// Rows of at most 64 events, one wave each; longer rows are listed for the
-// other kernels: up to min(kMidSmall, mid_cap) events at the front of
-// mid_list, up to mid_cap at its back, the rest in long_list.  With T >
-// kRegMaxTaus every non-empty row is listed.  V as for k_mt_reg2.
+// Row classes of the sparse path, listed by k_mt_classify for the kernels
+// after k_mt_reg (list: counter slot, front or back):
+//   k_mt_reg2 / k_mt_sparse<kMidSmall>  rows of 65..kMidSmall events (every
+//                                       non-empty row up to kMidSmall when
+//                                       T > kRegMaxTaus): mid_list front, [2]
+//   k_mt_sparse<kSparseCapUsed>         up to kSparseCapUsed: mid_list back, [4]
+//   k_mt_sparse<kSparseCapMid>          up to kSparseCapMid: long_list front, [5]
+//   k_mt_sparse<kSparseCapLong>,        longer rows: long_list back, [0]; the
+//   k_mt_long                           LDS kernel takes those up to
+//                                       kSparseCapLong, the series kernel the rest
+constexpr int kSparseCapUsed = 512;
+constexpr int kSparseCapMid = 1024;
+constexpr int kSparseCapLong = 2048;
+constexpr int kClassifyPer = 8;      // pixels per thread of the classification pass
+
+// kClassifyPer x 256 consecutive compact pixels per workgroup: each row's
+// class, appended to its list.  Positions come from LDS counters of the
+// workgroup (one LDS atomic per wave and class) and one global atomic per
+// class and workgroup, so the shared counters see few atomics.  The lists'
+// order does not matter: every row writes only its own sums.
+__global__ void __launch_bounds__(256)
+k_mt_classify(const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap, int32_t P, int32_t T,
+              int32_t* __restrict__ mid_list, int32_t* __restrict__ long_list, uint32_t* __restrict__ counters)
+{
+    __shared__ uint32_t s_cnt[4];
+    __shared__ uint32_t s_base[4];
+    if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t p0 = (int64_t)blockIdx.x * (256 * kClassifyPer) + threadIdx.x;
+    int cls[kClassifyPer];
+    uint32_t pos[kClassifyPer];
+#pragma unroll
+    for (int i = 0; i < kClassifyPer; i++) {
+        const int64_t p = p0 + (int64_t)i * 256;
+        cls[i] = -1;
+        pos[i] = 0u;
+        if (p < P) {
+            const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+            const int64_t n = off[row + 1] - off[row];
+            const bool reg = n == 0 || (n <= kWave && T <= kRegMaxTaus);
+            if (!reg) cls[i] = n <= kMidSmall ? 0 : n <= kSparseCapUsed ? 1 : n <= kSparseCapMid ? 2 : 3;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint64_t m = __ballot(cls[i] == c);
+            if (m == 0ull) continue;
+            const int leader = (int)__builtin_ctzll(m);
+            uint32_t b = 0u;
+            if (lane == leader) b = atomicAdd(&s_cnt[c], (uint32_t)__popcll(m));
+            b = read_lane_u32(b, leader);
+            if (cls[i] == c) pos[i] = b + (uint32_t)mask_rank(m);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int c = (int)threadIdx.x;
+        s_base[c] = s_cnt[c] ? atomicAdd(&counters[c == 0 ? 2 : c == 1 ? 4 : c == 2 ? 5 : 0], s_cnt[c]) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kClassifyPer; i++) {
+        const int c = cls[i];
+        if (c < 0) continue;
+        const int64_t p = p0 + (int64_t)i * 256;
+        const int64_t q = (int64_t)s_base[c] + pos[i];
+        if (c == 0) mid_list[q] = (int32_t)p;
+        else if (c == 1) mid_list[P - 1 - q] = (int32_t)p;
+        else if (c == 2) long_list[q] = (int32_t)p;
+        else long_list[P - 1 - q] = (int32_t)p;
+    }
+}
+
+// Rows of at most 64 events, one wave each (T <= kRegMaxTaus), and the empty
+// rows; k_mt_classify lists the others for the kernels after this one.  V as
+// for k_mt_reg2.
 // 88 scalar registers leave room for 8 waves per SIMD (106 would allow 7)
 template <int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_num_sgpr(88)))
@@ -725,15 +831,16 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
          const int4* __restrict__ tau_bnd, const uint16_t* __restrict__ frame_rank, int32_t nb,
          const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
-         uint64_t* __restrict__ pix_total, int32_t* __restrict__ mid_list,
-         int32_t* __restrict__ long_list, uint32_t* __restrict__ counters, int32_t mid_cap,
-         const int32_t* __restrict__ rowmap, const double* __restrict__ fw)
+         uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
+         int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
+    __shared__ uint32_t s_ser[WAVES][sizeof(V) == 4 ? kDeepMaxFrames : 1];
     const int lane = lane_id();
     const int w = wave_id();
     A* acc = s_acc[w];
+    const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
     const int64_t nw = (int64_t)gridDim.x * WAVES;
     const bool reg_ok = T <= kRegMaxTaus;
 
@@ -741,23 +848,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;   // CSR row of pixel p
         const int64_t base = off[row];
         const int64_t n64 = off[row + 1] - base;
-        if (n64 > kWave || (!reg_ok && n64 > 0)) {
-            if (lane == 0) {
-                // rows of up to kMidSmall events fill mid_list from the front,
-                // longer ones up to mid_cap from the back
-                if (n64 <= kMidSmall && n64 <= mid_cap) {
-                    const uint32_t q = atomicAdd(&counters[2], 1u);
-                    mid_list[q] = (int32_t)p;
-                } else if (n64 <= mid_cap) {
-                    const uint32_t q = atomicAdd(&counters[4], 1u);
-                    mid_list[P - 1 - (int64_t)q] = (int32_t)p;
-                } else {
-                    const uint32_t q = atomicAdd(&counters[0], 1u);
-                    long_list[q] = (int32_t)p;
-                }
-            }
-            continue;
-        }
+        if (n64 > kWave || (!reg_ok && n64 > 0)) continue;   // listed by k_mt_classify
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
@@ -804,17 +895,138 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         for (int L = 0; L < nlev; L++) {
             const LevelParam lp = levels[L];
             if (L > 0) nl = reg_level(k, v, nl, 1, (uint32_t)lp.lf, true);
-            if (nl < 2) break;
+            if constexpr (sizeof(V) == 4) {
+                if (L == ldeep) {
+                    write_deep_row(s_ser[w], dstride, k, v, lane < nl, 0u, 0u, false, deep + p * dstride);
+                    break;
+                }
+            }
+            if (nl < 2) {
+                if (ldeep >= nlev) break;
+                continue;   // no more pairs; bin what is left down to the hand-over level
+            }
             reg_lags(k, v, nl, lp, acc);
         }
         wave_sync();
-        for (int i = lane; i < T; i += kWave) {
+        for (int i = lane; i < tdeep; i += kWave) {
             A g = A(0);
 #pragma unroll
             for (int c = 0; c < kAccCopies; c++) g += acc[i * kAccCopies + c];
             g2row[i] = sum_bits(g);
         }
         wave_sync();
+    }
+}
+
+// The deep levels of the rows the register kernels handled (1..kMidSmall
+// events), one lane per pixel: levels ldeep .. nlev-1, whose series are
+// short, cost a wave per row one lag search each but only a few instructions
+// per value here.  A wave takes 64 consecutive pixels: their series rows
+// (contiguous in HBM) go to LDS, then each lane streams its own series.  A
+// level value x[t] adds x[t-d] x[t] to the lag-d sum for d = dpl+1 .. 2 dpl
+// (the last 2 dpl values in registers, indexed modulo 2 dpl over an unrolled
+// block, so no value moves) and, at odd t, x[t-1] + x[t] goes to index t/2:
+// the next level's series, in place.  The deep taus' G2 leave through the
+// same LDS area, one contiguous run per pixel.  Exact: u32 values, 64-bit
+// integer sums.
+template <int DPL>
+__global__ void __launch_bounds__(kWave)
+k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __restrict__ off,
+          const int32_t* __restrict__ rowmap, int32_t P, int32_t T, const LevelParam* __restrict__ levels,
+          int32_t ldeep, int32_t nlev, int64_t* __restrict__ G2)
+{
+    constexpr int W = 2 * DPL;
+    constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the lanes' reads of one index hit distinct banks
+    __shared__ uint32_t s_x[kWave * SP];
+    const int lane = lane_id();
+    const int64_t p0 = (int64_t)blockIdx.x * kWave;
+    const int np = (int)min<int64_t>(kWave, (int64_t)P - p0);
+    {
+        // the wave's np rows are one contiguous run of np * stride u32 (stride
+        // % 4 == 0): every 16-byte load in flight at once, then into LDS
+        typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+        constexpr int kV4 = kDeepMaxFrames / 4;        // 16-byte loads per lane at most
+        const int s4 = stride >> 2;
+        const int n4 = np * s4;
+        const v4u* src = reinterpret_cast<const v4u*>(series + p0 * (int64_t)stride);
+        v4u v[kV4];
+#pragma unroll
+        for (int k = 0; k < kV4; k++) {
+            const int i = k * kWave + lane;
+            if (i < n4) v[k] = __builtin_nontemporal_load(src + i);
+        }
+#pragma unroll
+        for (int k = 0; k < kV4; k++) {
+            const int i = k * kWave + lane;
+            if (i < n4) {
+                const int r = i / s4;
+                uint32_t* d = s_x + r * SP + (i - r * s4) * 4;
+                d[0] = v[k].x;
+                d[1] = v[k].y;
+                d[2] = v[k].z;
+                d[3] = v[k].w;
+            }
+        }
+    }
+    bool mine = false;
+    if (lane < np) {
+        const int64_t p = p0 + lane;
+        const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+        const int64_t n = off[row + 1] - off[row];
+        mine = n >= 1 && n <= kMidSmall;
+    }
+    wave_sync();
+    uint32_t* x = s_x + lane * SP;
+    unsigned long long res[kDeepMaxLevels][DPL];
+#pragma unroll
+    for (int j = 0; j < kDeepMaxLevels; j++) {
+#pragma unroll
+        for (int e = 0; e < DPL; e++) res[j][e] = 0ull;
+        if (ldeep + j >= nlev) continue;
+        const int lf = levels[ldeep + j].lf;
+        uint32_t buf[W];
+#pragma unroll
+        for (int u = 0; u < W; u++) buf[u] = 0u;
+        for (int t0 = 0; t0 < lf; t0 += W) {
+#pragma unroll
+            for (int u = 0; u < W; u++) {
+                const int t = t0 + u;
+                if (t < lf) {
+                    const uint32_t val = x[t];
+#pragma unroll
+                    for (int e = 0; e < DPL; e++)
+                        res[j][e] += (unsigned long long)buf[(u + W - (DPL + 1 + e)) % W] * val;
+                    if (u & 1) x[t >> 1] = buf[u - 1] + val;
+                    buf[u] = val;
+                }
+            }
+        }
+    }
+    wave_sync();
+    // results to LDS (two u32 per value, odd pitch), then one run per pixel
+    const int tdeep = levels[ldeep].first;
+    const int nd = T - tdeep;
+    const int gp = 2 * nd + 1;
+#pragma unroll
+    for (int j = 0; j < kDeepMaxLevels; j++) {
+        if (ldeep + j >= nlev) continue;
+        const LevelParam lp = levels[ldeep + j];
+#pragma unroll
+        for (int e = 0; e < DPL; e++) {
+            if (e < lp.count) {
+                const int i = lp.first + e - tdeep;
+                s_x[lane * gp + 2 * i] = (uint32_t)res[j][e];
+                s_x[lane * gp + 2 * i + 1] = (uint32_t)(res[j][e] >> 32);
+            }
+        }
+    }
+    wave_sync();
+    const uint64_t mine_mask = __ballot(mine);
+    for (int r = 0; r < np; r++) {
+        if (((mine_mask >> r) & 1ull) == 0ull) continue;
+        int64_t* dst = G2 + (p0 + r) * (int64_t)T + tdeep;
+        for (int i = lane; i < nd; i += kWave)
+            dst[i] = (int64_t)(((uint64_t)s_x[r * gp + 2 * i + 1] << 32) | s_x[r * gp + 2 * i]);
     }
 }
 
@@ -883,7 +1095,8 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
-        const int n = (int)n64;         // 0 < n <= CAP by construction of the list
+        if (n64 > CAP) continue;        // a list shared with the series kernel (its longer rows)
+        const int n = (int)n64;         // 0 < n <= CAP
         const int m = next_pow2(n);
         bool m_sorted = false;
         // ---- load and sort by frame ----
@@ -1125,7 +1338,8 @@ __global__ void __launch_bounds__(kLongBlock)
 k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
           const uint32_t* __restrict__ rows, int32_t P, int32_t T,
           const LevelParam* __restrict__ levels, int32_t nlev, int32_t F,
-          const int32_t* __restrict__ long_list, const uint32_t* __restrict__ counters, int dense,
+          const int32_t* __restrict__ long_list, int32_t step, int64_t min_len, const uint32_t* __restrict__ counters,
+          int dense,
           const double* __restrict__ fw, uint64_t* __restrict__ scratch, int64_t* __restrict__ G2,
           int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
           const int32_t* __restrict__ rowmap)
@@ -1135,7 +1349,12 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
     AccT* x = reinterpret_cast<AccT*>(scratch + (size_t)blockIdx.x * (size_t)F);
     const int64_t nrows = dense ? (int64_t)P : (int64_t)counters[0];
     for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
-        const int64_t p = dense ? r : (int64_t)long_list[r];
+        const int64_t p = dense ? r : (int64_t)long_list[r * step];
+        if (!dense) {
+            // the list is shared with the LDS kernel, which takes rows up to min_len
+            const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
+            if (off[row + 1] - off[row] <= min_len) continue;
+        }
         int64_t* g2row = G2 + p * T;
         int64_t* iprow = IP + p * T;
         int64_t* ifrow = IF + p * T;
@@ -1192,43 +1411,118 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
 
 // ---- launchers ------------------------------------------------------------------
 constexpr int kSparseWaves = 8;
-constexpr int kSparseCapUsed = 512;
+constexpr int kSparseMidWaves = 4;    // 4 x 16 KB of LDS rows per workgroup (counts): 2 waves per SIMD
+constexpr int kSparseLongWaves = 2;   // 2 x 32 KB of LDS rows per workgroup (counts)
 constexpr int kRegWaves = 8;
 constexpr int kReg2Waves = 4;
 
-void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
+// The row kernels of the sparse path, counts (V = uint32_t, exact 64-bit
+// sums) or frame-weighted values (V = double, normalize_by_framesum): the
+// classification pass, one wave per register row, then the listed rows.
+template <typename V>
+static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const char* t_lds, const char* t_long,
+                        const double* fw)
 {
     const HostTables& h = p->h;
-    timer_begin(p, s, "k_mt_reg");
-    k_mt_reg<kRegWaves, uint32_t><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
-        p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list,
-        p->d_long_list, p->d_counters, kSparseCapUsed, p->d_rowmap, nullptr);
+    timer_begin(p, s, "k_mt_classify");
+    if (h.P > 0)
+        k_mt_classify<<<(unsigned)((h.P + 256 * kClassifyPer - 1) / (256 * kClassifyPer)), 256, 0, s>>>(
+            p->d_off, p->d_rowmap, h.P, h.T, p->d_mid_list, p->d_long_list, p->d_counters);
     timer_end(p, s);
-    timer_begin(p, s, "k_mt_lds");
+    timer_begin(p, s, t_reg);
+    // the deep levels' hand-over (counts only)
+    const int nlev = (int)h.levels.size();
+    int dstride = 0;
+    const int ldeep = sizeof(V) == 4 ? deep_level(p, &dstride) : nlev;
+    k_mt_reg<kRegWaves, V><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
+        p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels, nlev, h.F, p->d_G2,
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep, dstride);
+    timer_end(p, s);
+    timer_begin(p, s, t_lds);
     if (h.T <= kRegMaxTaus)
-        k_mt_reg2<kReg2Waves, uint32_t><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
+        k_mt_reg2<kReg2Waves, V><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank,
-            p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap,
-            nullptr);
+            p->d_levels, nlev, h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep,
+            dstride);
     else
-        k_mt_sparse<kMidSmall, kSparseWaves, uint32_t><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
+        k_mt_sparse<kMidSmall, kSparseWaves, V><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
-    k_mt_sparse<kSparseCapUsed, kSparseWaves, uint32_t><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+    k_mt_sparse<kSparseCapUsed, kSparseWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, nullptr);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+    k_mt_sparse<kSparseCapMid, kSparseMidWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseMidWaves, 0, s>>>(
+        p->d_ev, p->d_off, p->d_long_list, 1, p->d_counters + 5, p->d_taus, h.T, p->d_levels,
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+    k_mt_sparse<kSparseCapLong, kSparseLongWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseLongWaves, 0, s>>>(
+        p->d_ev, p->d_off, p->d_long_list + (h.P - 1), -1, p->d_counters, p->d_taus, h.T, p->d_levels,
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
     timer_end(p, s);
+    // rows longer than kSparseCapLong: the dense-series kernel
+    timer_begin(p, s, t_long);
+    if (sizeof(V) == 4)
+        k_mt_long<uint64_t><<<p->long_blocks, kLongBlock, 0, s>>>(
+            p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F,
+            p->d_long_list + (h.P - 1), -1, (int64_t)kSparseCapLong, p->d_counters, 0, nullptr, p->d_scratch,
+            p->d_G2, p->d_IP, p->d_IF,
+            p->d_pix_total, p->d_rowmap);
+    else
+        k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
+            p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F,
+            p->d_long_list + (h.P - 1), -1, (int64_t)kSparseCapLong, p->d_counters, 0, fw, p->d_scratch,
+            p->d_G2, p->d_IP, p->d_IF,
+            p->d_pix_total, p->d_rowmap);
+    timer_end(p, s);
+    if (ldeep < nlev && h.P > 0) {
+        timer_begin(p, s, "k_mt_deep");
+        const unsigned nb = (unsigned)((h.P + kWave - 1) / kWave);
+        if (h.dpl == 4)
+            k_mt_deep<4><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
+                                              ldeep, nlev, p->d_G2);
+        else
+            k_mt_deep<8><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
+                                              ldeep, nlev, p->d_G2);
+        timer_end(p, s);
+    }
 }
 
+// First level of the one-lane-per-pixel kernel: the first level L >= 1 with
+// at most deep_frames binned frames (tuning; 0: kDeepMaxFrames, negative:
+// none), for counts (not the FP64 frame-sum mode), dpl 4 or 8 (the kernel's
+// instantiations), T <= kRegMaxTaus (the register kernels run) and at most
+// kDeepMaxTaus taus / kDeepMaxLevels levels from L on.
+int deep_level(const xpcs_mt_plan* p, int* stride)
+{
+    const HostTables& h = p->h;
+    const int nlev = (int)h.levels.size();
+    *stride = 0;
+    const int lim = p->tune.deep_frames < 0 ? 0 : (p->tune.deep_frames > 0 ? p->tune.deep_frames : kDeepMaxFrames);
+    if (lim == 0 || p->fsnorm || (h.dpl != 4 && h.dpl != 8) || h.T > kRegMaxTaus) return nlev;
+    for (int L = 1; L < nlev; L++) {
+        const LevelParam& lp = h.levels[(size_t)L];
+        if (lp.lf > lim) continue;
+        if (h.T - lp.first > kDeepMaxTaus || nlev - L > kDeepMaxLevels || lp.lo != h.dpl + 1) return nlev;
+        *stride = (lp.lf + 3) & ~3;
+        return L;
+    }
+    return nlev;
+}
+
+void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s)
+{
+    launch_rows<uint32_t>(p, s, "k_mt_reg", "k_mt_lds", "k_mt_long", nullptr);
+}
+
+// Every dense row through the dense-series kernel (the fallback of the dense
+// path for delays per level outside the streaming kernel's instantiations).
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
-    timer_begin(p, s, dense_rows ? "k_mt_long_dense" : "k_mt_long");
+    if (!dense_rows) return;   // sparse rows: launch_mt_sparse runs the long rows too
+    timer_begin(p, s, "k_mt_long_dense");
     k_mt_long<uint64_t><<<p->long_blocks, kLongBlock, 0, s>>>(
-        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
-        p->d_counters, dense_rows ? 1 : 0, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
-        p->d_rowmap);
+        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list, 1, 0,
+        p->d_counters, 1, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
     timer_end(p, s);
 }
 
@@ -1241,36 +1535,17 @@ void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
 {
     const HostTables& h = p->h;
     if (!dense_rows) {
-        timer_begin(p, s, "k_mt_reg_f64");
-        k_mt_reg<kRegWaves, double><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
-            p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_mid_list,
-            p->d_long_list, p->d_counters, kSparseCapUsed, p->d_rowmap, p->d_fw);
-        timer_end(p, s);
-        timer_begin(p, s, "k_mt_lds_f64");
-        if (h.T <= kRegMaxTaus)
-            k_mt_reg2<kReg2Waves, double><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
-                p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank,
-                p->nrank, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
-                p->d_rowmap, p->d_fw);
-        else
-            k_mt_sparse<kMidSmall, kSparseWaves, double><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
-                p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
-                (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
-        k_mt_sparse<kSparseCapUsed, kSparseWaves, double><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0,
-                                                            s>>>(
-            p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, p->d_fw);
-        timer_end(p, s);
+        launch_rows<double>(p, s, "k_mt_reg_f64", "k_mt_lds_f64", "k_mt_long_f64", p->d_fw);
+        return;
     }
     timer_begin(p, s, "k_mt_long_f64");
     k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
-        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list,
-        p->d_counters, dense_rows ? 1 : 0, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total,
-        p->d_rowmap);
+        p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list, 1, 0,
+        p->d_counters, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
     timer_end(p, s);
 }
 
+int sparse_cap_long() { return kSparseCapLong; }
 int sparse_cap() { return kSparseCapUsed; }
 
 // Workgroups of the row kernels: one row per wave up to 16384 workgroups by

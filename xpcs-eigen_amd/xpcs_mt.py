@@ -66,7 +66,7 @@ class Info(C.Structure):
     _fields_ = [("taus", C.c_int32), ("max_level", C.c_int32), ("dynamic_bins", C.c_int32),
                 ("static_bins", C.c_int32), ("pixels", C.c_int32), ("mapped_pixels", C.c_int32),
                 ("groups", C.c_int32), ("exact_pixels", C.c_int32), ("events", C.c_int64),
-                ("long_pixels", C.c_int64)]
+                ("long_pixels", C.c_int64), ("deep_level", C.c_int32), ("deep_stride", C.c_int32)]
 
 
 _LIB = None
@@ -208,7 +208,8 @@ class TTDesc(C.Structure):
 class Tuning(C.Structure):
     """Launch shapes of a plan (include/xpcs_mt.h, 0 = automatic)."""
     _fields_ = [("tr_bucket_shift", C.c_int32), ("tr_chunk_frames", C.c_int32), ("reg_blocks", C.c_int32),
-                ("lds_blocks", C.c_int32), ("dense_levels", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("lds_blocks", C.c_int32), ("dense_levels", C.c_int32), ("deep_frames", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -425,9 +426,9 @@ class Plan:
 
     def row_stats(self):
         """Pixels and level-0 events per row class (empty, <=64, 65..128,
-        129..512, longer) of the last sparse execute."""
-        px = np.zeros(5, np.int64)
-        ev = np.zeros(5, np.int64)
+        129..512, 513..2048, longer) of the last sparse execute."""
+        px = np.zeros(6, np.int64)
+        ev = np.zeros(6, np.int64)
         check(lib().xpcs_mt_row_stats(self._h, _p(px), _p(ev)))
         return px, ev
 

@@ -126,6 +126,42 @@ def test_deep_level_handover(xmt, orc, dpl, F, mu):
     assert nlev >= levels[1] >= levels[2] >= levels[3] >= 1 and levels[3] < nlev
 
 
+@pytest.mark.parametrize("dpl", [4, 8])
+def test_deep_level_large_totals(xmt, orc, dpl):
+    """Deep levels of rows with large counts, handed over at 96 frames: a
+    wave holding pixels of total 90000 (30 events) and 3000000 (100 events,
+    deep lag sums beyond 2^32) and a wave whose largest pixel totals 65400 in
+    20 consecutive frames, exact against the oracle."""
+    F = 2500
+    c = cases.sparse_case(W=24, H=24, F=F, dpl=dpl, mu=0.03, tau_c=15.0, seed=67 + dpl)
+    order = orc.run_exact(c["imm"], c["dq"], c["sq"], F, dpl)["qm"]["order"]
+    counts = c["counts"]
+    rng = np.random.default_rng(5 + dpl)
+    for k, nev, val in ((70, 30, 3000), (71, 100, 30000)):      # the second wave
+        counts[:, order[k]] = 0
+        counts[np.sort(rng.choice(F, nev, replace=False)), order[k]] = val
+    counts[:, order[200]] = 0                                     # the fourth wave
+    counts[1000:1020, order[200]] = 3270
+    c["imm"] = synth.imm_bytes_sparse(counts, 24, 24)
+    r = orc.run_exact(c["imm"], c["dq"], c["sq"], F, dpl)
+    assert np.array_equal(r["qm"]["order"], order)
+    tot = counts.astype(np.int64).sum(axis=0)[order]
+    assert tot[64:128].max() == 3000000 and tot[192:256].max() == 65400
+    assert np.diff(r["csr"]["off"]).max() <= 128
+    plan = xmt.Plan(c["W"], c["H"], F, dpl, c["dq"], c["sq"])
+    plan.set_tuning(deep_frames=96)
+    plan.ingest_imm(c["imm"])
+    plan.execute()
+    info = plan.info()
+    assert info.deep_stride > 0
+    G2, IP, IF = plan.sums()
+    assert np.array_equal(IP, r["IP"]) and np.array_equal(IF, r["IF"])
+    assert np.array_equal(G2, r["G2"])
+    lev, _ = plan.schedule()
+    assert int(G2[:, lev >= info.deep_level].max()) >= 2 ** 32     # 64-bit deep lag sums reached
+    plan.close()
+
+
 def test_mid_rows(xmt, orc):
     """Rows of 65..512 events take the LDS kernel."""
     c = cases.sparse_case(W=16, H=16, F=2000, dpl=4, mu=0.08, tau_c=12.0, seed=13)

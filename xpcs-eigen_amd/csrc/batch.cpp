@@ -140,6 +140,11 @@ extern "C" xpcs_status xpcs_mt_autotune(xpcs_mt_plan* p, xpcs_mt_tuning* best_ou
         }
         axes.push_back({&xpcs_mt_tuning::reg_blocks, {1024, 4096}});
         axes.push_back({&xpcs_mt_tuning::lds_blocks, {256, 2048, 4096, 8192}});
+        // deep-level hand-over: none, or from the first level with <= 48 frames
+        // (the default is <= 96)
+        int dstride = 0;
+        if (deep_level(p, &dstride) < (int)p->h.levels.size())
+            axes.push_back({&xpcs_mt_tuning::deep_frames, {-1, 48}});
     } else if (dense_stream) {
         axes.push_back({&xpcs_mt_tuning::dense_levels,
                         p->h.dpl == 4 ? std::vector<int32_t>{3, 4, 5} : std::vector<int32_t>{2, 3}});

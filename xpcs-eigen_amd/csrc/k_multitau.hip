@@ -918,6 +918,43 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
     }
 }
 
+// Levels ldeep .. nlev-1 of one lane's series x (in place), 64-bit lag sums.
+template <int DPL>
+__device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __restrict__ levels, int ldeep, int nlev,
+                                            unsigned long long (&res)[kDeepMaxLevels][DPL])
+{
+    constexpr int W = 2 * DPL;
+#pragma unroll
+    for (int j = 0; j < kDeepMaxLevels; j++) {
+        unsigned long long acc[DPL];
+#pragma unroll
+        for (int e = 0; e < DPL; e++) acc[e] = 0ull;
+        if (ldeep + j < nlev) {
+            const int lf = levels[ldeep + j].lf;
+            uint32_t buf[W];
+#pragma unroll
+            for (int u = 0; u < W; u++) buf[u] = 0u;
+            for (int t0 = 0; t0 < lf; t0 += W) {
+#pragma unroll
+                for (int u = 0; u < W; u++) {
+                    const int t = t0 + u;
+                    if (t < lf) {
+                        const uint32_t val = x[t];
+#pragma unroll
+                        for (int e = 0; e < DPL; e++) {
+                            acc[e] += (unsigned long long)buf[(u + W - (DPL + 1 + e)) % W] * val;
+                        }
+                        if (u & 1) x[t >> 1] = buf[u - 1] + val;
+                        buf[u] = val;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < DPL; e++) res[j][e] = acc[e];
+    }
+}
+
 // The deep levels of the rows the register kernels handled (1..kMidSmall
 // events), one lane per pixel: levels ldeep .. nlev-1, whose series are
 // short, cost a wave per row one lag search each but only a few instructions
@@ -928,14 +965,13 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
 // block, so no value moves) and, at odd t, x[t-1] + x[t] goes to index t/2:
 // the next level's series, in place.  The deep taus' G2 leave through the
 // same LDS area, one contiguous run per pixel.  Exact: u32 values, 64-bit
-// integer sums.
+// integer sums (32-bit sums for waves of small totals measured no faster).
 template <int DPL>
 __global__ void __launch_bounds__(kWave)
 k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __restrict__ off,
           const int32_t* __restrict__ rowmap, int32_t P, int32_t T, const LevelParam* __restrict__ levels,
           int32_t ldeep, int32_t nlev, int64_t* __restrict__ G2)
 {
-    constexpr int W = 2 * DPL;
     constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the lanes' reads of one index hit distinct banks
     __shared__ uint32_t s_x[kWave * SP];
     const int lane = lane_id();
@@ -978,30 +1014,7 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
     wave_sync();
     uint32_t* x = s_x + lane * SP;
     unsigned long long res[kDeepMaxLevels][DPL];
-#pragma unroll
-    for (int j = 0; j < kDeepMaxLevels; j++) {
-#pragma unroll
-        for (int e = 0; e < DPL; e++) res[j][e] = 0ull;
-        if (ldeep + j >= nlev) continue;
-        const int lf = levels[ldeep + j].lf;
-        uint32_t buf[W];
-#pragma unroll
-        for (int u = 0; u < W; u++) buf[u] = 0u;
-        for (int t0 = 0; t0 < lf; t0 += W) {
-#pragma unroll
-            for (int u = 0; u < W; u++) {
-                const int t = t0 + u;
-                if (t < lf) {
-                    const uint32_t val = x[t];
-#pragma unroll
-                    for (int e = 0; e < DPL; e++)
-                        res[j][e] += (unsigned long long)buf[(u + W - (DPL + 1 + e)) % W] * val;
-                    if (u & 1) x[t >> 1] = buf[u - 1] + val;
-                    buf[u] = val;
-                }
-            }
-        }
-    }
+    deep_levels<DPL>(x, levels, ldeep, nlev, res);
     wave_sync();
     // results to LDS (two u32 per value, odd pitch), then one run per pixel
     const int tdeep = levels[ldeep].first;

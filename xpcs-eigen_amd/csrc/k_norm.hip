@@ -124,7 +124,7 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
     const int nt = t1 - t0;
     const int lanes = min(nt, kNormLanes);              // taus per pass
     const int per_chunk = kNormSlices * lanes;
-    const int cpb = max(1, kNormBlock / per_chunk);     // chunks per workgroup
+    const int cpb = max(1, kNormBlock / per_chunk);     // chunks per workgroup (the launch sizes the block to it)
     const int cc = (int)threadIdx.x / per_chunk;
     const int r = (int)threadIdx.x % per_chunk;
     const int sl = r / lanes, lt = r % lanes;
@@ -343,13 +343,16 @@ void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     const int cpb = std::max(1, kNormBlock / (kNormSlices * std::min(nt, kNormLanes)));
     const int64_t nwg = (nchunks + cpb - 1) / cpb;
     const unsigned nb = nwg < 65536 ? (unsigned)nwg : 65536u;
+    // only the threads the chunks use, whole waves (T = 88: 192 threads, not 256)
+    const int used = cpb * kNormSlices * std::min(nt, kNormLanes);
+    const int nthr = std::min(kNormBlock, (used + 63) / 64 * 64);
     if (nchunks > 0) {
         if (fp)
-            k_norm_partial<true><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
-                                                           weight, out.part);
+            k_norm_partial<true><<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight,
+                                                     out.part);
         else
-            k_norm_partial<false><<<nb, kNormBlock, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF,
-                                                            weight, out.part);
+            k_norm_partial<false><<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight,
+                                                      out.part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)

@@ -29,10 +29,18 @@ struct NormAcc {
     double n = 0.0, mean = 0.0, m2 = 0.0;
 };
 
-// Pixels [pb, pe) of tau column i in order. FP: the sums are FP64
-// (frame-sum normalised values) stored in the same 8-byte buffers; they take
-// the weighted (FP64) route with weight 1 when the plan has no flat field.
-template <bool FP>
+// How a normalisation reads its sums: integers without weights (the exact
+// path), integers with flat-field weights (FP64 weighted sums), or FP64 sums
+// (frame-sum normalised values or reference floats; weights optional).  The
+// unweighted integer path carries no FP64 sums (94 VGPRs, five waves per
+// SIMD; it reads at the same rate as with three: the kernel is bound by the
+// HBM reads of the sums).
+enum NormMode { kNormInt = 0, kNormWeighted = 1, kNormFp = 2 };
+
+// Pixels [pb, pe) of tau column i in order. kNormFp: the sums are FP64 stored
+// in the same 8-byte buffers; they take the weighted (FP64) route with weight
+// 1 when the plan has no flat field.
+template <int MODE>
 __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                                                 const int64_t* __restrict__ IF, const double* __restrict__ weight,
                                                 int32_t S, int i, double N, int pb, int pe, NormAcc& a)
@@ -53,7 +61,7 @@ __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, 
             const int64_t ip = aa[u];
             const int64_t f = bb[u];
             double x;
-            if (FP) {
+            if constexpr (MODE == kNormFp) {
                 const double w = weight ? weight[p] : 1.0;
                 const double gd = __longlong_as_double(g), ad = __longlong_as_double(ip),
                              bd = __longlong_as_double(f);
@@ -61,7 +69,7 @@ __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, 
                 a.wp += w * ad;
                 a.wf += w * bd;
                 x = (w == 0.0) ? 0.0 : N * gd / (ad * bd);
-            } else if (weight) {
+            } else if constexpr (MODE == kNormWeighted) {
                 const double w = weight[p];
                 a.wg += w * w * (double)g;
                 a.wp += w * (double)ip;
@@ -83,14 +91,18 @@ __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, 
 }
 
 // a followed by b: sums added, Welford states merged (Chan et al.).
+template <int MODE>
 __device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b)
 {
-    a.sg += b.sg;
-    a.sp += b.sp;
-    a.sf += b.sf;
-    a.wg += b.wg;
-    a.wp += b.wp;
-    a.wf += b.wf;
+    if constexpr (MODE == kNormInt) {
+        a.sg += b.sg;
+        a.sp += b.sp;
+        a.sf += b.sf;
+    } else {
+        a.wg += b.wg;
+        a.wp += b.wp;
+        a.wf += b.wf;
+    }
     if (b.n == 0.0) return;
     if (a.n == 0.0) {
         a.n = b.n;
@@ -112,7 +124,7 @@ __device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b)
 // range is narrow, as in stream snapshots). Neither the tau range nor the
 // number of chunks per workgroup changes a chunk's computation, so a range of
 // taus gets the same partials as the full schedule.
-template <bool FP>
+template <int MODE>
 __global__ void __launch_bounds__(kNormBlock)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
                int32_t t0, int32_t t1, int32_t S, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
@@ -141,18 +153,19 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
             const bool act = live && i < t1;
             if (act) {
                 NormAcc a;
-                norm_accumulate<FP>(G2, IP, IF, weight, S, i, (double)taus[i].N, pb, pe, a);
+                norm_accumulate<MODE>(G2, IP, IF, weight, S, i, (double)taus[i].N, pb, pe, a);
                 s_acc[threadIdx.x] = a;
             }
             __syncthreads();
             if (act && sl == 0) {
                 NormAcc a = s_acc[threadIdx.x];
 #pragma unroll
-                for (int k = 1; k < kNormSlices; k++) norm_merge(a, s_acc[threadIdx.x + k * lanes]);
+                for (int k = 1; k < kNormSlices; k++) norm_merge<MODE>(a, s_acc[threadIdx.x + k * lanes]);
                 NormPartial o;
-                o.g2 = (FP || weight) ? __double_as_longlong(a.wg) : a.sg;
-                o.ip = (FP || weight) ? __double_as_longlong(a.wp) : a.sp;
-                o.ifs = (FP || weight) ? __double_as_longlong(a.wf) : a.sf;
+                constexpr bool fpsum = MODE != kNormInt;
+                o.g2 = fpsum ? __double_as_longlong(a.wg) : a.sg;
+                o.ip = fpsum ? __double_as_longlong(a.wp) : a.sp;
+                o.ifs = fpsum ? __double_as_longlong(a.wf) : a.sf;
                 o.n = a.n;
                 o.mean = a.mean;
                 o.m2 = a.m2;
@@ -347,12 +360,8 @@ void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     const int used = cpb * kNormSlices * std::min(nt, kNormLanes);
     const int nthr = std::min(kNormBlock, (used + 63) / 64 * 64);
     if (nchunks > 0) {
-        if (fp)
-            k_norm_partial<true><<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight,
-                                                     out.part);
-        else
-            k_norm_partial<false><<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight,
-                                                      out.part);
+        auto k = fp ? k_norm_partial<kNormFp> : weight ? k_norm_partial<kNormWeighted> : k_norm_partial<kNormInt>;
+        k<<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight, out.part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)

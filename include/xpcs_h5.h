@@ -25,6 +25,11 @@ int32_t xpcs_h5_open(const char* path, int32_t create, xpcs_h5_file** f);
 int32_t xpcs_h5_close(xpcs_h5_file* f);
 /* dims[0..*ndims) of a dataset (ndims 0 for a scalar); error if missing. */
 int32_t xpcs_h5_dims(xpcs_h5_file* f, const char* path, uint64_t* dims, int32_t cap, int32_t* ndims);
+/* Stored element type of a dataset, as H5Result leaves it for a reader
+ * (src/xpcs/h5_result.cpp:56-347 writes float32, or float64 for its double
+ * overload): klass 0 integer, 1 float, 3 string (HDF5's H5T_class_t); size in
+ * bytes; is_signed 1/0 for integers (-1 otherwise). */
+int32_t xpcs_h5_dtype(xpcs_h5_file* f, const char* path, int32_t* klass, int32_t* size, int32_t* is_signed);
 int32_t xpcs_h5_read_f64(xpcs_h5_file* f, const char* path, double* out, int64_t cap);
 int32_t xpcs_h5_read_f32(xpcs_h5_file* f, const char* path, float* out, int64_t cap);
 int32_t xpcs_h5_read_i32(xpcs_h5_file* f, const char* path, int32_t* out, int64_t cap);

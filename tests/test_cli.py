@@ -65,6 +65,30 @@ def test_usage():
     assert run("--bogus", "x").returncode == 1
 
 
+def test_h5_dtype_query(tmp_path):
+    """xpcs_h5_dtype reports the stored element type of every kind of dataset
+    the writer makes, so the output contract's dtypes can be checked."""
+    f = tmp_path / "t.h5"
+    with xpcs_h5.H5(f, create=True) as h:
+        for dt in (np.int32, np.int64, np.float32, np.float64, np.uint16, np.uint32):
+            h.write(f"/d/{np.dtype(dt).name}", np.arange(6, dtype=dt).reshape(2, 3))
+        h.write_string("/d/s", "ENABLED")
+    with xpcs_h5.H5(f) as h:
+        for dt in (np.int32, np.int64, np.float32, np.float64, np.uint16, np.uint32):
+            assert h.dtype(f"/d/{np.dtype(dt).name}") == np.dtype(dt)
+        assert h.dtype("/d/s") is str
+        with pytest.raises(RuntimeError):
+            h.dtype("/d/missing")
+
+
+# dtypes of the reference's outputs (h5_result.cpp:56-347: float32, float64 for
+# the double overload; main.cpp:339-478)
+OUT_DTYPES = {"pixelSum": "f4", "frameSum": "f4", "partition-mean-total": "f4",
+              "partition-mean-partial": "f4", "partition_norm_factor": "f4", "timestamp_clock": "f8",
+              "timestamp_tick": "f8", "tau": "f4", "norm-0-g2": "f4", "norm-0-stderr": "f4", "G2": "f4",
+              "IP": "f4", "IF": "f4"}
+
+
 @pytest.mark.gpu
 def test_cli_outputs_match_oracle(tmp_path, orc):
     job = make_job(tmp_path, static_window=40)
@@ -107,11 +131,45 @@ def test_cli_outputs_match_oracle(tmp_path, orc):
         clk = h.read("/exchange/timestamp_clock")
         assert clk.shape == (2, F) and np.allclose(clk[1], np.arange(F) * 1e-3)
         assert h.dims("/exchange/partition_norm_factor") == (1, 1)
+        for name, dt in OUT_DTYPES.items():
+            assert h.dtype("/exchange/" + name) == np.dtype(dt), name
         G2r = h.read("/exchange/G2", np.float32)
         assert G2r.shape == (T, W * H)
         G2c, _, _ = orc.multitau_compat(ref["csr"], ref["qm"]["order"], W * H, F, ref["level"], ref["tau"],
                                         stale_tail=False)
         assert np.allclose(G2r, G2c, rtol=1e-4, atol=0)
+
+
+@pytest.mark.gpu
+def test_cli_darkout(tmp_path, orc):
+    """Dense IMM with two leading dark frames and --darkout: DarkAvg and
+    DarkStd are float64 [H, W] (main.cpp:459-478) and equal the oracle's dark
+    image; g2 matches the oracle with the darks subtracted."""
+    W = H = 16
+    F = 200
+    dq, sq = synth.annular_qmaps(W, H, 6, 3)
+    sig = synth.ou_counts(W, H, F, 5.0, 15.0, seed=71)
+    dark = synth.dark_frames(W, H, 2, seed=72)
+    raw = np.concatenate([dark, sig + dark[0]], axis=0)
+    imm = tmp_path / "dense.imm"
+    imm.write_bytes(synth.imm_bytes_dense(raw, W, H))
+    conf = tmp_path / "cfg.h5"
+    xpcs_h5.write_config(conf, width=W, height=H, dqmap=dq, sqmap=sq, imm_path=imm, compression=False,
+                         dark_begin_todo=1, dark_end_todo=2, data_begin_todo=3, data_end_todo=F + 2, dpl=4)
+    print("corr dense job with darks", W, H, F, flush=True)
+    r = run(conf, "--darkout")
+    assert r.returncode == 0, r.stderr
+    buf = imm.read_bytes()
+    davg, dstd, _ = orc.dark_image(buf, orc.imm_index(buf), 2, W * H)
+    ref = orc.run_exact(buf, dq, sq, F, 4, darks=2, frame_from=2)
+    with xpcs_h5.H5(conf) as h:
+        for name, want in (("DarkAvg", davg), ("DarkStd", dstd)):
+            assert h.dtype("/exchange/" + name) == np.dtype("f8"), name
+            got = h.read("/exchange/" + name)
+            assert got.shape == (H, W) and np.allclose(got.reshape(-1), want, rtol=1e-12, atol=1e-12), name
+        g2 = h.read("/exchange/norm-0-g2", np.float32)
+        ok = ~np.isnan(ref["g2"])
+        assert np.allclose(g2[ok], ref["g2"][ok], rtol=1e-6)
 
 
 @pytest.mark.gpu

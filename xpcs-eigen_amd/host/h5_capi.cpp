@@ -49,6 +49,19 @@ extern "C" int32_t xpcs_h5_dims(xpcs_h5_file* f, const char* path, uint64_t* dim
     return 0;
 }
 
+extern "C" int32_t xpcs_h5_dtype(xpcs_h5_file* f, const char* path, int32_t* klass, int32_t* size,
+                                 int32_t* is_signed)
+{
+    int cls = -1, sign = -1;
+    size_t sz = 0;
+    if (!f || !path || !klass || !size || !is_signed || !f->f.type_of(path, &cls, &sz, &sign))
+        return err(std::string("no dataset ") + (path ? path : ""));
+    *klass = cls;
+    *size = (int32_t)sz;
+    *is_signed = sign;
+    return 0;
+}
+
 template <typename T, typename R>
 static int32_t read_vec(xpcs_h5_file* f, const char* path, T* out, int64_t cap, R reader)
 {

@@ -38,6 +38,7 @@ struct Api {
     htri_t (*Tis_variable_str)(hid_t);
     size_t (*Tget_size)(hid_t);
     int (*Tget_class)(hid_t);
+    int (*Tget_sign)(hid_t) = nullptr;   // optional: integer signedness
     hid_t (*Tcopy)(hid_t);
     herr_t (*Tset_size)(hid_t, size_t);
     herr_t (*Tclose)(hid_t);
@@ -108,6 +109,7 @@ void load()
               bind(g.T_NATIVE_FLOAT, "H5T_NATIVE_FLOAT_g") && bind(g.T_NATIVE_DOUBLE, "H5T_NATIVE_DOUBLE_g") &&
               bind(g.T_C_S1, "H5T_C_S1_g") && bind(g.P_CLS_LINK_CREATE, "H5P_CLS_LINK_CREATE_ID_g");
     if (!ok) return;
+    g.Tget_sign = reinterpret_cast<decltype(g.Tget_sign)>(dlsym(g.so, "H5Tget_sign"));
     g.Pset_chunk = reinterpret_cast<decltype(g.Pset_chunk)>(dlsym(g.so, "H5Pset_chunk"));
     g.Pset_deflate = reinterpret_cast<decltype(g.Pset_deflate)>(dlsym(g.so, "H5Pset_deflate"));
     g.P_CLS_DATASET_CREATE = reinterpret_cast<hid_t*>(dlsym(g.so, "H5P_CLS_DATASET_CREATE_ID_g"));
@@ -179,6 +181,20 @@ bool File::dims(const std::string& path, std::vector<hsize_t>* d) const
     g.Sclose(sp);
     g.Dclose(ds);
     return nd >= 0;
+}
+
+bool File::type_of(const std::string& path, int* cls, size_t* size, int* sign) const
+{
+    if (!exists(path)) return false;
+    hid_t ds = g.Dopen2(fid_, path.c_str(), 0);
+    if (ds < 0) return false;
+    hid_t ft = g.Dget_type(ds);
+    *cls = g.Tget_class(ft);
+    *size = g.Tget_size(ft);
+    *sign = (*cls == 0 && g.Tget_sign) ? g.Tget_sign(ft) : -1;
+    g.Tclose(ft);
+    g.Dclose(ds);
+    return *cls >= 0;
 }
 
 bool File::read_any(const std::string& path, hid_t memtype, void* buf, size_t elem, size_t cap,

@@ -32,6 +32,9 @@ public:
     bool exists(const std::string& path) const;
     // Shape of a dataset (empty for a scalar); false if missing.
     bool dims(const std::string& path, std::vector<hsize_t>* d) const;
+    // Stored element type: HDF5 class (0 integer, 1 float, 3 string), size in
+    // bytes, and for integers the sign (0 unsigned, 1 signed; -1 unknown).
+    bool type_of(const std::string& path, int* cls, size_t* size, int* sign) const;
     // Scalar readers converting from any stored numeric type (first element).
     bool read_i64(const std::string& path, int64_t* v) const;
     bool read_f64(const std::string& path, double* v) const;

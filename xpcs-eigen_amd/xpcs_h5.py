@@ -11,7 +11,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libxpcs_host.so"
 EXPORTS = ["xpcs_h5_last_error", "xpcs_h5_open", "xpcs_h5_close", "xpcs_h5_dims", "xpcs_h5_read_f64",
            "xpcs_h5_read_f32", "xpcs_h5_read_i32", "xpcs_h5_read_string", "xpcs_h5_write",
-           "xpcs_h5_write_string"]
+           "xpcs_h5_write_string", "xpcs_h5_dtype"]
 _LIB = None
 
 
@@ -31,6 +31,7 @@ def lib():
         L.xpcs_h5_read_string.argtypes = [vp, C.c_char_p, C.c_char_p, i64]
         L.xpcs_h5_write.argtypes = [vp, C.c_char_p, i32, vp, vp, i32]
         L.xpcs_h5_write_string.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.xpcs_h5_dtype.argtypes = [vp, C.c_char_p, vp, vp, vp]
         for n in EXPORTS[1:]:
             getattr(L, n).restype = i32
         _LIB = L
@@ -63,6 +64,18 @@ class H5:
         n = C.c_int32(0)
         _check(lib().xpcs_h5_dims(self._f, path.encode(), d, 8, C.byref(n)))
         return tuple(int(d[i]) for i in range(n.value))
+
+    def dtype(self, path):
+        """The stored element type: a numpy dtype, or str for strings."""
+        k, n, sg = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        _check(lib().xpcs_h5_dtype(self._f, path.encode(), C.byref(k), C.byref(n), C.byref(sg)))
+        if k.value == 3:
+            return str
+        if k.value == 1:
+            return np.dtype(f"f{n.value}")
+        if k.value == 0:
+            return np.dtype(f"{'u' if sg.value == 0 else 'i'}{n.value}")
+        raise RuntimeError(f"{path}: HDF5 type class {k.value}")
 
     def read(self, path, dtype=np.float64):
         shape = self.dims(path)

@@ -109,6 +109,11 @@ def dense_pass_bytes(P, F, avg, dpl, levels):
     return {"k_mt_dense_raw": raw, "k_mt_dense_series": int(np.mean(series)) if series else 0}
 
 
+def note(msg):
+    """A progress line on stderr (stdout carries the one JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic.txt), or None."""
@@ -394,8 +399,10 @@ def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
     streams arriving as a Poisson process (mean one per 10 s) and queued until a
     slot and HBM budget are free.  Runs `seconds` of wall time; frames beyond the
     window are not generated.  Reports p50/p99 frame-arrival -> covering-g2
-    latency and, for comparison, the plain path's cost of recomputing each live
-    stream's g2 from scratch at its final frame count."""
+    latency (frames of a queued stream wait for its admission), the same from
+    admission on, the admission waits, the final g2 of the streams that finish
+    against a plan of their frames and, for comparison, the plain path's cost
+    of recomputing each live stream's g2 from scratch at its frame count."""
     import torch
     import serve
     import synth
@@ -460,10 +467,26 @@ def run_streaming(dev, seconds, n_streams=16, fps=200.0, budget_gib=96.0):
         p.close()
     for k, e in live:
         e.stream.close()
+    # the final g2 of every stream that finished inside the window against a
+    # plan (the batch path) of the same frames
+    finished_equal = None
+    for k, res in svc.results.items():
+        sp = specs[k]
+        n = int(res["frames"])
+        p = xpcs_mt.Plan(sp["W"], sp["W"], n, sp["dpl"], sp["dq"], sp["sq"])
+        p.ingest_imm_device(sp["imm"].data_ptr(), sp["imm"].numel(), sp["off"][:n], sp["dl"][:n], compressed=True)
+        p.execute()
+        g2, se = p.g2()
+        p.close()
+        same = bool(np.array_equal(g2, res["g2"], equal_nan=True) and np.array_equal(se, res["stderr"], equal_nan=True))
+        finished_equal = same if finished_equal is None else (finished_equal and same)
     out = {"streams_submitted": len(specs), "streams_admitted": len(specs) - len(svc.queue),
-           "streams_finished": len(svc.results), "fps_per_stream": fps, "window_s": seconds,
+           "streams_finished": len(svc.results), "finished_g2_equal_plan": finished_equal,
+           "fps_per_stream": fps, "window_s": seconds,
            "frames_covered": svc.frames_done, "frames_per_s": svc.frames_done / seconds,
-           "latency": pc, "budget_gib": budget_gib, "peak_state_gib": svc.peak_used / 2**30,
+           "latency": pc, "latency_after_admission": svc.percentiles(service=True),
+           "admission_waits_s": [round(x, 3) for x in svc.admit_wait if x > 0],
+           "budget_gib": budget_gib, "peak_state_gib": svc.peak_used / 2**30,
            "loop_turns": steps, "admission_wait_s_max": max(svc.admit_wait) if svc.admit_wait else 0.0,
            "plain_recompute_ms": {"p50": float(np.percentile(plain_ms, 50)), "max": float(max(plain_ms)),
                                   "streams": len(plain_ms)} if plain_ms else None,
@@ -707,7 +730,7 @@ def main():
     ap.add_argument("--dense-steps", type=int, default=10,
                     help="timed jobs of the dense 512x512 x 20000-frame leg (0 = skip)")
     ap.add_argument("--batched-jobs", type=int, default=64, help="jobs of the batched leg (0 = skip)")
-    ap.add_argument("--stream-seconds", type=float, default=20.0,
+    ap.add_argument("--stream-seconds", type=float, default=60.0,
                     help="wall time of the streaming leg (0 = skip)")
     ap.add_argument("--no-formats", action="store_true", help="skip the Rigaku / UFXC / raw-frame leg")
     ap.add_argument("--no-fsnorm", action="store_true", help="skip the normalize_by_framesum leg")
@@ -737,6 +760,7 @@ def main():
                           "launches": 3}))
         return
 
+    note(f"main job: {args.workload}, {args.steps} timed steps")
     main_job, ctx = run_job(args.workload, args.steps, args.warmup, autotune=not args.no_autotune,
                             e2e=not args.no_e2e)
     if ctx["host_imm"] is None and (not args.no_cpu or not args.no_formats):
@@ -747,31 +771,41 @@ def main():
 
     cpu = None
     if not args.no_cpu and not dense:
+        note("cpu baseline")
         cpu = cpu_baseline_quarter(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl)
     corr_cli = None
     if not args.no_corr and not dense:
         if ctx["host_imm"] is None:
             ctx["host_imm"] = ctx["imm"].cpu().numpy()
             host_imm = ctx["host_imm"]
+        note("corr command")
         corr_cli = run_corr_cli(ctx["host_imm"], W, H, F, dpl, dq, sq, ctx["g2"])
     fsnorm = None
     if not args.no_fsnorm and not dense:
+        note("normalize_by_framesum")
         fsnorm = run_fsnorm(dev, imm, payload_off, dlen, W, H, F, dq, sq, dpl)
     formats = None
     if not args.no_formats and not dense:
+        note("other formats")
         formats = run_formats(host_imm, payload_off, dlen, W, H, F, dq, sq, dpl, ctx["g2"])
     twotime = None
     if args.twotime_frames > 0 and not dense:
+        note("two-time")
         twotime = run_twotime(dev, imm, payload_off, dlen, W, H, dq, sq, min(args.twotime_frames, F),
                               cpu=not args.no_cpu)
     del ctx, imm, host_imm
     torch.cuda.empty_cache()
     dense_leg = None
     if args.dense_steps > 0 and not dense:
+        note("dense-512")
         dense_leg, dctx = run_job("dense-512", args.dense_steps, 2, autotune=not args.no_autotune, e2e=False)
         del dctx
         torch.cuda.empty_cache()
+    if args.batched_jobs > 0:
+        note("batched jobs")
     batched = run_batched(dev, args.batched_jobs, 3) if args.batched_jobs > 0 else None
+    if args.stream_seconds > 0:
+        note(f"streams, {args.stream_seconds:.0f} s")
     streaming = run_streaming(dev, args.stream_seconds) if args.stream_seconds > 0 else None
 
     out = {

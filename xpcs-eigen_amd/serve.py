@@ -12,7 +12,9 @@ device.  `step(now)` is one turn of the request loop:
   frames pushed so far -- frames that arrive while a snapshot is on the GPU
   join the next one instead of queueing behind it;
 * finished snapshots are detected without blocking; every frame's latency is
-  (turn that saw its first covering snapshot finished) - (its arrival time);
+  (turn that saw its first covering snapshot finished) - (its arrival time),
+  and its service latency the same from the later of its arrival and its
+  stream's admission (frames of a queued stream wait for a slot first);
 * a stream whose last frame is covered is read out and closed, which frees its
   budget for streams waiting in the admission queue (first come, first served).
 
@@ -58,6 +60,7 @@ class StreamService:
         self.active: dict = {}
         self.results: dict = {}
         self.latencies: list[float] = []
+        self.service_latencies: list[float] = []
         self.admit_wait: list[float] = []
         self.peak_used = 0
         self.frames_done = 0
@@ -92,6 +95,7 @@ class StreamService:
                 lat = now - arr
                 e.lat.extend(lat.tolist())
                 self.latencies.extend(lat.tolist())
+                self.service_latencies.extend((now - np.maximum(arr, e.admitted)).tolist())
                 self.frames_done += done - e.reported
                 e.reported = done
             avail = e.source.available(now)
@@ -116,10 +120,12 @@ class StreamService:
     def idle(self):
         return not self.active and not self.queue
 
-    def percentiles(self):
-        if not self.latencies:
+    def percentiles(self, service=False):
+        """p50 / p99 / max of the frames' latencies (service=True: from admission on)."""
+        src = self.service_latencies if service else self.latencies
+        if not src:
             return None
-        a = np.array(self.latencies)
+        a = np.array(src)
         return {"p50_ms": 1e3 * float(np.percentile(a, 50)), "p99_ms": 1e3 * float(np.percentile(a, 99)),
                 "max_ms": 1e3 * float(a.max()), "frames": int(a.size)}
 

@@ -424,6 +424,54 @@ def test_repeated_execute_is_stable(xmt, orc):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("dpl,deep", [(4, 96), (8, 96), (8, -1)])
+def test_lds_rows_dense_levels_and_large_totals(xmt, orc, dpl, deep):
+    """Rows of 129..2048 events handed over pixel-major: levels that turn
+    dense run as LDS series, the deep levels go to the one-lane-per-pixel
+    kernel, and rows whose total passes 2^32 (counts of 3e6, 1500 events)
+    stay in the LDS kernel to the last level; exact against the oracle."""
+    rng = np.random.default_rng(300 + dpl + deep)
+    W = H = 10
+    F = 3000
+    dq, sq = synth.annular_qmaps(W, H, 4, 2)
+    npd = W * H
+    lens = rng.integers(129, 2049, npd)
+    lens[:8] = rng.integers(1, 129, 8)                 # a few register rows beside them
+    big = {20, 21, 55}
+    frames, counts = [], []
+    for j, n in enumerate(lens):
+        n = min(int(n), F)
+        lens[j] = n
+        frames.append(np.sort(rng.choice(F, n, replace=False)).astype(np.int32))
+        c = rng.integers(1, 40, n).astype(np.uint32)
+        if j in big:
+            lens[j] = n = 1500
+            frames[-1] = np.sort(rng.choice(F, n, replace=False)).astype(np.int32)
+            c = np.full(n, 3_000_000, np.uint32)        # total 4.5e9 > 2^32
+        counts.append(c)
+    roff = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    print(f"LDS rows dpl={dpl} deep_frames={deep}", flush=True)
+    plan = xmt.Plan(W, H, F, dpl, dq, sq)
+    plan.set_tuning(deep_frames=deep)
+    plan.ingest_rows(roff, np.concatenate(frames), np.concatenate(counts))
+    plan.execute()
+    qm = orc.build_qmap(dq, sq)
+    order = qm["order"]
+    off = np.concatenate([[0], np.cumsum(lens[order])]).astype(np.int64)
+    csr = dict(off=off, frame=np.concatenate([frames[j] for j in order]),
+               cnt=np.concatenate([counts[j] for j in order]).astype(np.int64))
+    lev, tau = orc.schedule(F, dpl)
+    G2, IP, IF = orc.multitau_exact(csr, F, lev, tau)
+    g, a, b = plan.sums()
+    assert np.array_equal(a, IP) and np.array_equal(b, IF)
+    assert np.array_equal(g, G2)
+    px, _ = plan.row_stats()
+    assert px[3] > 0 and px[4] > 0
+    if deep > 0:
+        assert plan.info().deep_stride > 0
+    plan.close()
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_shuffled_rows_two_per_lane(xmt, orc, seed):
     """Rows of 65..128 events in random frame order (the two-events-per-lane

@@ -956,7 +956,8 @@ __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __res
 }
 
 // The deep levels of the rows the register kernels handled (1..kMidSmall
-// events), one lane per pixel: levels ldeep .. nlev-1, whose series are
+// events) and of the LDS kernels' rows (up to kSparseCapLong events) whose
+// total fits 32 bits, one lane per pixel: levels ldeep .. nlev-1, whose series are
 // short, cost a wave per row one lag search each but only a few instructions
 // per value here.  A wave takes 64 consecutive pixels: their series rows
 // (contiguous in HBM) go to LDS, then each lane streams its own series.  A
@@ -970,7 +971,7 @@ template <int DPL>
 __global__ void __launch_bounds__(kWave)
 k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __restrict__ off,
           const int32_t* __restrict__ rowmap, int32_t P, int32_t T, const LevelParam* __restrict__ levels,
-          int32_t ldeep, int32_t nlev, int64_t* __restrict__ G2)
+          int32_t ldeep, int32_t nlev, const uint64_t* __restrict__ pix_total, int64_t* __restrict__ G2)
 {
     constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the lanes' reads of one index hit distinct banks
     __shared__ uint32_t s_x[kWave * SP];
@@ -1009,7 +1010,8 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         const int64_t p = p0 + lane;
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
         const int64_t n = off[row + 1] - off[row];
-        mine = n >= 1 && n <= kMidSmall;
+        // rows of the register kernels, and the LDS rows whose total fits 32 bits
+        mine = n >= 1 && (n <= kMidSmall || (n <= kSparseCapLong && pix_total[p] <= 0xFFFFFFFFull));
     }
     wave_sync();
     uint32_t* x = s_x + lane * SP;
@@ -1075,9 +1077,94 @@ __device__ bool lds_sort_rounds(uint32_t* key, V* val, int m)
     return false;
 }
 
+// Dense levels of an LDS row (counts): once a level's events fill at least a
+// quarter of its frames (and the frames fit the row's 2 CAP words), the row
+// becomes the series x[0, lf) over those words.  A lag sum is then
+// sum_t x[t] x[t+d] with lanes over t: independent, pipelined LDS reads and
+// one wave reduction per lag, instead of each lane scanning its events'
+// partners one dependent read at a time (which many partners per event make
+// slow).  Binning a dense level is a pairwise sum in place.  Values are at
+// most the row's total, which the caller keeps below 2^32.
+
+// The nl events (keys < lf ascending, values at key + CAP) as x[0, lf).
+template <int CAP>
+__device__ void lds_to_dense(uint32_t* key, int nl, int lf)
+{
+    constexpr int R = CAP / kWave;
+    const int lane = lane_id();
+    uint32_t rk[R], rv[R];
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        const int i = c * kWave + lane;
+        rk[c] = i < nl ? key[i] : kNoKey;
+        rv[c] = i < nl ? key[CAP + i] : 0u;
+    }
+    wave_sync();
+    for (int t = lane; t < lf; t += kWave) key[t] = 0u;
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < R; c++)
+        if (rk[c] != kNoKey) key[rk[c]] = rv[c];
+    wave_sync();
+}
+
+// x[0, lf) -> x[0, lf / 2), pair sums (lf / 2 <= CAP).
+template <int CAP>
+__device__ void lds_dense_bin(uint32_t* x, int lf)
+{
+    constexpr int R = CAP / kWave;
+    const int lane = lane_id();
+    const int h = lf >> 1;
+    uint32_t sm[R];
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        const int t = c * kWave + lane;
+        sm[c] = t < h ? x[2 * t] + x[2 * t + 1] : 0u;
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        const int t = c * kWave + lane;
+        if (t < h) x[t] = sm[c];
+    }
+    wave_sync();
+}
+
+// The level's lags lo..hi over x[0, lf) into copy 0 of their accumulators.
+__device__ void lds_dense_lags(const uint32_t* x, int lf, const LevelParam& lp, unsigned long long* acc, int nc)
+{
+    constexpr int kLagBlock = 16;
+    const int lane = lane_id();
+    for (int d0 = lp.lo; d0 <= lp.hi; d0 += kLagBlock) {
+        unsigned long long sd[kLagBlock];
+#pragma unroll
+        for (int j = 0; j < kLagBlock; j++) sd[j] = 0ull;
+        for (int t = lane; t < lf; t += kWave) {
+            const unsigned long long xt = x[t];
+#pragma unroll
+            for (int j = 0; j < kLagBlock; j++) {
+                const int d = d0 + j;
+                if (d <= lp.hi && t + d < lf) sd[j] += xt * x[t + d];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kLagBlock; j++) {
+            const int d = d0 + j;
+            if (d <= lp.hi) {
+                const unsigned long long g = wave_sum_u64(sd[j]);
+                if (lane == 0) acc[(lp.first + d - lp.lo) * nc] += g;
+            }
+        }
+    }
+    wave_sync();
+}
+
 // Rows r < *count of the list rows[r * step] (step -1 walks a list filled from
 // the back).  AUX slots hold the level-0 prefix sums (n <= CAP), then the lag
 // accumulators (T taus x copies).
+// Counts (V = uint32_t): dense levels as above, and the levels from ldeep on
+// handed to k_mt_deep like the register kernels' rows (rows whose total is
+// below 2^32).
 template <int CAP, int WAVES, typename V>
 __global__ void __launch_bounds__(kWave * WAVES)
 k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
@@ -1085,18 +1172,20 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
             int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
             int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap,
-            const double* __restrict__ fw)
+            const double* __restrict__ fw, int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
 {
     using A = typename LagAcc<V>::type;
+    constexpr bool U = sizeof(V) == 4;
     constexpr int AUX = CAP > 512 ? CAP : 512;
-    __shared__ uint32_t s_key[WAVES][CAP];
-    __shared__ V s_val[WAVES][CAP];
+    // keys, then values: 2 CAP contiguous words per wave for counts
+    __shared__ __align__(8) uint32_t s_kv[WAVES][CAP * (1 + sizeof(V) / 4)];
     __shared__ A s_aux[WAVES][AUX];   // prefix sums, then lag accumulators
     const int lane = lane_id();
     const int w = wave_id();
-    uint32_t* key = s_key[w];
-    V* val = s_val[w];
+    uint32_t* key = s_kv[w];
+    V* val = reinterpret_cast<V*>(s_kv[w] + CAP);
     A* aux = s_aux[w];
+    const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus before the hand-over
     const int64_t nw = (int64_t)gridDim.x * WAVES;
 
     const int64_t nrows = (int64_t)*count;
@@ -1196,12 +1285,17 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 ifrow[i] = sum_bits((A)(s_end - s_if));
             }
         }
+        // counts: the row total bounds every level value (32 bits for the
+        // dense levels and the hand-over)
+        bool small = false;
         if constexpr (sizeof(V) == 4) {
             if (lane == 0) pix_total[p] = carry;
+            small = carry <= 0xFFFFFFFFull;
         } else {
             const uint64_t tot = wave_sum_u64((uint64_t)counts);
             if (lane == 0) pix_total[p] = tot;
         }
+        const bool hand = U && small && ldeep < nlev;
         wave_sync();
         // lag accumulators: nc copies per tau while T * nc fits the CAP slots
         const int nc = T * kAccCopies <= AUX ? kAccCopies : 1;
@@ -1209,10 +1303,37 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         wave_sync();
         // ---- levels: bin in place, then lag products ----
         int nl = n0;
+        bool dense = false;
         for (int L = 0; L < nlev; L++) {
             const LevelParam lp = levels[L];
-            if (L > 0) nl = level_compact(key, val, nl, 1, (uint32_t)lp.lf);
-            if (nl < 2) break;
+            if constexpr (U) {
+                if (dense) {
+                    lds_dense_bin<CAP>(key, levels[L - 1].lf);
+                } else {
+                    if (L > 0) nl = level_compact(key, val, nl, 1, (uint32_t)lp.lf);
+                    if (small && ((lp.lf <= 2 * CAP && 4 * nl >= lp.lf) || (hand && L == ldeep))) {
+                        lds_to_dense<CAP>(key, nl, lp.lf);
+                        dense = true;
+                    }
+                }
+                if (hand && L == ldeep) {
+                    // the series of this level, zero-padded to the row length
+                    uint32_t* dst = deep + p * dstride;
+                    for (int t = lane; t < dstride; t += kWave) dst[t] = t < lp.lf ? key[t] : 0u;
+                    wave_sync();
+                    break;
+                }
+                if (dense) {
+                    lds_dense_lags(key, lp.lf, lp, reinterpret_cast<unsigned long long*>(aux), nc);
+                    continue;
+                }
+            } else {
+                if (L > 0) nl = level_compact(key, val, nl, 1, (uint32_t)lp.lf);
+            }
+            if (nl < 2) {
+                if (!hand) break;
+                continue;   // no more pairs; bin what is left down to the hand-over level
+            }
             const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
             for (int i = lane; i < nl; i += kWave) {
                 const uint32_t ki = key[i];
@@ -1226,7 +1347,8 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             }
             wave_sync();
         }
-        for (int i = lane; i < T; i += kWave) {
+        const int tg = hand ? tdeep : T;   // the deep kernel writes the others
+        for (int i = lane; i < tg; i += kWave) {
             A g = A(0);
             for (int c = 0; c < nc; c++) g += aux[i * nc + c];
             g2row[i] = sum_bits(g);
@@ -1460,16 +1582,20 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     else
         k_mt_sparse<kMidSmall, kSparseWaves, V><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
-            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+            (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
+            p->d_deep, dstride);
     k_mt_sparse<kSparseCapUsed, kSparseWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
+        p->d_deep, dstride);
     k_mt_sparse<kSparseCapMid, kSparseMidWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseMidWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_long_list, 1, p->d_counters + 5, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
+        p->d_deep, dstride);
     k_mt_sparse<kSparseCapLong, kSparseLongWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseLongWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_long_list + (h.P - 1), -1, p->d_counters, p->d_taus, h.T, p->d_levels,
-        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw);
+        (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
+        p->d_deep, dstride);
     timer_end(p, s);
     // rows longer than kSparseCapLong: the dense-series kernel
     timer_begin(p, s, t_long);
@@ -1491,10 +1617,10 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
         const unsigned nb = (unsigned)((h.P + kWave - 1) / kWave);
         if (h.dpl == 4)
             k_mt_deep<4><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
-                                              ldeep, nlev, p->d_G2);
+                                              ldeep, nlev, p->d_pix_total, p->d_G2);
         else
             k_mt_deep<8><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
-                                              ldeep, nlev, p->d_G2);
+                                              ldeep, nlev, p->d_pix_total, p->d_G2);
         timer_end(p, s);
     }
 }

@@ -424,13 +424,17 @@ def test_repeated_execute_is_stable(xmt, orc):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("dpl,deep", [(4, 96), (8, 96), (8, -1)])
-def test_lds_rows_dense_levels_and_large_totals(xmt, orc, dpl, deep):
+@pytest.mark.parametrize("dpl,deep,row_order", [(4, 96, "sorted"), (8, 96, "local"), (8, -1, "shuffled"),
+                                                 (4, 48, "local")])
+def test_lds_rows_dense_levels_and_large_totals(xmt, orc, dpl, deep, row_order):
     """Rows of 129..2048 events handed over pixel-major: levels that turn
     dense run as LDS series, the deep levels go to the one-lane-per-pixel
     kernel, and rows whose total passes 2^32 (counts of 3e6, 1500 events)
-    stay in the LDS kernel to the last level; exact against the oracle."""
-    rng = np.random.default_rng(300 + dpl + deep)
+    stay in the LDS kernel to the last level; rows in frame order, permuted
+    within stretches of 40 events (what the transpose leaves: the register
+    block sorts), or fully shuffled (the whole-row network); exact against
+    the oracle."""
+    rng = np.random.default_rng(300 + dpl + deep + len(row_order))
     W = H = 10
     F = 3000
     dq, sq = synth.annular_qmaps(W, H, 4, 2)
@@ -449,8 +453,17 @@ def test_lds_rows_dense_levels_and_large_totals(xmt, orc, dpl, deep):
             frames[-1] = np.sort(rng.choice(F, n, replace=False)).astype(np.int32)
             c = np.full(n, 3_000_000, np.uint32)        # total 4.5e9 > 2^32
         counts.append(c)
+    for j in range(npd):
+        perm = np.arange(lens[j])
+        if row_order == "local":
+            for a in range(0, lens[j], 40):
+                rng.shuffle(perm[a:a + 40])
+        elif row_order == "shuffled":
+            rng.shuffle(perm)
+        frames[j] = frames[j][perm]
+        counts[j] = counts[j][perm]
     roff = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
-    print(f"LDS rows dpl={dpl} deep_frames={deep}", flush=True)
+    print(f"LDS rows dpl={dpl} deep_frames={deep} order={row_order}", flush=True)
     plan = xmt.Plan(W, H, F, dpl, dq, sq)
     plan.set_tuning(deep_frames=deep)
     plan.ingest_rows(roff, np.concatenate(frames), np.concatenate(counts))

@@ -1045,38 +1045,6 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
     }
 }
 
-// LDS rows of m (a power of two > 64) entries, kNoKey-padded: up to 3
-// odd-even transposition rounds; true when the row ends up sorted.
-template <typename V>
-__device__ bool lds_sort_rounds(uint32_t* key, V* val, int m)
-{
-    const int lane = lane_id();
-    auto sorted = [&]() {
-        bool bad = false;
-        for (int i = lane; i + 1 < m; i += kWave) bad |= key[i] > key[i + 1];
-        return __builtin_amdgcn_ballot_w64(bad) == 0ull;
-    };
-    if (sorted()) return true;
-    for (int r = 0; r < 3; r++) {
-        for (int ph = 0; ph < 2; ph++) {
-            for (int t = lane; 2 * t + ph + 1 < m; t += kWave) {
-                const int i = 2 * t + ph;
-                const uint32_t a = key[i], b = key[i + 1];
-                if (b < a) {
-                    key[i] = b;
-                    key[i + 1] = a;
-                    const V va = val[i];
-                    val[i] = val[i + 1];
-                    val[i + 1] = va;
-                }
-            }
-            wave_sync();
-        }
-        if (sorted()) return true;
-    }
-    return false;
-}
-
 // Dense levels of an LDS row (counts): once a level's events fill at least a
 // quarter of its frames (and the frames fit the row's 2 CAP words), the row
 // becomes the series x[0, lf) over those words.  A lag sum is then
@@ -1130,33 +1098,101 @@ __device__ void lds_dense_bin(uint32_t* x, int lf)
     wave_sync();
 }
 
-// The level's lags lo..hi over x[0, lf) into copy 0 of their accumulators.
+// Sum over the wave of each of 8 per-lane values: a butterfly that halves the
+// set at each of the first three exchanges (lanes ^ 32, 16, 8 keep 4, 2, 1
+// of them), then sums the one left over lanes ^ 4, 2, 1.  Lane l ends with
+// the total of value (l >> 3) & 7, in 10 exchanges instead of 48.
+__device__ __forceinline__ unsigned long long wave_sum8_u64(unsigned long long (&v)[8])
+{
+    const int lane = lane_id();
+    unsigned long long a[4], b[2], c;
+    const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0, h8 = (lane & 8) != 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        // keep v[j] (lanes < 32) or v[j + 4] (lanes >= 32), add the partner's copy
+        const unsigned long long keep = h32 ? v[j + 4] : v[j], give = h32 ? v[j] : v[j + 4];
+        a[j] = keep + __shfl_xor(give, 32, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const unsigned long long keep = h16 ? a[j + 2] : a[j], give = h16 ? a[j] : a[j + 2];
+        b[j] = keep + __shfl_xor(give, 16, 64);
+    }
+    {
+        const unsigned long long keep = h8 ? b[1] : b[0], give = h8 ? b[0] : b[1];
+        c = keep + __shfl_xor(give, 8, 64);
+    }
+    c += __shfl_xor(c, 4, 64);
+    c += __shfl_xor(c, 2, 64);
+    c += __shfl_xor(c, 1, 64);
+    return c;
+}
+
+// The level's lags lo..hi over x[0, lf) into copy 0 of their accumulators,
+// 8 lags at a time (t + d < lf checked only in the last hi + 1 frames).
 __device__ void lds_dense_lags(const uint32_t* x, int lf, const LevelParam& lp, unsigned long long* acc, int nc)
 {
-    constexpr int kLagBlock = 16;
+    constexpr int kLagBlock = 8;
     const int lane = lane_id();
     for (int d0 = lp.lo; d0 <= lp.hi; d0 += kLagBlock) {
         unsigned long long sd[kLagBlock];
 #pragma unroll
         for (int j = 0; j < kLagBlock; j++) sd[j] = 0ull;
-        for (int t = lane; t < lf; t += kWave) {
+        const int tmain = lf - (d0 + kLagBlock - 1);   // every lag of the block in range below it
+        int t = lane;
+        for (; t < tmain; t += kWave) {
             const unsigned long long xt = x[t];
 #pragma unroll
-            for (int j = 0; j < kLagBlock; j++) {
-                const int d = d0 + j;
-                if (d <= lp.hi && t + d < lf) sd[j] += xt * x[t + d];
-            }
+            for (int j = 0; j < kLagBlock; j++) sd[j] += xt * x[t + d0 + j];
         }
+        for (; t < lf; t += kWave) {
+            const unsigned long long xt = x[t];
 #pragma unroll
-        for (int j = 0; j < kLagBlock; j++) {
-            const int d = d0 + j;
-            if (d <= lp.hi) {
-                const unsigned long long g = wave_sum_u64(sd[j]);
-                if (lane == 0) acc[(lp.first + d - lp.lo) * nc] += g;
-            }
+            for (int j = 0; j < kLagBlock; j++)
+                if (t + d0 + j < lf) sd[j] += xt * x[t + d0 + j];
         }
+        const unsigned long long g = wave_sum8_u64(sd);
+        // lane 8 j holds lag d0 + j
+        const int j = lane >> 3;
+        if ((lane & 7) == 0 && d0 + j <= lp.hi) acc[(lp.first + d0 + j - lp.lo) * nc] += g;
     }
     wave_sync();
+}
+
+// LDS rows whose events are out of order only locally (the transpose places
+// the events of one pixel within one slice of its bucket concurrently, so a
+// long row is a run of short unordered stretches in frame order): 64-entry
+// blocks sorted in registers (DPP bitonic network), alternately aligned at 0
+// and at 32, so an entry moves up to 32 places per round.  True when the row
+// ends up sorted within kBlockRounds rounds.
+constexpr int kBlockRounds = 4;
+template <typename V>
+__device__ bool lds_block_sort(uint32_t* key, V* val, int m)
+{
+    const int lane = lane_id();
+    auto sorted = [&]() {
+        bool bad = false;
+        for (int i = lane; i + 1 < m; i += kWave) bad |= key[i] > key[i + 1];
+        return __builtin_amdgcn_ballot_w64(bad) == 0ull;
+    };
+    if (sorted()) return true;
+    for (int r = 0; r < kBlockRounds; r++) {
+        for (int b = (r & 1) ? kWave / 2 : 0; b < m; b += kWave) {
+            const int i = b + lane;
+            uint32_t k = i < m ? key[i] : kNoKey;
+            V v = i < m ? val[i] : V(0);
+            if (reg_unsorted(k)) {
+                bitonic_sort64(k, v, kWave);
+                if (i < m) {
+                    key[i] = k;
+                    val[i] = v;
+                }
+            }
+        }
+        wave_sync();
+        if (sorted()) return true;
+    }
+    return false;
 }
 
 // Rows r < *count of the list rows[r * step] (step -1 walks a list filled from
@@ -1216,21 +1252,29 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             key[lane] = k;
             val[lane] = v;
         } else {
-            for (int i = lane; i < m; i += kWave) {
-                uint32_t k = 0xFFFFFFFFu, c = 0u;
-                if (i < n) {
-                    uint64_t e = ev[base + i];
-                    k = (uint32_t)(e >> 32);
-                    c = (uint32_t)e;
+            // every load of the row in flight at once, then the LDS stores
+            constexpr int R = CAP / kWave;
+            uint64_t e[R];
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                const int i = c * kWave + lane;
+                e[c] = i < n ? ev[base + i] : ~0ull;
+            }
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                const int i = c * kWave + lane;
+                if (i < m) {
+                    const uint32_t k = i < n ? (uint32_t)(e[c] >> 32) : 0xFFFFFFFFu;
+                    const uint32_t cnt = i < n ? (uint32_t)e[c] : 0u;
+                    counts += cnt;
+                    key[i] = k;
+                    val[i] = event_value<V>(cnt, k, fw, F);
                 }
-                counts += c;
-                key[i] = k;
-                val[i] = event_value<V>(c, k, fw, F);
             }
             wave_sync();
-            // nearly sorted rows: odd-even transposition rounds, the bitonic
-            // network only for what is still out of order
-            if (lds_sort_rounds(key, val, m)) m_sorted = true;
+            // locally permuted rows: register block sorts; the bitonic
+            // network over the whole row only for what is still out of order
+            if (lds_block_sort(key, val, m)) m_sorted = true;
             if (!m_sorted)
             for (int kk = 2; kk <= m; kk <<= 1) {
                 for (int j = kk >> 1; j > 0; j >>= 1) {

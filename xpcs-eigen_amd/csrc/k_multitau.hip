@@ -837,8 +837,11 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
     __shared__ uint32_t s_ser[WAVES][sizeof(V) == 4 ? kDeepMaxFrames : 1];
+    __shared__ int4 s_tb[kRegMaxTaus];   // each tau's boundary indices, read by every row
     const int lane = lane_id();
     const int w = wave_id();
+    for (int i = (int)threadIdx.x; i < T && i < kRegMaxTaus; i += kWave * WAVES) s_tb[i] = tau_bnd[i];
+    __syncthreads();
     A* acc = s_acc[w];
     const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
     const int64_t nw = (int64_t)gridDim.x * WAVES;
@@ -873,8 +876,13 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             const uint32_t tot = read_lane_u32(wave_incl_scan_u32(c), 63);
             if (lane == 0) pix_total[p] = tot;
         }
+        // each event's boundary rank, gathered before the sort: rows usually
+        // arrive in order, and then it needs no second gather
+        uint32_t r = lane < n ? (uint32_t)frame_rank[k] : kNoKey;
+        const bool moved = reg_unsorted(k);
         reg_sort(k, v, n);
         int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
+        if (moved || nl != n) r = lane < nl ? (uint32_t)frame_rank[k] : kNoKey;
         // IP and IF of every tau from the level-0 prefix sums:
         // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau'), read at the
         // boundaries (the accumulator slots serve as the boundary slots first)
@@ -885,10 +893,9 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if constexpr (sizeof(V) == 4) {
                 if (lane == 0) pix_total[p] = read_lane_u32(pre, 63);
             }
-            const uint32_t r = lane < nl ? (uint32_t)frame_rank[k] : kNoKey;
             const uint32_t rn = next_lane_u32(r, kNoKey);
             if (lane < nl && r != rn && r < (uint32_t)nb) M[r] = pre;
-            boundary_sums(M, nb, tau_bnd, T, iprow, ifrow);
+            boundary_sums(M, nb, s_tb, T, iprow, ifrow);
         }
         for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
         wave_sync();

@@ -352,6 +352,11 @@ xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* plan);
  *                    at most deep_frames binned frames (16..96; negative: no
  *                    hand-over; 0: 96).  Only for dpl 4 or 8, T <= 128 and at
  *                    most 48 taus from that level on (otherwise no hand-over).
+ *   sums64           0: integer IP and IF sums are kept on the device as 32-bit
+ *                    words plus the high words of the pixels whose total
+ *                    passes 2^32 - 1 (the normalisation reads 16 instead of
+ *                    24 bytes per pixel and tau); 1: as int64.  Results are
+ *                    the same either way; read_sums returns int64.
  */
 typedef struct xpcs_mt_tuning {
     int32_t tr_bucket_shift;
@@ -360,7 +365,8 @@ typedef struct xpcs_mt_tuning {
     int32_t lds_blocks;
     int32_t dense_levels;
     int32_t deep_frames;
-    int32_t reserved[2];
+    int32_t sums64;
+    int32_t reserved;
 } xpcs_mt_tuning;
 
 xpcs_status xpcs_mt_set_tuning(xpcs_mt_plan* plan, const xpcs_mt_tuning* tuning);

@@ -485,6 +485,68 @@ def test_lds_rows_dense_levels_and_large_totals(xmt, orc, dpl, deep, row_order):
     plan.close()
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_sum_layouts_and_wide_rows(xmt, orc, wide):
+    """IP/IF kept as 32-bit words (high words only for pixels whose total
+    passes 2^32 - 1) against int64 sums: rows of every class (register,
+    LDS, beyond the LDS capacity) with totals on both sides of 2^32; with
+    `wide`, rows of at most 128 events total more than 2^32 (u32 counts
+    handed over pixel-major) and the register kernels step aside.  Sums
+    exact against the oracle, g2/stderr identical between the layouts and
+    close to the oracle's normalisation."""
+    rng = np.random.default_rng(77 + wide)
+    W = H = 12
+    F = 4000
+    dq, sq = synth.annular_qmaps(W, H, 4, 2)
+    npd = W * H
+    lens = rng.integers(1, 129, npd)
+    lens[40:60] = rng.integers(129, 2049, 20)          # LDS rows
+    lens[60:63] = [2600, 3100, 3900]                   # beyond the LDS capacity
+    lens[[41, 45]] = 1500
+    if wide:
+        lens[[3, 7]] = 5
+    # large rows total 4.5e9 .. 5e9: above 2^32, and every lag sum stays below
+    # total^2 / 4 < 2^63
+    big = {41: 3_000_000, 45: 3_000_000, 61: 1_500_000}
+    if wide:
+        big.update({3: 1_000_000_000, 7: 1_000_000_000})
+    frames, counts = [], []
+    for j, n in enumerate(lens):
+        frames.append(np.sort(rng.choice(F, int(n), replace=False)).astype(np.int32))
+        c = rng.integers(1, 50, int(n)).astype(np.uint32)
+        if j in big:
+            c[:] = big[j]
+        counts.append(c)
+    roff = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    qm = orc.build_qmap(dq, sq)
+    order = qm["order"]
+    off = np.concatenate([[0], np.cumsum(lens[order])]).astype(np.int64)
+    csr = dict(off=off, frame=np.concatenate([frames[j] for j in order]),
+               cnt=np.concatenate([counts[j] for j in order]).astype(np.int64))
+    lev, tau = orc.schedule(F, 4)
+    G2, IP, IF = orc.multitau_exact(csr, F, lev, tau)
+    g2o, seo = orc.normalize_f64(G2, IP, IF, orc.schedule_N(F, lev, tau), qm)
+    res = []
+    for sums64 in (0, 1):
+        print(f"sum layout sums64={sums64} wide={wide}", flush=True)
+        plan = xmt.Plan(W, H, F, 4, dq, sq)
+        plan.set_tuning(sums64=sums64)
+        plan.ingest_rows(roff, np.concatenate(frames), np.concatenate(counts))
+        plan.execute()
+        g, a, b = plan.sums()
+        assert np.array_equal(a, IP) and np.array_equal(b, IF)
+        assert np.array_equal(g, G2)
+        res.append(plan.g2())
+        assert plan.get_tuning()["sums64"] == sums64
+        plan.close()
+    tot = np.array([int(c.astype(np.int64).sum()) for c in counts])
+    assert tot.max() > 2 ** 32 and (tot[lens <= 128].max() > 2 ** 32) == wide
+    for x, y in zip(res[0], res[1]):
+        assert np.array_equal(x, y)
+    assert_g2_close(res[0][0], g2o)
+    assert_g2_close(res[0][1], seo)
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_shuffled_rows_two_per_lane(xmt, orc, seed):
     """Rows of 65..128 events in random frame order (the two-events-per-lane

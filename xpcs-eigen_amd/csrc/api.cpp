@@ -23,6 +23,7 @@ using xpcs::fail;
 
 namespace xpcs {
 int sparse_cap_long();
+int reg_cap();
 
 template <typename T>
 static xpcs_status dalloc(T** p, size_t n)
@@ -882,15 +883,19 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     } catch (const std::bad_alloc&) {
         return fail(XPCS_ERR_NOMEM, "ingest_rows: %lld events do not fit host memory", (long long)off[(size_t)h.P]);
     }
+    bool wide = false;   // a row short enough for the register kernels totals 2^32 or more
     for (int c = 0; c < h.P; c++) {
         const int64_t j = h.order[(size_t)c];
         int64_t o = off[(size_t)c];
+        uint64_t tot = 0;
         for (int64_t e = row_off[j]; e < row_off[j + 1]; e++) {
             if (frame[e] < 0 || frame[e] >= h.F)
                 return fail(XPCS_ERR_RANGE, "ingest_rows: frame %d of pixel %lld outside [0, %d)", frame[e],
                             (long long)j, h.F);
             ev[(size_t)o++] = ((uint64_t)(uint32_t)frame[e] << 32) | count[e];
+            tot += count[e];
         }
+        if (row_off[j + 1] - row_off[j] <= reg_cap() && tot > 0xFFFFFFFFull) wide = true;
     }
     xpcs_status st = upload(&p->d_ev, ev, p->stream);
     if (st) return st;
@@ -900,6 +905,7 @@ extern "C" xpcs_status xpcs_mt_ingest_rows(xpcs_mt_plan* p, const int64_t* row_o
     p->nscale = 1;
     p->fmt = kFmtImm;
     p->mode = 3;
+    p->wide_rows = wide;
     p->nrows = h.P;
     p->d_rowmap = nullptr;
     p->executed = false;
@@ -927,6 +933,8 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
     if (p->fsnorm && p->mode == 3)
         return fail(XPCS_ERR_UNSUPPORTED, "execute: normalize_by_framesum needs frame sums (IMM or packed ingest)");
     HostTables& h = p->h;
+    // integer IP/IF in the 32-bit layout (unless tuned off); FP64 sums stay 8 bytes
+    p->ipif32 = !p->fsnorm && p->tune.sums64 == 0;
     const bool dense_stream = p->mode == 2 && dense_stream_ok(p);
     const bool long_rows = !dense_stream;
     if (long_rows && !p->d_scratch) {
@@ -1068,6 +1076,27 @@ extern "C" xpcs_status xpcs_mt_synchronize(xpcs_mt_plan* p)
     return sync_and_check(p);
 }
 
+// IP or IF of an integer plan as int64 [P][T]: the 32-bit layout widened with
+// the high words of the pixels whose total passes 2^32 - 1.
+static xpcs_status read_ipif(xpcs_mt_plan* p, const int64_t* d, int64_t* out)
+{
+    const size_t n = (size_t)p->h.P * p->h.T;
+    if (!p->ipif32) {
+        HIP_TRY(hipMemcpy(out, d, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+        return XPCS_OK;
+    }
+    std::vector<uint32_t> w(2 * n);
+    std::vector<uint64_t> tot((size_t)p->h.P);
+    HIP_TRY(hipMemcpy(w.data(), d, 2 * n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tot.data(), p->d_pix_total, tot.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const size_t T = (size_t)p->h.T;
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t hi = tot[i / T] > 0xFFFFFFFFull ? (uint64_t)w[n + i] << 32 : 0;
+        out[i] = (int64_t)(hi | w[i]);
+    }
+    return XPCS_OK;
+}
+
 extern "C" xpcs_status xpcs_mt_read_sums_f64(xpcs_mt_plan* p, double* G2, double* IP, double* IF)
 {
     if (!p || !G2 || !IP || !IF) return fail(XPCS_ERR_INVALID_ARG, "read_sums_f64: NULL argument");
@@ -1079,7 +1108,12 @@ extern "C" xpcs_status xpcs_mt_read_sums_f64(xpcs_mt_plan* p, double* G2, double
     double* out[3] = {G2, IP, IF};
     const int64_t* src[3] = {p->d_G2, p->d_IP, p->d_IF};
     for (int k = 0; k < 3; k++) {
-        HIP_TRY(hipMemcpy(tmp.data(), src[k], n * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (k > 0 && !p->fsnorm) {
+            st = read_ipif(p, src[k], tmp.data());
+            if (st) return st;
+        } else {
+            HIP_TRY(hipMemcpy(tmp.data(), src[k], n * sizeof(int64_t), hipMemcpyDeviceToHost));
+        }
         for (size_t i = 0; i < n; i++) {
             double v;
             if (p->fsnorm) std::memcpy(&v, &tmp[i], sizeof(v));
@@ -1099,8 +1133,8 @@ extern "C" xpcs_status xpcs_mt_read_sums_i64(xpcs_mt_plan* p, int64_t* G2, int64
     if (st) return st;
     const size_t n = (size_t)p->h.P * p->h.T * sizeof(int64_t);
     if (G2) HIP_TRY(hipMemcpy(G2, p->d_G2, n, hipMemcpyDeviceToHost));
-    if (IP) HIP_TRY(hipMemcpy(IP, p->d_IP, n, hipMemcpyDeviceToHost));
-    if (IF) HIP_TRY(hipMemcpy(IF, p->d_IF, n, hipMemcpyDeviceToHost));
+    if (IP && (st = read_ipif(p, p->d_IP, IP))) return st;
+    if (IF && (st = read_ipif(p, p->d_IF, IF))) return st;
     return XPCS_OK;
 }
 
@@ -1165,7 +1199,7 @@ extern "C" xpcs_status xpcs_mt_normalize_ref_f32(xpcs_mt_plan* p, const float* G
     if (e == hipSuccess) {
         launch_from_ref_f32(p, s, d_in, d_in + n_in, d_in + 2 * n_in, d_sum, d_sum + n_out, d_sum + 2 * n_out);
         // FP64 sums, no flat-field weights (the values carry them)
-        launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, d_sum, d_sum + n_out, d_sum + 2 * n_out, true,
+        launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, d_sum, d_sum + n_out, d_sum + 2 * n_out, true, false,
                               nullptr, plan_norm_out(p));
         e = hipGetLastError();
     }

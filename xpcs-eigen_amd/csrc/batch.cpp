@@ -28,6 +28,7 @@ bool tuning_valid(const xpcs_mt_plan* p, const xpcs_mt_tuning& t)
     if (t.lds_blocks < 0 || t.lds_blocks > 65535) return false;
     if (t.dense_levels != 0 && !dense_levels_allowed(p->h.dpl, t.dense_levels)) return false;
     if (t.deep_frames > 0 && (t.deep_frames < 16 || t.deep_frames > 96)) return false;
+    if (t.sums64 != 0 && t.sums64 != 1) return false;
     return true;
 }
 
@@ -95,6 +96,7 @@ extern "C" xpcs_status xpcs_mt_get_tuning(xpcs_mt_plan* p, xpcs_mt_tuning* t)
     t->lds_blocks = lds_blocks(p);
     t->dense_levels = (p->h.dpl == 4 || p->h.dpl == 8) ? dense_levels(p) : 0;
     t->deep_frames = p->tune.deep_frames < 0 ? -1 : (p->tune.deep_frames > 0 ? p->tune.deep_frames : 96);
+    t->sums64 = p->tune.sums64;
     return XPCS_OK;
 }
 

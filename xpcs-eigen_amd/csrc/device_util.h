@@ -57,6 +57,32 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x)
     return x;
 }
 
+// IP and IF per (pixel, tau), idx = p T + tau: int64 (or FP64 bit patterns),
+// or (w32, integer sums of the batch path) the low 32 bits in 32-bit words
+// [0, P T) and, for pixels whose total passes 2^32 - 1, the high 32 bits at
+// hi + idx (hi = P T; 0: the pixel needs none).  The normaliser then reads 16
+// instead of 24 bytes per pixel and tau, and a pixel's total says whether its
+// high words exist: IP and IF never exceed it.
+__device__ __forceinline__ void put_sum(int64_t* base, int64_t idx, int64_t v, bool w32, int64_t hi = 0)
+{
+    if (w32) {
+        uint32_t* b = reinterpret_cast<uint32_t*>(base);
+        b[idx] = (uint32_t)v;
+        if (hi) b[hi + idx] = (uint32_t)((uint64_t)v >> 32);
+    } else {
+        base[idx] = v;
+    }
+}
+__device__ __forceinline__ int64_t get_sum(const int64_t* base, int64_t idx, bool w32, int64_t hi = 0)
+{
+    if (!w32) return base[idx];
+    const uint32_t* b = reinterpret_cast<const uint32_t*>(base);
+    const uint64_t lo = b[idx];
+    return (int64_t)(hi ? lo | ((uint64_t)b[hi + idx] << 32) : lo);
+}
+// the offset of a pixel's high words: hw = P T when the layout is w32
+__device__ __forceinline__ int64_t sum_hi(uint64_t total, int64_t hw) { return total > 0xFFFFFFFFull ? hw : 0; }
+
 __device__ __forceinline__ int next_pow2(int n)
 {
     int m = 1;

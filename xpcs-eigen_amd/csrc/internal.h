@@ -209,8 +209,10 @@ struct xpcs_mt_plan {
 
     // outputs
     int64_t* d_G2 = nullptr;             // [P][T]
-    int64_t* d_IP = nullptr;
-    int64_t* d_IF = nullptr;
+    int64_t* d_IP = nullptr;             // [P][T] int64, or with ipif32 the 32-bit layout
+    int64_t* d_IF = nullptr;             //   (low words [P*T], high words of pixels with totals >= 2^32 after them)
+    bool ipif32 = false;                 // integer sums of the batch path (set by prepare_execute)
+    bool wide_rows = false;              // ingest_rows: a row of <= 128 events totals 2^32 or more
     uint64_t* d_pix_total = nullptr;     // [P] sum of level-0 values
     xpcs::NormPartial* d_part = nullptr; // [nchunks][T]
     double* d_g2s = nullptr;             // [ngroups][T]
@@ -283,7 +285,7 @@ struct NormOut {
 NormOut plan_norm_out(xpcs_mt_plan* p);
 void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
-                           const double* weight, const NormOut& out);
+                           bool w32, const double* weight, const NormOut& out);
 // the statically placed scratch of the allocation-free entry point
 // (noalloc.cpp): its address is resolved once per device, at plan creation
 xpcs_status noalloc_setup(int device);

@@ -164,7 +164,8 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
                int32_t nbnd, int32_t T, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
                int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
                std::conditional_t<FP, double, uint32_t>* __restrict__ series_out, uint32_t lf_out,
-               uint32_t* __restrict__ redo_count, int32_t* __restrict__ redo_list, const double* __restrict__ fw)
+               uint32_t* __restrict__ redo_count, int32_t* __restrict__ redo_list, const double* __restrict__ fw,
+               bool w32)
 {
     using ST = std::conditional_t<FP, double, uint32_t>;   // series element
     constexpr bool RAW = MODE != kSeries;
@@ -234,9 +235,9 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
 #pragma unroll
                     for (int u = 0; u < kGroup; u++) part += (u < n) ? x[u] : 0u;
                     if (active) {
-                        if (b.y == 0) IP[rowT + b.z] = (int64_t)part;
-                        else if (b.y == 1) IF[rowT + b.z] = (int64_t)part;
-                        else IF[rowT + b.z] = (int64_t)part - IF[rowT + b.z];
+                        if (b.y == 0) put_sum(IP, rowT + b.z, (int64_t)part, w32);
+                        else if (b.y == 1) put_sum(IF, rowT + b.z, (int64_t)part, w32);
+                        else put_sum(IF, rowT + b.z, (int64_t)part - get_sum(IF, rowT + b.z, w32), w32);
                     }
                 }
                 bi++;
@@ -400,8 +401,8 @@ k_mt_dense_f64(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_
         for (int i = 0; i < nbnd; i++) {
             const int4 b = bnd[i];
             if (b.x != 0) break;
-            if (b.y == 0) IP[rowT + b.z] = 0;
-            else if (b.y == 1) IF[rowT + b.z] = 0;
+            if (b.y == 0) put_sum(IP, rowT + b.z, 0, !FP && w32);
+            else if (b.y == 1) put_sum(IF, rowT + b.z, 0, !FP && w32);
         }
         if (!FP && S > kExactF64Total) redo_list[atomicAdd(redo_count, 1u)] = (int32_t)gid;
     }
@@ -724,12 +725,12 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
                 k_mt_dense_f64<DPL, K, kRawTiles, FP><<<grid, 256, 0, s>>>(
                     p->d_imm, p->d_sel_off, p->tile_off0, p->tile_step, h.avg, p->d_pix2c, npd, p->d_dark_sum,
                     p->d_thr, (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T,
-                    p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw);
+                    p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw, !FP && p->ipif32);
             else
                 k_mt_dense_f64<DPL, K, kRawLanes, FP><<<grid, 256, 0, s>>>(
                     p->d_imm, p->d_sel_off, 0, 0, h.avg, p->d_pix2c, npd, p->d_dark_sum, p->d_thr,
                     (int32_t)p->nscale, nullptr, n_in, p->d_levels, L0, k, p->d_bnd, p->nbnd, h.T, p->d_G2,
-                    p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw);
+                    p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, redo_count, p->d_long_list, fw, !FP && p->ipif32);
             timer_end(p, s);
             if constexpr (!FP) {
                 timer_begin(p, s, "k_mt_dense_exact");
@@ -742,7 +743,8 @@ static void run_passes(xpcs_mt_plan* p, hipStream_t s)
             timer_begin(p, s, FP ? "k_mt_dense_series_f64" : "k_mt_dense_series");
             k_mt_dense_f64<DPL, K, kSeries, FP><<<grid, 256, 0, s>>>(
                 nullptr, nullptr, 0, 0, 1, p->d_pix2c, npd, nullptr, nullptr, 1, in, n_in, p->d_levels, L0, k,
-                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr, fw);
+                nullptr, 0, h.T, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, out, lf_out, nullptr, nullptr, fw,
+                !FP && p->ipif32);
             timer_end(p, s);
             if constexpr (!FP) {
                 timer_begin(p, s, "k_mt_dense_exact");

@@ -337,7 +337,8 @@ __device__ __forceinline__ double running_max(double x)
 // for every tau (indices per tau in tau_bnd).
 template <typename V>
 __device__ __forceinline__ void boundary_sums(V* M, int nb, const int4* __restrict__ tau_bnd, int T,
-                                              int64_t* __restrict__ iprow, int64_t* __restrict__ ifrow)
+                                              int64_t* __restrict__ IP, int64_t* __restrict__ IF, int64_t row0,
+                                              bool w32)
 {
     const int lane = lane_id();
     wave_sync();
@@ -353,8 +354,8 @@ __device__ __forceinline__ void boundary_sums(V* M, int nb, const int4* __restri
     wave_sync();
     for (int i = lane; i < T; i += kWave) {
         const int4 j = tau_bnd[i];
-        iprow[i] = sum_bits(M[j.x]);
-        ifrow[i] = sum_bits((V)(M[j.z] - M[j.y]));
+        put_sum(IP, row0 + i, sum_bits(M[j.x]), w32);
+        put_sum(IF, row0 + i, sum_bits((V)(M[j.z] - M[j.y])), w32);
     }
     wave_sync();
 }
@@ -658,7 +659,7 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
           const uint16_t* __restrict__ frame_rank, int32_t nb, const LevelParam* __restrict__ levels,
           int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
           uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
-          int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
+          int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
@@ -676,8 +677,6 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
         const int64_t base = off[row];
         const int n = (int)min<int64_t>(off[row + 1] - base, 2 * kWave);   // 65..128 by construction
         int64_t* g2row = G2 + p * T;
-        int64_t* iprow = IP + p * T;
-        int64_t* ifrow = IF + p * T;
         uint32_t k0 = kNoKey, k1 = kNoKey, c0 = 0u, c1 = 0u;
         if (lane < n) {
             const uint64_t e = ev[base + lane];
@@ -711,7 +710,7 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
             const uint32_t rn0 = next_lane_u32(r0, read_lane_u32(r1, 0)), rn1 = next_lane_u32(r1, kNoKey);
             if (lane < nl && r0 != rn0 && r0 < (uint32_t)nb) M[r0] = pre0;
             if (lane + kWave < nl && r1 != rn1 && r1 < (uint32_t)nb) M[r1] = pre1;
-            boundary_sums(M, nb, tau_bnd, T, iprow, ifrow);
+            boundary_sums(M, nb, tau_bnd, T, IP, IF, p * T, w32);
         }
         for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
         wave_sync();
@@ -769,7 +768,7 @@ constexpr int kClassifyPer = 8;      // pixels per thread of the classification 
 // class and workgroup, so the shared counters see few atomics.  The lists'
 // order does not matter: every row writes only its own sums.
 __global__ void __launch_bounds__(256)
-k_mt_classify(const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap, int32_t P, int32_t T,
+k_mt_classify(const int64_t* __restrict__ off, const int32_t* __restrict__ rowmap, int32_t P, bool regs,
               int32_t* __restrict__ mid_list, int32_t* __restrict__ long_list, uint32_t* __restrict__ counters)
 {
     __shared__ uint32_t s_cnt[4];
@@ -788,7 +787,7 @@ k_mt_classify(const int64_t* __restrict__ off, const int32_t* __restrict__ rowma
         if (p < P) {
             const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
             const int64_t n = off[row + 1] - off[row];
-            const bool reg = n == 0 || (n <= kWave && T <= kRegMaxTaus);
+            const bool reg = n == 0 || (n <= kWave && regs);
             if (!reg) cls[i] = n <= kMidSmall ? 0 : n <= kSparseCapUsed ? 1 : n <= kSparseCapMid ? 2 : 3;
         }
 #pragma unroll
@@ -832,7 +831,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
          const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
          uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
-         int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
+         int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32, bool reg_ok)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
@@ -845,7 +844,6 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
     A* acc = s_acc[w];
     const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
     const int64_t nw = (int64_t)gridDim.x * WAVES;
-    const bool reg_ok = T <= kRegMaxTaus;
 
     for (int64_t p = (int64_t)blockIdx.x * WAVES + w; p < P; p += nw) {
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;   // CSR row of pixel p
@@ -853,14 +851,12 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         const int64_t n64 = off[row + 1] - base;
         if (n64 > kWave || (!reg_ok && n64 > 0)) continue;   // listed by k_mt_classify
         int64_t* g2row = G2 + p * T;
-        int64_t* iprow = IP + p * T;
-        int64_t* ifrow = IF + p * T;
         const int n = (int)n64;
         if (n == 0) {
             for (int i = lane; i < T; i += kWave) {
                 g2row[i] = 0;
-                iprow[i] = 0;
-                ifrow[i] = 0;
+                put_sum(IP, p * T + i, 0, w32);
+                put_sum(IF, p * T + i, 0, w32);
             }
             if (lane == 0) pix_total[p] = 0;
             continue;
@@ -895,7 +891,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             }
             const uint32_t rn = next_lane_u32(r, kNoKey);
             if (lane < nl && r != rn && r < (uint32_t)nb) M[r] = pre;
-            boundary_sums(M, nb, s_tb, T, iprow, ifrow);
+            boundary_sums(M, nb, s_tb, T, IP, IF, p * T, w32);
         }
         for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
         wave_sync();
@@ -1017,8 +1013,9 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         const int64_t p = p0 + lane;
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
         const int64_t n = off[row + 1] - off[row];
-        // rows of the register kernels, and the LDS rows whose total fits 32 bits
-        mine = n >= 1 && (n <= kMidSmall || (n <= kSparseCapLong && pix_total[p] <= 0xFFFFFFFFull));
+        // the rows handed over: those of the register kernels (totals below
+        // 2^32) and the LDS rows whose total fits 32 bits
+        mine = n >= 1 && n <= kSparseCapLong && pix_total[p] <= 0xFFFFFFFFull;
     }
     wave_sync();
     uint32_t* x = s_x + lane * SP;
@@ -1215,7 +1212,8 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const TauParam* __restrict__ taus, int32_t T, const LevelParam* __restrict__ levels,
             int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP,
             int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap,
-            const double* __restrict__ fw, int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride)
+            const double* __restrict__ fw, int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32,
+            int64_t hw)
 {
     using A = typename LagAcc<V>::type;
     constexpr bool U = sizeof(V) == 4;
@@ -1238,8 +1236,6 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
         const int64_t base = off[row];
         const int64_t n64 = off[row + 1] - base;
         int64_t* g2row = G2 + p * T;
-        int64_t* iprow = IP + p * T;
-        int64_t* ifrow = IF + p * T;
         if (n64 > CAP) continue;        // a list shared with the series kernel (its longer rows)
         const int n = (int)n64;         // 0 < n <= CAP
         const int m = next_pow2(n);
@@ -1321,6 +1317,7 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             const int j_end = lds_lower_bound(key, n0, (uint32_t)levels[lane].lf << lane);
             s_lvl = j_end > 0 ? aux[j_end - 1] : A(0);
         }
+        const int64_t hi = U ? sum_hi((uint64_t)carry, hw) : 0;
         for (int i0 = 0; i0 < T; i0 += kWave) {
             const int i = i0 + lane;
             const TauParam tp = taus[i < T ? i : T - 1];
@@ -1332,8 +1329,8 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 const int j_if = lds_lower_bound(key, n0, b_if);
                 const A s_ip = j_ip > 0 ? aux[j_ip - 1] : A(0);
                 const A s_if = j_if > 0 ? aux[j_if - 1] : A(0);
-                iprow[i] = sum_bits(s_ip);
-                ifrow[i] = sum_bits((A)(s_end - s_if));
+                put_sum(IP, p * T + i, sum_bits(s_ip), w32, hi);
+                put_sum(IF, p * T + i, sum_bits((A)(s_end - s_if)), w32, hi);
             }
         }
         // counts: the row total bounds every level value (32 bits for the
@@ -1455,8 +1452,8 @@ __device__ __forceinline__ double slot_value<double>(unsigned long long v) { ret
 // uint64_t (exact integer sums) or double (frame-sum normalised values).
 template <typename AccT, typename SrcT>
 __device__ void long_level(const SrcT* __restrict__ src, int lf, const LevelParam& lp,
-                           int64_t* __restrict__ g2row, int64_t* __restrict__ iprow,
-                           int64_t* __restrict__ ifrow, unsigned long long* s_red,
+                           int64_t* __restrict__ g2row, int64_t* __restrict__ IP,
+                           int64_t* __restrict__ IF, int64_t row0, bool w32, int64_t hi, unsigned long long* s_red,
                            unsigned long long* s_lag, uint64_t* total0)
 {
     // sum over the level
@@ -1495,8 +1492,8 @@ __device__ void long_level(const SrcT* __restrict__ src, int lf, const LevelPara
         AccT head = 0, tail = 0;
         for (int t = 0; t < tp && t < lf; t++) head += (AccT)src[t];
         for (int t = max(lf - tp, 0); t < lf; t++) tail += (AccT)src[t];
-        iprow[lp.first + d] = sum_bits((AccT)(S - tail));
-        ifrow[lp.first + d] = sum_bits((AccT)(S - head));
+        put_sum(IP, row0 + lp.first + d, sum_bits((AccT)(S - tail)), w32, hi);
+        put_sum(IF, row0 + lp.first + d, sum_bits((AccT)(S - head)), w32, hi);
     }
     __syncthreads();
 }
@@ -1528,10 +1525,12 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
           int dense,
           const double* __restrict__ fw, uint64_t* __restrict__ scratch, int64_t* __restrict__ G2,
           int64_t* __restrict__ IP, int64_t* __restrict__ IF, uint64_t* __restrict__ pix_total,
-          const int32_t* __restrict__ rowmap)
+          const int32_t* __restrict__ rowmap, bool w32)
 {
     __shared__ unsigned long long s_red[kLongBlock / 64];
     __shared__ unsigned long long s_lag[kMaxLagBlock];
+    // w32: the long rows write every high word (their totals are found late)
+    const int64_t hi = w32 ? (int64_t)P * T : 0;
     AccT* x = reinterpret_cast<AccT*>(scratch + (size_t)blockIdx.x * (size_t)F);
     const int64_t nrows = dense ? (int64_t)P : (int64_t)counters[0];
     for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
@@ -1542,17 +1541,15 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
             if (off[row + 1] - off[row] <= min_len) continue;
         }
         int64_t* g2row = G2 + p * T;
-        int64_t* iprow = IP + p * T;
-        int64_t* ifrow = IF + p * T;
         for (int i = threadIdx.x; i < T; i += blockDim.x) {
             g2row[i] = sum_bits((AccT)0);
-            iprow[i] = sum_bits((AccT)0);
-            ifrow[i] = sum_bits((AccT)0);
+            put_sum(IP, p * T + i, sum_bits((AccT)0), w32, hi);
+            put_sum(IF, p * T + i, sum_bits((AccT)0), w32, hi);
         }
         __syncthreads();
         if (dense && !fw) {
             const uint32_t* row = rows + p * (int64_t)F;
-            long_level<AccT, uint32_t>(row, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, pix_total + p);
+            long_level<AccT, uint32_t>(row, F, levels[0], g2row, IP, IF, p * T, w32, hi, s_red, s_lag, pix_total + p);
             if (nlev > 1) long_bin<AccT, uint32_t>(row, x, levels[1].lf);
         } else {
             uint64_t counts = 0;   // the pixel's raw count total (pixelSum precedes the normalisation)
@@ -1583,12 +1580,12 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 const uint64_t c = block_sum_u64(counts, s_red);
                 if (threadIdx.x == 0) pix_total[p] = c;
             }
-            long_level<AccT, AccT>(x, F, levels[0], g2row, iprow, ifrow, s_red, s_lag, fw ? nullptr : pix_total + p);
+            long_level<AccT, AccT>(x, F, levels[0], g2row, IP, IF, p * T, w32, hi, s_red, s_lag, fw ? nullptr : pix_total + p);
             if (nlev > 1) long_bin<AccT, AccT>(x, x, levels[1].lf);
         }
         for (int L = 1; L < nlev; L++) {
             const LevelParam lp = levels[L];
-            long_level<AccT, AccT>(x, lp.lf, lp, g2row, iprow, ifrow, s_red, s_lag, nullptr);
+            long_level<AccT, AccT>(x, lp.lf, lp, g2row, IP, IF, p * T, w32, hi, s_red, s_lag, nullptr);
             if (L + 1 < nlev) long_bin<AccT, AccT>(x, x, levels[L + 1].lf);
         }
         __syncthreads();
@@ -1610,43 +1607,48 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
                         const double* fw)
 {
     const HostTables& h = p->h;
+    // register rows (32-bit level values): T <= kRegMaxTaus, and no row of at
+    // most kMidSmall events totals 2^32 or more (pixel-major rows of u32 counts)
+    const bool regs = h.T <= kRegMaxTaus && !(p->mode == 3 && p->wide_rows);
     timer_begin(p, s, "k_mt_classify");
     if (h.P > 0)
         k_mt_classify<<<(unsigned)((h.P + 256 * kClassifyPer - 1) / (256 * kClassifyPer)), 256, 0, s>>>(
-            p->d_off, p->d_rowmap, h.P, h.T, p->d_mid_list, p->d_long_list, p->d_counters);
+            p->d_off, p->d_rowmap, h.P, regs, p->d_mid_list, p->d_long_list, p->d_counters);
     timer_end(p, s);
     timer_begin(p, s, t_reg);
     // the deep levels' hand-over (counts only)
     const int nlev = (int)h.levels.size();
     int dstride = 0;
     const int ldeep = sizeof(V) == 4 ? deep_level(p, &dstride) : nlev;
+    const bool w32 = sizeof(V) == 4 && p->ipif32;
+    const int64_t hw = (int64_t)h.P * h.T;   // high words of the w32 layout
     k_mt_reg<kRegWaves, V><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels, nlev, h.F, p->d_G2,
-        p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep, dstride);
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep, dstride, w32, regs);
     timer_end(p, s);
     timer_begin(p, s, t_lds);
-    if (h.T <= kRegMaxTaus)
+    if (regs)
         k_mt_reg2<kReg2Waves, V><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank,
             p->d_levels, nlev, h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep,
-            dstride);
+            dstride, w32);
     else
         k_mt_sparse<kMidSmall, kSparseWaves, V><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
             (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
-            p->d_deep, dstride);
+            p->d_deep, dstride, w32, hw);
     k_mt_sparse<kSparseCapUsed, kSparseWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_mid_list + (h.P - 1), -1, p->d_counters + 4, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
-        p->d_deep, dstride);
+        p->d_deep, dstride, w32, hw);
     k_mt_sparse<kSparseCapMid, kSparseMidWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseMidWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_long_list, 1, p->d_counters + 5, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
-        p->d_deep, dstride);
+        p->d_deep, dstride, w32, hw);
     k_mt_sparse<kSparseCapLong, kSparseLongWaves, V><<<std::max(1, lds_blocks(p) / 4), kWave * kSparseLongWaves, 0, s>>>(
         p->d_ev, p->d_off, p->d_long_list + (h.P - 1), -1, p->d_counters, p->d_taus, h.T, p->d_levels,
         (int32_t)h.levels.size(), h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep,
-        p->d_deep, dstride);
+        p->d_deep, dstride, w32, hw);
     timer_end(p, s);
     // rows longer than kSparseCapLong: the dense-series kernel
     timer_begin(p, s, t_long);
@@ -1655,13 +1657,13 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
             p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F,
             p->d_long_list + (h.P - 1), -1, (int64_t)kSparseCapLong, p->d_counters, 0, nullptr, p->d_scratch,
             p->d_G2, p->d_IP, p->d_IF,
-            p->d_pix_total, p->d_rowmap);
+            p->d_pix_total, p->d_rowmap, w32);
     else
         k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
             p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F,
             p->d_long_list + (h.P - 1), -1, (int64_t)kSparseCapLong, p->d_counters, 0, fw, p->d_scratch,
             p->d_G2, p->d_IP, p->d_IF,
-            p->d_pix_total, p->d_rowmap);
+            p->d_pix_total, p->d_rowmap, false);
     timer_end(p, s);
     if (ldeep < nlev && h.P > 0) {
         timer_begin(p, s, "k_mt_deep");
@@ -1712,7 +1714,8 @@ void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
     timer_begin(p, s, "k_mt_long_dense");
     k_mt_long<uint64_t><<<p->long_blocks, kLongBlock, 0, s>>>(
         p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list, 1, 0,
-        p->d_counters, 1, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+        p->d_counters, 1, nullptr, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap,
+        p->ipif32);
     timer_end(p, s);
 }
 
@@ -1731,11 +1734,13 @@ void launch_mt_fsnorm(xpcs_mt_plan* p, hipStream_t s, bool dense_rows)
     timer_begin(p, s, "k_mt_long_f64");
     k_mt_long<double><<<p->long_blocks, kWave, 0, s>>>(
         p->d_ev, p->d_off, p->d_rows, h.P, h.T, p->d_levels, (int32_t)h.levels.size(), h.F, p->d_long_list, 1, 0,
-        p->d_counters, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap);
+        p->d_counters, 1, p->d_fw, p->d_scratch, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap,
+        false);
     timer_end(p, s);
 }
 
 int sparse_cap_long() { return kSparseCapLong; }
+int reg_cap() { return kMidSmall; }
 int sparse_cap() { return kSparseCapUsed; }
 
 // Workgroups of the row kernels: one row per wave up to 16384 workgroups by

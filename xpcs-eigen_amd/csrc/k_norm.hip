@@ -39,21 +39,41 @@ enum NormMode { kNormInt = 0, kNormWeighted = 1, kNormFp = 2 };
 
 // Pixels [pb, pe) of tau column i in order. kNormFp: the sums are FP64 stored
 // in the same 8-byte buffers; they take the weighted (FP64) route with weight
-// 1 when the plan has no flat field.
-template <int MODE>
+// 1 when the plan has no flat field.  W32: integer IP and IF in 32-bit words,
+// with high words at hw + idx for the pixels whose total passes 2^32 - 1.
+template <int MODE, bool W32>
 __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                                                 const int64_t* __restrict__ IF, const double* __restrict__ weight,
-                                                int32_t S, int i, double N, int pb, int pe, NormAcc& a)
+                                                const uint64_t* __restrict__ pix_total, int64_t hw, int32_t S,
+                                                int i, double N, int pb, int pe, NormAcc& a)
 {
     // kNormBatch pixels' loads in flight per thread
     for (int p0 = pb; p0 < pe; p0 += kNormBatch) {
         int64_t gg[kNormBatch], aa[kNormBatch], bb[kNormBatch];
+        uint64_t tt[kNormBatch];
 #pragma unroll
         for (int u = 0; u < kNormBatch; u++) {
             const int pu = min(p0 + u, pe - 1);
-            gg[u] = G2[(int64_t)pu * S + i];
-            aa[u] = IP[(int64_t)pu * S + i];
-            bb[u] = IF[(int64_t)pu * S + i];
+            const int64_t idx = (int64_t)pu * S + i;
+            gg[u] = G2[idx];
+            if constexpr (W32) {
+                aa[u] = (int64_t)reinterpret_cast<const uint32_t*>(IP)[idx];
+                bb[u] = (int64_t)reinterpret_cast<const uint32_t*>(IF)[idx];
+                tt[u] = pix_total[pu];
+            } else {
+                aa[u] = IP[idx];
+                bb[u] = IF[idx];
+            }
+        }
+        if constexpr (W32) {
+#pragma unroll
+            for (int u = 0; u < kNormBatch; u++) {
+                if (tt[u] > 0xFFFFFFFFull) {
+                    const int64_t idx = (int64_t)min(p0 + u, pe - 1) * S + i;
+                    aa[u] = get_sum(IP, idx, true, hw);
+                    bb[u] = get_sum(IF, idx, true, hw);
+                }
+            }
         }
         for (int u = 0; u < kNormBatch && p0 + u < pe; u++) {
             const int p = p0 + u;
@@ -124,12 +144,12 @@ __device__ __forceinline__ void norm_merge(NormAcc& a, const NormAcc& b)
 // range is narrow, as in stream snapshots). Neither the tau range nor the
 // number of chunks per workgroup changes a chunk's computation, so a range of
 // taus gets the same partials as the full schedule.
-template <int MODE>
+template <int MODE, bool W32>
 __global__ void __launch_bounds__(kNormBlock)
 k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauParam* __restrict__ taus,
                int32_t t0, int32_t t1, int32_t S, const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                const int64_t* __restrict__ IF, const double* __restrict__ weight,
-               NormPartial* __restrict__ part)
+               const uint64_t* __restrict__ pix_total, int64_t hw, NormPartial* __restrict__ part)
 {
     __shared__ __align__(16) unsigned char s_raw[kNormBlock * sizeof(NormAcc)];
     NormAcc* s_acc = reinterpret_cast<NormAcc*>(s_raw);
@@ -153,7 +173,7 @@ k_norm_partial(const NormChunk* __restrict__ chunks, int32_t nchunks, const TauP
             const bool act = live && i < t1;
             if (act) {
                 NormAcc a;
-                norm_accumulate<MODE>(G2, IP, IF, weight, S, i, (double)taus[i].N, pb, pe, a);
+                norm_accumulate<MODE, W32>(G2, IP, IF, weight, pix_total, hw, S, i, (double)taus[i].N, pb, pe, a);
                 s_acc[threadIdx.x] = a;
             }
             __syncthreads();
@@ -298,6 +318,7 @@ __device__ __forceinline__ double sum_value(int64_t v, bool fp) { return fp ? __
 template <bool FP>
 __global__ void __launch_bounds__(256)
 k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const int64_t* __restrict__ IF,
+          bool w32, const uint64_t* __restrict__ pix_total,
           const int32_t* __restrict__ c2pix, int32_t P, const TauParam* __restrict__ taus, int32_t T,
           const double* __restrict__ weight, double inv_scale, int64_t npix_det,
           float* __restrict__ oG2, float* __restrict__ oIP, float* __restrict__ oIF)
@@ -312,8 +333,9 @@ k_ref_f32(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP, const 
     const double N = (double)tp.N;
     const int64_t o = (int64_t)i * npix_det + c2pix[c];
     oG2[o] = (float)(sum_value(G2[c * T + i], FP) * s * s / (lv * lv * N));
-    oIP[o] = (float)(sum_value(IP[c * T + i], FP) * s / (lv * N));
-    oIF[o] = (float)(sum_value(IF[c * T + i], FP) * s / (lv * N));
+    const int64_t hi = w32 ? sum_hi(pix_total[c], (int64_t)P * T) : 0;
+    oIP[o] = (float)(sum_value(get_sum(IP, c * T + i, w32, hi), FP) * s / (lv * N));
+    oIF[o] = (float)(sum_value(get_sum(IF, c * T + i, w32, hi), FP) * s / (lv * N));
 }
 
 // Reference-unit float arrays [T][W*H] (the G2s/IPs/IFs of a caller, as
@@ -338,14 +360,15 @@ k_from_ref_f32(const float* __restrict__ G2, const float* __restrict__ IP, const
 }
 
 // Normalisation of taus [t0, t1) of the sums G2/IP/IF: rows of stride S
-// ([P][S]), int64 integers or (fp) FP64 bit patterns, each pixel weighted by
+// ([P][S]), int64 integers (w32: IP and IF in the 32-bit layout, the plan's
+// pixel totals telling which rows have high words) or (fp) FP64 bit patterns, each pixel weighted by
 // weight[c] (NULL: 1); taus gives N per tau; the chunk partials and group
 // values use the same stride, g2 and stderr are [tau][q].  Values of other
 // taus are left as they are.  The partials, group values and results go to
 // `out` (the plan's buffers, or caller / scratch memory).
 void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus, const int64_t* G2, const int64_t* IP, const int64_t* IF, bool fp,
-                           const double* weight, const NormOut& out)
+                           bool w32, const double* weight, const NormOut& out)
 {
     const HostTables& h = p->h;
     const int32_t nchunks = (int32_t)h.chunks.size();
@@ -360,8 +383,11 @@ void launch_normalize_sums(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t
     const int used = cpb * kNormSlices * std::min(nt, kNormLanes);
     const int nthr = std::min(kNormBlock, (used + 63) / 64 * 64);
     if (nchunks > 0) {
-        auto k = fp ? k_norm_partial<kNormFp> : weight ? k_norm_partial<kNormWeighted> : k_norm_partial<kNormInt>;
-        k<<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight, out.part);
+        auto k = fp       ? k_norm_partial<kNormFp, false>
+                 : weight ? (w32 ? k_norm_partial<kNormWeighted, true> : k_norm_partial<kNormWeighted, false>)
+                 : (w32 ? k_norm_partial<kNormInt, true> : k_norm_partial<kNormInt, false>);
+        k<<<nb, nthr, 0, s>>>(p->d_chunks, nchunks, taus, t0, t1, S, G2, IP, IF, weight, p->d_pix_total,
+                              (int64_t)h.P * S, out.part);
     }
     const int64_t ng = (int64_t)h.ngroups * nt;
     if (ng > 0)
@@ -379,8 +405,8 @@ NormOut plan_norm_out(xpcs_mt_plan* p) { return NormOut{p->d_part, p->d_g2s, p->
 void launch_normalize_with(xpcs_mt_plan* p, hipStream_t s, int32_t t0, int32_t t1, int32_t S,
                            const TauParam* taus)
 {
-    launch_normalize_sums(p, s, t0, t1, S, taus, p->d_G2, p->d_IP, p->d_IF, p->fsnorm, p->d_weight,
-                          plan_norm_out(p));
+    launch_normalize_sums(p, s, t0, t1, S, taus, p->d_G2, p->d_IP, p->d_IF, p->fsnorm, p->ipif32 && !p->fsnorm,
+                          p->d_weight, plan_norm_out(p));
 }
 
 void launch_from_ref_f32(xpcs_mt_plan* p, hipStream_t s, const float* G2, const float* IP, const float* IF,
@@ -404,7 +430,8 @@ void launch_ref_f32(xpcs_mt_plan* p, hipStream_t s, float* G2, float* IP, float*
     const int64_t n = (int64_t)h.P * h.T;
     const double inv_scale = 1.0 / ((double)h.avg * (double)p->nscale);
     auto k = p->fsnorm ? k_ref_f32<true> : k_ref_f32<false>;
-    k<<<(int)((n + 255) / 256), 256, 0, s>>>(p->d_G2, p->d_IP, p->d_IF, p->d_c2pix, h.P, p->d_taus, h.T,
+    k<<<(int)((n + 255) / 256), 256, 0, s>>>(p->d_G2, p->d_IP, p->d_IF, p->ipif32 && !p->fsnorm, p->d_pix_total,
+                                             p->d_c2pix, h.P, p->d_taus, h.T,
                                              p->d_weight, inv_scale, (int64_t)h.W * h.H, G2, IP, IF);
 }
 

@@ -95,7 +95,7 @@ extern "C" xpcs_status xpcs_mt_normalize_ref_f32_noalloc(xpcs_mt_plan* p, const 
     // no timer events here (they would be created on demand)
     const bool was = p->capturing;
     p->capturing = true;
-    launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, sums, sums + PT, sums + 2 * PT, true, nullptr, out);
+    launch_normalize_sums(p, s, 0, h.T, h.T, p->d_taus, sums, sums + PT, sums + 2 * PT, true, false, nullptr, out);
     p->capturing = was;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(XPCS_ERR_HIP, "normalize_ref_f32_noalloc: %s", hipGetErrorString(e));

@@ -209,7 +209,7 @@ class Tuning(C.Structure):
     """Launch shapes of a plan (include/xpcs_mt.h, 0 = automatic)."""
     _fields_ = [("tr_bucket_shift", C.c_int32), ("tr_chunk_frames", C.c_int32), ("reg_blocks", C.c_int32),
                 ("lds_blocks", C.c_int32), ("dense_levels", C.c_int32), ("deep_frames", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("sums64", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -48,8 +48,9 @@ def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev, tdeep=N
     which lists the others at 4 B each, and the register kernel); 65..2048 ->
     the k_mt_lds timer (k_mt_reg2 and the two LDS kernels); longer ->
     k_mt_long.  Per row kernel: its rows' events (8 B), compact-to-row map
-    (4 B) and row offsets (2 x 8 B) in, the int64 G2/IP/IF of every tau
-    (24 B per tau) and the pixel total (8 B) out.  With the deep-level
+    (4 B) and row offsets (2 x 8 B) in, the int64 G2 and the 32-bit IP/IF
+    of every tau (16 B per tau; pixels whose total passes 2^32 - 1, none
+    here, add the IP/IF high words) and the pixel total (8 B) out.  With the deep-level
     hand-over (tdeep = first tau of the hand-over level, dstride = u32 per
     series row) the non-empty rows up to 128 events write G2 of the taus
     before tdeep and their series row (4 B x dstride) instead, and k_mt_deep
@@ -65,7 +66,7 @@ def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev, tdeep=N
 
     def rows(k):
         n = int(px[list(k)].sum())
-        b = 28 * n + 8 * int(ev[list(k)].sum()) + 16 * n * T + 8 * n
+        b = 28 * n + 8 * int(ev[list(k)].sum()) + 8 * n * T + 8 * n
         for c in k:
             handed = deep and c in (1, 2)
             b += int(px[c]) * (8 * (tdeep if handed else T) + (4 * dstride if handed else 0))
@@ -80,7 +81,7 @@ def sparse_kernel_bytes(T, E_raw, nsel, nrows, n_mapped, tuning, px, ev, tdeep=N
         "k_mt_reg": rows((0, 1)),
         "k_mt_lds": rows((2, 3, 4)) + 4 * int(px[2:5].sum()),   # + their list entries in
         "k_mt_long": rows((5,)) + 4 * int(px[5]),
-        "k_norm": 3 * 8 * n_mapped * T,                         # int64 sums in (partials out are ~1e-3 of it)
+        "k_norm": 16 * n_mapped * T + 8 * n_mapped,             # G2 int64 + IP/IF u32 + totals in (partials ~1e-3)
     }
     if deep:
         out["k_mt_deep"] = P * (4 * dstride + 4 + 16) + 8 * (T - tdeep) * int(px[1] + px[2])
@@ -91,15 +92,15 @@ def dense_pass_bytes(P, F, avg, dpl, levels):
     """Algorithmic bytes of the dense streaming passes (raw pass, mean series pass).
 
     Raw pass: 2-byte values of the P mapped pixels for every frame in, IP and IF
-    for every tau and G2 for the taus of its levels out (8 bytes each), level-K
-    series out (4 bytes).  Series pass: its input series in, its G2 out, the
+    for every tau (4 bytes each) and G2 for the taus of its levels (8 bytes)
+    out, level-K series out (4 bytes).  Series pass: its input series in, its G2 out, the
     next series out."""
     K = 4 if dpl == 4 else 3
     nlev = int(levels.max()) + 1
     lf = [F >> L for L in range(nlev + 1)]
     t_of = np.bincount(levels, minlength=nlev)
     T = len(levels)
-    raw = 2 * P * F * avg + 16 * P * T + 8 * P * int(t_of[:K].sum()) + (4 * P * lf[K] if nlev > K else 0)
+    raw = 2 * P * F * avg + 8 * P * T + 8 * P * int(t_of[:K].sum()) + (4 * P * lf[K] if nlev > K else 0)
     series = []
     L0 = K
     while L0 < nlev:
@@ -652,7 +653,7 @@ def run_job(name, steps, warmup, autotune=True, e2e=True):
     levels, _ = plan.schedule()
     if dense:
         abytes = dense_pass_bytes(P, F, wl.get("avg", 1), dpl, np.asarray(levels))
-        abytes["k_norm"] = 3 * 8 * info.mapped_pixels * T
+        abytes["k_norm"] = 16 * info.mapped_pixels * T + 8 * info.mapped_pixels
         px = ev = None
     else:
         px, ev = plan.row_stats()

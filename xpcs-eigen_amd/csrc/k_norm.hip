@@ -41,16 +41,21 @@ enum NormMode { kNormInt = 0, kNormWeighted = 1, kNormFp = 2 };
 // in the same 8-byte buffers; they take the weighted (FP64) route with weight
 // 1 when the plan has no flat field.  W32: integer IP and IF in 32-bit words,
 // with high words at hw + idx for the pixels whose total passes 2^32 - 1.
+// The spread of x_p over the slice is kept as sums of x - K and (x - K)^2
+// about the slice's first value K (no division per pixel besides x_p's own;
+// the slice is short, so the shifted sums lose nothing a Welford update
+// would keep) and leaves as (n, mean, M2) for the ordered merges.
 template <int MODE, bool W32>
 __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, const int64_t* __restrict__ IP,
                                                 const int64_t* __restrict__ IF, const double* __restrict__ weight,
                                                 const uint64_t* __restrict__ pix_total, int64_t hw, int32_t S,
                                                 int i, double N, int pb, int pe, NormAcc& a)
 {
+    double K = 0.0, s1 = 0.0, s2 = 0.0;
     // kNormBatch pixels' loads in flight per thread
     for (int p0 = pb; p0 < pe; p0 += kNormBatch) {
         int64_t gg[kNormBatch], aa[kNormBatch], bb[kNormBatch];
-        uint64_t tt[kNormBatch];
+        uint64_t tmax = 0;
 #pragma unroll
         for (int u = 0; u < kNormBatch; u++) {
             const int pu = min(p0 + u, pe - 1);
@@ -59,23 +64,29 @@ __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, 
             if constexpr (W32) {
                 aa[u] = (int64_t)reinterpret_cast<const uint32_t*>(IP)[idx];
                 bb[u] = (int64_t)reinterpret_cast<const uint32_t*>(IF)[idx];
-                tt[u] = pix_total[pu];
+                tmax = max(tmax, pix_total[pu]);
             } else {
                 aa[u] = IP[idx];
                 bb[u] = IF[idx];
             }
         }
         if constexpr (W32) {
+            // pixels with high words (totals of 2^32 or more) are rare
+            if (tmax > 0xFFFFFFFFull) {
 #pragma unroll
-            for (int u = 0; u < kNormBatch; u++) {
-                if (tt[u] > 0xFFFFFFFFull) {
-                    const int64_t idx = (int64_t)min(p0 + u, pe - 1) * S + i;
-                    aa[u] = get_sum(IP, idx, true, hw);
-                    bb[u] = get_sum(IF, idx, true, hw);
+                for (int u = 0; u < kNormBatch; u++) {
+                    const int pu = min(p0 + u, pe - 1);
+                    const int64_t idx = (int64_t)pu * S + i;
+                    const int64_t hi = sum_hi(pix_total[pu], hw);
+                    aa[u] = get_sum(IP, idx, true, hi);
+                    bb[u] = get_sum(IF, idx, true, hi);
                 }
             }
         }
-        for (int u = 0; u < kNormBatch && p0 + u < pe; u++) {
+        const int nu = min(kNormBatch, pe - p0);
+#pragma unroll
+        for (int u = 0; u < kNormBatch; u++) {
+            if (u >= nu) continue;
             const int p = p0 + u;
             const int64_t g = gg[u];
             const int64_t ip = aa[u];
@@ -102,11 +113,17 @@ __device__ __forceinline__ void norm_accumulate(const int64_t* __restrict__ G2, 
                 x = N * (double)g / ((double)ip * (double)f);
             }
             if (x != x) x = 0.0;
-            a.n += 1.0;
-            const double d = x - a.mean;
-            a.mean += d / a.n;
-            a.m2 += d * (x - a.mean);
+            if (p == pb) K = x;
+            const double d = x - K;
+            s1 += d;
+            s2 += d * d;
         }
+    }
+    const double n = (double)max(pe - pb, 0);
+    a.n = n;
+    if (n > 0.0) {
+        a.mean = K + s1 / n;
+        a.m2 = fmax(s2 - s1 * (s1 / n), 0.0);
     }
 }
 

@@ -57,45 +57,18 @@ __device__ __forceinline__ void tr_decode(const uint8_t* base, uint32_t dlen, bo
     }
 }
 
-__device__ __forceinline__ bool tr_event(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j, EvFormat fm,
-                                         const int32_t* __restrict__ pix2c, uint32_t npix_det,
-                                         uint32_t* __restrict__ counters, int32_t& c, uint32_t& v)
-{
-    uint32_t idx;
-    tr_decode(base, dlen, al4, j, fm, idx, v);
-    if (idx >= npix_det) {
-        if (counters) atomicOr(&counters[1], 1u);
-        return false;
-    }
-    c = pix2c[idx];
-    return c >= 0 && v != 0;
-}
-
-// An event of the transpose: raster pixel r, count v; kept when the pixel is
-// mapped (bit r of `mapped`, 128 KB for a megapixel: cache resident, unlike a
-// per-pixel id table) and v != 0.  Rows are per raster pixel.
-__device__ __forceinline__ bool tr_event_raw(const uint8_t* base, uint32_t dlen, bool al4, uint32_t j, EvFormat fm,
-                                             const uint32_t* __restrict__ mapped, uint32_t npix_det,
-                                             uint32_t* __restrict__ counters, uint32_t& r, uint32_t& v)
-{
-    uint32_t idx;
-    tr_decode(base, dlen, al4, j, fm, idx, v);
-    if (idx >= npix_det) {
-        if (counters) atomicOr(&counters[1], 1u);
-        return false;
-    }
-    r = idx;
-    return v != 0 && ((mapped[idx >> 5] >> (idx & 31u)) & 1u);
-}
-
 // The events of a chunk's raw frames form one flat index space, visited in
 // groups of up to kTrGroupFrames frames: the prefix sums of the group's record
 // lengths and the records' offsets sit in LDS, and each thread keeps
 // kTrInFlight events in flight across frame boundaries (frames hold a few
 // thousand events, so a thread's frame index advances by at most a step or two
-// per event; 2 measured faster than 1, 4 or 8).
+// per event).  Measured on a megapixel job: the count pass 0.125 ms with 2
+// (3: 0.130, 4: 0.132); the bucket pass 0.318 ms with 1 and 0.413 ms with 2,
+// whose slot reservations and scattered writes do not gain from overlapped
+// reads.
 constexpr int kTrGroupFrames = 256;
 constexpr int kTrInFlight = 2;
+constexpr int kTrBucketInFlight = 1;
 
 struct ChunkGroup {
     uint32_t pre[kTrGroupFrames + 1];   // events before each frame of the group
@@ -124,18 +97,72 @@ __device__ uint32_t chunk_group_load(ChunkGroup& cg, const int64_t* __restrict__
     return (uint32_t)tot;
 }
 
-// Event q of the group (q < its event count); kf is the frame of the thread's
-// previous event at or below q.
-__device__ __forceinline__ bool chunk_event(const ChunkGroup& cg, const uint8_t* __restrict__ imm, uint32_t q,
-                                            int& kf, EvFormat fm, const uint32_t* __restrict__ mapped,
-                                            uint32_t npix_det, uint32_t* __restrict__ counters, uint32_t& r,
-                                            uint32_t& v)
+// Events q0 + u * step (u < N) of the group, each with its own frame pointer
+// kf[u]: raster pixel r, count v, and ok when the event is inside the group,
+// its pixel on the detector and mapped, and its count nonzero; `bad` records
+// a pixel index outside the detector.  The word format is fixed at compile
+// time and the N events' loads are issued together, after every address is
+// known and before any result is used (then the N mask words).  IMM indices
+// are only 2-byte aligned in general: the aligned 8 bytes around one are read
+// and joined (inside the record, whose values follow its indices; an aligned
+// word never straddles a page).  Needs total >= 1.
+template <int KIND, int N>
+__device__ __forceinline__ void chunk_events(const ChunkGroup& cg, const uint8_t* __restrict__ imm, uint32_t q0,
+                                             uint32_t step, uint32_t total, int (&kf)[N], const EvFormat& fm,
+                                             const uint32_t* __restrict__ mapped, uint32_t npix_det, bool& bad,
+                                             uint32_t (&r)[N], uint32_t (&v)[N], bool (&ok)[N])
 {
-    while (cg.pre[kf + 1] <= q) kf++;
-    const uint8_t* base = imm + cg.rec[kf];
-    const uint32_t dlen = cg.pre[kf + 1] - cg.pre[kf];
-    const bool al4 = (reinterpret_cast<uintptr_t>(base) & 3u) == 0;
-    return tr_event_raw(base, dlen, al4, q - cg.pre[kf], fm, mapped, npix_det, counters, r, v);
+    bool in[N];
+    const uint8_t* wp[N];   // the event's word (IMM: the aligned 8 bytes around its index)
+    const uint8_t* vp[N];   // IMM: its value
+    uint32_t sh[N];         // IMM: 2 when the index starts in the middle of a word
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const uint32_t q = q0 + (uint32_t)u * step;
+        in[u] = q < total;
+        const uint32_t qc = in[u] ? q : total - 1u;
+        while (cg.pre[kf[u] + 1] <= qc) kf[u]++;
+        const uint32_t b = cg.pre[kf[u]];
+        const uint32_t j = qc - b;
+        const uint8_t* base = imm + cg.rec[kf[u]];
+        if constexpr (KIND == kFmtImm) {
+            const uint8_t* at = base + 4 * (size_t)j;
+            sh[u] = (uint32_t)reinterpret_cast<uintptr_t>(at) & 2u;
+            wp[u] = at - sh[u];
+            vp[u] = base + 4 * (size_t)(cg.pre[kf[u] + 1] - b) + 2 * (size_t)j;
+        } else if constexpr (KIND == kFmtUfxc) {
+            wp[u] = base + 4 * (size_t)j;
+        } else {
+            wp[u] = base + 8 * (size_t)j;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        if constexpr (KIND == kFmtImm) {
+            const uint2 w = *reinterpret_cast<const uint2*>(wp[u]);
+            r[u] = sh[u] ? __builtin_amdgcn_alignbyte(w.y, w.x, 2) : w.x;
+            v[u] = *reinterpret_cast<const uint16_t*>(vp[u]);
+        } else if constexpr (KIND == kFmtUfxc) {
+            const uint32_t w = *reinterpret_cast<const uint32_t*>(wp[u]);
+            const uint32_t pix = w & 0x7fffu;
+            r[u] = (pix % fm.H) * fm.W + pix / fm.H;
+            v[u] = (w >> 15) & 3u;
+        } else {
+            const uint64_t w = *reinterpret_cast<const uint64_t*>(wp[u]);
+            const uint32_t pix = (uint32_t)(w >> 16) & 0xfffffu;
+            r[u] = (pix % fm.H) * fm.W + pix / fm.H;
+            v[u] = (uint32_t)w & 0x7ffu;
+        }
+    }
+    uint32_t m[N];
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const bool inb = r[u] < npix_det;
+        bad |= in[u] && !inb;
+        m[u] = mapped[(inb ? r[u] : 0u) >> 5];
+    }
+#pragma unroll
+    for (int u = 0; u < N; u++) ok[u] = in[u] && r[u] < npix_det && v[u] != 0u && ((m[u] >> (r[u] & 31u)) & 1u);
 }
 
 // SIDE: the side statistics' per-(output frame, static bin) sums of the
@@ -143,7 +170,7 @@ __device__ __forceinline__ bool chunk_event(const ChunkGroup& cg, const uint8_t*
 // static bins < 256 from the 1-byte per-pixel table): an LDS table
 // [output frames of the chunk][S], runs of equal (frame, bin) summed across
 // the wave first, flushed into fs with one global atomic per nonzero entry.
-template <bool SIDE>
+template <bool SIDE, int KIND>
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
            const uint32_t* __restrict__ sel_dlen, int64_t nsel, int64_t cf, EvFormat fm,
@@ -165,6 +192,7 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         f_span = sel_frame[k1 - 1] - f_lo + 1;
         for (int i = threadIdx.x; i < f_span * S; i += blockDim.x) s_fs[i] = 0.0;
     }
+    bool bad = false;
     for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, SIDE ? sel_frame : nullptr, g0, ng);
@@ -174,12 +202,8 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
             for (uint32_t qb = 0; qb < total; qb += kTrInFlight * bd) {
                 uint32_t c[kTrInFlight], v[kTrInFlight];
                 bool ok[kTrInFlight];
-#pragma unroll
-                for (int u = 0; u < kTrInFlight; u++) {
-                    const uint32_t q = qb + u * bd + threadIdx.x;
-                    ok[u] = q < total &&
-                            chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, counters, c[u], v[u]);
-                }
+                chunk_events<KIND, kTrInFlight>(cg, imm, qb + threadIdx.x, bd, total, kf, fm, mapped, npix_det, bad,
+                                                c, v, ok);
 #pragma unroll
                 for (int u = 0; u < kTrInFlight; u++) {
                     if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
@@ -200,17 +224,14 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
         for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kTrInFlight * bd) {
             uint32_t c[kTrInFlight], v[kTrInFlight];
             bool ok[kTrInFlight];
-#pragma unroll
-            for (int u = 0; u < kTrInFlight; u++) {
-                const uint32_t q = q0 + u * bd;
-                ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, counters, c[u], v[u]);
-            }
+            chunk_events<KIND, kTrInFlight>(cg, imm, q0, bd, total, kf, fm, mapped, npix_det, bad, c, v, ok);
 #pragma unroll
             for (int u = 0; u < kTrInFlight; u++)
                 if (ok[u]) atomicAdd(&s_h[c[u] >> bshift], 1u);
         }
         __syncthreads();   // the group tables are reloaded
     }
+    if (bad && counters) atomicOr(&counters[1], 1u);
     for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t)blockIdx.x * nb + b] = s_h[b];
     if constexpr (SIDE) {
         for (int i = threadIdx.x; i < f_span * S; i += blockDim.x)
@@ -218,6 +239,7 @@ k_tr_count(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
     }
 }
 
+template <int KIND>
 __global__ void __launch_bounds__(kTrChunkBlock)
 k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off,
             const uint32_t* __restrict__ sel_dlen, const int32_t* __restrict__ sel_frame, int64_t nsel,
@@ -238,21 +260,19 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
     for (int64_t g0 = k0; g0 < k1; g0 += kTrGroupFrames) {
         const int ng = (int)min<int64_t>(kTrGroupFrames, k1 - g0);
         const uint32_t total = chunk_group_load(cg, sel_off, sel_dlen, sel_frame, g0, ng);
-        int kf[kTrInFlight] = {};
+        int kf[kTrBucketInFlight] = {};
         // uniform trip count: every lane takes part in the run detection
-        for (uint32_t qb = 0; qb < total; qb += kTrInFlight * bd) {
-            uint32_t c[kTrInFlight], v[kTrInFlight], slot[kTrInFlight];
-            bool ok[kTrInFlight];
-#pragma unroll
-            for (int u = 0; u < kTrInFlight; u++) {
-                const uint32_t q = qb + u * bd + threadIdx.x;
-                ok[u] = q < total && chunk_event(cg, imm, q, kf[u], fm, mapped, npix_det, nullptr, c[u], v[u]);
-            }
+        for (uint32_t qb = 0; qb < total; qb += kTrBucketInFlight * bd) {
+            uint32_t c[kTrBucketInFlight], v[kTrBucketInFlight], slot[kTrBucketInFlight];
+            bool ok[kTrBucketInFlight];
+            bool bad = false;   // reported by the count pass
+            chunk_events<KIND, kTrBucketInFlight>(cg, imm, qb + threadIdx.x, bd, total, kf, fm, mapped, npix_det, bad, c,
+                                            v, ok);
             // one LDS cursor atomic per run of equal buckets: the run's last
             // lane takes the run's slots and hands the base to the run's first
             // lane, from which every lane of the run reads it (slots in lane order)
 #pragma unroll
-            for (int u = 0; u < kTrInFlight; u++) {
+            for (int u = 0; u < kTrBucketInFlight; u++) {
                 const uint32_t key = ok[u] ? c[u] >> bshift : 0xFFFFFFFFu;
                 bool tail;
                 const uint32_t lane = (uint32_t)lane_id();
@@ -266,7 +286,7 @@ k_tr_bucket(const uint8_t* __restrict__ imm, const int64_t* __restrict__ sel_off
                 slot[u] = pull_from_lane((int)start, at_head) + (lane - start);
             }
 #pragma unroll
-            for (int u = 0; u < kTrInFlight; u++)
+            for (int u = 0; u < kTrBucketInFlight; u++)
                 if (ok[u])
                     b1[s_base[c[u] >> bshift] + slot[u]] = ((uint64_t)(uint32_t)cg.frame[kf[u]] << 32) |
                                                            ((uint64_t)(c[u] & lmask) << 16) | (uint64_t)v[u];
@@ -800,26 +820,40 @@ void launch_transpose(xpcs_mt_plan* p, hipStream_t s, bool with_side)
     const TransposeShape& t = p->tshape;
     const uint32_t npd = (uint32_t)((int64_t)p->h.W * p->h.H);
     timer_begin(p, s, "k_tr_count");
+    const EvFormat fm = ev_format(p);
+    // the kernels are instantiated per word format (no decode branches)
+    auto by_format = [&](auto launch) {
+        if (fm.kind == kFmtUfxc) launch(std::integral_constant<int, kFmtUfxc>{});
+        else if (fm.kind == kFmtRigaku) launch(std::integral_constant<int, kFmtRigaku>{});
+        else launch(std::integral_constant<int, kFmtImm>{});
+    };
     if (with_side) {
         // the chunk's output frames never exceed its cf raw frames
         const size_t lds = (size_t)((t.nb + 1) & ~1) * sizeof(uint32_t) + (size_t)t.cf * p->h.ns * sizeof(double);
-        k_tr_count<true><<<t.nblk, kTrChunkBlock, lds, s>>>(
-            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
-            t.nblk, p->d_hist, p->d_counters, p->d_sel_frame, static_cast<const uint8_t*>(p->d_sbin_pix), p->h.ns,
-            p->d_fs);
+        by_format([&](auto kind) {
+            k_tr_count<true, decltype(kind)::value><<<t.nblk, kTrChunkBlock, lds, s>>>(
+                p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, fm, p->d_mapped, npd, t.bshift, t.nb, t.nblk,
+                p->d_hist, p->d_counters, p->d_sel_frame, static_cast<const uint8_t*>(p->d_sbin_pix), p->h.ns,
+                p->d_fs);
+        });
     } else {
-        k_tr_count<false><<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
-            p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, ev_format(p), p->d_mapped, npd, t.bshift, t.nb,
-            t.nblk, p->d_hist, p->d_counters, nullptr, nullptr, 0, nullptr);
+        by_format([&](auto kind) {
+            k_tr_count<false, decltype(kind)::value><<<t.nblk, kTrChunkBlock, (size_t)t.nb * sizeof(uint32_t), s>>>(
+                p->d_imm, p->d_sel_off, p->d_sel_dlen, p->nsel, t.cf, fm, p->d_mapped, npd, t.bshift, t.nb, t.nblk,
+                p->d_hist, p->d_counters, nullptr, nullptr, 0, nullptr);
+        });
     }
     timer_end(p, s);
     timer_begin(p, s, "k_tr_scan");
     bucket_scan(p->d_hist, t.nb, t.nblk, p->d_hoff, p->d_bsum, s);
     timer_end(p, s);
     timer_begin(p, s, "k_tr_bucket");
-    k_tr_bucket<<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(
-        p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, ev_format(p), p->d_mapped, npd,
-        t.bshift, t.nb, t.nblk, p->d_hoff, p->d_b1);
+    by_format([&](auto kind) {
+        k_tr_bucket<decltype(kind)::value>
+            <<<t.nblk, kTrChunkBlock, (size_t)t.nb * (sizeof(uint32_t) + sizeof(unsigned long long)), s>>>(
+                p->d_imm, p->d_sel_off, p->d_sel_dlen, p->d_sel_frame, p->nsel, t.cf, fm, p->d_mapped, npd, t.bshift,
+                t.nb, t.nblk, p->d_hoff, p->d_b1);
+    });
     timer_end(p, s);
     timer_begin(p, s, "k_tr_rows");
     k_tr_rows<<<t.nb < 256 ? t.nb : 256, kTrRowsBlock, (size_t)sizeof(uint32_t) << t.bshift, s>>>(

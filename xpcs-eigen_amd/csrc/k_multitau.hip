@@ -35,7 +35,10 @@ namespace xpcs {
 // In-place compaction of one level: keys are shifted by `shift`, entries with
 // key >= lf are dropped, runs of equal keys are summed.  Returns the new count.
 // Reads of a chunk complete before its writes, and writes only go to slots at
-// or below the slot being read, so the pass is safe in place.
+// or below the slot being read, so the pass is safe in place.  Each lane reads
+// its entry and the next one together (runs of two, the rule after level 0,
+// need no further read; longer runs, repeated frames at level 0, continue
+// from there), and the neighbour keys move by DPP, not through LDS.
 template <typename V>
 __device__ int level_compact(uint32_t* key, V* val, int n, int shift, uint32_t lf)
 {
@@ -45,16 +48,20 @@ __device__ int level_compact(uint32_t* key, V* val, int n, int shift, uint32_t l
     for (int b0 = 0; b0 < n; b0 += kWave) {
         const int i = b0 + lane;
         const bool valid = i < n;
-        const uint32_t k = valid ? (key[i] >> shift) : 0xFFFFFFFFu;
-        uint32_t kp = __shfl_up(k, 1, 64);
-        if (lane == 0) kp = prevk;
+        const bool has_next = i + 1 < n;
+        const uint32_t kr = valid ? key[i] : 0u;
+        const V vi = valid ? val[i] : V(0);
+        const uint32_t knr = has_next ? key[i + 1] : 0u;
+        const V vn = has_next ? val[i + 1] : V(0);
+        const uint32_t k = valid ? (kr >> shift) : 0xFFFFFFFFu;
+        const uint32_t kp = prev_lane_u32(k, prevk);
         const bool head = valid && k < lf && k != kp;
-        V sum = V(0);
-        if (head) {
-            sum = val[i];
-            for (int j = i + 1; j < n && (key[j] >> shift) == k; j++) sum += val[j];
+        V sum = vi;
+        if (head && has_next && (knr >> shift) == k) {
+            sum += vn;
+            for (int j = i + 2; j < n && (key[j] >> shift) == k; j++) sum += val[j];
         }
-        prevk = __shfl(k, kWave - 1, 64);
+        prevk = read_lane_u32(k, kWave - 1);
         const uint64_t mask = __ballot(head);
         const int pos = carry + mask_rank(mask);
         const bool stop = __ballot(valid && k >= lf) != 0;
@@ -75,6 +82,7 @@ __device__ int level_compact(uint32_t* key, V* val, int n, int shift, uint32_t l
 // normalize_by_framesum, on frame-weighted values (V = double, FP64 sums).
 constexpr int kRegMaxTaus = 128;
 constexpr int kMidSmall = 128;   // LDS rows up to this length run with 128-entry wave slices
+constexpr int kLagStep = 4;      // lag-search candidates read per step in the LDS kernel
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 // Lag accumulators are kept in kAccCopies interleaved copies per tau; lane l
 // adds into copy l % kAccCopies, so the lanes of a wave that hit the same lag
@@ -1170,6 +1178,77 @@ __device__ void lds_dense_lags(const uint32_t* x, int lf, const LevelParam& lp, 
 // and at 32, so an entry moves up to 32 places per round.  True when the row
 // ends up sorted within kBlockRounds rounds.
 constexpr int kBlockRounds = 4;
+
+// A row held in registers, element c * 64 + lane in kk[c] / vv[c] (rc
+// registers in use, kNoKey past the row): up to `rounds` odd-even
+// transposition rounds across the whole row, the pairs that straddle two
+// registers exchanged through lane reads.  Rows leave the transpose sorted
+// except among a pixel's events of one transpose chunk, which stand next to
+// each other, so a few rounds usually finish the row.  Returns whether it is
+// sorted.
+template <int R, typename V>
+__device__ __forceinline__ bool regs_oddeven(uint32_t (&kk)[R], V (&vv)[R], int rc, int rounds)
+{
+    const int lane = lane_id();
+    auto unsorted = [&]() {
+        bool bad = false;
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            if (c < rc) {
+                const uint32_t first = c + 1 < rc ? read_lane_u32(kk[c + 1], 0) : kNoKey;
+                bad |= next_lane_u32(kk[c], first) < kk[c];
+            }
+        }
+        return __ballot(bad) != 0ull;
+    };
+    if (!unsorted()) return true;
+    for (int r = 0; r < rounds; r++) {
+        // even phase: (2j, 2j+1), inside one register
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            if (c < rc) {
+                const uint32_t pk = xor_lane_u32<1>(kk[c]);
+                const V pv = xor_lane_v<1>(vv[c]);
+                if ((lane & 1) == 0 ? (pk < kk[c]) : (pk > kk[c])) {
+                    kk[c] = pk;
+                    vv[c] = pv;
+                }
+            }
+        }
+        // odd phase: (2j+1, 2j+2); lane 63 of register c pairs with lane 0 of
+        // register c + 1 (neighbours taken before any of them moves)
+        uint32_t lastk[R], firstk[R];
+        V lastv[R], firstv[R];
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            lastk[c] = read_lane_u32(kk[c], kWave - 1);
+            lastv[c] = read_lane_v(vv[c], kWave - 1);
+            firstk[c] = read_lane_u32(kk[c], 0);
+            firstv[c] = read_lane_v(vv[c], 0);
+        }
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            if (c < rc) {
+                const uint32_t nk = next_lane_u32(kk[c], c + 1 < rc ? firstk[c + 1] : kNoKey);
+                const V nv = next_lane_v(vv[c], c + 1 < rc ? firstv[c + 1] : V(0));
+                const uint32_t bk = prev_lane_u32(kk[c], c > 0 ? lastk[c - 1] : 0u);
+                const V bv = prev_lane_v(vv[c], c > 0 ? lastv[c - 1] : V(0));
+                if ((lane & 1) != 0) {
+                    if (nk < kk[c]) {
+                        kk[c] = nk;
+                        vv[c] = nv;
+                    }
+                } else if ((c > 0 || lane > 0) && bk > kk[c]) {
+                    kk[c] = bk;
+                    vv[c] = bv;
+                }
+            }
+        }
+        if (!unsorted()) return true;
+    }
+    return false;
+}
+constexpr int kRowRounds = 4;   // odd-even rounds on a loaded row before the LDS block sorts
 template <typename V>
 __device__ bool lds_block_sort(uint32_t* key, V* val, int m)
 {
@@ -1263,21 +1342,30 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 const int i = c * kWave + lane;
                 e[c] = i < n ? ev[base + i] : ~0ull;
             }
+            uint32_t kk[R];
+            V vv[R];
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                const int i = c * kWave + lane;
+                kk[c] = i < n ? (uint32_t)(e[c] >> 32) : kNoKey;
+                const uint32_t cnt = i < n ? (uint32_t)e[c] : 0u;
+                counts += cnt;
+                vv[c] = i < n ? event_value<V>(cnt, kk[c], fw, F) : V(0);
+            }
+            // locally permuted rows: odd-even rounds in registers first, then
+            // (what is still out of order) register block sorts in LDS and the
+            // bitonic network over the whole row
+            const bool reg_ok = regs_oddeven<R>(kk, vv, (m + kWave - 1) / kWave, kRowRounds);
 #pragma unroll
             for (int c = 0; c < R; c++) {
                 const int i = c * kWave + lane;
                 if (i < m) {
-                    const uint32_t k = i < n ? (uint32_t)(e[c] >> 32) : 0xFFFFFFFFu;
-                    const uint32_t cnt = i < n ? (uint32_t)e[c] : 0u;
-                    counts += cnt;
-                    key[i] = k;
-                    val[i] = event_value<V>(cnt, k, fw, F);
+                    key[i] = kk[c];
+                    val[i] = vv[c];
                 }
             }
             wave_sync();
-            // locally permuted rows: register block sorts; the bitonic
-            // network over the whole row only for what is still out of order
-            if (lds_block_sort(key, val, m)) m_sorted = true;
+            if (reg_ok || lds_block_sort(key, val, m)) m_sorted = true;
             if (!m_sorted)
             for (int kk = 2; kk <= m; kk <<= 1) {
                 for (int j = kk >> 1; j > 0; j >>= 1) {
@@ -1382,15 +1470,30 @@ k_mt_sparse(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
                 if (!hand) break;
                 continue;   // no more pairs; bin what is left down to the hand-over level
             }
+            // each event's partners up to hi frames later, kLagStep candidates
+            // at a time: their keys and values are read together (one LDS
+            // round trip per step instead of one per candidate); keys are
+            // sorted and distinct, so a step whose last key is past hi ends it
             const uint32_t lo = (uint32_t)lp.lo, hi = (uint32_t)lp.hi;
             for (int i = lane; i < nl; i += kWave) {
                 const uint32_t ki = key[i];
                 const V vi = val[i];
-                for (int j = i + 1; j < nl; j++) {
-                    const uint32_t d = key[j] - ki;
-                    if (d > hi) break;
-                    if (d >= lo)
-                        atomicAdd(&aux[(lp.first + (int)(d - lo)) * nc + (lane & (nc - 1))], lag_prod(vi, val[j]));
+                for (int j0 = i + 1; j0 < nl; j0 += kLagStep) {
+                    uint32_t kj[kLagStep];
+                    V vj[kLagStep];
+#pragma unroll
+                    for (int u = 0; u < kLagStep; u++) {
+                        const int j = min(j0 + u, nl - 1);
+                        kj[u] = key[j];
+                        vj[u] = val[j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLagStep; u++) {
+                        const uint32_t d = kj[u] - ki;
+                        if (j0 + u < nl && d >= lo && d <= hi)
+                            atomicAdd(&aux[(lp.first + (int)(d - lo)) * nc + (lane & (nc - 1))], lag_prod(vi, vj[u]));
+                    }
+                    if (kj[kLagStep - 1] - ki > hi) break;
                 }
             }
             wave_sync();

@@ -929,22 +929,31 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
     }
 }
 
-// Levels ldeep .. nlev-1 of one lane's series x (in place), 64-bit lag sums.
+// Levels ldeep .. nlev-1 of one pixel's series x (in place), 64-bit lag sums,
+// shared by two lanes: the lane of half h sums lags dpl + 1 + h dpl/2 + e
+// (e < dpl/2).  Both lanes index one register window of the last 2 dpl
+// values with the same constants (modulo 2 dpl over an unrolled block); the
+// window of half 1 is rotated by dpl/2 (each value stored dpl/2 slots further
+// on), so the same slots give it values dpl/2 frames older.  Only half 0
+// writes the pair sums x[t-1] + x[t] at odd t to x[t/2], the next level's
+// series; both lanes are at the same t, so every overwritten value has been
+// read by both.
 template <int DPL>
 __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __restrict__ levels, int ldeep, int nlev,
-                                            unsigned long long (&res)[kDeepMaxLevels][DPL])
+                                            bool h1, unsigned long long (&res)[kDeepMaxLevels][DPL / 2])
 {
-    constexpr int W = 2 * DPL;
+    constexpr int W = 2 * DPL, NE = DPL / 2;
 #pragma unroll
     for (int j = 0; j < kDeepMaxLevels; j++) {
-        unsigned long long acc[DPL];
+        unsigned long long acc[NE];
 #pragma unroll
-        for (int e = 0; e < DPL; e++) acc[e] = 0ull;
+        for (int e = 0; e < NE; e++) acc[e] = 0ull;
         if (ldeep + j < nlev) {
             const int lf = levels[ldeep + j].lf;
             uint32_t buf[W];
 #pragma unroll
             for (int u = 0; u < W; u++) buf[u] = 0u;
+            uint32_t last = 0u;   // x[t-1] for the pair sums
             for (int t0 = 0; t0 < lf; t0 += W) {
 #pragma unroll
                 for (int u = 0; u < W; u++) {
@@ -952,48 +961,53 @@ __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __res
                     if (t < lf) {
                         const uint32_t val = x[t];
 #pragma unroll
-                        for (int e = 0; e < DPL; e++) {
+                        for (int e = 0; e < NE; e++)
                             acc[e] += (unsigned long long)buf[(u + W - (DPL + 1 + e)) % W] * val;
-                        }
-                        if (u & 1) x[t >> 1] = buf[u - 1] + val;
-                        buf[u] = val;
+                        if ((u & 1) && !h1) x[t >> 1] = last + val;
+                        last = val;
+                        const int ur = (u + NE) % W;   // a constant once the block is unrolled
+                        buf[u] = h1 ? buf[u] : val;
+                        buf[ur] = h1 ? val : buf[ur];
                     }
                 }
             }
         }
 #pragma unroll
-        for (int e = 0; e < DPL; e++) res[j][e] = acc[e];
+        for (int e = 0; e < NE; e++) res[j][e] = acc[e];
     }
 }
 
 // The deep levels of the rows the register kernels handled (1..kMidSmall
 // events) and of the LDS kernels' rows (up to kSparseCapLong events) whose
-// total fits 32 bits, one lane per pixel: levels ldeep .. nlev-1, whose series are
-// short, cost a wave per row one lag search each but only a few instructions
-// per value here.  A wave takes 64 consecutive pixels: their series rows
-// (contiguous in HBM) go to LDS, then each lane streams its own series.  A
-// level value x[t] adds x[t-d] x[t] to the lag-d sum for d = dpl+1 .. 2 dpl
-// (the last 2 dpl values in registers, indexed modulo 2 dpl over an unrolled
-// block, so no value moves) and, at odd t, x[t-1] + x[t] goes to index t/2:
-// the next level's series, in place.  The deep taus' G2 leave through the
-// same LDS area, one contiguous run per pixel.  Exact: u32 values, 64-bit
-// integer sums (32-bit sums for waves of small totals measured no faster).
+// total fits 32 bits, two lanes per pixel: levels ldeep .. nlev-1, whose
+// series are short, cost a wave per row one lag search each but only a few
+// instructions per value here.  A wave takes 32 consecutive pixels: their
+// series rows (contiguous in HBM, all 16-byte loads in flight) go to LDS (odd
+// pitch 97, 12 KB per wave, so four waves share a SIMD); each lane pair
+// streams its pixel's series once per level, each lane half the level's lags
+// (see deep_levels), and the deep taus' G2 leave through the same LDS area,
+// one contiguous run per pixel.  Exact: u32 values, 64-bit integer sums
+// (32-bit sums for waves of small totals measured no faster).
 template <int DPL>
 __global__ void __launch_bounds__(kWave)
 k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __restrict__ off,
           const int32_t* __restrict__ rowmap, int32_t P, int32_t T, const LevelParam* __restrict__ levels,
           int32_t ldeep, int32_t nlev, const uint64_t* __restrict__ pix_total, int64_t* __restrict__ G2)
 {
-    constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the lanes' reads of one index hit distinct banks
-    __shared__ uint32_t s_x[kWave * SP];
+    constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the pixels' reads of one index hit distinct banks
+    constexpr int NPW = kWave / 2;           // pixels per wave
+    constexpr int NE = DPL / 2;              // lags per lane and level
+    __shared__ uint32_t s_x[NPW * SP];
     const int lane = lane_id();
-    const int64_t p0 = (int64_t)blockIdx.x * kWave;
-    const int np = (int)min<int64_t>(kWave, (int64_t)P - p0);
+    const int q = lane >> 1;                 // the lane pair's pixel
+    const bool h1 = (lane & 1) != 0;
+    const int64_t p0 = (int64_t)blockIdx.x * NPW;
+    const int np = (int)min<int64_t>(NPW, (int64_t)P - p0);
     {
         // the wave's np rows are one contiguous run of np * stride u32 (stride
         // % 4 == 0): every 16-byte load in flight at once, then into LDS
         typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-        constexpr int kV4 = kDeepMaxFrames / 4;        // 16-byte loads per lane at most
+        constexpr int kV4 = NPW * (kDeepMaxFrames / 4) / kWave;   // 16-byte loads per lane at most
         const int s4 = stride >> 2;
         const int n4 = np * s4;
         const v4u* src = reinterpret_cast<const v4u*>(series + p0 * (int64_t)stride);
@@ -1017,8 +1031,8 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         }
     }
     bool mine = false;
-    if (lane < np) {
-        const int64_t p = p0 + lane;
+    if (q < np) {
+        const int64_t p = p0 + q;
         const int64_t row = rowmap ? (int64_t)rowmap[p] : p;
         const int64_t n = off[row + 1] - off[row];
         // the rows handed over: those of the register kernels (totals below
@@ -1026,9 +1040,9 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         mine = n >= 1 && n <= kSparseCapLong && pix_total[p] <= 0xFFFFFFFFull;
     }
     wave_sync();
-    uint32_t* x = s_x + lane * SP;
-    unsigned long long res[kDeepMaxLevels][DPL];
-    deep_levels<DPL>(x, levels, ldeep, nlev, res);
+    uint32_t* x = s_x + min(q, NPW - 1) * SP;
+    unsigned long long res[kDeepMaxLevels][NE];
+    deep_levels<DPL>(x, levels, ldeep, nlev, h1, res);
     wave_sync();
     // results to LDS (two u32 per value, odd pitch), then one run per pixel
     const int tdeep = levels[ldeep].first;
@@ -1039,18 +1053,19 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         if (ldeep + j >= nlev) continue;
         const LevelParam lp = levels[ldeep + j];
 #pragma unroll
-        for (int e = 0; e < DPL; e++) {
-            if (e < lp.count) {
-                const int i = lp.first + e - tdeep;
-                s_x[lane * gp + 2 * i] = (uint32_t)res[j][e];
-                s_x[lane * gp + 2 * i + 1] = (uint32_t)(res[j][e] >> 32);
+        for (int e = 0; e < NE; e++) {
+            const int ee = e + (h1 ? NE : 0);
+            if (ee < lp.count && q < np) {
+                const int i = lp.first + ee - tdeep;
+                s_x[q * gp + 2 * i] = (uint32_t)res[j][e];
+                s_x[q * gp + 2 * i + 1] = (uint32_t)(res[j][e] >> 32);
             }
         }
     }
     wave_sync();
-    const uint64_t mine_mask = __ballot(mine);
+    const uint64_t mine_mask = __ballot(mine && !h1);
     for (int r = 0; r < np; r++) {
-        if (((mine_mask >> r) & 1ull) == 0ull) continue;
+        if (((mine_mask >> (2 * r)) & 1ull) == 0ull) continue;
         int64_t* dst = G2 + (p0 + r) * (int64_t)T + tdeep;
         for (int i = lane; i < nd; i += kWave)
             dst[i] = (int64_t)(((uint64_t)s_x[r * gp + 2 * i + 1] << 32) | s_x[r * gp + 2 * i]);
@@ -1770,7 +1785,7 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     timer_end(p, s);
     if (ldeep < nlev && h.P > 0) {
         timer_begin(p, s, "k_mt_deep");
-        const unsigned nb = (unsigned)((h.P + kWave - 1) / kWave);
+        const unsigned nb = (unsigned)((h.P + kWave / 2 - 1) / (kWave / 2));   // 32 pixels per wave
         if (h.dpl == 4)
             k_mt_deep<4><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
                                               ldeep, nlev, p->d_pix_total, p->d_G2);

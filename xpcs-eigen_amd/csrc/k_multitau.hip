@@ -639,8 +639,9 @@ __device__ __forceinline__ V event_value(uint32_t c, uint32_t f, const double* _
 // with at most kDeepMaxFrames binned frames, written by the register kernels
 // as one dense row of `stride` u32 per pixel ([P][stride]).
 constexpr int kDeepMaxFrames = 96;
-constexpr int kDeepMaxTaus = 48;     // deep G2 staged through the series area: 64 x 48 x 8 B fits
+constexpr int kDeepMaxTaus = 48;     // deep G2 staged through the series area: 48 x 8 B fit a pixel's 97 words
 constexpr int kDeepMaxLevels = 6;
+constexpr int kDeepLanes = 4;        // lanes per pixel in k_mt_deep (divides 4, the smallest dpl)
 
 // One row's level series (keys < stride, distinct) as a dense row at dst,
 // staged through the wave's LDS words ser[0, stride).
@@ -930,19 +931,19 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
 }
 
 // Levels ldeep .. nlev-1 of one pixel's series x (in place), 64-bit lag sums,
-// shared by two lanes: the lane of half h sums lags dpl + 1 + h dpl/2 + e
-// (e < dpl/2).  Both lanes index one register window of the last 2 dpl
-// values with the same constants (modulo 2 dpl over an unrolled block); the
-// window of half 1 is rotated by dpl/2 (each value stored dpl/2 slots further
-// on), so the same slots give it values dpl/2 frames older.  Only half 0
+// shared by LPP lanes: lane h of the pixel sums lags dpl + 1 + h NE + e
+// (e < NE = dpl / LPP).  Every lane indexes one register window of the last
+// 2 dpl values with the same constants (modulo 2 dpl over an unrolled block);
+// lane h's window is rotated by h NE (each value stored h NE slots further
+// on), so the same slots give it values h NE frames older.  Only lane 0
 // writes the pair sums x[t-1] + x[t] at odd t to x[t/2], the next level's
-// series; both lanes are at the same t, so every overwritten value has been
-// read by both.
-template <int DPL>
+// series; the lanes are at the same t, so every overwritten value has been
+// read by all of them.
+template <int DPL, int LPP>
 __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __restrict__ levels, int ldeep, int nlev,
-                                            bool h1, unsigned long long (&res)[kDeepMaxLevels][DPL / 2])
+                                            int h, unsigned long long (&res)[kDeepMaxLevels][DPL / LPP])
 {
-    constexpr int W = 2 * DPL, NE = DPL / 2;
+    constexpr int W = 2 * DPL, NE = DPL / LPP;
 #pragma unroll
     for (int j = 0; j < kDeepMaxLevels; j++) {
         unsigned long long acc[NE];
@@ -963,11 +964,13 @@ __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __res
 #pragma unroll
                         for (int e = 0; e < NE; e++)
                             acc[e] += (unsigned long long)buf[(u + W - (DPL + 1 + e)) % W] * val;
-                        if ((u & 1) && !h1) x[t >> 1] = last + val;
+                        if ((u & 1) && h == 0) x[t >> 1] = last + val;
                         last = val;
-                        const int ur = (u + NE) % W;   // a constant once the block is unrolled
-                        buf[u] = h1 ? buf[u] : val;
-                        buf[ur] = h1 ? val : buf[ur];
+#pragma unroll
+                        for (int g = 0; g < LPP; g++) {
+                            const int ur = (u + g * NE) % W;   // constants once the block is unrolled
+                            buf[ur] = h == g ? val : buf[ur];
+                        }
                     }
                 }
             }
@@ -979,12 +982,14 @@ __device__ __forceinline__ void deep_levels(uint32_t* x, const LevelParam* __res
 
 // The deep levels of the rows the register kernels handled (1..kMidSmall
 // events) and of the LDS kernels' rows (up to kSparseCapLong events) whose
-// total fits 32 bits, two lanes per pixel: levels ldeep .. nlev-1, whose
-// series are short, cost a wave per row one lag search each but only a few
-// instructions per value here.  A wave takes 32 consecutive pixels: their
-// series rows (contiguous in HBM, all 16-byte loads in flight) go to LDS (odd
-// pitch 97, 12 KB per wave, so four waves share a SIMD); each lane pair
-// streams its pixel's series once per level, each lane half the level's lags
+// total fits 32 bits, kDeepLanes lanes per pixel: levels ldeep .. nlev-1,
+// whose series are short, cost a wave per row one lag search each but only a
+// few instructions per value here.  A wave takes 64 / kDeepLanes consecutive
+// pixels: their series rows (contiguous in HBM, all 16-byte loads in flight)
+// go to LDS (odd pitch 97; with four lanes per pixel 6 KB per wave, so seven
+// waves share a SIMD, against 1.5 with one lane per pixel: 0.194 -> 0.163 ms
+// with two lanes, 0.150 ms with four on the 1 Mpx job); the lanes of a
+// pixel stream its series once per level, each a share of the level's lags
 // (see deep_levels), and the deep taus' G2 leave through the same LDS area,
 // one contiguous run per pixel.  Exact: u32 values, 64-bit integer sums
 // (32-bit sums for waves of small totals measured no faster).
@@ -995,12 +1000,13 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
           int32_t ldeep, int32_t nlev, const uint64_t* __restrict__ pix_total, int64_t* __restrict__ G2)
 {
     constexpr int SP = kDeepMaxFrames + 1;   // odd pitch: the pixels' reads of one index hit distinct banks
-    constexpr int NPW = kWave / 2;           // pixels per wave
-    constexpr int NE = DPL / 2;              // lags per lane and level
+    constexpr int LPP = kDeepLanes;
+    constexpr int NPW = kWave / LPP;         // pixels per wave
+    constexpr int NE = DPL / LPP;            // lags per lane and level
     __shared__ uint32_t s_x[NPW * SP];
     const int lane = lane_id();
-    const int q = lane >> 1;                 // the lane pair's pixel
-    const bool h1 = (lane & 1) != 0;
+    const int q = lane / LPP;                // the lanes' pixel
+    const int h = lane % LPP;
     const int64_t p0 = (int64_t)blockIdx.x * NPW;
     const int np = (int)min<int64_t>(NPW, (int64_t)P - p0);
     {
@@ -1042,7 +1048,7 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
     wave_sync();
     uint32_t* x = s_x + min(q, NPW - 1) * SP;
     unsigned long long res[kDeepMaxLevels][NE];
-    deep_levels<DPL>(x, levels, ldeep, nlev, h1, res);
+    deep_levels<DPL, LPP>(x, levels, ldeep, nlev, h, res);
     wave_sync();
     // results to LDS (two u32 per value, odd pitch), then one run per pixel
     const int tdeep = levels[ldeep].first;
@@ -1054,7 +1060,7 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         const LevelParam lp = levels[ldeep + j];
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-            const int ee = e + (h1 ? NE : 0);
+            const int ee = e + h * NE;
             if (ee < lp.count && q < np) {
                 const int i = lp.first + ee - tdeep;
                 s_x[q * gp + 2 * i] = (uint32_t)res[j][e];
@@ -1063,9 +1069,9 @@ k_mt_deep(const uint32_t* __restrict__ series, int32_t stride, const int64_t* __
         }
     }
     wave_sync();
-    const uint64_t mine_mask = __ballot(mine && !h1);
+    const uint64_t mine_mask = __ballot(mine && h == 0);
     for (int r = 0; r < np; r++) {
-        if (((mine_mask >> (2 * r)) & 1ull) == 0ull) continue;
+        if (((mine_mask >> (LPP * r)) & 1ull) == 0ull) continue;
         int64_t* dst = G2 + (p0 + r) * (int64_t)T + tdeep;
         for (int i = lane; i < nd; i += kWave)
             dst[i] = (int64_t)(((uint64_t)s_x[r * gp + 2 * i + 1] << 32) | s_x[r * gp + 2 * i]);
@@ -1785,7 +1791,8 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     timer_end(p, s);
     if (ldeep < nlev && h.P > 0) {
         timer_begin(p, s, "k_mt_deep");
-        const unsigned nb = (unsigned)((h.P + kWave / 2 - 1) / (kWave / 2));   // 32 pixels per wave
+        const int npw = kWave / kDeepLanes;   // pixels per wave
+        const unsigned nb = (unsigned)((h.P + npw - 1) / npw);
         if (h.dpl == 4)
             k_mt_deep<4><<<nb, kWave, 0, s>>>(p->d_deep, dstride, p->d_off, p->d_rowmap, h.P, h.T, p->d_levels,
                                               ldeep, nlev, p->d_pix_total, p->d_G2);

@@ -721,7 +721,7 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
             if (lane + kWave < nl && r1 != rn1 && r1 < (uint32_t)nb) M[r1] = pre1;
             boundary_sums(M, nb, tau_bnd, T, IP, IF, p * T, w32);
         }
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        for (int i = lane; i < tdeep * kAccCopies; i += kWave) acc[i] = A(0);   // the lags the row kernel sums
         wave_sync();
         bool two = nl > kWave;
         for (int L = 0; L < nlev; L++) {
@@ -902,7 +902,7 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
             if (lane < nl && r != rn && r < (uint32_t)nb) M[r] = pre;
             boundary_sums(M, nb, s_tb, T, IP, IF, p * T, w32);
         }
-        for (int i = lane; i < T * kAccCopies; i += kWave) acc[i] = A(0);
+        for (int i = lane; i < tdeep * kAccCopies; i += kWave) acc[i] = A(0);   // the lags the row kernel sums
         wave_sync();
         for (int L = 0; L < nlev; L++) {
             const LevelParam lp = levels[L];

@@ -313,6 +313,11 @@ constexpr int kTrRowsWaves = kTrRowsBlock / 64;
 constexpr int kTrRowsCap = 15872;     // staged events per range (124 KB of LDS)
 constexpr int kTrRowsTable = 4096;    // per-group count and cursor entries
 constexpr int kTrRowsRegs = 48;       // register-held events per lane (2 VGPRs each; 128 in all)
+// batches of the large buckets (8 events per thread; the batch's per-pixel
+// group starts, 4096 u32, sit in the staging area behind them)
+constexpr int kTrRowsBatch = 8 * kTrRowsBlock;
+static_assert(kTrRowsBatch + kTrRowsTable / 2 <= kTrRowsCap, "batch staging");
+constexpr int kTrBatchSteps = (kTrRowsBatch + kTrRowsBlock - 1) / kTrRowsBlock;
 
 // Index of the first of the n nondecreasing values s[] that is >= x.
 __device__ __forceinline__ int lds_first_ge(const uint32_t* s, int n, uint32_t x)
@@ -407,8 +412,65 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
         if (b == nb - 1 && threadIdx.x == 0) off[P] = s1;
         __syncthreads();
         const uint32_t maxrow = s_max;
-        const bool direct = 2 * maxrow > (uint32_t)kTrRowsCap || n > 4 * ((uint32_t)kTrRowsCap - maxrow);
-        const uint32_t span = direct ? n + 1u : (uint32_t)kTrRowsCap - maxrow;
+        if (2 * maxrow > (uint32_t)kTrRowsCap || n > 4 * ((uint32_t)kTrRowsCap - maxrow)) {
+            // long rows or a large bucket: the bucket in batches of kTrRowsBatch
+            // events in bucket order; a batch is grouped by pixel in LDS and
+            // each pixel's group is written to its row as one run, where
+            // scattering single events straight to their rows wrote 3.3 times
+            // the bytes (largest batched job: 4.1 -> 2.7 ms).  Ranks are taken
+            // one event per thread and step with a barrier between steps, so a
+            // row's events keep bucket order except among those of one step,
+            // which the row kernels' sorts put right (sorting each group here
+            // cost more than it saved there)
+            uint32_t* bc = s_tab;                                                // [nloc] batch counts
+            uint32_t* bs = reinterpret_cast<uint32_t*>(s_stage + kTrRowsBatch);  // [nloc] batch group starts
+            uint32_t* rowpos = s_start;                                          // [nloc] next slot of each row
+            for (uint32_t b0 = 0; b0 < n; b0 += kTrRowsBatch) {
+                const uint32_t nbt = min(n - b0, (uint32_t)kTrRowsBatch);
+                for (int i = threadIdx.x; i < nloc; i += kTrRowsBlock) bc[i] = 0u;
+                __syncthreads();
+                // (the events are read again for the staging below, from cache:
+                // holding them here as well would spill)
+                uint32_t rk[kTrBatchSteps];
+#pragma unroll
+                for (int k = 0; k < kTrBatchSteps; k++) {
+                    const uint32_t i = (uint32_t)k * kTrRowsBlock + threadIdx.x;
+                    rk[k] = 0u;
+                    if (i < nbt) rk[k] = atomicAdd(&bc[(uint32_t)(seg[b0 + i] >> 16) & lmask], 1u);
+                    __syncthreads();
+                }
+                uint32_t bsum = 0u;
+                for (int u = 0; u < per; u++)
+                    if (i0 + u < nloc) bsum += bc[i0 + u];
+                int64_t btot;
+                uint32_t bex = (uint32_t)block_excl_scan((int64_t)bsum, s_warp, btot);
+                for (int u = 0; u < per; u++)
+                    if (i0 + u < nloc) {
+                        bs[i0 + u] = bex;
+                        bex += bc[i0 + u];
+                    }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kTrBatchSteps; k++) {
+                    const uint32_t i = (uint32_t)k * kTrRowsBlock + threadIdx.x;
+                    if (i < nbt) {
+                        const uint64_t x = seg[b0 + i];
+                        s_stage[bs[(uint32_t)(x >> 16) & lmask] + rk[k]] = x;
+                    }
+                }
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < nbt; i += kTrRowsBlock) {
+                    const uint64_t x = s_stage[i];
+                    const uint32_t p = (uint32_t)(x >> 16) & lmask;
+                    ev[s0 + rowpos[p] + (i - bs[p])] = (x & 0xFFFFFFFF00000000ull) | (x & 0xFFFFull);
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < nloc; i += kTrRowsBlock) rowpos[i] += bc[i];
+                __syncthreads();
+            }
+            continue;   // (the batches end on a barrier)
+        }
+        const uint32_t span = (uint32_t)kTrRowsCap - maxrow;
         for (uint32_t lo = 0; lo < n; lo += span) {
             const uint32_t hi = lo + span;
             // pixels whose rows start in [lo, hi) and the window their rows fill
@@ -420,12 +482,8 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
             auto place = [&](uint64_t x) {
                 const uint32_t p = (uint32_t)(x >> 16) & lmask;
                 const uint64_t y = (x & 0xFFFFFFFF00000000ull) | (x & 0xFFFFull);
-                if (direct) {
-                    ev[s0 + atomicAdd(&cur[p], 1u)] = y;
-                } else {
-                    const uint32_t st = s_start[p];
-                    if (st >= lo && st < hi) s_stage[atomicAdd(&cur[p], 1u) - wlo] = y;
-                }
+                const uint32_t st = s_start[p];
+                if (st >= lo && st < hi) s_stage[atomicAdd(&cur[p], 1u) - wlo] = y;
             };
             // an empty register barrier per held event: without it the
             // compiler hoists each event's pixel and output word out of this
@@ -440,8 +498,7 @@ k_tr_rows(const uint64_t* __restrict__ b1, const int64_t* __restrict__ hoff, int
                 if (e < gb) place(seg[e]);
             }
             __syncthreads();
-            if (!direct)
-                for (uint32_t i = threadIdx.x; i < whi - wlo; i += kTrRowsBlock) ev[s0 + wlo + i] = s_stage[i];
+            for (uint32_t i = threadIdx.x; i < whi - wlo; i += kTrRowsBlock) ev[s0 + wlo + i] = s_stage[i];
             __syncthreads();
         }
         __syncthreads();   // s_max and the row tables are rewritten for the next bucket

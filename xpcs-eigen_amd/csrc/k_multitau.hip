@@ -1717,6 +1717,11 @@ k_mt_long(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off,
 }
 
 // ---- launchers ------------------------------------------------------------------
+// Waves per workgroup of the LDS row kernel (counts: 8 / 16 / 32 KB of LDS per
+// wave for CAP 512 / 1024 / 2048).  These leave a quarter of the 160 KB of a
+// CU unused, but filling it measured slower: 10 or 5 waves for CAP 512, 5 for
+// 1024 and 2048 took the largest batched jobs' k_mt_lds from 12.0 and 11.7 ms
+// to 12.9-16.4 and 17.8-17.9 ms.
 constexpr int kSparseWaves = 8;
 constexpr int kSparseMidWaves = 4;    // 4 x 16 KB of LDS rows per workgroup (counts): 2 waves per SIMD
 constexpr int kSparseLongWaves = 2;   // 2 x 32 KB of LDS rows per workgroup (counts)

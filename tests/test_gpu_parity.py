@@ -532,12 +532,18 @@ def test_sum_layouts_and_wide_rows(xmt, orc, wide):
         plan = xmt.Plan(W, H, F, 4, dq, sq)
         plan.set_tuning(sums64=sums64)
         plan.ingest_rows(roff, np.concatenate(frames), np.concatenate(counts))
+        if sums64:
+            plan.autotune()             # launch shapes only: the layout stays
         plan.execute()
         g, a, b = plan.sums()
         assert np.array_equal(a, IP) and np.array_equal(b, IF)
         assert np.array_equal(g, G2)
         res.append(plan.g2())
         assert plan.get_tuning()["sums64"] == sums64
+        # the readers follow the layout of the run that wrote the sums
+        plan.set_tuning(sums64=1 - sums64)
+        g, a, b = plan.sums()
+        assert np.array_equal(a, IP) and np.array_equal(b, IF)
         plan.close()
     tot = np.array([int(c.astype(np.int64).sum()) for c in counts])
     assert tot.max() > 2 ** 32 and (tot[lens <= 128].max() > 2 ** 32) == wide

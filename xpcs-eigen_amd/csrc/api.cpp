@@ -933,8 +933,6 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
     if (p->fsnorm && p->mode == 3)
         return fail(XPCS_ERR_UNSUPPORTED, "execute: normalize_by_framesum needs frame sums (IMM or packed ingest)");
     HostTables& h = p->h;
-    // integer IP/IF in the 32-bit layout (unless tuned off); FP64 sums stay 8 bytes
-    p->ipif32 = !p->fsnorm && p->tune.sums64 == 0;
     const bool dense_stream = p->mode == 2 && dense_stream_ok(p);
     const bool long_rows = !dense_stream;
     if (long_rows && !p->d_scratch) {
@@ -1003,6 +1001,10 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
 void enqueue_execute(xpcs_mt_plan* p, hipStream_t s)
 {
     HostTables& h = p->h;
+    // integer IP/IF in the 32-bit layout (unless tuned off; FP64 sums stay 8
+    // bytes), recorded with the launches that write them, so the readers follow
+    // the last multi-tau run whatever ran on the plan since
+    p->ipif32 = !p->fsnorm && p->tune.sums64 == 0;
     launch_fill_u32(p->d_counters, 8, s);
     if (p->d_fs) launch_fill_u64(reinterpret_cast<uint64_t*>(p->d_fs), (size_t)h.F * h.ns, s);
     if (p->fsnorm && p->mode == 2 && dense_stream_ok(p)) {

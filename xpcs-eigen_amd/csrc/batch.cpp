@@ -105,13 +105,16 @@ extern "C" xpcs_status xpcs_mt_autotune(xpcs_mt_plan* p, xpcs_mt_tuning* best_ou
     if (!p) return fail(XPCS_ERR_INVALID_ARG, "autotune: NULL plan");
     if (p->mode == 0) return fail(XPCS_ERR_STATE, "autotune: nothing ingested");
     const TuneKey key = key_of(p);
+    // the sums layout is the caller's choice, not a launch shape: kept as set
+    const int32_t sums64 = p->tune.sums64;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
             p->tune = it->second;
+            p->tune.sums64 = sums64;
             p->generation++;
-            if (best_out) *best_out = it->second;
+            if (best_out) *best_out = p->tune;
             return XPCS_OK;
         }
     }
@@ -125,6 +128,7 @@ extern "C" xpcs_status xpcs_mt_autotune(xpcs_mt_plan* p, xpcs_mt_tuning* best_ou
     const bool was = p->capturing;
     p->capturing = true;   // no timer events while tuning
     xpcs_mt_tuning cur{};
+    cur.sums64 = sums64;
     p->tune = cur;
     double cur_ms = time_job(p, s, a, b);
     xpcs_status st = XPCS_OK;

@@ -211,7 +211,7 @@ struct xpcs_mt_plan {
     int64_t* d_G2 = nullptr;             // [P][T]
     int64_t* d_IP = nullptr;             // [P][T] int64, or with ipif32 the 32-bit layout
     int64_t* d_IF = nullptr;             //   (low words [P*T], high words of pixels with totals >= 2^32 after them)
-    bool ipif32 = false;                 // integer sums of the batch path (set by prepare_execute)
+    bool ipif32 = false;                 // integer sums of the batch path (set by enqueue_execute)
     bool wide_rows = false;              // ingest_rows: a row of <= 128 events totals 2^32 or more
     uint64_t* d_pix_total = nullptr;     // [P] sum of level-0 values
     xpcs::NormPartial* d_part = nullptr; // [nchunks][T]

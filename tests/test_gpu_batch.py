@@ -175,3 +175,33 @@ def test_autotune_keeps_results_and_caches(xmt):
             b.add(p)
         b.execute(autotune=True)
         b.synchronize()
+
+
+def test_two_plans_alternating_overlap_ingest(xmt):
+    """A series of jobs of one geometry through two plans used in turn: each
+    job is ingested while the other plan's job is still executing, and every
+    job's sums and g2 equal a lone plan's."""
+    jobs = [cases.sparse_case(W=64, H=48, F=1200, dpl=8, mu=mu, tau_c=tc, seed=300 + j, S=8, Q=4)
+            for j, (mu, tc) in enumerate([(0.02, 20.0), (0.2, 5.0), (0.005, 80.0), (0.05, 40.0), (0.6, 10.0)])]
+    want = []
+    for c in jobs:
+        p = make_plan(xmt, c)
+        p.execute()
+        want.append(results(p))
+        p.close()
+    c0 = jobs[0]
+    plans = [xmt.Plan(c0["W"], c0["H"], c0["F"], c0["dpl"], c0["dq"], c0["sq"]) for _ in range(2)]
+    try:
+        prev = None
+        for k, c in enumerate(jobs):
+            print("job", k, flush=True)
+            cur = plans[k % 2]
+            cur.ingest_imm(c["imm"])
+            if prev is not None:
+                assert_same(results(prev), want[k - 1])
+            cur.execute()
+            prev = cur
+        assert_same(results(prev), want[-1])
+    finally:
+        for p in plans:
+            p.close()

@@ -71,6 +71,18 @@ def test_config1_plain(xmt, orc):
     assert plan.T == 34
 
 
+@pytest.mark.parametrize("F,dpl", [(3000, 4), (16000, 8)])
+def test_register_rows_boundary_ranks(xmt, orc, F, dpl):
+    """The register row kernel finds each event's boundary rank in per-64-frame
+    words staged in LDS when they fit beside its accumulators (3000 frames) and
+    in the per-frame table otherwise (16000 frames: 250 words, more than the
+    LDS room that keeps its occupancy); both give the oracle's sums."""
+    c = cases.sparse_case(W=24, H=24, F=F, dpl=dpl, mu=0.004, tau_c=40.0, seed=41 + F, S=8, Q=4)
+    plan, r = check_against_oracle(xmt, orc, c)
+    px, _ = plan.row_stats()
+    assert px[1] > 0
+
+
 def test_long_rows(xmt, orc):
     """High occupancy: rows longer than the on-chip capacity take the dense-row kernel."""
     c = cases.sparse_case(W=16, H=16, F=3000, dpl=4, mu=2.0, tau_c=30.0, seed=3)

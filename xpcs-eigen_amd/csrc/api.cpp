@@ -169,8 +169,31 @@ extern "C" xpcs_status xpcs_mt_plan_create(const xpcs_mt_desc* desc, xpcs_mt_pla
             rank[(size_t)f] = (uint16_t)j;
         }
         p->nrank = (int32_t)b.size();
+        // per 64 frames: the bits of the boundary frames in the word and the
+        // rank before it; rank(f) = z + popcount(bits at or below f & 63)
+        const int32_t nw = (h.F + 63) / 64;
+        std::vector<uint4> words((size_t)std::max(nw, 1));
+        for (int32_t w = 0; w < nw; w++) {
+            uint64_t bits = 0;
+            for (int32_t f = 64 * w; f < std::min(h.F, 64 * w + 64); f++)
+                if (rank[(size_t)f] != (f > 0 ? rank[(size_t)f - 1] : 0)) bits |= 1ull << (f - 64 * w);
+            words[(size_t)w] = make_uint4((uint32_t)bits, (uint32_t)(bits >> 32), w > 0 ? rank[(size_t)(64 * w - 1)] : 0u, 0u);
+        }
+        // ranks rise by one at each boundary (distinct, sorted), so the words
+        // give back the table; checked here, and left unused otherwise
+        bool words_ok = true;
+        for (int32_t f = 0; f < h.F && words_ok; f++) {
+            const uint4 q = words[(size_t)(f / 64)];
+            const uint64_t bits = ((uint64_t)q.y << 32) | q.x;
+            const uint32_t r = q.z + (uint32_t)__builtin_popcountll(bits & (~0ull >> (63 - (f & 63))));
+            words_ok = r == rank[(size_t)f];
+        }
         TRY(upload(&p->d_tau_bnd, tj, s));
         TRY(upload(&p->d_frame_rank, rank, s));
+        if (words_ok && nw > 0) {
+            TRY(upload(&p->d_rank_words, words, s));
+            p->nrank_words = nw;
+        }
     }
     TRY(upload(&p->d_chunks, h.chunks, s));
     TRY(upload(&p->d_group_chunk_first, h.group_chunk_first, s));
@@ -267,7 +290,7 @@ extern "C" xpcs_status xpcs_mt_plan_destroy(xpcs_mt_plan* p)
 {
     if (!p) return XPCS_OK;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_tau_bnd); dfree(p->d_frame_rank); dfree(p->d_chunks);
+    dfree(p->d_pix2c); dfree(p->d_mapped); dfree(p->d_taus); dfree(p->d_levels); dfree(p->d_tau_bnd); dfree(p->d_frame_rank); dfree(p->d_rank_words); dfree(p->d_chunks);
     dfree(p->d_group_chunk_first); dfree(p->d_group_q); dfree(p->d_group_size);
     dfree(p->d_q_chunk_first); dfree(p->d_q_first_group); dfree(p->d_weight); dfree(p->d_c2pix);
     dfree(p->d_sbin_of_c); dfree(p->d_sbin_pix); dfree(p->d_imm_owned); dfree(p->d_sel_off); dfree(p->d_sel_dlen);
@@ -940,6 +963,7 @@ xpcs_status prepare_execute(xpcs_mt_plan* p)
         if (st) return st;
     }
     if (p->mode == 1 || p->mode == 3) {
+        rank_lds_setup(p);
         int stride = 0;
         if (deep_level(p, &stride) < (int)h.levels.size()) {
             const size_t need = (size_t)h.P * (size_t)stride;

@@ -195,6 +195,12 @@ struct xpcs_mt_plan {
     uint16_t* d_frame_rank = nullptr;    // [F] number of B_j <= frame
     int4* d_tau_bnd = nullptr;           // [T] (j of IP, j of the IF head, j of the level end, 0)
     int32_t nrank = 0;                   // number of distinct B_j
+    // the same ranks per 64 frames: (boundary bits lo, hi, rank before the
+    // word, 0); k_mt_reg stages them in LDS when that costs no occupancy
+    // (rank_lds: dynamic LDS bytes, 0 = gather d_frame_rank)
+    uint4* d_rank_words = nullptr;
+    int32_t nrank_words = 0;
+    int32_t rank_lds[2] = {-1, -1};   // [counts / FP64]; -1: undecided
 
     // long pixels
     int32_t* d_long_list = nullptr;      // [P]
@@ -260,6 +266,7 @@ void launch_mt_sparse(xpcs_mt_plan* p, hipStream_t s);
 // first level of the one-lane-per-pixel kernel of the sparse path (the level
 // count: none) and its series row length in u32 (0: none)
 int deep_level(const xpcs_mt_plan* p, int* stride);
+void rank_lds_setup(xpcs_mt_plan* p);
 void launch_mt_long(xpcs_mt_plan* p, hipStream_t s, bool dense_rows);
 void launch_dense_transpose(xpcs_mt_plan* p, hipStream_t s);
 void launch_side(xpcs_mt_plan* p, hipStream_t s);

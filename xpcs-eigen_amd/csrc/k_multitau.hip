@@ -624,6 +624,21 @@ __device__ __forceinline__ void reg2_lags(uint32_t k0, V v0, uint32_t k1, V v1, 
     }
 }
 
+// Boundary rank of frame f (the number of prefix-sum boundaries at or below
+// it): from the per-64-frame words staged in LDS (bits of the boundary frames,
+// rank before the word) when the launch gave them room, else gathered from the
+// per-frame table (one dependent global load less per row with the words).
+__device__ __forceinline__ uint32_t boundary_rank(const uint4* s_rw, bool staged,
+                                                  const uint16_t* __restrict__ frame_rank, uint32_t f)
+{
+    if (staged) {
+        const uint4 q = s_rw[f >> 6];
+        const uint64_t bits = ((uint64_t)q.y << 32) | q.x;
+        return q.z + (uint32_t)__popcll(bits & (~0ull >> (63u - (f & 63u))));
+    }
+    return (uint32_t)frame_rank[f];
+}
+
 // The level-0 value of an event: its count, or count x fw[frame].
 template <typename V>
 __device__ __forceinline__ V event_value(uint32_t c, uint32_t f, const double* __restrict__ fw, int32_t F)
@@ -840,15 +855,18 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
          const LevelParam* __restrict__ levels, int32_t nlev,
          int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
          uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
-         int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32, bool reg_ok)
+         int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32, bool reg_ok,
+         const uint4* __restrict__ rank_words, int32_t nrw)
 {
     using A = typename LagAcc<V>::type;
     __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
     __shared__ uint32_t s_ser[WAVES][sizeof(V) == 4 ? kDeepMaxFrames : 1];
     __shared__ int4 s_tb[kRegMaxTaus];   // each tau's boundary indices, read by every row
+    extern __shared__ uint4 s_rw[];      // nrw rank words (dynamic LDS)
     const int lane = lane_id();
     const int w = wave_id();
     for (int i = (int)threadIdx.x; i < T && i < kRegMaxTaus; i += kWave * WAVES) s_tb[i] = tau_bnd[i];
+    for (int i = (int)threadIdx.x; i < nrw; i += kWave * WAVES) s_rw[i] = rank_words[i];
     __syncthreads();
     A* acc = s_acc[w];
     const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
@@ -883,11 +901,11 @@ k_mt_reg(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, int32
         }
         // each event's boundary rank, gathered before the sort: rows usually
         // arrive in order, and then it needs no second gather
-        uint32_t r = lane < n ? (uint32_t)frame_rank[k] : kNoKey;
+        uint32_t r = lane < n ? boundary_rank(s_rw, nrw > 0, frame_rank, k) : kNoKey;
         const bool moved = reg_unsorted(k);
         reg_sort(k, v, n);
         int nl = reg_level(k, v, n, 0, (uint32_t)F, false);
-        if (moved || nl != n) r = lane < nl ? (uint32_t)frame_rank[k] : kNoKey;
+        if (moved || nl != n) r = lane < nl ? boundary_rank(s_rw, nrw > 0, frame_rank, k) : kNoKey;
         // IP and IF of every tau from the level-0 prefix sums:
         // IP = S0(2^L (lf - tau')), IF = S0(2^L lf) - S0(2^L tau'), read at the
         // boundaries (the accumulator slots serve as the boundary slots first)
@@ -1728,6 +1746,32 @@ constexpr int kSparseLongWaves = 2;   // 2 x 32 KB of LDS rows per workgroup (co
 constexpr int kRegWaves = 8;
 constexpr int kReg2Waves = 4;
 
+// Dynamic LDS bytes for the rank words of k_mt_reg: all of them, when that
+// leaves the kernel's workgroups per CU unchanged, else 0 (the kernel gathers
+// the per-frame table).  1 Mpx job: k_mt_reg 1.134 -> 1.121 ms; k_mt_reg2,
+// whose rows are fewer and longer, took 0.371 against 0.356 ms with them and
+// gathers.
+static int rank_lds_bytes(const void* kernel, int block, int words)
+{
+    if (words <= 0) return 0;
+    const int bytes = words * (int)sizeof(uint4);
+    int base = 0, with = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, kernel, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kernel, block, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return base > 0 && with >= base ? bytes : 0;
+}
+
+void rank_lds_setup(xpcs_mt_plan* p)
+{
+    if (p->rank_lds[0] >= 0) return;   // the tables are fixed at plan creation
+    const int w = p->nrank_words;
+    p->rank_lds[0] = rank_lds_bytes(reinterpret_cast<const void*>(k_mt_reg<kRegWaves, uint32_t>), kWave * kRegWaves, w);
+    p->rank_lds[1] = rank_lds_bytes(reinterpret_cast<const void*>(k_mt_reg<kRegWaves, double>), kWave * kRegWaves, w);
+}
+
 // The row kernels of the sparse path, counts (V = uint32_t, exact 64-bit
 // sums) or frame-weighted values (V = double, normalize_by_framesum): the
 // classification pass, one wave per register row, then the listed rows.
@@ -1751,9 +1795,12 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     const int ldeep = sizeof(V) == 4 ? deep_level(p, &dstride) : nlev;
     const bool w32 = sizeof(V) == 4 && p->ipif32;
     const int64_t hw = (int64_t)h.P * h.T;   // high words of the w32 layout
-    k_mt_reg<kRegWaves, V><<<reg_blocks(p), kWave * kRegWaves, 0, s>>>(
+    const int vi = sizeof(V) == 4 ? 0 : 1;
+    const int rl_reg = std::max(p->rank_lds[vi], 0);
+    k_mt_reg<kRegWaves, V><<<reg_blocks(p), kWave * kRegWaves, rl_reg, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels, nlev, h.F, p->d_G2,
-        p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep, dstride, w32, regs);
+        p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep, dstride, w32, regs, p->d_rank_words,
+        rl_reg / (int)sizeof(uint4));
     timer_end(p, s);
     timer_begin(p, s, t_lds);
     if (regs)

@@ -64,6 +64,9 @@ static void dfree(T*& p)
 void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name)
 {
     if (!(p->h.flags & XPCS_MT_TIMING) || p->capturing) return;
+    // timers follow one another (none is nested), so one flag serves timer_end
+    p->timer_skipped = !p->timer_only.empty() && p->timer_only != name;
+    if (p->timer_skipped) return;
     KernelTimer t;
     t.name = name;
     if (p->event_pool.size() >= 2) {
@@ -79,7 +82,7 @@ void timer_begin(xpcs_mt_plan* p, hipStream_t s, const char* name)
 
 void timer_end(xpcs_mt_plan* p, hipStream_t s)
 {
-    if (!(p->h.flags & XPCS_MT_TIMING) || p->capturing) return;
+    if (!(p->h.flags & XPCS_MT_TIMING) || p->capturing || p->timer_skipped) return;
     (void)hipEventRecord(p->pending.back().b, s);
 }
 
@@ -1370,6 +1373,13 @@ extern "C" xpcs_status xpcs_mt_kernel_times_reset(xpcs_mt_plan* p)
     if (p->executed) (void)hipStreamSynchronize(p->stream_last ? p->stream_last : p->stream);
     collect_timers(p);
     p->times.clear();
+    return XPCS_OK;
+}
+
+extern "C" xpcs_status xpcs_mt_set_timing_filter(xpcs_mt_plan* p, const char* name)
+{
+    if (!p) return fail(XPCS_ERR_INVALID_ARG, "set_timing_filter: NULL plan");
+    p->timer_only = name ? name : "";
     return XPCS_OK;
 }
 

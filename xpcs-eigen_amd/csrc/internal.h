@@ -233,6 +233,8 @@ struct xpcs_mt_plan {
     std::vector<xpcs::KernelTimer> pending;
     std::map<std::string, std::pair<double, int64_t>> times;
     std::vector<hipEvent_t> event_pool;
+    std::string timer_only;              // time only this timer (empty: all)
+    bool timer_skipped = false;          // the open timer is filtered out
 };
 
 namespace xpcs {

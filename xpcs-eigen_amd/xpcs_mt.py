@@ -33,7 +33,8 @@ EXPORTS = [
     "xpcs_mt_schedule", "xpcs_mt_pixel_order", "xpcs_mt_ingest_imm", "xpcs_mt_ingest_imm_device",
     "xpcs_imm_index", "xpcs_mt_ingest_rows", "xpcs_mt_ingest_events", "xpcs_mt_execute", "xpcs_mt_synchronize",
     "xpcs_mt_read_sums_i64", "xpcs_mt_read_ref_f32", "xpcs_mt_read_g2", "xpcs_mt_device_results",
-    "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_build_qmap",
+    "xpcs_mt_read_side", "xpcs_mt_kernel_times", "xpcs_mt_kernel_times_reset", "xpcs_mt_set_timing_filter",
+    "xpcs_build_qmap",
     "xpcs_mt_read_dark", "xpcs_mt_set_tuning", "xpcs_mt_get_tuning", "xpcs_mt_autotune",
     "xpcs_mt_batch_create", "xpcs_mt_batch_add", "xpcs_mt_batch_execute", "xpcs_mt_batch_synchronize",
     "xpcs_mt_batch_destroy", "xpcs_mt_stream_bytes", "xpcs_mt_stream_open", "xpcs_mt_stream_push_imm",
@@ -106,6 +107,7 @@ def lib():
             "xpcs_mt_read_side": ([vp, i32, vp, vp, vp, vp], i32),
             "xpcs_mt_kernel_times": ([vp, vp, C.c_size_t, vp, vp, i32, vp], i32),
             "xpcs_mt_kernel_times_reset": ([vp], i32),
+            "xpcs_mt_set_timing_filter": ([vp, C.c_char_p], i32),
             "xpcs_build_qmap": ([i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, vp], i32),
             "xpcs_mt_read_dark": ([vp, vp, vp], i32),
             "xpcs_mt_set_tuning": ([vp, vp], i32),
@@ -467,6 +469,10 @@ class Plan:
 
     def kernel_times_reset(self):
         check(lib().xpcs_mt_kernel_times_reset(self._h))
+
+    def set_timing_filter(self, name=None):
+        """Time only the kernel timer called `name` (None: every timer)."""
+        check(lib().xpcs_mt_set_timing_filter(self._h, name.encode() if name else None))
 
 
 class Batch:

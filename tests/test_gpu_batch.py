@@ -205,3 +205,29 @@ def test_two_plans_alternating_overlap_ingest(xmt):
     finally:
         for p in plans:
             p.close()
+
+
+def test_timing_filter(xmt):
+    """With a timing filter only the named kernel timer records; without it
+    every timer does, and the sums are the same either way."""
+    c = cases.sparse_case(W=48, H=48, F=1200, dpl=8, mu=0.03, tau_c=20.0, seed=95, S=8, Q=4)
+    p = xmt.Plan(c["W"], c["H"], c["F"], c["dpl"], c["dq"], c["sq"], timing=True)
+    try:
+        p.ingest_imm(c["imm"])
+        p.execute()
+        want = results(p)
+        every = set(p.kernel_times())
+        assert "k_mt_reg" in every and len(every) > 3
+        p.set_timing_filter("k_mt_reg")
+        p.kernel_times_reset()
+        p.execute()
+        assert_same(results(p), want)
+        kt = p.kernel_times()
+        assert set(kt) == {"k_mt_reg"} and kt["k_mt_reg"][1] == 1
+        p.set_timing_filter(None)
+        p.kernel_times_reset()
+        p.execute()
+        p.synchronize()
+        assert set(p.kernel_times()) == every
+    finally:
+        p.close()

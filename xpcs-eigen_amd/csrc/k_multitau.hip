@@ -683,15 +683,18 @@ k_mt_reg2(const uint64_t* __restrict__ ev, const int64_t* __restrict__ off, cons
           const uint16_t* __restrict__ frame_rank, int32_t nb, const LevelParam* __restrict__ levels,
           int32_t nlev, int32_t F, int64_t* __restrict__ G2, int64_t* __restrict__ IP, int64_t* __restrict__ IF,
           uint64_t* __restrict__ pix_total, const int32_t* __restrict__ rowmap, const double* __restrict__ fw,
-          int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32)
+          int32_t ldeep, uint32_t* __restrict__ deep, int32_t dstride, bool w32, int32_t acc_words)
 {
     using A = typename LagAcc<V>::type;
-    __shared__ A s_acc[WAVES][kRegMaxTaus * kAccCopies];
+    // each wave's accumulators (acc_words entries: the taus before the
+    // hand-over in kAccCopies copies, or the boundary slots if more) in
+    // dynamic LDS, sized by the launch to what the schedule needs
+    extern __shared__ __align__(16) unsigned char s_acc_dyn[];
     __shared__ uint32_t s_stage[WAVES][reg2_stage_words<V>()];
     const int tdeep = ldeep < nlev ? levels[ldeep].first : T;   // taus of the levels before the hand-over
     const int lane = lane_id();
     const int w = wave_id();
-    A* acc = s_acc[w];
+    A* acc = reinterpret_cast<A*>(s_acc_dyn) + (int64_t)w * acc_words;
     uint32_t* stage = s_stage[w];
     const int64_t nrows = (int64_t)*count;
     const int64_t nw = (int64_t)gridDim.x * WAVES;
@@ -1796,6 +1799,13 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     const bool w32 = sizeof(V) == 4 && p->ipif32;
     const int64_t hw = (int64_t)h.P * h.T;   // high words of the w32 layout
     const int vi = sizeof(V) == 4 ? 0 : 1;
+    // k_mt_reg2's accumulators per wave: the taus it sums (before the
+    // hand-over level) in kAccCopies copies, at least the nb boundary slots,
+    // even (16-byte steps between waves)
+    using A2 = typename LagAcc<V>::type;
+    const int tsum = ldeep < nlev ? h.levels[(size_t)ldeep].first : h.T;
+    int reg2_acc_words = std::max(tsum * kAccCopies, (int)((p->nrank * (int)sizeof(V) + (int)sizeof(A2) - 1) / (int)sizeof(A2)));
+    reg2_acc_words = (reg2_acc_words + 1) & ~1;
     const int rl_reg = std::max(p->rank_lds[vi], 0);
     k_mt_reg<kRegWaves, V><<<reg_blocks(p), kWave * kRegWaves, rl_reg, s>>>(
         p->d_ev, p->d_off, h.P, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank, p->d_levels, nlev, h.F, p->d_G2,
@@ -1804,10 +1814,10 @@ static void launch_rows(xpcs_mt_plan* p, hipStream_t s, const char* t_reg, const
     timer_end(p, s);
     timer_begin(p, s, t_lds);
     if (regs)
-        k_mt_reg2<kReg2Waves, V><<<lds_blocks(p), kWave * kReg2Waves, 0, s>>>(
+        k_mt_reg2<kReg2Waves, V><<<lds_blocks(p), kWave * kReg2Waves, reg2_acc_words * (int)sizeof(A2) * kReg2Waves, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, p->d_counters + 2, h.T, p->d_tau_bnd, p->d_frame_rank, p->nrank,
             p->d_levels, nlev, h.F, p->d_G2, p->d_IP, p->d_IF, p->d_pix_total, p->d_rowmap, fw, ldeep, p->d_deep,
-            dstride, w32);
+            dstride, w32, reg2_acc_words);
     else
         k_mt_sparse<kMidSmall, kSparseWaves, V><<<lds_blocks(p), kWave * kSparseWaves, 0, s>>>(
             p->d_ev, p->d_off, p->d_mid_list, 1, p->d_counters + 2, p->d_taus, h.T, p->d_levels,
